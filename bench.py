@@ -1,0 +1,173 @@
+"""Widom-insertion throughput on one MI355X: rigid 3-site CO2 with Ewald
+electrostatics in a synthetic 600-atom triclinic framework (replicated 2x2x2
+to 4800 atoms), 298 K, r_c 12.8 A, Ewald precision 1e-6.  One step = one
+Widom run of 10^6 insertions (5 blocks x 200,000) through pm_widom, inputs
+resident on the device (the insertion stream is generated on the device).
+
+Prints one JSON line: insertions/s of the mixed-precision path (value), the
+plain FP64 path beside it, the roofline of the energy kernel (FP32 VALU
+bound; algorithmic flops per insertion from the per-pair constants below
+and the exact expected in-cutoff pair count) and the CPU baseline: the
+oracle's reference-form restatement (e^{ik.r} tables over every replicated
+charge and replicated-box k-vector, as the reference's _phi) timed on a
+bounded sample on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Widom insertions/s on 1 MI355X, mixed vs FP64 GPU vs CPU, at <=1e-4 rel K_H"
+
+# algorithmic flops per unit of work
+C_GEOM = 21   # per (site, replicated atom) candidate pair
+C_LJ = 10     # per in-cutoff LJ pair
+C_SR = 20     # per in-cutoff Ewald real-space pair
+C_K = 14      # per (charge site, unit-cell k-vector)
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6
+
+
+def flops_per_insertion(n_rep, omega_rep, r_cut, n_lj, n_q, n_k, sr_cut):
+    """Candidate pairs are all-pairs (every site against every replicated
+    atom); the expected in-cutoff count of a uniformly placed site is
+    N_rep * (4/3) pi r_c^3 / Omega_rep (the cutoff sphere fits in the box)."""
+    in_lj = n_rep * (4.0 / 3.0) * math.pi * r_cut ** 3 / omega_rep
+    in_sr = n_rep * (4.0 / 3.0) * math.pi * sr_cut ** 3 / omega_rep
+    n_sites = max(n_lj, n_q)  # CO2: LJ and charge sites coincide
+    return n_sites * n_rep * C_GEOM + n_lj * in_lj * C_LJ + n_q * in_sr * C_SR + n_q * n_k * C_K
+
+
+def cpu_baseline(pm, case, xr, seed, target_s):
+    from oracle import oracle as orc
+    x, ff, co2 = case.crystal, case.ljff, case.probes["CO2"]
+    lj_off, q_off = pm.site_offsets(co2)
+    o = orc.OracleSystem((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma),
+                         x.xf.T, [ff.index(s) for s in x.species], x.atom_q, ff.epsilon_table,
+                         ff.sigma2_table, math.sqrt(ff.r2_cutoff), lj_off.T,
+                         [ff.index(s) for s in co2.atom_species], q_off.T, co2.charges_q,
+                         ewald_precision=case.ewald_precision)
+    threads = orc.lib().orc_num_threads()
+    # probe the cost, then size the sample to ~target_s of CPU time
+    n0 = max(threads, 4)
+    t = time.perf_counter()
+    o.widom_energies(seed, 0, n0, es_mode=1)
+    dt = time.perf_counter() - t
+    n = max(n0, int(n0 * target_s / max(dt, 1e-6)))
+    t = time.perf_counter()
+    o.widom_energies(seed, n0, n, es_mode=1)
+    dt = time.perf_counter() - t
+    ref_rate = n / dt
+    # structure-factor CPU form, same cores (separates algorithm from device gains)
+    t = time.perf_counter()
+    m = max(threads * 8, 64)
+    o.widom_energies(seed, 0, m, es_mode=0)
+    sk_rate = m / (time.perf_counter() - t)
+    return {
+        "value": ref_rate, "unit": "insertions/s", "cores": threads, "kind": "port",
+        "sample": (f"{n} insertions of this workload (stream offset {n0}), reference form of the "
+                   f"Fourier sum (e^(ik.r) tables over {xr.n_atoms} replicated charges x "
+                   f"{o.wt.size} replicated-box k-vectors per site), OpenMP over insertions"),
+        "sk_form_insertions_per_s": sk_rate,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-per-block", type=int, default=200000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(Widom blocks would shard by insertion range with one 120-byte reduction).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__
+    pm = __graft_entry__.load_package()
+    case = pm.synthetic.co2_widom_case(n_per_block=args.n_per_block)
+    x, ff, co2 = case.crystal, case.ljff, case.probes["CO2"]
+    rep = pm.replication_factors(x.box, ff)
+    xr = pm.replicate(x, rep)
+    ep = pm.setup_Ewald_sum(xr.box, math.sqrt(ff.r2_cutoff), eps=case.ewald_precision)
+    fw = pm.device.framework(xr, ff)
+    ew = pm.device.ewald(xr, fw, ep)
+    pr = pm.device.probe(co2, ff)
+    n_k = pm.device.ewald_info(ew)
+    nb = args.n_per_block
+    seed = case.meta["insertion_seed"]
+    n_ins = 5 * nb
+
+    def run(precision):
+        return pm.device.widom(fw, ff, ew, pr, case.temperature, seed, 5, nb, precision)
+
+    for _ in range(args.warmup):
+        run("mixed")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sums = run("mixed")
+        kernel_ms.append(pm.device.last_kernel_stats()[0])
+    elapsed = time.perf_counter() - t0
+    value = args.steps * n_ins / elapsed
+    k_ms = float(np.mean(kernel_ms))
+    print(f"mixed: {args.steps} steps, {elapsed / args.steps * 1e3:.3f} ms/step, kernel "
+          f"{k_ms:.3f} ms", file=sys.stderr, flush=True)
+
+    flop = flops_per_insertion(xr.n_atoms, xr.box.Omega, math.sqrt(ff.r2_cutoff), co2.n_atoms,
+                               co2.n_charges, n_k, ep.sr_cutoff_r)
+    achieved = flop * n_ins / (k_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "insertions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 wrap/r^2/Fourier/Boltzmann sums", "data": "synthetic",
+        "config": {
+            "workload": ("co2_widom: rigid 3-site CO2 at 298 K, Ewald precision 1e-6, r_c 12.8 A, "
+                         f"synthetic triclinic 600-atom framework replicated {rep} to "
+                         f"{xr.n_atoms} atoms, {n_k} unit-cell k-vectors, "
+                         f"{n_ins} insertions = 5 blocks x {nb}"),
+            "insertions_per_step": n_ins, "precision": "mixed"},
+        "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "energy_kernel<3, true, float>", "kernel_ms": k_ms,
+                     "flop_per_insertion": flop},
+    }
+    if not args.no_fp64:
+        run("fp64")
+        t = time.perf_counter()
+        s64 = run("fp64")
+        dt64 = time.perf_counter() - t
+        k64 = pm.device.last_kernel_stats()[0]
+        r64 = pm.block_statistics(s64, nb, case.temperature, 1.0)
+        r32 = pm.block_statistics(sums, nb, case.temperature, 1.0)
+        kh = "henry coefficient [mol/(m³-bar)]"
+        out["fp64"] = {"value": n_ins / dt64, "kernel_ms": k64,
+                       "achieved_tflops": flop * n_ins / (k64 * 1e-3) / 1e12,
+                       "peak_tflops": PEAK_FP64_TFLOPS}
+        out["mixed_over_fp64"] = value / (n_ins / dt64)
+        out["kh_rel_diff_mixed_vs_fp64"] = abs(r32[kh] - r64[kh]) / abs(r64[kh])
+        out["u_rel_diff_mixed_vs_fp64"] = abs(r32["⟨U⟩ (K)"] - r64["⟨U⟩ (K)"]) / abs(r64["⟨U⟩ (K)"])
+    if not args.no_cpu:
+        cb = cpu_baseline(pm, case, xr, seed, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

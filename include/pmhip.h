@@ -1,0 +1,175 @@
+/*
+ * pmhip.h -- C ABI of the MI355X (gfx950) guest-host energy library
+ * libpmhip.so: Lennard-Jones + Ewald energies of rigid adsorbates in
+ * periodic frameworks, Widom-insertion block sums and energy grids.
+ *
+ * The reference (PorousMaterials.jl v0.4.2) is pure Julia and has no FFI of
+ * its own; each entry point below names the Julia function whose numerical
+ * work it replaces, so a Julia `ccall` shim can keep the reference's call
+ * surface.  Paths are relative to the reference tree.
+ *
+ * Conventions
+ *   - Every function returns PM_OK (0) or a PM_ERR_* code; the message of
+ *     the last failure on the calling thread is pm_last_error().
+ *   - 3x3 matrices are column-major (Julia layout): m[row + 3*col].
+ *     f_to_c must be upper-triangular (Xtals Box convention, a along x).
+ *   - Coordinates are AoS xyz per point (a Julia 3xN Matrix{Float64}).
+ *   - Caller buffers are borrowed for the duration of a call only; device
+ *     state lives in the opaque handles until the matching *_destroy.
+ *   - Calls are synchronous.  All calls are serialised by one library
+ *     mutex, so any thread may call; handles must not be destroyed while in
+ *     use by another thread.
+ *   - precision: PM_FP64 computes every pair in FP64; PM_MIXED computes pair
+ *     arithmetic (LJ, erfc/r) in FP32 with FP64 fractional wrap, r^2 and
+ *     cutoff tests, FP64 per-point accumulation, FP64 Fourier phases and
+ *     structure factors and FP64 Boltzmann sums.
+ */
+#ifndef PMHIP_H
+#define PMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PM_OK = 0,
+    PM_ERR_ARG = 1,     /* invalid argument (shape, index, range)      */
+    PM_ERR_DEVICE = 2,  /* HIP runtime error                           */
+    PM_ERR_OOM = 3,     /* device allocation failed                    */
+    PM_ERR_STATE = 4    /* library not initialised / bad handle        */
+};
+
+enum { PM_FP64 = 0, PM_MIXED = 1 };
+
+#define PM_MAX_SITES 8          /* sites (LJ spheres + point charges) per probe */
+#define PM_MAX_FW_SPECIES 64    /* distinct LJ species within one framework     */
+
+typedef struct pm_framework pm_framework;
+typedef struct pm_forcefield pm_forcefield;
+typedef struct pm_ewald pm_ewald;
+typedef struct pm_probe pm_probe;
+
+/* Library version string. */
+const char* pm_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pm_last_error(void);
+
+/* Select the HIP device and create the library stream.  Must precede every
+ * other call except pm_version/pm_last_error.  Idempotent per device. */
+int pm_init(int32_t device);
+
+/*
+ * Framework = the crystal replicated by `rep` (src/henry.jl:80-86,
+ * src/grid.jl:331-332, Xtals `replicate`).  f_to_c is the UNIT cell's;
+ * xf (n_atoms x 3) are unit-cell fractional coordinates; species are LJ
+ * species indices into the force field's tables; charge may be NULL
+ * (uncharged framework).  The library builds the replicated atom list in
+ * Xtals order (ra outermost, then rb, rc, then atoms).
+ * Replaces the data preparation of `_conduct_Widom_insertions`
+ * (src/henry.jl:185-203) and `energy_grid` (src/grid.jl:326-332).
+ */
+int pm_framework_create(const double f_to_c[9], const int32_t rep[3], int64_t n_atoms,
+                        const double* xf, const int32_t* species, const double* charge,
+                        pm_framework** out);
+int pm_framework_destroy(pm_framework* fw);
+/* Replicated atom count, number of distinct species, total charge. */
+int pm_framework_info(const pm_framework* fw, int64_t* n_replicated, int32_t* n_species,
+                      double* net_charge);
+
+/*
+ * LJ force field as dense pair tables over its species (the Dict-of-Dict
+ * `LJForceField.epsilon` / `sigma^2` after mixing, src/forcefield.jl:12-22,
+ * 46-100): epsilon[i*n + j] (K), sigma2[i*n + j] (A^2), r2_cutoff (A^2).
+ */
+int pm_forcefield_create(int32_t n_species, const double* epsilon, const double* sigma2,
+                         double r2_cutoff, pm_forcefield** out);
+int pm_forcefield_destroy(pm_forcefield* ff);
+
+/*
+ * Ewald set-up on the REPLICATED box (src/electrostatics.jl:256-272 output):
+ * alpha (1/A), real-space cutoff (A) and the k-vector list exactly as
+ * `precompute_kvec_wts` enumerates it (kidx n_k x 3 integer multiples of the
+ * replicated box's reciprocal vectors, weights wt).  The library keeps the
+ * vectors whose unit-cell structure factor does not vanish identically
+ * (n divisible by rep) and computes S(k) on the device in FP64.
+ * Replaces `setup_Ewald_sum` + `Eikr` (src/electrostatics.jl:90-101,256-272)
+ * as the input to the Fourier sum.
+ */
+int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int64_t n_kvecs,
+                    const int32_t* kidx, const double* wt, pm_ewald** out);
+int pm_ewald_destroy(pm_ewald* ew);
+/* Number of k-vectors kept after folding onto the unit cell. */
+int pm_ewald_info(const pm_ewald* ew, int64_t* n_kept);
+
+/*
+ * Rigid probe (adsorbate template, src/molecule.jl:10-15,56-91): LJ spheres
+ * (species index, Cartesian offset from the centre of mass) and point charges
+ * (charge, Cartesian offset).  Coincident LJ and charge sites are merged.
+ * n_lj + n_q (after merging) <= PM_MAX_SITES.
+ */
+int pm_probe_create(int32_t n_lj, const double* lj_xyz, const int32_t* lj_species,
+                    int32_t n_q, const double* q_xyz, const double* q, pm_probe** out);
+int pm_probe_destroy(pm_probe* pr);
+
+/*
+ * Guest-host LJ energy (K) of n_poses rigid poses: `vdw_energy(crystal,
+ * molecule, ljff)` (src/vdw_energetics.jl:25-47,62-64) per pose.
+ * site_xf: n_poses x n_sites x 3, replicated-box fractional coordinates.
+ * Overlap (r^2 < 0.1 A^2 inside the cutoff) gives +Inf for that pose.
+ */
+int pm_vdw_energy(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sites,
+                  const int32_t* site_species, int64_t n_poses, const double* site_xf,
+                  int32_t precision, double* out);
+
+/*
+ * Guest-host Ewald energy (K) of n_poses sets of point charges:
+ * `electrostatic_potential_energy(crystal, molecule, eparams, eikr)`
+ * (src/electrostatics.jl:331-388) per pose, split into the real-space (sr)
+ * and Fourier (lr) parts of the returned `EwaldSum`.  Summing over poses
+ * gives `total_electrostatic_potential_energy` (src/electrostatics.jl:558-568).
+ * charge_xf: n_poses x n_charges x 3, replicated-box fractional coordinates.
+ */
+int pm_electrostatic_energy(const pm_framework* fw, const pm_ewald* ew, int32_t n_charges,
+                            const double* charges, int64_t n_poses, const double* charge_xf,
+                            int32_t precision, double* out_sr, double* out_lr);
+
+/*
+ * Widom insertions (src/henry.jl:185-265): n_blocks x n_per_block insertions
+ * of the probe, insertion g = b*n_per_block + i drawing its centre of mass
+ * (uniform in the replicated box) and an absolute Arvo rotation from the
+ * Philox4x32-10 stream keyed by `seed`.  ew may be NULL (no electrostatics,
+ * `charged_system == false`).  out_block_sums[3*b + {0,1,2}] =
+ * (sum e^{-E/T}, sum E_vdw e^{-E/T}, sum E_es e^{-E/T}) with Inf energies
+ * excluded from the weighted sums (src/henry.jl:236-243).
+ * out_energies (nullable): n_blocks*n_per_block x 2 (E_vdw, E_es) in K.
+ */
+int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+             const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
+             int64_t n_per_block, int32_t precision, double* out_block_sums,
+             double* out_energies);
+
+/*
+ * Energy grid (src/grid.jl:291-362, single-site probe): point (i,j,k) of an
+ * n_pts grid including fractional 0 and 1 of the UNIT cell, energy in K of
+ * the probe centred there; out is i-fastest (Julia Array{Float64,3}).
+ * A charged single-site probe adds q * phi (the point-charge potential
+ * energy of `electrostatic_potential_energy(crystal, ion(q, xf), ...)`),
+ * which needs ew.  PM_MIXED keeps the device grid in FP32.
+ */
+int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+                   const pm_probe* pr, const int32_t n_pts[3], int32_t precision,
+                   double* out);
+
+/* Duration (ms, HIP events on the library stream) of the energy kernel of
+ * the last pm_widom / pm_energy_grid / pm_vdw_energy / pm_electrostatic_energy
+ * call, and the replicated-atom x site pair count it evaluated. */
+int pm_last_kernel_stats(double* kernel_ms, double* pair_evals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMHIP_H */

@@ -1,0 +1,695 @@
+/*
+ * oracle.c -- CPU FP64 restatement of PorousMaterials.jl's guest-host energy,
+ * Ewald, Widom-insertion and energy-grid path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
+ * check and the CPU-baseline leg of bench.py may load it; the product
+ * library (porousmaterials.jl_amd/csrc) never links, calls or falls back to
+ * it.
+ *
+ * Every function cites the reference source it restates (paths relative to
+ * the PorousMaterials.jl v0.4.2 tree).  Xtals.jl (Box, nearest_image!,
+ * replicate) is not vendored in the reference; its semantics are restated as
+ * documented by the reference's own doctests (docs/src/box.md:39-44,
+ * docs/src/distance.md:15-35) and pinned by the known-answer tests under
+ * tests/.
+ *
+ * Storage conventions (match Julia's column-major arrays):
+ *   3x3 matrices   m[row + 3*col]
+ *   coordinates    x[3*i + d]   (a 3xN Julia Matrix, column i = atom i)
+ *   pair tables    t[s_i*n_species + s_j]
+ *
+ * Parity status per function (see the test files named in brackets):
+ *   box / replicate / nearest image / LJ    pinned  [test_oracle_kat.py]
+ *   Ewald params, kreps, k-vector weights   pinned  [test_oracle_kat.py]
+ *   _phi (Eikr form), S(k) form             pinned  [test_oracle_kat.py]
+ *   Philox stream                           pinned  [test_oracle_kat.py]
+ *   Arvo rotation                           pinned  (orthogonality/det/
+ *                                           uniformity) [test_oracle_kat.py]
+ *   Widom block statistics                  pinned  (closed forms)
+ *   energy grid                             pinned  (direct vdw calls,
+ *                                           face periodicity)
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* src/vdw_energetics.jl:2 */
+#define R2_OVERLAP 0.1
+
+/* src/electrostatics.jl:6-7 -- evaluated in the same left-to-right order */
+static double eps0(void) {
+    const double e = 1.602176565e-19;
+    return 8.854187817e-12 / (e * e) / 1.0e10 * 1.3806488e-23;
+}
+static double four_pi_eps0(void) { return 4.0 * M_PI * eps0(); }
+
+double orc_eps0(void) { return eps0(); }
+double orc_four_pi_eps0(void) { return four_pi_eps0(); }
+
+/* ------------------------------------------------------------------ */
+/* Box (Xtals.jl Box(a,b,c,alpha,beta,gamma)), upper-triangular f_to_c */
+/* with a along x; columns are the lattice vectors (docs/src/box.md).  */
+/* ------------------------------------------------------------------ */
+void orc_box_f_to_c(double a, double b, double c, double al, double be,
+                    double ga, double F[9]) {
+    double ca = cos(al), cb = cos(be), cg = cos(ga), sg = sin(ga);
+    double omega =
+        a * b * c * sqrt(1.0 - ca * ca - cb * cb - cg * cg + 2.0 * ca * cb * cg);
+    F[0] = a;      F[1] = 0.0;    F[2] = 0.0;
+    F[3] = b * cg; F[4] = b * sg; F[5] = 0.0;
+    F[6] = c * cb; F[7] = c * (ca - cb * cg) / sg;
+    F[8] = omega / (a * b * sg);
+}
+
+double orc_det3(const double m[9]) {
+    return m[0] * (m[4] * m[8] - m[7] * m[5]) -
+           m[3] * (m[1] * m[8] - m[7] * m[2]) +
+           m[6] * (m[1] * m[5] - m[4] * m[2]);
+}
+
+void orc_inv3(const double m[9], double out[9]) {
+    double d = orc_det3(m);
+    out[0] = (m[4] * m[8] - m[7] * m[5]) / d;
+    out[3] = -(m[3] * m[8] - m[6] * m[5]) / d;
+    out[6] = (m[3] * m[7] - m[6] * m[4]) / d;
+    out[1] = -(m[1] * m[8] - m[7] * m[2]) / d;
+    out[4] = (m[0] * m[8] - m[6] * m[2]) / d;
+    out[7] = -(m[0] * m[7] - m[6] * m[1]) / d;
+    out[2] = (m[1] * m[5] - m[4] * m[2]) / d;
+    out[5] = -(m[0] * m[5] - m[3] * m[2]) / d;
+    out[8] = (m[0] * m[4] - m[3] * m[1]) / d;
+}
+
+static void matvec3(const double m[9], const double v[3], double out[3]) {
+    for (int r = 0; r < 3; ++r)
+        out[r] = m[r] * v[0] + m[r + 3] * v[1] + m[r + 6] * v[2];
+}
+
+static void matmul3(const double a[9], const double b[9], double out[9]) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            out[r + 3 * c] = a[r] * b[3 * c] + a[r + 3] * b[3 * c + 1] +
+                             a[r + 6] * b[3 * c + 2];
+}
+
+/* ------------------------------------------------------------------ */
+/* src/forcefield.jl:120-158  replication_factors                     */
+/* ------------------------------------------------------------------ */
+static void cross3(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double a[3], const double b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static double norm3(const double a[3]) { return sqrt(dot3(a, a)); }
+
+void orc_replication_factors(const double F[9], double r_cutoff, int32_t rep[3]) {
+    double a[3] = {F[0], F[1], F[2]}, b[3] = {F[3], F[4], F[5]},
+           c[3] = {F[6], F[7], F[8]};
+    double n_ab[3], n_ac[3], n_bc[3], c0[3];
+    cross3(a, b, n_ab);
+    cross3(a, c, n_ac);
+    cross3(b, c, n_bc);
+#define C0()                                                   \
+    for (int d = 0; d < 3; ++d) c0[d] = (a[d] * 0.5 + b[d] * 0.5) + c[d] * 0.5;
+    C0();
+    rep[0] = rep[1] = rep[2] = 1;
+    while (fabs(dot3(n_bc, c0)) / norm3(n_bc) < r_cutoff) {
+        rep[0] += 1;
+        for (int d = 0; d < 3; ++d) a[d] += F[d];
+        C0();
+    }
+    while (fabs(dot3(n_ac, c0)) / norm3(n_ac) < r_cutoff) {
+        rep[1] += 1;
+        for (int d = 0; d < 3; ++d) b[d] += F[3 + d];
+        C0();
+    }
+    while (fabs(dot3(n_ab, c0)) / norm3(n_ab) < r_cutoff) {
+        rep[2] += 1;
+        for (int d = 0; d < 3; ++d) c[d] += F[6 + d];
+        C0();
+    }
+#undef C0
+}
+
+/* Xtals replicate(crystal, repfactors): ra outermost, then rb, rc, then
+ * atoms; xf_new = (xf + [ra, rb, rc]) ./ repfactors.  Returns n*prod(rep). */
+int64_t orc_replicate(int64_t n, const double* xf, const int32_t rep[3],
+                      double* out_xf) {
+    int64_t k = 0;
+    for (int ra = 0; ra < rep[0]; ++ra)
+        for (int rb = 0; rb < rep[1]; ++rb)
+            for (int rc = 0; rc < rep[2]; ++rc)
+                for (int64_t i = 0; i < n; ++i, ++k) {
+                    out_xf[3 * k + 0] = (xf[3 * i + 0] + 1.0 * ra) / rep[0];
+                    out_xf[3 * k + 1] = (xf[3 * i + 1] + 1.0 * rb) / rep[1];
+                    out_xf[3 * k + 2] = (xf[3 * i + 2] + 1.0 * rc) / rep[2];
+                }
+    return k;
+}
+
+/* Xtals nearest_image!: subtract sign(dx) once if |dx| > 0.5 (valid for
+ * |dx| < 1.5, which every call below satisfies). */
+static inline double nearest_image(double d) {
+    if (fabs(d) > 0.5) d -= (d > 0.0) ? 1.0 : -1.0;
+    return d;
+}
+
+/* ------------------------------------------------------------------ */
+/* src/vdw_energetics.jl:19-22  lennard_jones                          */
+/* ------------------------------------------------------------------ */
+double orc_lennard_jones(double r2, double sig2, double eps) {
+    double s = sig2 / r2;
+    double ratio = s * s * s;
+    return 4.0 * eps * ratio * (ratio - 1.0);
+}
+
+/* src/vdw_energetics.jl:25-47 (via :62-64): atoms_i = adsorbate sites,
+ * atoms_j = (replicated) framework atoms; both in fractional coords of the
+ * box whose f_to_c is F.  Overlap inside the cutoff returns +Inf. */
+double orc_vdw_energy(int32_t ni, const double* xi, const int32_t* si,
+                      int64_t nj, const double* xj, const int32_t* sj,
+                      const double F[9], int32_t n_species, const double* eps,
+                      const double* sig2, double r2_cutoff) {
+    double energy = 0.0;
+    for (int32_t i = 0; i < ni; ++i) {
+        for (int64_t j = 0; j < nj; ++j) {
+            double d[3], c[3];
+            for (int k = 0; k < 3; ++k) d[k] = nearest_image(xj[3 * j + k] - xi[3 * i + k]);
+            matvec3(F, d, c);
+            double r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+            if (r2 < r2_cutoff) {
+                if (r2 < R2_OVERLAP) return INFINITY;
+                int64_t p = (int64_t)si[i] * n_species + sj[j];
+                energy += orc_lennard_jones(r2, sig2[p], eps[p]);
+            }
+        }
+    }
+    return energy;
+}
+
+/* Guest-guest total LJ energy of a set of single-site molecules with the
+ * nearest-image convention (src/vdw_energetics.jl:82-114,143-149 for
+ * one-atom molecules); used only to pin lennard_jones + nearest image with
+ * the NIST LJ reference configurations (test/vdw_energetics.jl:53-80). */
+double orc_total_vdw_gg(int64_t n, const double* xf, const double F[9],
+                        double sig2, double eps, double r2_cutoff) {
+    double total = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = i + 1; j < n; ++j) {
+            double d[3], c[3];
+            for (int k = 0; k < 3; ++k) d[k] = nearest_image(xf[3 * j + k] - xf[3 * i + k]);
+            matvec3(F, d, c);
+            double r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+            if (r2 < r2_cutoff) total += orc_lennard_jones(r2, sig2, eps);
+        }
+    return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* Ewald set-up: src/electrostatics.jl:123-141, 157-175, 198-232       */
+/* ------------------------------------------------------------------ */
+/* Bisection on a sign-changing bracket down to adjacent doubles (the
+ * Roots.jl fzero(f, a, b) bracketing method; the returned point can differ
+ * from Roots' in the last ulp). */
+typedef double (*scalar_fn)(double, const double*);
+static double bisect(scalar_fn f, const double* p, double lo, double hi) {
+    double flo = f(lo, p);
+    for (int it = 0; it < 4000; ++it) {
+        double mid = lo + 0.5 * (hi - lo);
+        if (mid <= lo || mid >= hi) break;
+        double fm = f(mid, p);
+        if (fm == 0.0) return mid;
+        if ((fm > 0.0) == (flo > 0.0)) { lo = mid; flo = fm; }
+        else hi = mid;
+    }
+    double fl = fabs(f(lo, p)), fh = fabs(f(hi, p));
+    return (fh < fl) ? hi : lo;
+}
+static double sr_error(double alpha, const double* p) {
+    return erfc(p[0] * alpha) / p[0] - p[1];
+}
+static double lr_error(double k2, const double* p) {
+    return 2.0 * exp(-k2 / (4.0 * p[0] * p[0])) / k2 - p[1];
+}
+void orc_determine_ewald_params(double sr_cutoff, double eps, double* alpha,
+                                double* max_k2) {
+    double p1[2] = {sr_cutoff, eps};
+    *alpha = bisect(sr_error, p1, 0.0, 25.0);
+    double p2[2] = {*alpha, eps};
+    *max_k2 = bisect(lr_error, p2, 0.0, 100.0);
+}
+
+/* reciprocal_lattice = 2*pi*inv(f_to_c); rows are the reciprocal vectors */
+void orc_reciprocal_lattice(const double F[9], double R[9]) {
+    double inv[9];
+    orc_inv3(F, inv);
+    double twopi = 2.0 * M_PI;
+    for (int i = 0; i < 9; ++i) R[i] = twopi * inv[i];
+}
+
+/* k = transpose(reciprocal_lattice) * n */
+static void kvec_cart(const double R[9], int a, int b, int c, double k[3]) {
+    for (int i = 0; i < 3; ++i)
+        k[i] = R[0 + 3 * i] * a + R[1 + 3 * i] * b + R[2 + 3 * i] * c;
+}
+
+void orc_required_kreps(const double F[9], double max_k2, int32_t kreps[3]) {
+    double R[9];
+    orc_reciprocal_lattice(F, R);
+    for (int abc = 0; abc < 3; ++abc) {
+        int n[3] = {0, 0, 0};
+        for (;;) {
+            n[abc] += 1;
+            double k[3];
+            kvec_cart(R, n[0], n[1], n[2], k);
+            if (dot3(k, k) > max_k2) { kreps[abc] = n[abc] - 1; break; }
+        }
+    }
+}
+
+/* src/electrostatics.jl:198-232.  Writes up to `cap` vectors; returns the
+ * count (call with cap=0 to size).  max_k2 = INFINITY keeps every vector. */
+int64_t orc_precompute_kvec_wts(const int32_t kreps[3], const double F[9],
+                                double alpha, double max_k2, int64_t cap,
+                                int32_t* kidx, double* wt) {
+    double R[9];
+    orc_reciprocal_lattice(F, R);
+    double omega = fabs(orc_det3(F));
+    double e0 = eps0();
+    int64_t n = 0;
+    for (int ka = 0; ka <= kreps[0]; ++ka)
+        for (int kb = -kreps[1]; kb <= kreps[1]; ++kb)
+            for (int kc = -kreps[2]; kc <= kreps[2]; ++kc) {
+                if (ka == 0 && kb == 0 && kc == 0) continue;
+                if (ka == 0 && kb < 0) continue;
+                if (ka == 0 && kb == 0 && kc < 0) continue;
+                double k[3];
+                kvec_cart(R, ka, kb, kc, k);
+                double k2 = dot3(k, k);
+                if (k2 < max_k2) {
+                    if (n < cap) {
+                        kidx[3 * n + 0] = ka;
+                        kidx[3 * n + 1] = kb;
+                        kidx[3 * n + 2] = kc;
+                        wt[n] = 2.0 * exp(-k2 / (4.0 * alpha * alpha)) / k2 / e0 / omega;
+                    }
+                    ++n;
+                }
+            }
+    return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* src/electrostatics.jl:347-388  _phi, reference form: e^{ik.r} tables  */
+/* (fill_eikr!, :288-306) over every charge and every k-vector.          */
+/* Returns the potential per unit probe charge in sr[0], lr[0].          */
+/* ------------------------------------------------------------------ */
+typedef struct { double re, im; } cplx;
+static inline cplx cmul(cplx a, cplx b) {
+    cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+
+void orc_phi_eikr(int64_t nq, const double* xq, const double* q,
+                  const double xf[3], const double F[9], const int32_t kreps[3],
+                  int64_t nk, const int32_t* kidx, const double* wt, double alpha,
+                  double sr_cutoff, double* sr_out, double* lr_out) {
+    double R[9], M[9];
+    orc_reciprocal_lattice(F, R);
+    matmul3(R, F, M); /* reciprocal_lattice * f_to_c (~2 pi I) */
+    int K0 = kreps[0], K1 = kreps[1], K2 = kreps[2];
+    int w0 = K0 + 1, w1 = 2 * K1 + 1, w2 = 2 * K2 + 1;
+    cplx* ea = (cplx*)malloc(sizeof(cplx) * (size_t)nq * w0);
+    cplx* eb = (cplx*)malloc(sizeof(cplx) * (size_t)nq * w1);
+    cplx* ec = (cplx*)malloc(sizeof(cplx) * (size_t)nq * w2);
+    double* dxf = (double*)malloc(sizeof(double) * 3 * (size_t)nq);
+    for (int64_t c = 0; c < nq; ++c) {
+        double kdx[3];
+        for (int d = 0; d < 3; ++d) dxf[3 * c + d] = xq[3 * c + d] - xf[d];
+        matvec3(M, &dxf[3 * c], kdx);
+        /* fill_eikr!: n = 0, 1 explicitly, higher n by recurrence,
+         * negative n as complex conjugates */
+        cplx one = {1.0, 0.0};
+        cplx e1;
+        cplx* tabs[3] = {ea + c * w0, eb + c * w1, ec + c * w2};
+        int kr[3] = {K0, K1, K2};
+        int off[3] = {0, K1, K2};
+        for (int d = 0; d < 3; ++d) {
+            cplx* t = tabs[d] + off[d];
+            t[0] = one;
+            if (kr[d] >= 1) {
+                e1.re = cos(kdx[d]);
+                e1.im = sin(kdx[d]);
+                t[1] = e1;
+                for (int k = 2; k <= kr[d]; ++k) t[k] = cmul(t[k - 1], e1);
+            }
+            if (d > 0)
+                for (int k = 1; k <= kr[d]; ++k) {
+                    t[-k].re = t[k].re;
+                    t[-k].im = -t[k].im;
+                }
+        }
+    }
+    double lr = 0.0;
+    for (int64_t k = 0; k < nk; ++k) {
+        int ka = kidx[3 * k], kb = kidx[3 * k + 1], kc = kidx[3 * k + 2];
+        double kvec_pot = 0.0;
+        for (int64_t c = 0; c < nq; ++c) {
+            cplx p = cmul(cmul(ea[c * w0 + ka], eb[c * w1 + K1 + kb]), ec[c * w2 + K2 + kc]);
+            kvec_pot += q[c] * p.re;
+        }
+        lr += wt[k] * kvec_pot;
+    }
+    double sr = 0.0, fpe = four_pi_eps0();
+    for (int64_t c = 0; c < nq; ++c) {
+        double d[3], cc[3];
+        for (int k = 0; k < 3; ++k) d[k] = nearest_image(dxf[3 * c + k]);
+        matvec3(F, d, cc);
+        double r = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+        if (r < sr_cutoff) sr += q[c] / r * erfc(r * alpha) / fpe;
+    }
+    free(ea); free(eb); free(ec); free(dxf);
+    *sr_out = sr;
+    *lr_out = lr;
+}
+
+/* ------------------------------------------------------------------ */
+/* Structure-factor form of the same Fourier sum.  The replicated        */
+/* framework is R_a x R_b x R_c copies of the unit cell, so                */
+/*   sum_{replicated c} q_c e^{i 2pi n.(x_c - x)}                           */
+/* vanishes unless R_a | n_a (all a) and then equals                        */
+/*   N_R * S(m) e^{-i 2pi m.(R o x)},  m = n / R,  S(m) = sum_unit q e^{i2pi m.x}.
+ * orc_sk_build folds the replicated-box k-list (reference enumeration order)
+ * into A_k = wt_k N_R S(m_k); orc_phi_lr_sk evaluates the sum at a point.  */
+/* ------------------------------------------------------------------ */
+int64_t orc_sk_build(int64_t n_unit, const double* xq_unit, const double* q,
+                     const int32_t rep[3], int64_t nk, const int32_t* kidx,
+                     const double* wt, int32_t* m_out, double* a_re, double* a_im) {
+    int64_t nr = (int64_t)rep[0] * rep[1] * rep[2];
+    int64_t n = 0;
+    for (int64_t k = 0; k < nk; ++k) {
+        int na = kidx[3 * k], nb = kidx[3 * k + 1], nc = kidx[3 * k + 2];
+        if (na % rep[0] || nb % rep[1] || nc % rep[2]) continue;
+        int ma = na / rep[0], mb = nb / rep[1], mc = nc / rep[2];
+        double sre = 0.0, sim = 0.0;
+        for (int64_t c = 0; c < n_unit; ++c) {
+            double t = ma * xq_unit[3 * c] + mb * xq_unit[3 * c + 1] + mc * xq_unit[3 * c + 2];
+            t -= nearbyint(t);
+            double ang = 2.0 * M_PI * t;
+            sre += q[c] * cos(ang);
+            sim += q[c] * sin(ang);
+        }
+        m_out[3 * n] = ma; m_out[3 * n + 1] = mb; m_out[3 * n + 2] = mc;
+        a_re[n] = wt[k] * (double)nr * sre;
+        a_im[n] = wt[k] * (double)nr * sim;
+        ++n;
+    }
+    return n;
+}
+
+/* x_rep: point in replicated fractional coordinates */
+double orc_phi_lr_sk(int64_t nk, const int32_t* m, const double* a_re,
+                     const double* a_im, const int32_t rep[3], const double x_rep[3]) {
+    double xu[3] = {x_rep[0] * rep[0], x_rep[1] * rep[1], x_rep[2] * rep[2]};
+    double lr = 0.0;
+    for (int64_t k = 0; k < nk; ++k) {
+        double t = m[3 * k] * xu[0] + m[3 * k + 1] * xu[1] + m[3 * k + 2] * xu[2];
+        t -= nearbyint(t);
+        double ang = 2.0 * M_PI * t;
+        lr += a_re[k] * cos(ang) + a_im[k] * sin(ang);
+    }
+    return lr;
+}
+
+/* Real-space part of _phi (src/electrostatics.jl:377-386) per unit probe charge */
+double orc_phi_sr(int64_t nq, const double* xq, const double* q, const double xf[3],
+                  const double F[9], double alpha, double sr_cutoff) {
+    double sr = 0.0, fpe = four_pi_eps0();
+    for (int64_t c = 0; c < nq; ++c) {
+        double d[3], cc[3];
+        for (int k = 0; k < 3; ++k) d[k] = nearest_image(xq[3 * c + k] - xf[k]);
+        matvec3(F, d, cc);
+        double r = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+        if (r < sr_cutoff) sr += q[c] / r * erfc(r * alpha) / fpe;
+    }
+    return sr;
+}
+
+/* Guest-guest Ewald components of a periodic set of point-charge molecules
+ * (src/electrostatics.jl:391-409, 434-544), used only to pin alpha/erfc,
+ * the k-vector weights and the unit constant against the NIST SPC/E
+ * reference energies (test/electrostatics.jl:51-125).  mol[i] is the
+ * molecule index of charge i.  out = {real, fourier, self, intra}. */
+void orc_ewald_gg_components(int64_t n, const double* xf, const double* q,
+                             const int32_t* mol, const double F[9], double alpha,
+                             double sr_cutoff, int64_t nk, const int32_t* kidx,
+                             const double* wt, double out[4]) {
+    double fpe = four_pi_eps0();
+    double real = 0.0, intra = 0.0, self = 0.0, four = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        self += q[i] * q[i];
+        for (int64_t j = i + 1; j < n; ++j) {
+            double d[3], c[3];
+            for (int k = 0; k < 3; ++k) d[k] = nearest_image(xf[3 * j + k] - xf[3 * i + k]);
+            matvec3(F, d, c);
+            double r = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            if (mol[i] == mol[j]) {
+                intra -= q[i] * q[j] * erf(alpha * r) / r / fpe;
+            } else if (r < sr_cutoff) {
+                real += q[i] * q[j] * erfc(alpha * r) / r / fpe;
+            }
+        }
+    }
+    self *= -alpha / sqrt(M_PI) / fpe;
+    for (int64_t k = 0; k < nk; ++k) {
+        double sre = 0.0, sim = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            double t = kidx[3 * k] * xf[3 * i] + kidx[3 * k + 1] * xf[3 * i + 1] +
+                       kidx[3 * k + 2] * xf[3 * i + 2];
+            double ang = 2.0 * M_PI * t;
+            sre += q[i] * cos(ang);
+            sim += q[i] * sin(ang);
+        }
+        /* the weights carry the factor 2 of the half-space enumeration */
+        four += 0.5 * wt[k] * (sre * sre + sim * sim);
+    }
+    out[0] = real; out[1] = four; out[2] = self; out[3] = intra;
+}
+
+/* ------------------------------------------------------------------ */
+/* Insertion stream: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11,  */
+/* "Parallel random numbers: as easy as 1, 2, 3").  Insertion g draws    */
+/* three blocks: counter (g_lo, g_hi, w, 0x57494430), key (seed_lo,      */
+/* seed_hi), w = 0, 1, 2 -> six uniforms in [0,1) with 53-bit mantissas: */
+/* u[0..2] = fractional COM position (replicated box), u[3..5] = Arvo.    */
+/* The reference draws from the unseeded global RNG (src/henry.jl:214),   */
+/* so this stream is shared by both sides instead.                        */
+/* ------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
+                       uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+#define STREAM_TAG 0x57494430u
+
+static inline double u53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+void orc_insertion_uniforms(uint64_t seed, uint64_t g, double u[6]) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (uint32_t w = 0; w < 3; ++w) {
+        uint32_t ctr[4] = {(uint32_t)g, (uint32_t)(g >> 32), w, STREAM_TAG};
+        uint32_t o[4];
+        orc_philox4x32_10(ctr, key, o);
+        u[2 * w] = u53(o[0], o[1]);
+        u[2 * w + 1] = u53(o[2], o[3]);
+    }
+}
+
+/* src/molecule.jl:233-244 random_rotation_matrix (Arvo) from three
+ * uniforms: Rz(2 pi u0), v = (cos(2pi u1) sqrt(u2), sin(2pi u1) sqrt(u2),
+ * sqrt(1-u2)), H = I - 2 v v^T, return -H * Rz. */
+void orc_arvo_rotation(const double u[3], double Rm[9]) {
+    double u1 = u[0] * 2.0 * M_PI;
+    double r[9] = {cos(u1), -sin(u1), 0.0, sin(u1), cos(u1), 0.0, 0.0, 0.0, 1.0};
+    double u2 = 2.0 * M_PI * u[1], u3 = u[2];
+    double v[3] = {cos(u2) * sqrt(u3), sin(u2) * sqrt(u3), sqrt(1.0 - u3)};
+    double h[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) h[i + 3 * j] = (i == j ? 1.0 : 0.0) - 2.0 * v[i] * v[j];
+    double hr[9];
+    matmul3(h, r, hr);
+    for (int i = 0; i < 9; ++i) Rm[i] = -hr[i];
+}
+
+/* Site positions (replicated fractional coords) of insertion g: COM at u[0..2],
+ * template offsets (Cartesian, relative to the COM) rotated by the Arvo
+ * matrix of u[3..5] and mapped by c_to_f (src/henry.jl:214-220,
+ * src/molecule.jl:176-179,246,269-281, with an absolute rotation of the
+ * template per insertion). rotate=0 for single-site probes (needs_rotations,
+ * src/mc_helpers.jl:8). */
+void orc_insertion_pose(uint64_t seed, uint64_t g, const double c_to_f[9],
+                        int32_t n_sites, const double* offs, int32_t rotate,
+                        double* site_xf) {
+    double u[6], Rm[9];
+    orc_insertion_uniforms(seed, g, u);
+    if (rotate) orc_arvo_rotation(&u[3], Rm);
+    for (int32_t s = 0; s < n_sites; ++s) {
+        double y[3], f[3];
+        if (rotate) matvec3(Rm, &offs[3 * s], y);
+        else { y[0] = offs[3 * s]; y[1] = offs[3 * s + 1]; y[2] = offs[3 * s + 2]; }
+        matvec3(c_to_f, y, f);
+        for (int d = 0; d < 3; ++d) site_xf[3 * s + d] = u[d] + f[d];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Widom insertions: src/henry.jl:185-265 per-insertion energies and     */
+/* per-block sums.  Framework: replicated atoms (rep frac) with species   */
+/* and charge; LJ sites and charge sites of the probe as template offsets.*/
+/* es_mode 0: structure-factor Fourier form (a_* from orc_sk_build),      */
+/* es_mode 1: reference e^{ik.r} form (kidx/wt on the replicated box).    */
+/* Energies of insertions [g0, g0+count) go to out_vdw/out_es; block sums */
+/* (S, W_vdw, W_es) are formed sequentially in insertion order.           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int64_t n_rep;
+    const double* xf; const int32_t* species; const double* q;
+    int64_t nq; const double* xq; const double* qq; /* nonzero charges only */
+    double F[9], c_to_f[9];
+    int32_t rep[3];
+    int32_t n_species; const double* eps; const double* sig2; double r2_cutoff;
+    int32_t n_lj; const double* lj_off; const int32_t* lj_species;
+    int32_t n_q; const double* q_off; const double* q_site;
+    int32_t charged; double alpha; double sr_cutoff;
+    int32_t es_mode;
+    int64_t nk_sk; const int32_t* m_sk; const double* a_re; const double* a_im;
+    int32_t kreps[3]; int64_t nk; const int32_t* kidx; const double* wt;
+    uint64_t seed;
+} orc_widom_cfg;
+
+static void widom_energy(const orc_widom_cfg* c, uint64_t g, double* vdw, double* es) {
+    int32_t rotate = (c->n_lj + c->n_q) > 1;
+    double u[6], Rm[9];
+    orc_insertion_uniforms(c->seed, g, u);
+    if (rotate) orc_arvo_rotation(&u[3], Rm);
+    double sxf[3 * 16], qxf[3 * 16];
+    for (int pass = 0; pass < 2; ++pass) {
+        int32_t ns = pass ? c->n_q : c->n_lj;
+        const double* offs = pass ? c->q_off : c->lj_off;
+        double* dst = pass ? qxf : sxf;
+        for (int32_t s = 0; s < ns; ++s) {
+            double y[3], f[3];
+            if (rotate) matvec3(Rm, &offs[3 * s], y);
+            else { y[0] = offs[3 * s]; y[1] = offs[3 * s + 1]; y[2] = offs[3 * s + 2]; }
+            matvec3(c->c_to_f, y, f);
+            for (int d = 0; d < 3; ++d) dst[3 * s + d] = u[d] + f[d];
+        }
+    }
+    *vdw = orc_vdw_energy(c->n_lj, sxf, c->lj_species, c->n_rep, c->xf, c->species,
+                          c->F, c->n_species, c->eps, c->sig2, c->r2_cutoff);
+    double e = 0.0;
+    if (c->charged) {
+        for (int32_t s = 0; s < c->n_q; ++s) {
+            double sr, lr;
+            if (c->es_mode == 1) {
+                orc_phi_eikr(c->nq, c->xq, c->qq, &qxf[3 * s], c->F, c->kreps, c->nk,
+                             c->kidx, c->wt, c->alpha, c->sr_cutoff, &sr, &lr);
+            } else {
+                sr = orc_phi_sr(c->nq, c->xq, c->qq, &qxf[3 * s], c->F, c->alpha, c->sr_cutoff);
+                lr = orc_phi_lr_sk(c->nk_sk, c->m_sk, c->a_re, c->a_im, c->rep, &qxf[3 * s]);
+            }
+            e += c->q_site[s] * (sr + lr);
+        }
+    }
+    *es = e;
+}
+
+int32_t orc_widom_energies(const orc_widom_cfg* c, uint64_t g0, int64_t count,
+                           double* out_vdw, double* out_es) {
+    if (c->n_lj > 16 || c->n_q > 16) return 1;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = 0; i < count; ++i)
+        widom_energy(c, g0 + (uint64_t)i, &out_vdw[i], &out_es[i]);
+    return 0;
+}
+
+/* Block sums from energies, src/henry.jl:236-243 */
+void orc_widom_block_sums(int64_t n, const double* vdw, const double* es,
+                          double temperature, double out[3]) {
+    double S = 0.0, Wv = 0.0, We = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double w = exp(-(vdw[i] + es[i]) / temperature);
+        S += w;
+        if (!isinf(vdw[i]) && !isinf(es[i])) { Wv += w * vdw[i]; We += w * es[i]; }
+    }
+    out[0] = S; out[1] = Wv; out[2] = We;
+}
+
+int32_t orc_num_threads(void) {
+#ifdef _OPENMP
+    return (int32_t)omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* Energy grid, src/grid.jl:313-359 (single-site probe, the branch that  */
+/* runs in the reference): point (i,j,k) at ([i,j,k]/(n-1)) ./ rep in the */
+/* replicated box.  Charged single-site probes return the per-point       */
+/* electrostatic_potential_energy(crystal, ion(q, xf), ...) instead (the  */
+/* reference's charged branch does not run: src/grid.jl:342,347).         */
+/* out is i-fastest (Julia column-major Array{T,3}), units K.             */
+/* ------------------------------------------------------------------ */
+int32_t orc_energy_grid(const orc_widom_cfg* c, const int32_t n_pts[3], double* out) {
+    int64_t n1 = n_pts[0], n2 = n_pts[1], n3 = n_pts[2];
+    int64_t total = n1 * n2 * n3;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < total; ++p) {
+        int64_t i = p % n1, j = (p / n1) % n2, k = p / (n1 * n2);
+        double x[3] = {(n1 > 1 ? (double)i / (double)(n1 - 1) : 0.0) / c->rep[0],
+                       (n2 > 1 ? (double)j / (double)(n2 - 1) : 0.0) / c->rep[1],
+                       (n3 > 1 ? (double)k / (double)(n3 - 1) : 0.0) / c->rep[2]};
+        double e = 0.0;
+        if (c->n_lj == 1)
+            e += orc_vdw_energy(1, x, c->lj_species, c->n_rep, c->xf, c->species, c->F,
+                                c->n_species, c->eps, c->sig2, c->r2_cutoff);
+        if (c->charged && c->n_q == 1) {
+            double sr = orc_phi_sr(c->nq, c->xq, c->qq, x, c->F, c->alpha, c->sr_cutoff);
+            double lr = orc_phi_lr_sk(c->nk_sk, c->m_sk, c->a_re, c->a_im, c->rep, x);
+            e += c->q_site[0] * (sr + lr);
+        }
+        out[p] = e;
+    }
+    return 0;
+}

@@ -1,0 +1,96 @@
+"""ctypes binding of libpmhip.so (declared in include/pmhip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded
+every energy call raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpmhip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "pmhip.h")
+
+PM_OK, PM_ERR_ARG, PM_ERR_DEVICE, PM_ERR_OOM, PM_ERR_STATE = 0, 1, 2, 3, 4
+PM_FP64, PM_MIXED = 0, 1
+PM_MAX_SITES = 8
+
+_ERR_NAMES = {1: "PM_ERR_ARG", 2: "PM_ERR_DEVICE", 3: "PM_ERR_OOM", 4: "PM_ERR_STATE"}
+
+
+class PMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+P = ctypes.c_void_p
+PP = ctypes.POINTER(ctypes.c_void_p)
+D = ctypes.POINTER(ctypes.c_double)
+I32 = ctypes.POINTER(ctypes.c_int32)
+I64 = ctypes.POINTER(ctypes.c_int64)
+
+_SIGS = {
+    "pm_version": (ctypes.c_char_p, []),
+    "pm_last_error": (ctypes.c_char_p, []),
+    "pm_init": (ctypes.c_int, [ctypes.c_int32]),
+    "pm_framework_create": (ctypes.c_int, [D, I32, ctypes.c_int64, D, I32, D, PP]),
+    "pm_framework_destroy": (ctypes.c_int, [P]),
+    "pm_framework_info": (ctypes.c_int, [P, I64, I32, D]),
+    "pm_forcefield_create": (ctypes.c_int, [ctypes.c_int32, D, D, ctypes.c_double, PP]),
+    "pm_forcefield_destroy": (ctypes.c_int, [P]),
+    "pm_ewald_create": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_int64, I32, D, PP]),
+    "pm_ewald_destroy": (ctypes.c_int, [P]),
+    "pm_ewald_info": (ctypes.c_int, [P, I64]),
+    "pm_probe_create": (ctypes.c_int, [ctypes.c_int32, D, I32, ctypes.c_int32, D, D, PP]),
+    "pm_probe_destroy": (ctypes.c_int, [P]),
+    "pm_vdw_energy": (ctypes.c_int, [P, P, ctypes.c_int32, I32, ctypes.c_int64, D, ctypes.c_int32, D]),
+    "pm_electrostatic_energy": (ctypes.c_int, [P, P, ctypes.c_int32, D, ctypes.c_int64, D,
+                                               ctypes.c_int32, D, D]),
+    "pm_widom": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
+                                ctypes.c_int64, ctypes.c_int32, D, D]),
+    "pm_energy_grid": (ctypes.c_int, [P, P, P, P, I32, ctypes.c_int32, D]),
+    "pm_last_kernel_stats": (ctypes.c_int, [D, D]),
+}
+
+
+def header_functions(path: str = HEADER):
+    """Names of the functions include/pmhip.h declares."""
+    with open(path) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pm_[a-z_0-9]+)\s*\(", text)))
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                           "from the repository root")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != PM_OK:
+        raise PMError(rc, load().pm_last_error().decode(errors="replace"))
+
+
+def dptr(a):
+    return a.ctypes.data_as(D)
+
+
+def iptr(a):
+    return a.ctypes.data_as(I32)

@@ -1,0 +1,222 @@
+"""Device handles behind the reference call surface.
+
+Frameworks, force fields, Ewald settings and probes are uploaded once and
+cached on the Python objects they come from (the Julia shim does the same with
+finalizers).  A crystal produced by ``replicate`` is uploaded as its unit cell
+plus replication factors, so the library can fold the replicated box's
+k-vectors onto the unit-cell structure factor.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import threading
+import weakref
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr
+
+_init_lock = threading.Lock()
+_initialised = False
+
+
+def init(device: int = 0) -> None:
+    global _initialised
+    with _init_lock:
+        if not _initialised:
+            check(_lib.load().pm_init(int(device)))
+            _initialised = True
+
+
+def precision_code(precision) -> int:
+    p = str(precision).lower()
+    if p in ("mixed", "fp32", "pm_mixed"):
+        return _lib.PM_MIXED
+    if p in ("fp64", "float64", "double", "pm_fp64"):
+        return _lib.PM_FP64
+    raise ValueError(f"precision must be 'mixed' or 'fp64', got {precision!r}")
+
+
+class _Handle:
+    def __init__(self, ptr, destroy_name):
+        self.ptr = ptr
+        self._fin = weakref.finalize(self, _destroy, destroy_name, ptr.value)
+
+    @property
+    def value(self):
+        return self.ptr
+
+
+def _destroy(name, raw):
+    try:
+        getattr(_lib.load(), name)(ctypes.c_void_p(raw))
+    except Exception:
+        pass
+
+
+def framework(crystal, ljff=None) -> _Handle:
+    """Upload ``crystal`` (species indexed by ``ljff``; all 0 without one)."""
+    key = ("fw", tuple(ljff.species) if ljff is not None else None)
+    h = crystal._device_cache.get(key)
+    if h is not None:
+        return h
+    init()
+    unit = crystal._unit if crystal._unit is not None else crystal
+    rep = crystal._rep if crystal._unit is not None else (1, 1, 1)
+    if not unit.box.is_upper_triangular:
+        raise ValueError("the device library needs an upper-triangular f_to_c (a along x)")
+    n = unit.n_atoms
+    if ljff is not None:
+        sp = np.array([ljff.index(s) for s in unit.species], dtype=np.int32)
+    else:
+        sp = np.zeros(n, dtype=np.int32)
+    xf = np.ascontiguousarray(unit.xf.T, dtype=np.float64)
+    q = np.ascontiguousarray(unit.atom_q, dtype=np.float64)
+    f = np.ascontiguousarray(unit.box.f_to_c.T.reshape(-1))  # column-major
+    r = np.array(rep, dtype=np.int32)
+    out = ctypes.c_void_p()
+    check(_lib.load().pm_framework_create(dptr(f), iptr(r), n, dptr(xf), iptr(sp), dptr(q),
+                                          ctypes.byref(out)))
+    h = _Handle(out, "pm_framework_destroy")
+    crystal._device_cache[key] = h
+    return h
+
+
+_ff_cache = weakref.WeakKeyDictionary()
+
+
+def forcefield(ljff) -> _Handle:
+    h = _ff_cache.get(ljff)
+    if h is not None:
+        return h
+    init()
+    eps = np.ascontiguousarray(ljff.epsilon_table, dtype=np.float64)
+    sig2 = np.ascontiguousarray(ljff.sigma2_table, dtype=np.float64)
+    out = ctypes.c_void_p()
+    check(_lib.load().pm_forcefield_create(len(ljff.species), dptr(eps), dptr(sig2),
+                                           float(ljff.r2_cutoff), ctypes.byref(out)))
+    h = _Handle(out, "pm_forcefield_destroy")
+    _ff_cache[ljff] = h
+    return h
+
+
+def ewald(crystal, fw: _Handle, eparams) -> _Handle:
+    key = ("ew", id(eparams), fw.ptr.value)
+    h = crystal._device_cache.get(key)
+    if h is not None:
+        return h
+    kidx = np.ascontiguousarray(eparams.kidx, dtype=np.int32)
+    wt = np.ascontiguousarray(eparams.wt, dtype=np.float64)
+    out = ctypes.c_void_p()
+    check(_lib.load().pm_ewald_create(fw.ptr, float(eparams.alpha), float(eparams.sr_cutoff_r),
+                                      int(wt.size), iptr(kidx), dptr(wt), ctypes.byref(out)))
+    h = _Handle(out, "pm_ewald_destroy")
+    h._eparams_ref = eparams  # keep id() stable while cached
+    crystal._device_cache[key] = h
+    return h
+
+
+def probe(molecule, ljff) -> _Handle:
+    """Rigid template from a Cartesian molecule (offsets from its COM)."""
+    from .molecule import site_offsets
+    init()
+    lj_off, q_off = site_offsets(molecule)
+    n_lj, n_q = lj_off.shape[1], q_off.shape[1]
+    sp = np.array([ljff.index(s) for s in molecule.atom_species] if n_lj else [0], dtype=np.int32)
+    lj = np.ascontiguousarray(lj_off.T if n_lj else np.zeros((1, 3)), dtype=np.float64)
+    qx = np.ascontiguousarray(q_off.T if n_q else np.zeros((1, 3)), dtype=np.float64)
+    qq = np.ascontiguousarray(molecule.charges_q if n_q else np.zeros(1), dtype=np.float64)
+    out = ctypes.c_void_p()
+    check(_lib.load().pm_probe_create(n_lj, dptr(lj), iptr(sp), n_q, dptr(qx), dptr(qq),
+                                      ctypes.byref(out)))
+    return _Handle(out, "pm_probe_destroy")
+
+
+def last_kernel_stats():
+    ms = ctypes.c_double()
+    pairs = ctypes.c_double()
+    check(_lib.load().pm_last_kernel_stats(ctypes.byref(ms), ctypes.byref(pairs)))
+    return ms.value, pairs.value
+
+
+def fw_info(fw: _Handle):
+    n = ctypes.c_int64()
+    ns = ctypes.c_int32()
+    q = ctypes.c_double()
+    check(_lib.load().pm_framework_info(fw.ptr, ctypes.byref(n), ctypes.byref(ns), ctypes.byref(q)))
+    return n.value, ns.value, q.value
+
+
+def ewald_info(ew: _Handle) -> int:
+    n = ctypes.c_int64()
+    check(_lib.load().pm_ewald_info(ew.ptr, ctypes.byref(n)))
+    return n.value
+
+
+SITE_CHUNK = 6  # site counts with a kernel instantiation: 1..6, 8
+
+
+def _chunks(n):
+    for s in range(0, n, SITE_CHUNK):
+        yield s, min(n, s + SITE_CHUNK)
+
+
+def vdw_energies(fw, ff, site_species, site_xf, precision) -> np.ndarray:
+    """site_xf: (n_poses, n_sites, 3) replicated-box fractional coordinates."""
+    site_xf = np.asarray(site_xf, dtype=np.float64)
+    n_poses, n_sites = site_xf.shape[0], site_xf.shape[1]
+    total_e = np.zeros(n_poses)
+    lib = _lib.load()
+    for a, b in _chunks(n_sites):
+        x = np.ascontiguousarray(site_xf[:, a:b, :])
+        sp = np.ascontiguousarray(np.asarray(site_species[a:b], dtype=np.int32))
+        out = np.zeros(n_poses)
+        check(lib.pm_vdw_energy(fw.ptr, ff.ptr, b - a, iptr(sp), n_poses, dptr(x),
+                                precision_code(precision), dptr(out)))
+        total_e += out
+    return total_e
+
+
+def es_energies(fw, ew, charges, charge_xf, precision):
+    charge_xf = np.asarray(charge_xf, dtype=np.float64)
+    n_poses, n_q = charge_xf.shape[0], charge_xf.shape[1]
+    sr = np.zeros(n_poses)
+    lr = np.zeros(n_poses)
+    lib = _lib.load()
+    for a, b in _chunks(n_q):
+        x = np.ascontiguousarray(charge_xf[:, a:b, :])
+        q = np.ascontiguousarray(np.asarray(charges[a:b], dtype=np.float64))
+        o_sr = np.zeros(n_poses)
+        o_lr = np.zeros(n_poses)
+        check(lib.pm_electrostatic_energy(fw.ptr, ew.ptr, b - a, dptr(q), n_poses, dptr(x),
+                                          precision_code(precision), dptr(o_sr), dptr(o_lr)))
+        sr += o_sr
+        lr += o_lr
+    return sr, lr
+
+
+def widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision,
+          return_energies=False):
+    lib = _lib.load()
+    sums = np.zeros(3 * n_blocks)
+    energies = np.zeros(2 * n_blocks * n_per_block) if return_energies else None
+    check(lib.pm_widom(fw.ptr, ff.ptr, ew.ptr if ew is not None else None, pr.ptr,
+                       float(temperature), int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_blocks),
+                       int(n_per_block), precision_code(precision), dptr(sums),
+                       dptr(energies) if energies is not None else None))
+    sums = sums.reshape(n_blocks, 3)
+    if return_energies:
+        return sums, energies.reshape(n_blocks * n_per_block, 2)
+    return sums
+
+
+def energy_grid(fw, ff, ew, pr, n_pts, precision) -> np.ndarray:
+    lib = _lib.load()
+    n = np.array(n_pts, dtype=np.int32)
+    out = np.zeros(int(np.prod(n_pts)))
+    check(lib.pm_energy_grid(fw.ptr, ff.ptr if ff is not None else None,
+                             ew.ptr if ew is not None else None, pr.ptr, iptr(n),
+                             precision_code(precision), dptr(out)))
+    return out

@@ -1,0 +1,186 @@
+"""Host-side logic of the package (no GPU): data model, CIF reading, mixing,
+replication, Ewald set-up, checked against the oracle and the reference's
+known answers; and the C-ABI library's exports."""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden
+
+
+def test_cif_reader(pm):
+    x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
+    assert x.n_atoms == 120 and x.charges_q.size == 0
+    assert math.isclose(x.box.Omega, 1456.472102, rel_tol=1e-8)
+    nu = pm.read_cif(golden("crystals", "NU-1000_Greg.cif"))
+    assert nu.n_atoms == 1116 and nu.charges_q.size > 0
+    assert abs(pm.net_charge(nu)) < 1e-3
+    z = pm.read_cif(golden("crystals", "zif71_bogus_charges.cif"))
+    assert z.species[0] == "Zn1"
+    pm.strip_numbers_from_atom_labels(z)
+    assert set(z.species) == {"Zn", "N", "C", "Cl", "H"}
+    assert np.all((z.xf >= 0.0) & (z.xf < 1.0))
+
+
+def test_box_matches_oracle(pm, orc):
+    for lat in [(11.6193, 5.5667, 22.9312, math.pi / 2, 100.897 * math.pi / 180, math.pi / 2),
+                (19.0, 21.0, 18.5, 1.45, 1.62, 1.58)]:
+        b = pm.Box(*lat)
+        np.testing.assert_allclose(b.f_to_c, orc.box_f_to_c(*lat), rtol=0, atol=1e-14)
+        assert b.is_upper_triangular
+        np.testing.assert_allclose(b.reciprocal_lattice @ b.f_to_c, 2 * math.pi * np.eye(3), atol=1e-12)
+        np.testing.assert_allclose(pm.replicate_box(b, (2, 3, 1)).f_to_c,
+                                   orc.box_f_to_c(lat[0] * 2, lat[1] * 3, lat[2], *lat[3:]), atol=1e-13)
+
+
+def test_replication_factors_match_oracle(pm, orc):
+    rng = np.random.Generator(np.random.PCG64(1))
+    for _ in range(50):
+        lat = (*rng.uniform(8, 40, 3), *(rng.uniform(70, 110, 3) * math.pi / 180))
+        b = pm.Box(*lat)
+        for rc in (12.5, 12.8, 14.0):
+            assert pm.replication_factors(b, rc) == orc.replication_factors(b.f_to_c, rc)
+
+
+def test_replicate_crystal_matches_oracle(pm, orc):
+    x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
+    xr = pm.replicate(x, (3, 6, 2))
+    np.testing.assert_array_equal(xr.xf.T, orc.replicate(x.xf.T, (3, 6, 2)))
+    assert xr._unit is x and xr._rep == (3, 6, 2)
+    assert xr.species == x.species * 36
+    # a replica of a replica keeps the unit-cell provenance
+    xrr = pm.replicate(pm.replicate(x, (1, 2, 1)), (3, 3, 2))
+    assert xrr._unit is x and xrr._rep == (3, 6, 2)
+
+
+def test_lorentz_berthelot_tables(pm):
+    ff = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
+    assert math.isclose(ff.r2_cutoff, 156.25)
+    e = ff.epsilon("Xe", "C")
+    assert math.isclose(e, math.sqrt(ff.pure_epsilon["Xe"] * ff.pure_epsilon["C"]))
+    assert math.isclose(ff.sigma2("Xe", "C"), ((ff.pure_sigma["Xe"] + ff.pure_sigma["C"]) / 2) ** 2)
+    # swapped column order (test/data/forcefields/Greg_bogus_ZIF71.csv)
+    g = pm.LJForceField.from_csv(golden("forcefields", "Greg_bogus_ZIF71.csv"))
+    assert g.pure_epsilon["Zn"] == 27.68 and g.pure_sigma["Zn"] == 4.04
+    geo = pm.LJForceField("g", {"A": 2.0, "B": 3.0}, {"A": 4.0, "B": 9.0}, 10.0, "geometric")
+    assert math.isclose(geo.sigma2("A", "B"), 6.0) and math.isclose(geo.epsilon("A", "B"), 6.0)
+    kong = pm.LJForceField("k", {"A": 3.0}, {"A": 50.0}, 10.0, "Kong")
+    assert math.isclose(kong.sigma2("A", "A"), 9.0) and math.isclose(kong.epsilon("A", "A"), 50.0)
+    with pytest.raises(ValueError):
+        pm.LJForceField("bad", {"A": 1.0}, {"A": 1.0}, 10.0, "magic")
+    with pytest.raises(KeyError):
+        ff.index("Unobtainium")
+
+
+def test_ewald_setup_matches_oracle(pm, orc):
+    nu = pm.read_cif(golden("crystals", "NU-1000_Greg.cif"))
+    ep = pm.setup_Ewald_sum(nu.box, 12.5, eps=1e-6)
+    assert ep.kreps == (9, 9, 9)
+    a, k2 = orc.determine_ewald_params(12.5, 1e-6)
+    assert abs(ep.alpha - a) <= 4 * np.spacing(a)
+    kidx, wt = orc.precompute_kvec_wts(ep.kreps, nu.box.f_to_c, a, k2)
+    np.testing.assert_array_equal(ep.kidx, kidx)
+    np.testing.assert_allclose(ep.wt, wt, rtol=1e-13)
+    box = pm.Box(0.5 * 2 * math.pi, 0.1 * 2 * math.pi, 0.2 * 2 * math.pi, math.pi / 2, math.pi / 2, math.pi / 2)
+    assert pm.required_kreps(box, 2.1 ** 2) == (1, 0, 0)
+    assert pm.required_kreps(box, 5.1 ** 2) == (2, 0, 1)
+    assert pm.required_kreps(box, 10.1 ** 2) == (5, 1, 2)
+
+
+def test_molecule_io_and_rotation(pm):
+    co2 = pm.Molecule.from_dir(golden("molecules", "CO2"))
+    assert co2.atom_species == ["C_CO2", "O_CO2", "O_CO2"]
+    np.testing.assert_allclose(co2.com, 0.0, atol=1e-15)
+    assert pm.needs_rotations(co2)
+    xe = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    assert not pm.needs_rotations(xe) and xe.n_charges == 0
+    lj, q = pm.site_offsets(co2)
+    np.testing.assert_allclose(np.linalg.norm(lj[:, 1] - lj[:, 0]), 1.16)
+    m = pm.ion(0.5, [0.1, 0.2, 0.3])
+    assert m.frac and m.n_atoms == 0 and m.charges_q[0] == 0.5
+
+
+def test_henry_savename(pm):
+    x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
+    ff = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
+    xe = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    assert pm.henry_result_savename(x, xe, 298.0, ff, 500, "run") == \
+        "henry_sim_Xe_in_SBMOF-1.cif_298.000000K_Dreiding_ff_500_insertions_per_volume_run.json"
+
+
+def test_synthetic_cases_deterministic(pm):
+    a = pm.synthetic.co2_widom_case(n_per_block=10)
+    b = pm.synthetic.co2_widom_case(n_per_block=10)
+    np.testing.assert_array_equal(a.crystal.xf, b.crystal.xf)
+    assert a.crystal.n_atoms == 600
+    assert abs(float(np.sum(a.crystal.atom_q))) < 1e-10
+    assert pm.replication_factors(a.crystal.box, a.ljff) == (2, 2, 2)
+    # minimum separation
+    x = a.crystal
+    d = x.xf.T[:, None, :] - x.xf.T[None, :, :]
+    d -= np.round(d)
+    c = d @ x.box.f_to_c.T
+    r2 = np.einsum("ijk,ijk->ij", c, c) + np.eye(x.n_atoms) * 1e9
+    assert r2.min() >= 1.0
+    g = pm.synthetic.lj_grid_case()
+    assert g.crystal.n_atoms == 424 and pm.replication_factors(g.crystal.box, g.ljff) == (1, 1, 1)
+
+
+# ------------------------------------------------------------- C ABI ----
+def test_library_exports_every_declared_symbol(pm):
+    from porousmaterials_jl_amd import _lib
+    names = _lib.header_functions()
+    assert "pm_widom" in names and "pm_energy_grid" in names
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"symbols declared in include/pmhip.h but not exported: {missing}"
+    assert set(names) == set(_lib._SIGS), "ctypes signatures out of sync with the header"
+    lib.pm_version.restype = ctypes.c_char_p
+    assert lib.pm_version().startswith(b"pmhip")
+
+
+def test_library_rejects_calls_before_init(pm):
+    """Device entry points refuse with PM_ERR_STATE before pm_init (checked in
+    a fresh process, so no earlier pm_init can interfere)."""
+    import subprocess
+    import sys
+    code = (
+        "import ctypes, numpy as np, sys\n"
+        f"lib = ctypes.CDLL({pm._lib.LIB_PATH!r})\n"
+        "F = np.eye(3).reshape(-1); rep = np.ones(3, dtype=np.int32); out = ctypes.c_void_p()\n"
+        "rc = lib.pm_framework_create(F.ctypes.data_as(ctypes.c_void_p), rep.ctypes.data_as(ctypes.c_void_p),"
+        " ctypes.c_int64(0), None, None, None, ctypes.byref(out))\n"
+        "sys.exit(0 if rc == 4 else 1)\n")
+    assert subprocess.run([sys.executable, "-c", code], timeout=120).returncode == 0
+
+
+def test_forcefield_handle_validation(pm):
+    from porousmaterials_jl_amd import _lib
+    out = ctypes.c_void_p()
+    e = np.array([1.0])
+    s = np.array([-1.0])
+    assert _lib.load().pm_forcefield_create(1, _lib.dptr(e), _lib.dptr(s), 100.0, ctypes.byref(out)) \
+        == _lib.PM_ERR_ARG
+    assert b"sigma" in _lib.load().pm_last_error()
+    s = np.array([9.0])
+    assert _lib.load().pm_forcefield_create(1, _lib.dptr(e), _lib.dptr(s), 100.0, ctypes.byref(out)) == 0
+    assert _lib.load().pm_forcefield_destroy(out) == 0
+
+
+def test_probe_merge_and_limits(pm):
+    from porousmaterials_jl_amd import _lib
+    out = ctypes.c_void_p()
+    lj = np.array([[0.0, 0, 0], [-1.16, 0, 0], [1.16, 0, 0]])
+    q = np.array([0.7, -0.35, -0.35])
+    sp = np.zeros(3, dtype=np.int32)
+    assert _lib.load().pm_probe_create(3, _lib.dptr(lj), _lib.iptr(sp), 3, _lib.dptr(lj), _lib.dptr(q),
+                                       ctypes.byref(out)) == 0
+    _lib.load().pm_probe_destroy(out)
+    many = np.zeros((9, 3))
+    many[:, 0] = np.arange(9)
+    sp9 = np.zeros(9, dtype=np.int32)
+    assert _lib.load().pm_probe_create(9, _lib.dptr(many), _lib.iptr(sp9), 0, None, None,
+                                       ctypes.byref(out)) == _lib.PM_ERR_ARG
