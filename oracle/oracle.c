@@ -693,3 +693,105 @@ int32_t orc_energy_grid(const orc_widom_cfg* c, const int32_t n_pts[3], double* 
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Condition scales: the sum of absolute pair contributions behind each  */
+/* energy, sum_ij |lj_ij| + sum_s |q_s| (sum_j |sr_sj| + |lr_s|).  An     */
+/* energy summed from terms with relative error e has absolute error at   */
+/* most e * scale, so mixed-precision tolerances are stated against it.   */
+/* ------------------------------------------------------------------ */
+static double vdw_abs(int32_t ni, const double* xi, const int32_t* si, int64_t nj,
+                      const double* xj, const int32_t* sj, const double F[9],
+                      int32_t n_species, const double* eps, const double* sig2,
+                      double r2_cutoff) {
+    double total = 0.0;
+    for (int32_t i = 0; i < ni; ++i)
+        for (int64_t j = 0; j < nj; ++j) {
+            double d[3], c[3];
+            for (int k = 0; k < 3; ++k) d[k] = nearest_image(xj[3 * j + k] - xi[3 * i + k]);
+            matvec3(F, d, c);
+            double r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+            if (r2 < r2_cutoff) {
+                int64_t p = (int64_t)si[i] * n_species + sj[j];
+                total += fabs(orc_lennard_jones(r2, sig2[p], eps[p]));
+            }
+        }
+    return total;
+}
+
+static double sr_abs(int64_t nq, const double* xq, const double* q, const double xf[3],
+                     const double F[9], double alpha, double sr_cutoff) {
+    double s = 0.0, fpe = four_pi_eps0();
+    for (int64_t c = 0; c < nq; ++c) {
+        double d[3], cc[3];
+        for (int k = 0; k < 3; ++k) d[k] = nearest_image(xq[3 * c + k] - xf[k]);
+        matvec3(F, d, cc);
+        double r = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+        if (r < sr_cutoff) s += fabs(q[c] / r * erfc(r * alpha) / fpe);
+    }
+    return s;
+}
+
+double orc_vdw_energy_abs(int32_t ni, const double* xi, const int32_t* si, int64_t nj,
+                          const double* xj, const int32_t* sj, const double F[9],
+                          int32_t n_species, const double* eps, const double* sig2,
+                          double r2_cutoff) {
+    return vdw_abs(ni, xi, si, nj, xj, sj, F, n_species, eps, sig2, r2_cutoff);
+}
+
+double orc_phi_sr_abs(int64_t nq, const double* xq, const double* q, const double xf[3],
+                      const double F[9], double alpha, double sr_cutoff) {
+    return sr_abs(nq, xq, q, xf, F, alpha, sr_cutoff);
+}
+
+static double point_scale(const orc_widom_cfg* c, int32_t n_lj, const double* sxf,
+                          int32_t n_q, const double* qxf) {
+    double s = vdw_abs(n_lj, sxf, c->lj_species, c->n_rep, c->xf, c->species, c->F,
+                       c->n_species, c->eps, c->sig2, c->r2_cutoff);
+    if (c->charged)
+        for (int32_t k = 0; k < n_q; ++k) {
+            double a = sr_abs(c->nq, c->xq, c->qq, &qxf[3 * k], c->F, c->alpha, c->sr_cutoff);
+            double lr = orc_phi_lr_sk(c->nk_sk, c->m_sk, c->a_re, c->a_im, c->rep, &qxf[3 * k]);
+            s += fabs(c->q_site[k]) * (a + fabs(lr));
+        }
+    return s;
+}
+
+int32_t orc_widom_scales(const orc_widom_cfg* c, uint64_t g0, int64_t count, double* out) {
+    if (c->n_lj > 16 || c->n_q > 16) return 1;
+    int32_t rotate = (c->n_lj + c->n_q) > 1;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = 0; i < count; ++i) {
+        double u[6], Rm[9], sxf[48], qxf[48];
+        orc_insertion_uniforms(c->seed, g0 + (uint64_t)i, u);
+        if (rotate) orc_arvo_rotation(&u[3], Rm);
+        for (int pass = 0; pass < 2; ++pass) {
+            int32_t ns = pass ? c->n_q : c->n_lj;
+            const double* offs = pass ? c->q_off : c->lj_off;
+            double* dst = pass ? qxf : sxf;
+            for (int32_t s = 0; s < ns; ++s) {
+                double y[3], f[3];
+                if (rotate) matvec3(Rm, &offs[3 * s], y);
+                else { y[0] = offs[3 * s]; y[1] = offs[3 * s + 1]; y[2] = offs[3 * s + 2]; }
+                matvec3(c->c_to_f, y, f);
+                for (int d = 0; d < 3; ++d) dst[3 * s + d] = u[d] + f[d];
+            }
+        }
+        out[i] = point_scale(c, c->n_lj, sxf, c->n_q, qxf);
+    }
+    return 0;
+}
+
+int32_t orc_energy_grid_scales(const orc_widom_cfg* c, const int32_t n_pts[3], double* out) {
+    int64_t n1 = n_pts[0], n2 = n_pts[1], n3 = n_pts[2];
+    int64_t total = n1 * n2 * n3;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < total; ++p) {
+        int64_t i = p % n1, j = (p / n1) % n2, k = p / (n1 * n2);
+        double x[3] = {(n1 > 1 ? (double)i / (double)(n1 - 1) : 0.0) / c->rep[0],
+                       (n2 > 1 ? (double)j / (double)(n2 - 1) : 0.0) / c->rep[1],
+                       (n3 > 1 ? (double)k / (double)(n3 - 1) : 0.0) / c->rep[2]};
+        out[p] = point_scale(c, c->n_lj == 1 ? 1 : 0, x, c->n_q == 1 ? 1 : 0, x);
+    }
+    return 0;
+}

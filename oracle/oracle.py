@@ -68,6 +68,12 @@ def lib():
             "orc_widom_block_sums": (None, [ctypes.c_int64, D, D, ctypes.c_double, D]),
             "orc_energy_grid": (ctypes.c_int32, [ctypes.c_void_p, I32, D]),
             "orc_num_threads": (ctypes.c_int32, []),
+            "orc_vdw_energy_abs": (ctypes.c_double, [ctypes.c_int32, D, I32, ctypes.c_int64, D, I32, D,
+                                                     ctypes.c_int32, D, D, ctypes.c_double]),
+            "orc_phi_sr_abs": (ctypes.c_double, [ctypes.c_int64, D, D, D, D, ctypes.c_double,
+                                                 ctypes.c_double]),
+            "orc_widom_scales": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, D]),
+            "orc_energy_grid_scales": (ctypes.c_int32, [ctypes.c_void_p, I32, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -336,6 +342,22 @@ class OracleSystem:
         return vdw_energy(site_xf, sp, self.xf, self.species, self.F, self.eps, self.sig2,
                           self.r2_cutoff)
 
+    def vdw_scale(self, site_xf, site_species=None):
+        """sum of |LJ pair terms| inside the cutoff (condition scale of vdw)."""
+        sp = self.lj_species if site_species is None else site_species
+        sx = _c(site_xf).reshape(-1, 3)
+        ss = _c(sp, np.int32)
+        return lib().orc_vdw_energy_abs(sx.shape[0], _d(sx), _i(ss), self.xf.shape[0], _d(self.xf),
+                                        _i(self.species), _d(_colmajor(self.F)), self.eps.shape[0],
+                                        _d(self.eps), _d(self.sig2), self.r2_cutoff)
+
+    def phi_scale(self, x):
+        """sum |q_j erfc(alpha r)/(4 pi eps0 r)| + |lr| per unit probe charge."""
+        x = _c(x)
+        a = lib().orc_phi_sr_abs(self.qq.size, _d(self.xq), _d(self.qq), _d(x), _d(_colmajor(self.F)),
+                                 self.alpha, self.sr_cutoff)
+        return a + abs(self.phi(x)[1])
+
     def phi(self, x, form="sk"):
         """Potential per unit charge at x (replicated fractional): (sr, lr)."""
         x = _c(x)
@@ -359,6 +381,19 @@ class OracleSystem:
         if rc != 0:
             raise ValueError("oracle: too many probe sites")
         return v, e
+
+    def widom_scales(self, seed, g0, count):
+        """per-insertion condition scale (sum of absolute pair contributions)."""
+        self.cfg.seed = int(seed)
+        out = np.zeros(count)
+        lib().orc_widom_scales(ctypes.byref(self.cfg), int(g0), int(count), _d(out))
+        return out
+
+    def energy_grid_scales(self, n_pts):
+        n = np.array(n_pts, dtype=np.int32)
+        out = np.zeros(int(np.prod(n_pts)))
+        lib().orc_energy_grid_scales(ctypes.byref(self.cfg), _i(n), _d(out))
+        return out
 
     def block_sums(self, vdw, es, temperature):
         out = np.zeros(3)

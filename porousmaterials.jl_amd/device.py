@@ -165,6 +165,7 @@ def _chunks(n):
 
 def vdw_energies(fw, ff, site_species, site_xf, precision) -> np.ndarray:
     """site_xf: (n_poses, n_sites, 3) replicated-box fractional coordinates."""
+    ff = _ff_handle(ff)
     site_xf = np.asarray(site_xf, dtype=np.float64)
     n_poses, n_sites = site_xf.shape[0], site_xf.shape[1]
     total_e = np.zeros(n_poses)
@@ -197,9 +198,16 @@ def es_energies(fw, ew, charges, charge_xf, precision):
     return sr, lr
 
 
+def _ff_handle(ff):
+    return ff if ff is None or isinstance(ff, _Handle) else forcefield(ff)
+
+
 def widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision,
           return_energies=False):
+    """Block sums (n_blocks, 3) of a Widom run; ``ff`` is a force-field
+    handle or an LJForceField."""
     lib = _lib.load()
+    ff = _ff_handle(ff)
     sums = np.zeros(3 * n_blocks)
     energies = np.zeros(2 * n_blocks * n_per_block) if return_energies else None
     check(lib.pm_widom(fw.ptr, ff.ptr, ew.ptr if ew is not None else None, pr.ptr,
@@ -214,6 +222,7 @@ def widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision,
 
 def energy_grid(fw, ff, ew, pr, n_pts, precision) -> np.ndarray:
     lib = _lib.load()
+    ff = _ff_handle(ff)
     n = np.array(n_pts, dtype=np.int32)
     out = np.zeros(int(np.prod(n_pts)))
     check(lib.pm_energy_grid(fw.ptr, ff.ptr if ff is not None else None,
