@@ -163,10 +163,16 @@ int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewa
                    const pm_probe* pr, const int32_t n_pts[3], int32_t precision,
                    double* out);
 
-/* Duration (ms, HIP events on the library stream) of the energy kernel of
- * the last pm_widom / pm_energy_grid / pm_vdw_energy / pm_electrostatic_energy
- * call, and the replicated-atom x site pair count it evaluated. */
+/* Timing of the last pm_widom / pm_energy_grid / pm_vdw_energy /
+ * pm_electrostatic_energy call (HIP events on the library stream): the whole
+ * device pipeline of the call in ms, and the site x replicated-atom pair
+ * count of the all-pairs formulation it covers. */
 int pm_last_kernel_stats(double* kernel_ms, double* pair_evals);
+
+/* For the last pm_widom / pm_energy_grid call: time of the pair kernel
+ * alone (summed over batches, ms) and the (point, candidate image) pairs it
+ * evaluated after spatial pruning. */
+int pm_last_pair_kernel_stats(double* pair_kernel_ms, double* candidate_pairs);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,7 @@ _SIGS = {
                                 ctypes.c_int64, ctypes.c_int32, D, D]),
     "pm_energy_grid": (ctypes.c_int, [P, P, P, P, I32, ctypes.c_int32, D]),
     "pm_last_kernel_stats": (ctypes.c_int, [D, D]),
+    "pm_last_pair_kernel_stats": (ctypes.c_int, [D, D]),
 }
 
 

@@ -1,27 +1,11 @@
-// pmhip.hip -- gfx950 (CDNA4, wave64) kernels and C ABI for guest-host
-// Lennard-Jones + Ewald energies, Widom insertions and energy grids.
+// pmhip.hip -- host side and C ABI of libpmhip.so (declared in include/pmhip.h).
 //
-// Numerical contract (restating the reference, PorousMaterials.jl v0.4.2):
-//   LJ pair        src/vdw_energetics.jl:19-22    4 eps s^3 (s^3 - 1), s = sigma^2/r^2
-//   pair loop      src/vdw_energetics.jl:25-47    nearest image in fractional space,
-//                                                 strict r^2 < rc^2, overlap r^2 < 0.1 -> +Inf
-//   real space     src/electrostatics.jl:377-386  q erfc(alpha r) / (4 pi eps0 r), r < rc
-//   Fourier        src/electrostatics.jl:354-372  sum_k wt_k sum_c q_c cos(k.(x_c - x)),
-//                                                 evaluated here from the unit-cell
-//                                                 structure factor S(k) (identical sum)
-//   Widom          src/henry.jl:212-243           per-block sum e^{-E/T}, E e^{-E/T}
-//   energy grid    src/grid.jl:313-355            point (i,j,k)/(n-1) ./ rep
-//
-// Layout in HBM: replicated framework atoms as double4 (x, y, z fractional in the
-// replicated box, charge) plus an int32 compact species index; k-vectors as
-// KVec {m_a, m_b, m_c, Re A, Im A} with A = wt * N_R * S(m) in FP64.
-//
-// Execution: one work-item per insertion / pose / grid point, 256 work-items per
-// workgroup; the workgroup streams framework atoms through LDS in tiles of 256
-// (coalesced double4 loads), every lane reads the same LDS record (broadcast), so
-// LDS traffic is one record per atom per wave and the pair arithmetic is VALU-bound.
+// Device code is in pm_device.h.  Handles own device copies of their data;
+// every entry point validates its arguments on the host, launches on the
+// library stream and synchronises before returning.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -29,407 +13,29 @@
 #include <string>
 #include <vector>
 
+#include "pm_device.h"
 #include "pmhip.h"
 
-#define PM_VERSION "pmhip 0.1.0 (gfx950)"
+#define PM_VERSION "pmhip 0.2.0 (gfx950)"
+
+using namespace pmk;
 
 namespace {
 
-constexpr int WG = 256;      // work-items per workgroup (4 waves)
-constexpr int TILE = 256;    // framework atoms per LDS tile
-constexpr int KTILE = 128;   // k-vectors per LDS tile
-constexpr double R2_OVERLAP = 0.1;  // src/vdw_energetics.jl:2
+// Point-engine tunables.
+constexpr double CHUNK_R_MAX = 3.0;       // A: largest chunk radius served by the image list
+constexpr double CELL_POINTS = 640.0;     // target points per Widom cell
+constexpr long long MAX_POINT_SLOTS = 1LL << 24;  // point slots per Widom batch
+constexpr long long MAX_IMAGES = 1LL << 26;
 
-enum { MODE_WIDOM = 0, MODE_POSES = 1, MODE_GRID = 2 };
-
-struct KVec {
-    double ma, mb, mc, are, aim, pad;
-};
-
-struct Site {
-    double ox, oy, oz;  // Cartesian offset from the centre of mass (template)
-    double q;           // point charge (e), 0 if none
-    int lj;             // 1 if an LJ sphere
-    int pad;
-};
-
-struct KArgs {
-    int mode, ns, n_atoms, n_types, nk, rotate;
-    int n_blocks, chunks_per_block;
-    long long n_per_block, n_items;
-    int grid_n[3];
-    int pad0;
-    double f11, f12, f13, f22, f23, f33;  // replicated-box f_to_c (upper triangular)
-    double c2f[9];                        // its inverse, column-major
-    double repd[3];                       // replication factors (unit-cell phases)
-    double r2_lj, r2_sr, alpha, inv4pe, temperature;
-    unsigned long long seed;
-    Site site[PM_MAX_SITES];
-    const double4* atoms;
-    const int* species;
-    const void* ljtab;      // [n_types][ns] of (sigma^2, 4 eps) in Real
-    const KVec* kv;
-    const double* pose_xf;  // MODE_POSES: n_items x ns x 3
-    double* out_e;          // per-item outputs (mode dependent)
-    float* out_f;           // MODE_GRID, FP32 grid
-    double* partials;       // MODE_WIDOM: per-workgroup (S, Wvdw, Wes)
-};
-
-template <typename Real>
-struct LJP {
-    Real sig2, eps4;
-};
-
-// ---------------------------------------------------------------- RNG ----
-// Philox4x32-10 (Salmon et al., SC'11); same stream layout as the oracle:
-// counter (g_lo, g_hi, w, 0x57494430), key (seed_lo, seed_hi), w = 0, 1, 2.
-__device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
-        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-}
-
-__device__ inline double u53(uint32_t a, uint32_t b) {
-    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
-}
-
-__device__ inline void insertion_uniforms(unsigned long long seed, unsigned long long g,
-                                          double u[6]) {
-#pragma unroll
-    for (uint32_t w = 0; w < 3; ++w) {
-        uint32_t c[4] = {(uint32_t)g, (uint32_t)(g >> 32), w, 0x57494430u};
-        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-        u[2 * w] = u53(c[0], c[1]);
-        u[2 * w + 1] = u53(c[2], c[3]);
-    }
-}
-
-// ---------------------------------------------------------- pair maths ----
-// FP32 erfc: Numerical Recipes "erfcc" Chebyshev fit (Press et al., Numerical
-// Recipes in C, 2nd ed., section 6.2), fractional error < 1.2e-7 for x >= 0.
-__device__ inline float erfc_f32(float x) {
-    float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * x);
-    float p = 0.17087277f;
-    p = fmaf(p, t, -0.82215223f);
-    p = fmaf(p, t, 1.48851587f);
-    p = fmaf(p, t, -1.13520398f);
-    p = fmaf(p, t, 0.27886807f);
-    p = fmaf(p, t, -0.18628806f);
-    p = fmaf(p, t, 0.09678418f);
-    p = fmaf(p, t, 0.37409196f);
-    p = fmaf(p, t, 1.00002368f);
-    p = fmaf(p, t, -1.26551223f);
-    float y = p - x * x;  // exponent, natural log units
-    return t * __builtin_amdgcn_exp2f(y * 1.4426950408889634f);
-}
-
-template <typename Real>
-struct PairMath;
-
-template <>
-struct PairMath<float> {
-    // 4 eps s^3 (s^3 - 1), s = sigma^2 / r^2
-    __device__ static inline float lj(double r2, float sig2, float eps4) {
-        float s = sig2 * __builtin_amdgcn_rcpf((float)r2);
-        float s3 = s * s * s;
-        return eps4 * fmaf(s3, s3, -s3);
-    }
-    // q erfc(alpha r) / r
-    __device__ static inline float coul(double r2, double q, double alpha) {
-        float rf2 = (float)r2;
-        float rinv = __builtin_amdgcn_rsqf(rf2);
-        float r = rf2 * rinv;
-        return (float)q * erfc_f32((float)alpha * r) * rinv;
-    }
-};
-
-template <>
-struct PairMath<double> {
-    __device__ static inline double lj(double r2, double sig2, double eps4) {
-        double s = sig2 / r2;
-        double s3 = s * s * s;
-        return eps4 * s3 * (s3 - 1.0);
-    }
-    __device__ static inline double coul(double r2, double q, double alpha) {
-        double r = sqrt(r2);
-        return q / r * erfc(r * alpha);
-    }
-};
-
-// --------------------------------------------------------- reductions ----
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// ------------------------------------------------------------- kernel ----
-template <int NS, bool ES, typename Real>
-__global__ __launch_bounds__(WG) void energy_kernel(KArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    // [ max(atom tile, k tile) | types | LJ table ]
-    constexpr int ATOM_BYTES = TILE * (int)sizeof(double4);
-    constexpr int K_BYTES = KTILE * (int)sizeof(KVec);
-    constexpr int REGION = ATOM_BYTES > K_BYTES ? ATOM_BYTES : K_BYTES;
-    double4* s_atoms = reinterpret_cast<double4*>(smem);
-    KVec* s_k = reinterpret_cast<KVec*>(smem);
-    int* s_type = reinterpret_cast<int*>(smem + REGION);
-    LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + REGION + TILE * sizeof(int));
-
-    const int tid = threadIdx.x;
-    const int ntab = a.n_types * NS;
-    const LJP<Real>* gtab = reinterpret_cast<const LJP<Real>*>(a.ljtab);
-    for (int i = tid; i < ntab; i += WG) s_lj[i] = gtab[i];
-
-    // ---- work item and its site positions (replicated-box fractional) ----
-    long long item;
-    bool valid;
-    if (a.mode == MODE_WIDOM) {
-        int b = blockIdx.x / a.chunks_per_block;
-        int c = blockIdx.x - b * a.chunks_per_block;
-        long long i = (long long)c * WG + tid;
-        valid = i < a.n_per_block;
-        item = (long long)b * a.n_per_block + i;
-    } else {
-        item = (long long)blockIdx.x * WG + tid;
-        valid = item < a.n_items;
-    }
-
-    double sx[NS], sy[NS], sz[NS];
-    if (a.mode == MODE_WIDOM) {
-        double u[6];
-        insertion_uniforms(a.seed, (unsigned long long)item, u);
-        double R[9];  // column-major rotation, Arvo: -H * Rz (src/molecule.jl:233-244)
-        if (a.rotate) {
-            double s1, c1, s2, c2;
-            sincospi(2.0 * u[3], &s1, &c1);
-            sincospi(2.0 * u[4], &s2, &c2);
-            double sq = sqrt(u[5]);
-            double v0 = c2 * sq, v1 = s2 * sq, v2 = sqrt(1.0 - u[5]);
-            double H[9] = {1.0 - 2.0 * v0 * v0, -2.0 * v1 * v0,       -2.0 * v2 * v0,
-                           -2.0 * v0 * v1,       1.0 - 2.0 * v1 * v1, -2.0 * v2 * v1,
-                           -2.0 * v0 * v2,       -2.0 * v1 * v2,       1.0 - 2.0 * v2 * v2};
-            // Rz columns: (c1, -s1, 0), (s1, c1, 0), (0, 0, 1)
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                R[r] = -(H[r] * c1 - H[r + 3] * s1);
-                R[r + 3] = -(H[r] * s1 + H[r + 3] * c1);
-                R[r + 6] = -H[r + 6];
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            double y0 = a.site[s].ox, y1 = a.site[s].oy, y2 = a.site[s].oz;
-            if (a.rotate) {
-                double t0 = R[0] * y0 + R[3] * y1 + R[6] * y2;
-                double t1 = R[1] * y0 + R[4] * y1 + R[7] * y2;
-                double t2 = R[2] * y0 + R[5] * y1 + R[8] * y2;
-                y0 = t0; y1 = t1; y2 = t2;
-            }
-            sx[s] = u[0] + (a.c2f[0] * y0 + a.c2f[3] * y1 + a.c2f[6] * y2);
-            sy[s] = u[1] + (a.c2f[1] * y0 + a.c2f[4] * y1 + a.c2f[7] * y2);
-            sz[s] = u[2] + (a.c2f[2] * y0 + a.c2f[5] * y1 + a.c2f[8] * y2);
-        }
-    } else if (a.mode == MODE_POSES) {
-        long long it = valid ? item : 0;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const double* p = a.pose_xf + (it * NS + s) * 3;
-            sx[s] = p[0]; sy[s] = p[1]; sz[s] = p[2];
-        }
-    } else {
-        long long it = valid ? item : 0;
-        int n1 = a.grid_n[0], n2 = a.grid_n[1], n3 = a.grid_n[2];
-        long long i = it % n1, j = (it / n1) % n2, k = it / ((long long)n1 * n2);
-        double x = (n1 > 1 ? (double)i / (double)(n1 - 1) : 0.0) / a.repd[0];
-        double y = (n2 > 1 ? (double)j / (double)(n2 - 1) : 0.0) / a.repd[1];
-        double z = (n3 > 1 ? (double)k / (double)(n3 - 1) : 0.0) / a.repd[2];
-#pragma unroll
-        for (int s = 0; s < NS; ++s) { sx[s] = x; sy[s] = y; sz[s] = z; }
-    }
-
-    // ---- pair loop over framework atoms (LJ + real-space Ewald) ----
-    double acc_lj[NS], acc_sr[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) { acc_lj[s] = 0.0; acc_sr[s] = 0.0; }
-    bool ovl = false;
-    const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
-    const double r2lj = a.r2_lj, r2sr = a.r2_sr, alpha = a.alpha;
-
-    for (int base = 0; base < a.n_atoms; base += TILE) {
-        int nt = min(TILE, a.n_atoms - base);
-        __syncthreads();
-        if (tid < nt) {
-            s_atoms[tid] = a.atoms[base + tid];
-            s_type[tid] = a.species[base + tid];
-        }
-        __syncthreads();
-        for (int j = 0; j < nt; ++j) {
-            const double4 at = s_atoms[j];
-            const int t = s_type[j];
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                double dx = at.x - sx[s], dy = at.y - sy[s], dz = at.z - sz[s];
-                dx -= rint(dx);
-                dy -= rint(dy);
-                dz -= rint(dz);
-                double cx = f11 * dx + f12 * dy + f13 * dz;
-                double cy = f22 * dy + f23 * dz;
-                double cz = f33 * dz;
-                double r2 = cx * cx + cy * cy + cz * cz;
-                const LJP<Real> p = s_lj[t * NS + s];
-                bool in = r2 < r2lj;
-                ovl |= in & (r2 < R2_OVERLAP) & (a.site[s].lj != 0);
-                Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
-                acc_lj[s] += in ? (double)e : 0.0;
-                if (ES) {
-                    bool ins = (r2 < r2sr) & (at.w != 0.0);
-                    Real c = PairMath<Real>::coul(r2, at.w, alpha);
-                    acc_sr[s] += ins ? (double)c : 0.0;
-                }
-            }
-        }
-    }
-
-    // ---- Fourier part from the unit-cell structure factor (FP64) ----
-    double acc_lr[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) acc_lr[s] = 0.0;
-    if (ES) {
-        double ux[NS], uy[NS], uz[NS];
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            ux[s] = sx[s] * a.repd[0];
-            uy[s] = sy[s] * a.repd[1];
-            uz[s] = sz[s] * a.repd[2];
-        }
-        for (int base = 0; base < a.nk; base += KTILE) {
-            int nt = min(KTILE, a.nk - base);
-            __syncthreads();
-            if (tid < nt) s_k[tid] = a.kv[base + tid];
-            __syncthreads();
-            for (int k = 0; k < nt; ++k) {
-                const KVec kv = s_k[k];
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    if (a.site[s].q != 0.0) {
-                        double t = kv.ma * ux[s] + kv.mb * uy[s] + kv.mc * uz[s];
-                        t -= rint(t);
-                        double sn, cs;
-                        sincospi(2.0 * t, &sn, &cs);
-                        acc_lr[s] += kv.are * cs + kv.aim * sn;
-                    }
-                }
-            }
-        }
-    }
-
-    double vdw = 0.0, sr = 0.0, lr = 0.0;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        vdw += acc_lj[s];
-        sr += a.site[s].q * acc_sr[s];
-        lr += a.site[s].q * acc_lr[s];
-    }
-    if (ovl) vdw = INFINITY;
-    sr *= a.inv4pe;
-
-    if (a.mode == MODE_WIDOM) {
-        double es = sr + lr;
-        double w = exp(-(vdw + es) / a.temperature);
-        bool fin = !isinf(vdw) && !isinf(es);
-        double v0 = valid ? w : 0.0;
-        double v1 = (valid && fin) ? w * vdw : 0.0;
-        double v2 = (valid && fin) ? w * es : 0.0;
-        if (a.out_e != nullptr && valid) {
-            a.out_e[2 * item] = vdw;
-            a.out_e[2 * item + 1] = es;
-        }
-        __shared__ double red[3][WG / 64];
-        v0 = wave_sum(v0);
-        v1 = wave_sum(v1);
-        v2 = wave_sum(v2);
-        int wave = tid >> 6, lane = tid & 63;
-        if (lane == 0) { red[0][wave] = v0; red[1][wave] = v1; red[2][wave] = v2; }
-        __syncthreads();
-        if (tid == 0) {
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-            for (int w2 = 0; w2 < WG / 64; ++w2) { t0 += red[0][w2]; t1 += red[1][w2]; t2 += red[2][w2]; }
-            a.partials[3 * blockIdx.x + 0] = t0;
-            a.partials[3 * blockIdx.x + 1] = t1;
-            a.partials[3 * blockIdx.x + 2] = t2;
-        }
-    } else if (a.mode == MODE_POSES) {
-        if (valid) {
-            a.out_e[3 * item + 0] = vdw;
-            a.out_e[3 * item + 1] = sr;
-            a.out_e[3 * item + 2] = lr;
-        }
-    } else {
-        if (valid) {
-            double e = vdw + sr + lr;
-            if (a.out_f != nullptr) a.out_f[item] = (float)e;
-            else a.out_e[item] = e;
-        }
-    }
-}
-
-// Fixed-order per-block reduction of the workgroup partials (deterministic).
-__global__ void block_sums_kernel(const double* partials, int chunks, int n_blocks, double* out) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n_blocks) return;
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    for (int c = 0; c < chunks; ++c) {
-        const double* p = partials + 3LL * ((long long)b * chunks + c);
-        t0 += p[0]; t1 += p[1]; t2 += p[2];
-    }
-    out[3 * b] = t0; out[3 * b + 1] = t1; out[3 * b + 2] = t2;
-}
-
-// Unit-cell structure factor per kept k-vector: one wave per k-vector,
-// lanes stride over the unit-cell charges, FP64 phases and sums.
-__global__ __launch_bounds__(64) void structure_factor_kernel(const double4* uq, long long n,
-                                                              const int* m, const double* wt,
-                                                              double nr, KVec* out) {
-    int k = blockIdx.x, lane = threadIdx.x;
-    double ma = m[3 * k], mb = m[3 * k + 1], mc = m[3 * k + 2];
-    double sre = 0.0, sim = 0.0;
-    for (long long c = lane; c < n; c += 64) {
-        double4 p = uq[c];
-        double t = ma * p.x + mb * p.y + mc * p.z;
-        t -= rint(t);
-        double sn, cs;
-        sincospi(2.0 * t, &sn, &cs);
-        sre += p.w * cs;
-        sim += p.w * sn;
-    }
-    sre = wave_sum(sre);
-    sim = wave_sum(sim);
-    if (lane == 0) {
-        KVec kv;
-        kv.ma = ma; kv.mb = mb; kv.mc = mc;
-        kv.are = wt[k] * nr * sre;
-        kv.aim = wt[k] * nr * sim;
-        kv.pad = 0.0;
-        out[k] = kv;
-    }
-}
-
-// ------------------------------------------------------------ host side ----
 thread_local std::string g_err;
 std::mutex g_mu;
 
 struct Ctx {
     int device = -1;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0, last_pairs = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+    double last_ms = 0.0, last_pair_ms = 0.0, last_pairs = 0.0, last_cand = 0.0;
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     // host staging for the per-call LJ table: outlives the asynchronous copy
@@ -458,7 +64,7 @@ int ensure_scratch(size_t bytes) {
         g_ctx.scratch = nullptr;
         g_ctx.scratch_bytes = 0;
     }
-    size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+    size_t want = std::max(bytes, (size_t)1 << 20);
     PM_HIP(hipMalloc(&g_ctx.scratch, want));
     g_ctx.scratch_bytes = want;
     return PM_OK;
@@ -468,6 +74,18 @@ int require_init() {
     if (g_ctx.device < 0) return fail(PM_ERR_STATE, "pm_init has not been called");
     return PM_OK;
 }
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Carves aligned sub-buffers out of one scratch allocation.
+struct Arena {
+    size_t used = 0;
+    size_t take(size_t bytes) {
+        size_t off = used;
+        used += align256(bytes);
+        return off;
+    }
+};
 
 void inv3(const double m[9], double out[9]) {
     double d = m[0] * (m[4] * m[8] - m[7] * m[5]) - m[3] * (m[1] * m[8] - m[7] * m[2]) +
@@ -483,19 +101,63 @@ void inv3(const double m[9], double out[9]) {
     out[8] = (m[0] * m[4] - m[3] * m[1]) / d;
 }
 
+void cross(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+double norm3(const double* a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+// Perpendicular widths of the parallelepiped spanned by the columns of F.
+void widths(const double F[9], double w[3]) {
+    double n[3];
+    double vol = std::fabs(F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
+                           F[6] * (F[1] * F[5] - F[4] * F[2]));
+    cross(&F[3], &F[6], n);
+    w[0] = vol / norm3(n);
+    cross(&F[0], &F[6], n);
+    w[1] = vol / norm3(n);
+    cross(&F[0], &F[3], n);
+    w[2] = vol / norm3(n);
+}
+
+// Half of the longest diagonal of the parallelepiped with edges ea, eb, ec.
+double half_diagonal(const double* ea, const double* eb, const double* ec) {
+    double best = 0.0;
+    for (int sb = -1; sb <= 1; sb += 2)
+        for (int sc = -1; sc <= 1; sc += 2) {
+            double d[3];
+            for (int i = 0; i < 3; ++i) d[i] = ea[i] + sb * eb[i] + sc * ec[i];
+            best = std::max(best, norm3(d));
+        }
+    return 0.5 * best;
+}
+
+const double INV_4PI_EPS0 =
+    1.0 / (4.0 * M_PI * (8.854187817e-12 / (1.602176565e-19 * 1.602176565e-19) / 1.0e10 * 1.3806488e-23));
+
 }  // namespace
 
 struct pm_framework {
     double F[9];      // replicated-box f_to_c
     double c2f[9];
     int32_t rep[3];
+    double width[3];  // perpendicular widths of the replicated box
     int64_t n_unit = 0, n_rep = 0, n_unit_q = 0;
     std::vector<int32_t> species_global;  // compact -> force-field species
+    std::vector<double> unit_xf, unit_q;  // host copies for the image list
+    std::vector<int32_t> unit_type;
     double net_charge = 0.0;
     bool charged = false;
     double4* d_atoms = nullptr;
     int32_t* d_species = nullptr;
     double4* d_unit_q = nullptr;
+    // image list covering every image within r_img of any point of the box
+    double r_img = -1.0;
+    int64_t n_img = 0;
+    double4* d_img = nullptr;
+    int32_t* d_img_type = nullptr;
 };
 
 struct pm_forcefield {
@@ -508,6 +170,8 @@ struct pm_ewald {
     double alpha = 0.0, sr_cutoff = 0.0;
     int64_t nk = 0;
     KVec* d_k = nullptr;
+    int Ma = 0, Mb = 0, Mc = 0;
+    double2* d_box = nullptr;
 };
 
 struct pm_probe {
@@ -520,27 +184,13 @@ struct pm_probe {
 
 namespace {
 
-// Fill kernel arguments common to every launch.
-void fill_box(KArgs& a, const pm_framework* fw) {
-    a.f11 = fw->F[0]; a.f12 = fw->F[3]; a.f13 = fw->F[6];
-    a.f22 = fw->F[4]; a.f23 = fw->F[7]; a.f33 = fw->F[8];
-    for (int i = 0; i < 9; ++i) a.c2f[i] = fw->c2f[i];
-    for (int i = 0; i < 3; ++i) a.repd[i] = (double)fw->rep[i];
-    a.n_atoms = (int)fw->n_rep;
-    a.n_types = (int)fw->species_global.size();
-    a.atoms = fw->d_atoms;
-    a.species = fw->d_species;
-    a.inv4pe = 1.0 / (4.0 * M_PI * (8.854187817e-12 / (1.602176565e-19 * 1.602176565e-19) /
-                                    1.0e10 * 1.3806488e-23));
-}
-
 // LJ table [n_types][ns] of (sigma^2, 4 eps) for the framework's compact
 // species against each probe site, in the precision of the pair maths.
 template <typename Real>
 void build_ljtab(const pm_framework* fw, const pm_forcefield* ff, int ns, const Site* site,
                  const int32_t* site_species, std::vector<unsigned char>& out) {
     int nt = (int)fw->species_global.size();
-    out.resize(sizeof(LJP<Real>) * (size_t)(nt > 0 ? nt : 1) * ns);
+    out.assign(sizeof(LJP<Real>) * (size_t)(nt > 0 ? nt : 1) * ns, 0);
     LJP<Real>* t = reinterpret_cast<LJP<Real>*>(out.data());
     for (int i = 0; i < nt; ++i)
         for (int s = 0; s < ns; ++s) {
@@ -556,46 +206,22 @@ void build_ljtab(const pm_framework* fw, const pm_forcefield* ff, int ns, const 
         }
 }
 
-size_t lds_bytes(int n_types, int ns, bool fp64) {
-    size_t region = (size_t)TILE * sizeof(double4);
-    size_t kb = (size_t)KTILE * sizeof(KVec);
-    if (kb > region) region = kb;
-    size_t ljb = (size_t)(n_types > 0 ? n_types : 1) * ns * (fp64 ? 16 : 8);
-    return region + TILE * sizeof(int) + ljb;
-}
-
-template <int NS, bool ES, typename Real>
-void launch_t(const KArgs& a, int n_wg, size_t shm) {
-    hipLaunchKernelGGL((energy_kernel<NS, ES, Real>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
-}
-
-template <bool ES, typename Real>
-int launch_ns(const KArgs& a, int n_wg, size_t shm) {
-    switch (a.ns) {
-        case 1: launch_t<1, ES, Real>(a, n_wg, shm); break;
-        case 2: launch_t<2, ES, Real>(a, n_wg, shm); break;
-        case 3: launch_t<3, ES, Real>(a, n_wg, shm); break;
-        case 4: launch_t<4, ES, Real>(a, n_wg, shm); break;
-        case 5: launch_t<5, ES, Real>(a, n_wg, shm); break;
-        case 6: launch_t<6, ES, Real>(a, n_wg, shm); break;
-        case 8: launch_t<8, ES, Real>(a, n_wg, shm); break;
-        default: return fail(PM_ERR_ARG, "unsupported site count " + std::to_string(a.ns));
-    }
+// Builds the LJ table in the host staging buffer; returns its byte size.
+int stage_ljtab(const pm_framework* fw, const pm_forcefield* ff, int ns, const Site* site,
+                const int32_t* site_species, int precision, size_t* bytes) {
+    // no copy from host_tab may still be in flight (a failed call can return
+    // before its final synchronisation)
+    PM_HIP(hipStreamSynchronize(g_ctx.stream));
+    std::vector<unsigned char>& tab = g_ctx.host_tab;
+    if (precision == PM_FP64) build_ljtab<double>(fw, ff, ns, site, site_species, tab);
+    else build_ljtab<float>(fw, ff, ns, site, site_species, tab);
+    *bytes = tab.size();
     return PM_OK;
 }
 
-// Launch the energy kernel with timing events around it.
-int launch_energy(const KArgs& a, bool es, int precision, int n_wg) {
-    bool fp64 = precision == PM_FP64;
-    size_t shm = lds_bytes(a.n_types, a.ns, fp64);
-    if (n_wg <= 0) return PM_OK;
-    PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
-    int rc;
-    if (fp64) rc = es ? launch_ns<true, double>(a, n_wg, shm) : launch_ns<false, double>(a, n_wg, shm);
-    else rc = es ? launch_ns<true, float>(a, n_wg, shm) : launch_ns<false, float>(a, n_wg, shm);
-    if (rc != PM_OK) return rc;
-    PM_HIP(hipGetLastError());
-    PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
+int copy_ljtab(void* dst) {
+    PM_HIP(hipMemcpyAsync(dst, g_ctx.host_tab.data(), g_ctx.host_tab.size(), hipMemcpyHostToDevice,
+                          g_ctx.stream));
     return PM_OK;
 }
 
@@ -616,6 +242,221 @@ int check_species(const pm_framework* fw, const pm_forcefield* ff) {
                                         " outside the force field's " + std::to_string(ff->n) +
                                         " species");
     return PM_OK;
+}
+
+// ----------------------------------------------------------- poses ----
+size_t pose_lds_bytes(int n_types, int ns, bool fp64) {
+    size_t region = std::max((size_t)TILE * sizeof(double4), (size_t)KTILE * sizeof(KVec));
+    return region + TILE * sizeof(int) + (size_t)std::max(n_types, 1) * ns * (fp64 ? 16 : 8);
+}
+
+template <int NS, bool ES, typename Real>
+void pose_launch_t(const PoseArgs& a, int n_wg, size_t shm) {
+    hipLaunchKernelGGL((pose_kernel<NS, ES, Real>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
+}
+
+template <bool ES, typename Real>
+int pose_launch_ns(const PoseArgs& a, int n_wg, size_t shm) {
+    switch (a.ns) {
+        case 1: pose_launch_t<1, ES, Real>(a, n_wg, shm); break;
+        case 2: pose_launch_t<2, ES, Real>(a, n_wg, shm); break;
+        case 3: pose_launch_t<3, ES, Real>(a, n_wg, shm); break;
+        case 4: pose_launch_t<4, ES, Real>(a, n_wg, shm); break;
+        case 5: pose_launch_t<5, ES, Real>(a, n_wg, shm); break;
+        case 6: pose_launch_t<6, ES, Real>(a, n_wg, shm); break;
+        case 8: pose_launch_t<8, ES, Real>(a, n_wg, shm); break;
+        default: return fail(PM_ERR_ARG, "unsupported site count " + std::to_string(a.ns));
+    }
+    return PM_OK;
+}
+
+// Explicit-pose driver behind pm_vdw_energy and pm_electrostatic_energy:
+// out3 receives (vdw, sr, lr) per pose.
+int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, PoseArgs a,
+              const int32_t* site_species, int precision, bool es, const double* site_xf,
+              std::vector<double>& out3) {
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
+    const int64_t n = a.n_items;
+    a.f11 = fw->F[0]; a.f12 = fw->F[3]; a.f13 = fw->F[6];
+    a.f22 = fw->F[4]; a.f23 = fw->F[7]; a.f33 = fw->F[8];
+    for (int i = 0; i < 3; ++i) a.repd[i] = (double)fw->rep[i];
+    a.n_atoms = (int)fw->n_rep;
+    a.n_types = (int)fw->species_global.size();
+    a.atoms = fw->d_atoms;
+    a.species = fw->d_species;
+    a.inv4pe = INV_4PI_EPS0;
+    a.r2_lj = ff ? ff->r2_cutoff : 0.0;
+    if (es) {
+        a.alpha = ew->alpha;
+        a.r2_sr = ew->sr_cutoff * ew->sr_cutoff;
+        a.kv = ew->d_k;
+        a.nk = (int)ew->nk;
+    } else {
+        a.alpha = 1.0;
+        a.r2_sr = 0.0;
+        a.kv = nullptr;
+        a.nk = 0;
+    }
+    size_t tab_bytes = 0;
+    if (int rc = stage_ljtab(fw, ff, a.ns, a.site, site_species, precision, &tab_bytes)) return rc;
+    Arena ar;
+    size_t o_tab = ar.take(tab_bytes);
+    size_t o_pose = ar.take((size_t)n * a.ns * 3 * sizeof(double));
+    size_t o_out = ar.take((size_t)n * 3 * sizeof(double));
+    if (int rc = ensure_scratch(ar.used)) return rc;
+    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
+    if (int rc = copy_ljtab(base + o_tab)) return rc;
+    a.ljtab = base + o_tab;
+    double* d_pose = reinterpret_cast<double*>(base + o_pose);
+    double* d_out = reinterpret_cast<double*>(base + o_out);
+    PM_HIP(hipMemcpyAsync(d_pose, site_xf, (size_t)n * a.ns * 3 * sizeof(double), hipMemcpyHostToDevice,
+                          g_ctx.stream));
+    a.pose_xf = d_pose;
+    a.out_e = d_out;
+    const bool fp64 = precision == PM_FP64;
+    const size_t shm = pose_lds_bytes(a.n_types, a.ns, fp64);
+    const int n_wg = (int)((n + WG - 1) / WG);
+    PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
+    int rc;
+    if (fp64) rc = es ? pose_launch_ns<true, double>(a, n_wg, shm) : pose_launch_ns<false, double>(a, n_wg, shm);
+    else rc = es ? pose_launch_ns<true, float>(a, n_wg, shm) : pose_launch_ns<false, float>(a, n_wg, shm);
+    if (rc != PM_OK) return rc;
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
+    out3.resize((size_t)n * 3);
+    PM_HIP(hipMemcpyAsync(out3.data(), d_out, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                          g_ctx.stream));
+    return finish_timing((double)n * a.ns * fw->n_rep);
+}
+
+// ---------------------------------------------------- point engine ----
+// Image list of the framework: every periodic image (x_unit + t) / rep whose
+// replicated-box fractional coordinates lie within [-m_a, 1 + m_a] on each
+// axis, m_a = r / width_a, so that every image within r of any point of the
+// box [0,1)^3 is present.  Built once per framework and radius.
+int ensure_images(pm_framework* fw, double r) {
+    if (fw->r_img >= r && fw->d_img != nullptr) return PM_OK;
+    double m[3];
+    for (int i = 0; i < 3; ++i) m[i] = r / fw->width[i];
+    int64_t est = fw->n_unit;
+    for (int i = 0; i < 3; ++i) est *= (int64_t)std::ceil(fw->rep[i] * (1.0 + 2.0 * m[i])) + 1;
+    if (est > MAX_IMAGES) return fail(PM_ERR_ARG, "image list too large");
+    std::vector<double4> img;
+    std::vector<int32_t> ty;
+    img.reserve((size_t)est);
+    ty.reserve((size_t)est);
+    for (int64_t u = 0; u < fw->n_unit; ++u) {
+        const double* x = &fw->unit_xf[3 * u];
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = (int)std::ceil(-m[a] * fw->rep[a] - x[a]);
+            hi[a] = (int)std::floor((1.0 + m[a]) * fw->rep[a] - x[a]);
+        }
+        for (int ta = lo[0]; ta <= hi[0]; ++ta)
+            for (int tb = lo[1]; tb <= hi[1]; ++tb)
+                for (int tc = lo[2]; tc <= hi[2]; ++tc) {
+                    img.push_back(make_double4((x[0] + ta) / fw->rep[0], (x[1] + tb) / fw->rep[1],
+                                               (x[2] + tc) / fw->rep[2], fw->unit_q[u]));
+                    ty.push_back(fw->unit_type[u]);
+                }
+    }
+    PM_HIP(hipStreamSynchronize(g_ctx.stream));
+    if (fw->d_img) { PM_HIP(hipFree(fw->d_img)); fw->d_img = nullptr; }
+    if (fw->d_img_type) { PM_HIP(hipFree(fw->d_img_type)); fw->d_img_type = nullptr; }
+    fw->r_img = -1.0;
+    fw->n_img = (int64_t)img.size();
+    size_t n = std::max<size_t>(img.size(), 1);
+    PM_HIP(hipMalloc(&fw->d_img, n * sizeof(double4)));
+    PM_HIP(hipMalloc(&fw->d_img_type, n * sizeof(int32_t)));
+    if (!img.empty()) {
+        PM_HIP(hipMemcpy(fw->d_img, img.data(), img.size() * sizeof(double4), hipMemcpyHostToDevice));
+        PM_HIP(hipMemcpy(fw->d_img_type, ty.data(), ty.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    fw->r_img = r;
+    return PM_OK;
+}
+
+void fill_pargs_common(PArgs& a, const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+                       bool es, int nsite, const Site* site) {
+    a.f11 = fw->F[0]; a.f12 = fw->F[3]; a.f13 = fw->F[6];
+    a.f22 = fw->F[4]; a.f23 = fw->F[7]; a.f33 = fw->F[8];
+    for (int i = 0; i < 3; ++i) a.repd[i] = (double)fw->rep[i];
+    a.r2_lj = ff ? ff->r2_cutoff : 0.0;
+    a.r2_sr = es ? ew->sr_cutoff * ew->sr_cutoff : 0.0;
+    a.alpha = es ? ew->alpha : 1.0;
+    a.r_cmax = std::sqrt(std::max(a.r2_lj, a.r2_sr));
+    a.nsite = nsite;
+    a.n_types = (int)fw->species_global.size();
+    a.n_atoms = (int)fw->n_rep;
+    a.atoms = fw->d_atoms;
+    a.species = fw->d_species;
+    a.site_lj_mask = 0;
+    for (int s = 0; s < nsite; ++s)
+        if (site[s].lj) a.site_lj_mask |= 1u << s;
+}
+
+// Uses the image list (fast path) when it can cover r_cmax + chunk_r.
+int choose_path(pm_framework* fw, PArgs& a) {
+    a.fast = 0;
+    if (a.chunk_r <= CHUNK_R_MAX) {
+        int rc = ensure_images(fw, a.r_cmax + CHUNK_R_MAX + 0.01);
+        if (rc == PM_OK) a.fast = 1;
+        else if (rc != PM_ERR_ARG) return rc;
+    }
+    a.n_img = (int)fw->n_img;
+    a.img = fw->d_img;
+    a.img_type = fw->d_img_type;
+    return PM_OK;
+}
+
+size_t point_lds_bytes(int n_types, int ns, bool fp64) {
+    return (size_t)TILE * (sizeof(double4) + sizeof(int)) + (size_t)std::max(n_types, 1) * ns * (fp64 ? 16 : 8);
+}
+
+template <bool ES, typename Real>
+void point_pair_launch_t(const PArgs& a, int n_wg) {
+    const size_t shm = point_lds_bytes(a.n_types, a.nsite, sizeof(Real) == 8);
+    hipLaunchKernelGGL((point_pair_kernel<ES, Real>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
+}
+
+void point_pair_launch(const PArgs& a, int n_wg, bool es, int precision) {
+    if (precision == PM_FP64) {
+        if (es) point_pair_launch_t<true, double>(a, n_wg);
+        else point_pair_launch_t<false, double>(a, n_wg);
+    } else {
+        if (es) point_pair_launch_t<true, float>(a, n_wg);
+        else point_pair_launch_t<false, float>(a, n_wg);
+    }
+}
+
+void fill_fargs(FArgs& f, const pm_framework* fw, const pm_ewald* ew) {
+    for (int i = 0; i < 3; ++i) f.repd[i] = (double)fw->rep[i];
+    f.Ma = ew->Ma;
+    f.Mb = ew->Mb;
+    f.Mc = ew->Mc;
+    f.A = ew->d_box;
+}
+
+// Cells of the Widom counting sort: about CELL_POINTS points each, half
+// diagonal <= CHUNK_R_MAX where the point density allows.
+void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double* cell_r) {
+    double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
+    double h = std::cbrt(vol * CELL_POINTS / std::max(n_points, 1.0));
+    double scale = 1.0;
+    for (int iter = 0; iter < 64; ++iter) {
+        for (int i = 0; i < 3; ++i)
+            ncell[i] = (int)std::min(1024.0, std::max(1.0, std::round(fw->width[i] / h * scale)));
+        double ea[3], eb[3], ec[3];
+        for (int i = 0; i < 3; ++i) {
+            ea[i] = fw->F[i] / ncell[0];
+            eb[i] = fw->F[3 + i] / ncell[1];
+            ec[i] = fw->F[6 + i] / ncell[2];
+        }
+        *cell_r = half_diagonal(ea, eb, ec);
+        if (*cell_r <= CHUNK_R_MAX || (ncell[0] == 1024 && ncell[1] == 1024 && ncell[2] == 1024)) break;
+        scale *= 1.25;
+    }
 }
 
 }  // namespace
@@ -640,6 +481,8 @@ int pm_init(int32_t device) {
     PM_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     PM_HIP(hipEventCreate(&g_ctx.ev0));
     PM_HIP(hipEventCreate(&g_ctx.ev1));
+    PM_HIP(hipEventCreate(&g_ctx.evp0));
+    PM_HIP(hipEventCreate(&g_ctx.evp1));
     g_ctx.device = device;
     return PM_OK;
 }
@@ -665,11 +508,13 @@ int pm_framework_create(const double f_to_c[9], const int32_t rep[3], int64_t n_
     for (int c = 0; c < 3; ++c)
         for (int r = 0; r < 3; ++r) fw->F[r + 3 * c] = f_to_c[r + 3 * c] * rep[c];
     inv3(fw->F, fw->c2f);
+    widths(fw->F, fw->width);
     for (int i = 0; i < 3; ++i) fw->rep[i] = rep[i];
     fw->n_unit = n_atoms;
     fw->n_rep = n_atoms * nr;
-    // compact species
-    std::vector<int32_t> local(n_atoms);
+    fw->unit_type.resize((size_t)n_atoms);
+    if (n_atoms > 0) fw->unit_xf.assign(xf, xf + 3 * n_atoms);
+    fw->unit_q.assign((size_t)n_atoms, 0.0);
     for (int64_t i = 0; i < n_atoms; ++i) {
         int32_t s = species[i];
         if (s < 0) { delete fw; return fail(PM_ERR_ARG, "pm_framework_create: negative species index"); }
@@ -684,15 +529,18 @@ int pm_framework_create(const double f_to_c[9], const int32_t rep[3], int64_t n_
             found = (int32_t)fw->species_global.size();
             fw->species_global.push_back(s);
         }
-        local[i] = found;
+        fw->unit_type[i] = found;
         for (int d = 0; d < 3; ++d)
             if (!std::isfinite(xf[3 * i + d])) {
                 delete fw;
                 return fail(PM_ERR_ARG, "pm_framework_create: non-finite coordinate");
             }
-        if (charge && !std::isfinite(charge[i])) {
-            delete fw;
-            return fail(PM_ERR_ARG, "pm_framework_create: non-finite charge");
+        if (charge) {
+            if (!std::isfinite(charge[i])) {
+                delete fw;
+                return fail(PM_ERR_ARG, "pm_framework_create: non-finite charge");
+            }
+            fw->unit_q[i] = charge[i];
         }
     }
     std::vector<double4> atoms((size_t)fw->n_rep);
@@ -703,21 +551,20 @@ int pm_framework_create(const double f_to_c[9], const int32_t rep[3], int64_t n_
         for (int rb = 0; rb < rep[1]; ++rb)
             for (int rc = 0; rc < rep[2]; ++rc)
                 for (int64_t i = 0; i < n_atoms; ++i, ++k) {
-                    double q = charge ? charge[i] : 0.0;
                     atoms[k] = make_double4((xf[3 * i] + 1.0 * ra) / rep[0],
                                             (xf[3 * i + 1] + 1.0 * rb) / rep[1],
-                                            (xf[3 * i + 2] + 1.0 * rc) / rep[2], q);
-                    sp[k] = local[i];
+                                            (xf[3 * i + 2] + 1.0 * rc) / rep[2], fw->unit_q[i]);
+                    sp[k] = fw->unit_type[i];
                 }
     for (int64_t i = 0; i < n_atoms; ++i) {
-        double q = charge ? charge[i] : 0.0;
+        double q = fw->unit_q[i];
         fw->net_charge += q;
         if (q != 0.0) uq.push_back(make_double4(xf[3 * i], xf[3 * i + 1], xf[3 * i + 2], q));
     }
     fw->n_unit_q = (int64_t)uq.size();
     fw->charged = fw->n_unit_q > 0;
     auto cleanup = [&]() {
-        hipFree(fw->d_atoms); hipFree(fw->d_species); hipFree(fw->d_unit_q);
+        (void)hipFree(fw->d_atoms); (void)hipFree(fw->d_species); (void)hipFree(fw->d_unit_q);
         delete fw;
     };
     size_t na = (size_t)(fw->n_rep > 0 ? fw->n_rep : 1);
@@ -746,9 +593,11 @@ int pm_framework_create(const double f_to_c[9], const int32_t rep[3], int64_t n_
 int pm_framework_destroy(pm_framework* fw) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!fw) return PM_OK;
-    hipFree(fw->d_atoms);
-    hipFree(fw->d_species);
-    hipFree(fw->d_unit_q);
+    (void)hipFree(fw->d_atoms);
+    (void)hipFree(fw->d_species);
+    (void)hipFree(fw->d_unit_q);
+    (void)hipFree(fw->d_img);
+    (void)hipFree(fw->d_img_type);
     delete fw;
     return PM_OK;
 }
@@ -799,44 +648,69 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
         return fail(PM_ERR_ARG, "pm_ewald_create: alpha and sr_cutoff must be positive and finite");
     std::vector<int32_t> m;
     std::vector<double> w;
+    int Ma = 0, Mb = 0, Mc = 0;
     for (int64_t k = 0; k < n_kvecs; ++k) {
         int32_t na = kidx[3 * k], nb = kidx[3 * k + 1], nc = kidx[3 * k + 2];
         if (!std::isfinite(wt[k])) return fail(PM_ERR_ARG, "pm_ewald_create: non-finite weight");
+        if (na < 0 || (na == 0 && nb < 0) || (na == 0 && nb == 0 && nc <= 0))
+            return fail(PM_ERR_ARG, "pm_ewald_create: k-vectors must be the half-space set (ka >= 0, ...)");
         if (na % fw->rep[0] || nb % fw->rep[1] || nc % fw->rep[2]) continue;
-        m.push_back(na / fw->rep[0]);
-        m.push_back(nb / fw->rep[1]);
-        m.push_back(nc / fw->rep[2]);
+        int ma = na / fw->rep[0], mb = nb / fw->rep[1], mc = nc / fw->rep[2];
+        m.push_back(ma);
+        m.push_back(mb);
+        m.push_back(mc);
         w.push_back(wt[k]);
+        Ma = std::max(Ma, ma);
+        Mb = std::max(Mb, std::abs(mb));
+        Mc = std::max(Mc, std::abs(mc));
     }
+    if ((int64_t)(Ma + 1) * (2 * Mb + 1) * (2 * Mc + 1) > (1LL << 26) || 2 * Mc + 1 > FT)
+        return fail(PM_ERR_ARG, "pm_ewald_create: k-vector box too large");
     auto* ew = new pm_ewald();
     ew->alpha = alpha;
     ew->sr_cutoff = sr_cutoff;
     ew->nk = (int64_t)w.size();
-    if (ew->nk > 0) {
-        int* d_m = nullptr;
-        double* d_w = nullptr;
-        if (hipMalloc(&ew->d_k, ew->nk * sizeof(KVec)) != hipSuccess ||
-            hipMalloc(&d_m, m.size() * sizeof(int)) != hipSuccess ||
-            hipMalloc(&d_w, w.size() * sizeof(double)) != hipSuccess) {
-            hipFree(ew->d_k); hipFree(d_m); hipFree(d_w);
-            delete ew;
-            return fail(PM_ERR_OOM, "pm_ewald_create: device allocation failed");
-        }
-        hipMemcpyAsync(d_m, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, g_ctx.stream);
-        hipMemcpyAsync(d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream);
+    ew->Ma = Ma; ew->Mb = Mb; ew->Mc = Mc;
+    size_t box_n = (size_t)(Ma + 1) * (2 * Mb + 1) * (2 * Mc + 1);
+    int* d_m = nullptr;
+    double* d_w = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(ew->d_k); (void)hipFree(ew->d_box); (void)hipFree(d_m); (void)hipFree(d_w);
+        delete ew;
+    };
+    if (hipMalloc(&ew->d_box, box_n * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&ew->d_k, std::max<size_t>(ew->nk, 1) * sizeof(KVec)) != hipSuccess ||
+        hipMalloc(&d_m, std::max<size_t>(m.size(), 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_w, std::max<size_t>(w.size(), 1) * sizeof(double)) != hipSuccess) {
+        cleanup();
+        return fail(PM_ERR_OOM, "pm_ewald_create: device allocation failed");
+    }
+    hipError_t e = hipMemsetAsync(ew->d_box, 0, box_n * sizeof(double2), g_ctx.stream);
+    if (e == hipSuccess && ew->nk > 0) {
+        e = hipMemcpyAsync(d_m, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, g_ctx.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, g_ctx.stream);
         double nr = (double)fw->rep[0] * fw->rep[1] * fw->rep[2];
-        hipLaunchKernelGGL(structure_factor_kernel, dim3((unsigned)ew->nk), dim3(64), 0, g_ctx.stream,
-                           fw->d_unit_q, (long long)fw->n_unit_q, d_m, d_w, nr, ew->d_k);
-        hipError_t e1 = hipGetLastError();
-        hipError_t e2 = hipStreamSynchronize(g_ctx.stream);
-        hipFree(d_m);
-        hipFree(d_w);
-        if (e1 != hipSuccess || e2 != hipSuccess) {
-            hipFree(ew->d_k);
-            delete ew;
-            return fail(PM_ERR_DEVICE, std::string("pm_ewald_create: structure factor kernel: ") +
-                                           hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(structure_factor_kernel, dim3((unsigned)ew->nk), dim3(64), 0, g_ctx.stream,
+                               fw->d_unit_q, (long long)fw->n_unit_q, d_m, d_w, nr, ew->d_k);
+            e = hipGetLastError();
         }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(fourier_box_kernel, dim3((unsigned)((ew->nk + 255) / 256)), dim3(256), 0,
+                               g_ctx.stream, ew->d_k, (int)ew->nk, Mb, Mc, ew->d_box);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.stream);
+    (void)hipFree(d_m);
+    (void)hipFree(d_w);
+    d_m = nullptr;
+    d_w = nullptr;
+    if (e != hipSuccess) {
+        std::string msg = std::string("pm_ewald_create: ") + hipGetErrorString(e);
+        cleanup();
+        return fail(PM_ERR_DEVICE, msg);
     }
     *out = ew;
     return PM_OK;
@@ -845,7 +719,8 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
 int pm_ewald_destroy(pm_ewald* ew) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!ew) return PM_OK;
-    hipFree(ew->d_k);
+    (void)hipFree(ew->d_k);
+    (void)hipFree(ew->d_box);
     delete ew;
     return PM_OK;
 }
@@ -899,12 +774,6 @@ int pm_probe_create(int32_t n_lj, const double* lj_xyz, const int32_t* lj_specie
         }
         if (s.q != 0.0) pr->charged = true;
     }
-    if (pr->ns == 7) {  // no 7-site instantiation: pad with an inert site
-        Site s{};
-        pr->lj_species[7] = 0;
-        pr->site[7] = s;
-        pr->ns = 8;
-    }
     *out = pr;
     return PM_OK;
 }
@@ -913,78 +782,6 @@ int pm_probe_destroy(pm_probe* pr) {
     delete pr;
     return PM_OK;
 }
-
-}  // extern "C"
-
-namespace {
-
-// Shared set-up for the three modes: box and cut-offs into the kernel
-// arguments, the LJ table into the head of the scratch buffer, and a staging
-// region of `extra_bytes` right after it (returned in *extra) for the
-// caller's inputs and outputs.
-int prepare_energy(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, KArgs& a,
-                   const int32_t* site_species, int precision, bool es, size_t extra_bytes,
-                   unsigned char** extra) {
-    if (precision != PM_FP64 && precision != PM_MIXED)
-        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
-    fill_box(a, fw);
-    // no copy from host_tab may still be in flight (a failed call can return
-    // before its final synchronisation)
-    PM_HIP(hipStreamSynchronize(g_ctx.stream));
-    std::vector<unsigned char>& tab = g_ctx.host_tab;
-    if (precision == PM_FP64) build_ljtab<double>(fw, ff, a.ns, a.site, site_species, tab);
-    else build_ljtab<float>(fw, ff, a.ns, a.site, site_species, tab);
-    size_t tab_bytes = (tab.size() + 255) & ~(size_t)255;
-    if (int rc = ensure_scratch(tab_bytes + extra_bytes)) return rc;
-    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
-    PM_HIP(hipMemcpyAsync(base, tab.data(), tab.size(), hipMemcpyHostToDevice, g_ctx.stream));
-    a.ljtab = base;
-    *extra = base + tab_bytes;
-    a.r2_lj = ff ? ff->r2_cutoff : 0.0;
-    if (es) {
-        a.alpha = ew->alpha;
-        a.r2_sr = ew->sr_cutoff * ew->sr_cutoff;
-        a.kv = ew->d_k;
-        a.nk = (int)ew->nk;
-    } else {
-        a.alpha = 1.0;
-        a.r2_sr = 0.0;
-        a.kv = nullptr;
-        a.nk = 0;
-    }
-    return PM_OK;
-}
-
-size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-
-// Explicit-pose driver behind pm_vdw_energy and pm_electrostatic_energy:
-// out3 receives (vdw, sr, lr) per pose.
-int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, KArgs& a,
-              const int32_t* site_species, int precision, bool es, const double* site_xf,
-              std::vector<double>& out3) {
-    int64_t n = a.n_items;
-    size_t pose_bytes = align256((size_t)n * a.ns * 3 * sizeof(double));
-    size_t out_bytes = align256((size_t)n * 3 * sizeof(double));
-    unsigned char* extra = nullptr;
-    if (int rc = prepare_energy(fw, ff, ew, a, site_species, precision, es, pose_bytes + out_bytes, &extra))
-        return rc;
-    double* d_pose = reinterpret_cast<double*>(extra);
-    double* d_out = reinterpret_cast<double*>(extra + pose_bytes);
-    PM_HIP(hipMemcpyAsync(d_pose, site_xf, (size_t)n * a.ns * 3 * sizeof(double), hipMemcpyHostToDevice,
-                          g_ctx.stream));
-    a.pose_xf = d_pose;
-    a.out_e = d_out;
-    int n_wg = (int)((n + WG - 1) / WG);
-    if (int rc = launch_energy(a, es, precision, n_wg)) return rc;
-    out3.resize((size_t)n * 3);
-    PM_HIP(hipMemcpyAsync(out3.data(), d_out, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost,
-                          g_ctx.stream));
-    return finish_timing((double)n * a.ns * fw->n_rep);
-}
-
-}  // namespace
-
-extern "C" {
 
 int pm_vdw_energy(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sites,
                   const int32_t* site_species, int64_t n_poses, const double* site_xf,
@@ -1000,8 +797,7 @@ int pm_vdw_energy(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sit
             return fail(PM_ERR_ARG, "pm_vdw_energy: site species outside the force field");
     if (n_poses == 0) return PM_OK;
     if (n_poses > (int64_t)INT32_MAX - WG) return fail(PM_ERR_ARG, "pm_vdw_energy: too many poses");
-    KArgs a{};
-    a.mode = MODE_POSES;
+    PoseArgs a{};
     a.ns = n_sites;
     a.n_items = n_poses;
     for (int s = 0; s < n_sites; ++s) { a.site[s].lj = 1; a.site[s].q = 0.0; }
@@ -1023,8 +819,7 @@ int pm_electrostatic_energy(const pm_framework* fw, const pm_ewald* ew, int32_t 
         if (!std::isfinite(charges[s])) return fail(PM_ERR_ARG, "pm_electrostatic_energy: non-finite charge");
     if (n_poses == 0) return PM_OK;
     if (n_poses > (int64_t)INT32_MAX - WG) return fail(PM_ERR_ARG, "pm_electrostatic_energy: too many poses");
-    KArgs a{};
-    a.mode = MODE_POSES;
+    PoseArgs a{};
     a.ns = n_charges;
     a.n_items = n_poses;
     std::vector<int32_t> zero_species(n_charges, 0);
@@ -1040,76 +835,220 @@ int pm_electrostatic_energy(const pm_framework* fw, const pm_ewald* ew, int32_t 
     return PM_OK;
 }
 
-int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+int pm_widom(const pm_framework* fw_c, const pm_forcefield* ff, const pm_ewald* ew,
              const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
              int64_t n_per_block, int32_t precision, double* out_block_sums,
              double* out_energies) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = require_init()) return rc;
+    pm_framework* fw = const_cast<pm_framework*>(fw_c);  // the image-list cache only
     if (!fw || !ff || !pr || !out_block_sums || n_blocks < 1 || n_per_block < 1)
         return fail(PM_ERR_ARG, "pm_widom: null handle/pointer or empty block layout");
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
     if (!(temperature > 0.0) || !std::isfinite(temperature))
         return fail(PM_ERR_ARG, "pm_widom: temperature must be positive and finite");
     if (int rc = check_species(fw, ff)) return rc;
     for (int s = 0; s < pr->ns; ++s)
         if (pr->site[s].lj && pr->lj_species[s] >= ff->n)
             return fail(PM_ERR_ARG, "pm_widom: probe species outside the force field");
-    bool es = fw->charged && pr->charged;  // src/henry.jl:93
+    const bool es = fw->charged && pr->charged;  // src/henry.jl:93
     if (es && !ew) return fail(PM_ERR_ARG, "pm_widom: charged framework and probe need an Ewald handle");
-    int64_t chunks = (n_per_block + WG - 1) / WG;
-    int64_t n_wg = chunks * n_blocks;
+    const int ns = pr->ns;
+    const int64_t chunks = (n_per_block + WG - 1) / WG;
+    const int64_t n_wg = chunks * n_blocks;
     if (n_wg > (int64_t)INT32_MAX || (int64_t)n_blocks * n_per_block > ((int64_t)1 << 53))
         return fail(PM_ERR_ARG, "pm_widom: too many insertions for one call");
-    KArgs a{};
-    a.mode = MODE_WIDOM;
-    a.ns = pr->ns;
-    a.rotate = pr->rotate;
-    a.n_blocks = n_blocks;
-    a.chunks_per_block = (int)chunks;
-    a.n_per_block = n_per_block;
-    a.n_items = (long long)n_blocks * n_per_block;
-    a.temperature = temperature;
-    a.seed = seed;
-    for (int s = 0; s < pr->ns; ++s) a.site[s] = pr->site[s];
-    size_t part_bytes = align256((size_t)n_wg * 3 * sizeof(double));
-    size_t sums_bytes = align256((size_t)n_blocks * 3 * sizeof(double));
+    const int64_t batch_wg = std::max<int64_t>(1, MAX_POINT_SLOTS / ((int64_t)WG * ns));
+    const int64_t bw_max = std::min(batch_wg, n_wg);
+    const int64_t slots_max = bw_max * WG * ns;
+
+    int ncell[3];
+    double cell_r = 0.0;
+    choose_cells(fw, (double)std::min<int64_t>(n_per_block * n_blocks, bw_max * WG) * ns, ncell, &cell_r);
+    const int64_t n_cells = (int64_t)ncell[0] * ncell[1] * ncell[2];
+    PArgs pa{};
+    fill_pargs_common(pa, fw, ff, ew, es, ns, pr->site);
+    pa.mode = MODE_WIDOM;
+    pa.chunk_r = cell_r;
+    if (int rc = choose_path(fw, pa)) return rc;
+    for (int i = 0; i < 3; ++i) pa.ncell[i] = ncell[i];
+
+    size_t tab_bytes = 0;
+    if (int rc = stage_ljtab(fw, ff, ns, pr->site, pr->lj_species, precision, &tab_bytes)) return rc;
+    const int64_t items_max = n_cells + (slots_max + WG - 1) / WG + 1;
+    Arena ar;
+    const size_t o_tab = ar.take(tab_bytes);
+    const size_t o_part = ar.take((size_t)n_wg * 3 * sizeof(double));
+    const size_t o_sums = ar.take((size_t)n_blocks * 3 * sizeof(double));
+    const size_t o_pos = ar.take((size_t)slots_max * sizeof(double4));
+    const size_t o_cell = ar.take((size_t)slots_max * sizeof(unsigned));
+    const size_t o_order = ar.take((size_t)slots_max * sizeof(unsigned));
+    const size_t o_lj = ar.take((size_t)slots_max * sizeof(double));
+    const size_t o_sr = ar.take((size_t)slots_max * sizeof(double));
+    const size_t o_lr = ar.take((size_t)slots_max * sizeof(double));
+    const size_t o_count = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
+    const size_t o_cursor = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
+    const size_t o_poff = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
+    const size_t o_soff = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
+    const size_t o_nitems = ar.take(sizeof(unsigned) * 4);
+    const size_t o_cand = ar.take(sizeof(unsigned long long));
+    const size_t o_items = ar.take((size_t)items_max * sizeof(uint2));
+    if (int rc = ensure_scratch(ar.used)) return rc;
+    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
+    if (int rc = copy_ljtab(base + o_tab)) return rc;
+    double* d_part = reinterpret_cast<double*>(base + o_part);
+    double* d_sums = reinterpret_cast<double*>(base + o_sums);
+    double4* d_pos = reinterpret_cast<double4*>(base + o_pos);
+    unsigned* d_cell = reinterpret_cast<unsigned*>(base + o_cell);
+    unsigned* d_order = reinterpret_cast<unsigned*>(base + o_order);
+    double* d_lj = reinterpret_cast<double*>(base + o_lj);
+    double* d_sr = reinterpret_cast<double*>(base + o_sr);
+    double* d_lr = reinterpret_cast<double*>(base + o_lr);
+    unsigned* d_count = reinterpret_cast<unsigned*>(base + o_count);
+    unsigned* d_cursor = reinterpret_cast<unsigned*>(base + o_cursor);
+    unsigned* d_poff = reinterpret_cast<unsigned*>(base + o_poff);
+    unsigned* d_soff = reinterpret_cast<unsigned*>(base + o_soff);
+    unsigned* d_nitems = reinterpret_cast<unsigned*>(base + o_nitems);
+    unsigned long long* d_cand = reinterpret_cast<unsigned long long*>(base + o_cand);
+    uint2* d_items = reinterpret_cast<uint2*>(base + o_items);
+    pa.ljtab = base + o_tab;
+    pa.point_off = d_poff;
+    pa.items = d_items;
+    pa.n_items = d_nitems;
+    pa.order = d_order;
+    pa.pos = d_pos;
+    pa.out_lj = d_lj;
+    pa.out_sr = d_sr;
+    pa.cand_pairs = d_cand;
+
     double* d_energies = nullptr;
     if (out_energies) {
-        hipError_t e = hipMalloc(&d_energies, (size_t)a.n_items * 2 * sizeof(double));
+        hipError_t e = hipMalloc(&d_energies, (size_t)n_blocks * n_per_block * 2 * sizeof(double));
         if (e != hipSuccess) return fail(PM_ERR_OOM, "pm_widom: energy buffer allocation failed");
     }
-    a.out_e = d_energies;
-    unsigned char* extra = nullptr;
-    int rc = prepare_energy(fw, ff, es ? ew : nullptr, a, pr->lj_species, precision, es,
-                            part_bytes + sums_bytes, &extra);
-    if (rc == PM_OK) {
-        a.partials = reinterpret_cast<double*>(extra);
-        rc = launch_energy(a, es, precision, (int)n_wg);
+    auto bail = [&](int rc) {
+        (void)hipStreamSynchronize(g_ctx.stream);
+        (void)hipFree(d_energies);
+        return rc;
+    };
+
+    GenArgs ga{};
+    ga.seed = seed;
+    ga.rotate = pr->rotate;
+    ga.nsite = ns;
+    ga.n_blocks = n_blocks;
+    ga.chunks_per_block = (int)chunks;
+    ga.n_per_block = n_per_block;
+    for (int i = 0; i < 3; ++i) ga.ncell[i] = ncell[i];
+    for (int s = 0; s < ns; ++s) {
+        ga.off[s][0] = pr->site[s].ox;
+        ga.off[s][1] = pr->site[s].oy;
+        ga.off[s][2] = pr->site[s].oz;
     }
-    if (rc != PM_OK) { hipFree(d_energies); return rc; }
-    double* d_part = reinterpret_cast<double*>(extra);
-    double* d_sums = reinterpret_cast<double*>(extra + part_bytes);
-    hipLaunchKernelGGL(block_sums_kernel, dim3((n_blocks + 63) / 64), dim3(64), 0, g_ctx.stream,
-                       d_part, (int)chunks, n_blocks, d_sums);
-    hipError_t le = hipGetLastError();
-    if (le != hipSuccess) { hipFree(d_energies); return fail(PM_ERR_DEVICE, hipGetErrorString(le)); }
+    for (int i = 0; i < 9; ++i) ga.c2f[i] = fw->c2f[i];
+    ga.pos = d_pos;
+    ga.cell = d_cell;
+    ga.count = d_count;
+
+    FArgs fa{};
+    fa.mode = MODE_WIDOM;
+    fa.nsite = ns;
+    fa.pos = d_pos;
+    fa.out_lr = d_lr;
+    for (int s = 0; s < ns; ++s)
+        if (pr->site[s].q != 0.0) fa.site_q_mask |= 1u << s;
+    if (es) fill_fargs(fa, fw, ew);
+
+    CArgs ca{};
+    ca.nsite = ns;
+    ca.es = es ? 1 : 0;
+    ca.n_blocks = n_blocks;
+    ca.chunks_per_block = (int)chunks;
+    ca.n_per_block = n_per_block;
+    for (int s = 0; s < ns; ++s) ca.site_q[s] = pr->site[s].q;
+    ca.inv4pe = INV_4PI_EPS0;
+    ca.temperature = temperature;
+    ca.lj = d_lj;
+    ca.sr = d_sr;
+    ca.lr = d_lr;
+    ca.out_e = d_energies;
+    ca.partials = d_part;
+
+    if (hipMemsetAsync(d_cand, 0, sizeof(unsigned long long), g_ctx.stream) != hipSuccess)
+        return bail(fail(PM_ERR_DEVICE, "pm_widom: memset failed"));
+    if (hipEventRecord(g_ctx.ev0, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
+    float pair_ms = 0.f;
+    for (int64_t w0 = 0; w0 < n_wg; w0 += batch_wg) {
+        const int64_t bw = std::min(batch_wg, n_wg - w0);
+        const long long n_slots = bw * WG * ns;
+        const long long fill_n = n_cells + 1;
+        hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((fill_n + 255) / 256)), dim3(256), 0, g_ctx.stream,
+                           d_count, fill_n, 0u);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((fill_n + 255) / 256)), dim3(256), 0, g_ctx.stream,
+                           d_cursor, fill_n, 0u);
+        ga.w0 = (int)w0;
+        hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)bw), dim3(WG), 0, g_ctx.stream, ga);
+        hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, g_ctx.stream, d_count, d_poff, d_soff,
+                           d_nitems, (int)n_cells);
+        hipLaunchKernelGGL(scatter_points_kernel, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0,
+                           g_ctx.stream, d_cell, n_slots, d_poff, d_cursor, d_order);
+        hipLaunchKernelGGL(slice_items_kernel, dim3((unsigned)((n_cells + 255) / 256)), dim3(256), 0,
+                           g_ctx.stream, d_count, d_soff, d_items, (int)n_cells);
+        const int64_t n_items_bound = n_cells + (n_slots + WG - 1) / WG;
+        if (hipEventRecord(g_ctx.evp0, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
+        point_pair_launch(pa, (int)n_items_bound, es, precision);
+        if (hipEventRecord(g_ctx.evp1, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
+        if (es) {
+            fa.n_slots = n_slots;
+            hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)((n_slots + WG - 1) / WG)), dim3(WG), 0,
+                               g_ctx.stream, fa);
+        }
+        ca.w0 = (int)w0;
+        hipLaunchKernelGGL(widom_combine_kernel, dim3((unsigned)bw), dim3(WG), 0, g_ctx.stream, ca);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess)
+            return bail(fail(PM_ERR_DEVICE, std::string("pm_widom: launch: ") + hipGetErrorString(le)));
+        if (w0 + batch_wg < n_wg) {
+            // the next batch reuses the events: read this batch's pair-kernel time now
+            if (hipEventSynchronize(g_ctx.evp1) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event sync"));
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1);
+            pair_ms += ms;
+        }
+    }
+    hipLaunchKernelGGL(block_sums_kernel, dim3((unsigned)n_blocks), dim3(WG), 0, g_ctx.stream, d_part,
+                       (int)chunks, d_sums);
+    if (hipGetLastError() != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: block sums launch"));
+    if (hipEventRecord(g_ctx.ev1, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
+    unsigned long long cand = 0;
     hipError_t ce = hipMemcpyAsync(out_block_sums, d_sums, (size_t)n_blocks * 3 * sizeof(double),
                                    hipMemcpyDeviceToHost, g_ctx.stream);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(&cand, d_cand, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream);
     if (ce == hipSuccess && out_energies)
-        ce = hipMemcpyAsync(out_energies, d_energies, (size_t)a.n_items * 2 * sizeof(double),
+        ce = hipMemcpyAsync(out_energies, d_energies, (size_t)n_blocks * n_per_block * 2 * sizeof(double),
                             hipMemcpyDeviceToHost, g_ctx.stream);
-    int frc = finish_timing((double)a.n_items * pr->ns * fw->n_rep);
-    hipFree(d_energies);
+    int frc = finish_timing((double)n_blocks * n_per_block * ns * fw->n_rep);
+    {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1) == hipSuccess) pair_ms += ms;
+    }
+    g_ctx.last_pair_ms = pair_ms;
+    g_ctx.last_cand = (double)cand;
+    (void)hipFree(d_energies);
     if (ce != hipSuccess) return fail(PM_ERR_DEVICE, std::string("pm_widom: copy: ") + hipGetErrorString(ce));
     return frc;
 }
 
-int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+int pm_energy_grid(const pm_framework* fw_c, const pm_forcefield* ff, const pm_ewald* ew,
                    const pm_probe* pr, const int32_t n_pts[3], int32_t precision, double* out) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = require_init()) return rc;
+    pm_framework* fw = const_cast<pm_framework*>(fw_c);  // the image-list cache only
     if (!fw || !pr || !n_pts || !out)
         return fail(PM_ERR_ARG, "pm_energy_grid: null handle or pointer");
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
     if (pr->ns != 1) return fail(PM_ERR_ARG, "pm_energy_grid: probe must have exactly one site");
     if (pr->site[0].ox != 0.0 || pr->site[0].oy != 0.0 || pr->site[0].oz != 0.0)
         return fail(PM_ERR_ARG, "pm_energy_grid: single-site probe must sit at its centre of mass");
@@ -1120,47 +1059,113 @@ int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewa
         if (int rc = check_species(fw, ff)) return rc;
         if (pr->lj_species[0] >= ff->n) return fail(PM_ERR_ARG, "pm_energy_grid: probe species outside the force field");
     }
-    bool es = fw->charged && pr->charged;
+    const bool es = fw->charged && pr->charged;
     if (es && !ew) return fail(PM_ERR_ARG, "pm_energy_grid: charged probe needs an Ewald handle");
-    long long n_items = (long long)n_pts[0] * n_pts[1] * n_pts[2];
-    if ((n_items + WG - 1) / WG > INT32_MAX) return fail(PM_ERR_ARG, "pm_energy_grid: grid too large");
-    KArgs a{};
-    a.mode = MODE_GRID;
-    a.ns = 1;
-    a.n_items = n_items;
-    for (int i = 0; i < 3; ++i) a.grid_n[i] = n_pts[i];
-    a.site[0] = pr->site[0];
-    bool fp64 = precision == PM_FP64;
-    void* d_grid = nullptr;
-    hipError_t e = hipMalloc(&d_grid, (size_t)n_items * (fp64 ? sizeof(double) : sizeof(float)));
-    if (e != hipSuccess) return fail(PM_ERR_OOM, "pm_energy_grid: grid allocation failed");
-    if (fp64) a.out_e = static_cast<double*>(d_grid);
-    else a.out_f = static_cast<float*>(d_grid);
-    int n_wg = (int)((n_items + WG - 1) / WG);
-    unsigned char* extra = nullptr;
-    int rc = prepare_energy(fw, ff, es ? ew : nullptr, a, pr->lj_species, precision, es, 0, &extra);
-    if (rc == PM_OK) rc = launch_energy(a, es, precision, n_wg);
-    if (rc != PM_OK) { hipFree(d_grid); return rc; }
-    hipError_t ce;
-    if (fp64) {
-        ce = hipMemcpyAsync(out, d_grid, (size_t)n_items * sizeof(double), hipMemcpyDeviceToHost, g_ctx.stream);
-        int frc = finish_timing((double)n_items * fw->n_rep);
-        hipFree(d_grid);
-        if (ce != hipSuccess) return fail(PM_ERR_DEVICE, hipGetErrorString(ce));
-        return frc;
+    const long long n_items = (long long)n_pts[0] * n_pts[1] * n_pts[2];
+    const int brick[3] = {8, 8, 4};
+    int nbrick[3];
+    for (int i = 0; i < 3; ++i) nbrick[i] = (n_pts[i] + brick[i] - 1) / brick[i];
+    const long long n_bricks = (long long)nbrick[0] * nbrick[1] * nbrick[2];
+    if (n_bricks > INT32_MAX || (n_items + WG - 1) / WG > INT32_MAX)
+        return fail(PM_ERR_ARG, "pm_energy_grid: grid too large");
+    // Cartesian span of one brick and its half diagonal
+    double ed[3][3];
+    for (int d = 0; d < 3; ++d) {
+        double step = n_pts[d] > 1 ? (double)(brick[d] - 1) / (double)(n_pts[d] - 1) / fw->rep[d] : 0.0;
+        for (int i = 0; i < 3; ++i) ed[d][i] = fw->F[3 * d + i] * step;
     }
-    std::vector<float> tmp((size_t)n_items);
-    ce = hipMemcpyAsync(tmp.data(), d_grid, (size_t)n_items * sizeof(float), hipMemcpyDeviceToHost, g_ctx.stream);
-    int frc = finish_timing((double)n_items * fw->n_rep);
-    hipFree(d_grid);
-    if (ce != hipSuccess) return fail(PM_ERR_DEVICE, hipGetErrorString(ce));
-    for (long long i = 0; i < n_items; ++i) out[i] = (double)tmp[i];
-    return frc;
+    PArgs pa{};
+    fill_pargs_common(pa, fw, ff, ew, es, 1, pr->site);
+    pa.mode = MODE_GRID;
+    pa.chunk_r = half_diagonal(ed[0], ed[1], ed[2]);
+    if (int rc = choose_path(fw, pa)) return rc;
+    for (int i = 0; i < 3; ++i) {
+        pa.grid_n[i] = n_pts[i];
+        pa.brick[i] = brick[i];
+        pa.nbrick[i] = nbrick[i];
+    }
+    size_t tab_bytes = 0;
+    if (int rc = stage_ljtab(fw, ff, 1, pr->site, pr->lj_species, precision, &tab_bytes)) return rc;
+    const bool fp64 = precision == PM_FP64;
+    Arena ar;
+    const size_t o_tab = ar.take(tab_bytes);
+    const size_t o_lj = ar.take((size_t)n_items * sizeof(double));
+    const size_t o_sr = ar.take((size_t)n_items * sizeof(double));
+    const size_t o_lr = ar.take((size_t)n_items * sizeof(double));
+    const size_t o_grid = ar.take((size_t)n_items * (fp64 ? sizeof(double) : sizeof(float)));
+    const size_t o_cand = ar.take(sizeof(unsigned long long));
+    if (int rc = ensure_scratch(ar.used)) return rc;
+    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
+    if (int rc = copy_ljtab(base + o_tab)) return rc;
+    pa.ljtab = base + o_tab;
+    pa.out_lj = reinterpret_cast<double*>(base + o_lj);
+    pa.out_sr = reinterpret_cast<double*>(base + o_sr);
+    pa.cand_pairs = reinterpret_cast<unsigned long long*>(base + o_cand);
+    double* d_lr = reinterpret_cast<double*>(base + o_lr);
+    PM_HIP(hipMemsetAsync(pa.cand_pairs, 0, sizeof(unsigned long long), g_ctx.stream));
+
+    PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
+    PM_HIP(hipEventRecord(g_ctx.evp0, g_ctx.stream));
+    point_pair_launch(pa, (int)n_bricks, es, precision);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipEventRecord(g_ctx.evp1, g_ctx.stream));
+    if (es) {
+        FArgs fa{};
+        fa.mode = MODE_GRID;
+        fa.nsite = 1;
+        fa.n_slots = n_items;
+        for (int i = 0; i < 3; ++i) fa.grid_n[i] = n_pts[i];
+        fa.site_q_mask = 1u;
+        fa.out_lr = d_lr;
+        fill_fargs(fa, fw, ew);
+        hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)((n_items + WG - 1) / WG)), dim3(WG), 0,
+                           g_ctx.stream, fa);
+        PM_HIP(hipGetLastError());
+    }
+    GCArgs gc{};
+    gc.n = n_items;
+    gc.q = pr->site[0].q;
+    gc.inv4pe = INV_4PI_EPS0;
+    gc.es = es ? 1 : 0;
+    gc.lj = pa.out_lj;
+    gc.sr = pa.out_sr;
+    gc.lr = d_lr;
+    gc.out_d = fp64 ? reinterpret_cast<double*>(base + o_grid) : nullptr;
+    gc.out_f = fp64 ? nullptr : reinterpret_cast<float*>(base + o_grid);
+    hipLaunchKernelGGL(grid_combine_kernel, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0,
+                       g_ctx.stream, gc);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
+    unsigned long long cand = 0;
+    PM_HIP(hipMemcpyAsync(&cand, pa.cand_pairs, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream));
+    std::vector<float> tmp;
+    if (fp64) {
+        PM_HIP(hipMemcpyAsync(out, base + o_grid, (size_t)n_items * sizeof(double), hipMemcpyDeviceToHost,
+                              g_ctx.stream));
+    } else {
+        tmp.resize((size_t)n_items);
+        PM_HIP(hipMemcpyAsync(tmp.data(), base + o_grid, (size_t)n_items * sizeof(float), hipMemcpyDeviceToHost,
+                              g_ctx.stream));
+    }
+    if (int rc = finish_timing((double)n_items * fw->n_rep)) return rc;
+    float ms = 0.f;
+    PM_HIP(hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1));
+    g_ctx.last_pair_ms = ms;
+    g_ctx.last_cand = (double)cand;
+    if (!fp64)
+        for (long long i = 0; i < n_items; ++i) out[i] = (double)tmp[i];
+    return PM_OK;
 }
 
 int pm_last_kernel_stats(double* kernel_ms, double* pair_evals) {
     if (kernel_ms) *kernel_ms = g_ctx.last_ms;
     if (pair_evals) *pair_evals = g_ctx.last_pairs;
+    return PM_OK;
+}
+
+int pm_last_pair_kernel_stats(double* pair_kernel_ms, double* candidate_pairs) {
+    if (pair_kernel_ms) *pair_kernel_ms = g_ctx.last_pair_ms;
+    if (candidate_pairs) *candidate_pairs = g_ctx.last_cand;
     return PM_OK;
 }
 

@@ -141,6 +141,14 @@ def last_kernel_stats():
     return ms.value, pairs.value
 
 
+def last_pair_kernel_stats():
+    """(pair-kernel ms, evaluated candidate pairs) of the last Widom/grid call."""
+    ms = ctypes.c_double()
+    pairs = ctypes.c_double()
+    check(_lib.load().pm_last_pair_kernel_stats(ctypes.byref(ms), ctypes.byref(pairs)))
+    return ms.value, pairs.value
+
+
 def fw_info(fw: _Handle):
     n = ctypes.c_int64()
     ns = ctypes.c_int32()
