@@ -1,0 +1,382 @@
+"""
+    PorousMaterialsHIP
+
+`ccall` binding of libpmhip.so (include/pmhip.h) with the call surface of
+PorousMaterials.jl v0.4.2: `henry_coefficient`, `energy_grid`, `vdw_energy`,
+`electrostatic_potential_energy` and `total_electrostatic_potential_energy`
+take and return the reference's own types (`Crystal`, `Molecule`,
+`LJForceField`, `EwaldParams`, `Eikr`, `EwaldSum`, `Grid`, the 16-key results
+`Dict`).  Device handles are cached per object and released by finalizers.
+
+Usage: `using PorousMaterials, PorousMaterialsHIP`, then call
+`PorousMaterialsHIP.henry_coefficient(...)` exactly as the reference's, or
+`PorousMaterialsHIP.enable!()` to route the reference's own methods through
+the device.
+"""
+module PorousMaterialsHIP
+
+using PorousMaterials
+using LinearAlgebra
+using Printf
+using Statistics
+
+const libpmhip = normpath(joinpath(@__DIR__, "..", "libpmhip.so"))
+
+const PM_FP64 = Int32(0)
+const PM_MIXED = Int32(1)
+
+struct PMError <: Exception
+    code::Int
+    msg::String
+end
+Base.showerror(io::IO, e::PMError) = print(io, "libpmhip error ", e.code, ": ", e.msg)
+
+function check(rc::Integer)
+    if rc != 0
+        msg = unsafe_string(ccall((:pm_last_error, libpmhip), Cstring, ()))
+        throw(PMError(rc, msg))
+    end
+    return nothing
+end
+
+precision_code(p::Symbol) = p === :fp64 ? PM_FP64 : p === :mixed ? PM_MIXED :
+    error("precision must be :mixed or :fp64")
+
+function __init__()
+    check(ccall((:pm_init, libpmhip), Cint, (Int32,), Int32(0)))
+end
+
+# ------------------------------------------------------------ handles ----
+mutable struct Handle
+    ptr::Ptr{Cvoid}
+    destroy::Symbol
+    function Handle(ptr::Ptr{Cvoid}, destroy::Symbol)
+        h = new(ptr, destroy)
+        finalizer(h) do x
+            if x.ptr != C_NULL
+                if x.destroy === :pm_framework_destroy
+                    ccall((:pm_framework_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
+                elseif x.destroy === :pm_forcefield_destroy
+                    ccall((:pm_forcefield_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
+                elseif x.destroy === :pm_ewald_destroy
+                    ccall((:pm_ewald_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
+                else
+                    ccall((:pm_probe_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
+                end
+                x.ptr = C_NULL
+            end
+        end
+        return h
+    end
+end
+
+# species order of the dense tables: sorted symbols of the force field, plus a
+# zero-epsilon slot for framework charges that sit on no LJ atom
+ff_species(ljff::LJForceField) = sort(collect(keys(ljff.pure_σ)))
+
+function ff_tables(ljff::LJForceField)
+    sp = ff_species(ljff)
+    n = length(sp) + 1
+    eps = zeros(n, n)
+    sig2 = ones(n, n)
+    for (i, a) in enumerate(sp), (j, b) in enumerate(sp)
+        eps[i, j] = ljff.ϵ[a][b]      # row-major [i*n + j] == column-major transpose;
+        sig2[i, j] = ljff.σ²[a][b]    # the tables are symmetric
+    end
+    return sp, eps, sig2
+end
+
+const FF_CACHE = WeakKeyDict{LJForceField, Handle}()
+
+function forcefield_handle(ljff::LJForceField)
+    get!(FF_CACHE, ljff) do
+        sp, eps, sig2 = ff_tables(ljff)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:pm_forcefield_create, libpmhip), Cint,
+                    (Int32, Ptr{Float64}, Ptr{Float64}, Float64, Ref{Ptr{Cvoid}}),
+                    Int32(size(eps, 1)), eps, sig2, ljff.r²_cutoff, out))
+        Handle(out[], :pm_forcefield_destroy)
+    end
+end
+
+# Framework atoms for the device: LJ atoms carrying the charge at their
+# position, plus charge-only sites on the zero-epsilon species.
+function framework_arrays(crystal::Crystal, ljff::LJForceField)
+    sp = ff_species(ljff)
+    idx = Dict(s => Int32(i - 1) for (i, s) in enumerate(sp))
+    ghost = Int32(length(sp))
+    n = crystal.atoms.n
+    xf = copy(crystal.atoms.coords.xf)
+    species = Int32[idx[s] for s in crystal.atoms.species]
+    q = zeros(n)
+    extra_x = Vector{Vector{Float64}}()
+    extra_q = Float64[]
+    for c in 1:crystal.charges.n
+        x = crystal.charges.coords.xf[:, c]
+        k = findfirst(i -> xf[:, i] == x, 1:n)
+        if k === nothing
+            push!(extra_x, x)
+            push!(extra_q, crystal.charges.q[c])
+        else
+            q[k] += crystal.charges.q[c]
+        end
+    end
+    if !isempty(extra_q)
+        xf = hcat(xf, reduce(hcat, extra_x))
+        species = vcat(species, fill(ghost, length(extra_q)))
+        q = vcat(q, extra_q)
+    end
+    return xf, species, q
+end
+
+const FW_CACHE = WeakKeyDict{Crystal, Dict{Any, Handle}}()
+
+# species-only force field (every epsilon 0) for Ewald-only device frameworks
+const ES_FF_CACHE = WeakKeyDict{Crystal, LJForceField}()
+
+function es_only_forcefield(crystal::Crystal, r_cutoff::Float64)
+    get!(ES_FF_CACHE, crystal) do
+        species = unique(crystal.atoms.species)
+        pure_σ = Dict{Symbol, Float64}(s => 1.0 for s in species)
+        pure_ϵ = Dict{Symbol, Float64}(s => 0.0 for s in species)
+        σ² = Dict{Symbol, Dict{Symbol, Float64}}(a => Dict{Symbol, Float64}(b => 1.0 for b in species)
+                                                 for a in species)
+        ϵ = Dict{Symbol, Dict{Symbol, Float64}}(a => Dict{Symbol, Float64}(b => 0.0 for b in species)
+                                                for a in species)
+        LJForceField("electrostatics-only", pure_σ, pure_ϵ, σ², ϵ, r_cutoff^2)
+    end
+end
+
+# `crystal` is the unit cell; `rep` the replication factors the reference uses
+function framework_handle(crystal::Crystal, ljff::LJForceField, rep::Tuple{Int, Int, Int})
+    cache = get!(() -> Dict{Any, Handle}(), FW_CACHE, crystal)
+    get!(cache, (objectid(ljff), rep)) do
+        xf, species, q = framework_arrays(crystal, ljff)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:pm_framework_create, libpmhip), Cint,
+                    (Ptr{Float64}, Ptr{Int32}, Int64, Ptr{Float64}, Ptr{Int32}, Ptr{Float64},
+                     Ref{Ptr{Cvoid}}),
+                    Matrix{Float64}(crystal.box.f_to_c), Int32[rep...], Int64(size(xf, 2)), xf,
+                    species, q, out))
+        Handle(out[], :pm_framework_destroy)
+    end
+end
+
+function ewald_handle(fw::Handle, eparams::EwaldParams)
+    kidx = Int32[v for kv in eparams.kvecs for v in (kv.ka, kv.kb, kv.kc)]
+    wt = Float64[kv.wt for kv in eparams.kvecs]
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pm_ewald_create, libpmhip), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Int64, Ptr{Int32}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
+                fw.ptr, eparams.α, eparams.sr_cutoff_r, Int64(length(wt)), kidx, wt, out))
+    return Handle(out[], :pm_ewald_destroy)
+end
+
+function probe_handle(molecule::Molecule{Cart}, ljff::LJForceField)
+    sp = ff_species(ljff)
+    idx = Dict(s => Int32(i - 1) for (i, s) in enumerate(sp))
+    com = molecule.com.x
+    lj = molecule.atoms.coords.x .- com
+    qx = molecule.charges.coords.x .- com
+    species = Int32[idx[s] for s in molecule.atoms.species]
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pm_probe_create, libpmhip), Cint,
+                (Int32, Ptr{Float64}, Ptr{Int32}, Int32, Ptr{Float64}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
+                Int32(molecule.atoms.n), lj, species, Int32(molecule.charges.n), qx,
+                molecule.charges.q, out))
+    return Handle(out[], :pm_probe_destroy)
+end
+
+# ------------------------------------------------- reference surface ----
+"""
+    vdw_energy(crystal, molecule, ljff; precision=:fp64)
+
+`src/vdw_energetics.jl:62-64` on the device: the crystal is the (already
+replicated) framework, the molecule's fractional coordinates refer to its box.
+"""
+function vdw_energy(crystal::Crystal, molecule::Molecule, ljff::LJForceField;
+                    precision::Symbol=:fp64)
+    m = isa(molecule, Molecule{Cart}) ? Frac(molecule, crystal.box) : molecule
+    m.atoms.n == 0 && return 0.0
+    fw = framework_handle(crystal, ljff, (1, 1, 1))
+    ff = forcefield_handle(ljff)
+    sp = ff_species(ljff)
+    idx = Dict(s => Int32(i - 1) for (i, s) in enumerate(sp))
+    species = Int32[idx[s] for s in m.atoms.species]
+    out = zeros(1)
+    check(ccall((:pm_vdw_energy, libpmhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Int32}, Int64, Ptr{Float64}, Int32, Ptr{Float64}),
+                fw.ptr, ff.ptr, Int32(m.atoms.n), species, Int64(1),
+                Matrix{Float64}(m.atoms.coords.xf), precision_code(precision), out))
+    return out[1]
+end
+
+"""
+    electrostatic_potential_energy(crystal, molecule, eparams, eikr; precision=:fp64)
+
+`src/electrostatics.jl:331-342` on the device; returns an `EwaldSum`.
+"""
+function electrostatic_potential_energy(crystal::Crystal, molecule::Molecule{Frac},
+                                        eparams::EwaldParams, eikr::Eikr;
+                                        precision::Symbol=:fp64)
+    molecule.charges.n == 0 && return EwaldSum(0.0, 0.0)
+    fw = framework_handle(crystal, es_only_forcefield(crystal, eparams.sr_cutoff_r), (1, 1, 1))
+    ew = ewald_handle(fw, eparams)
+    sr = zeros(1)
+    lr = zeros(1)
+    check(ccall((:pm_electrostatic_energy, libpmhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Float64}, Int32,
+                 Ptr{Float64}, Ptr{Float64}),
+                fw.ptr, ew.ptr, Int32(molecule.charges.n), molecule.charges.q, Int64(1),
+                Matrix{Float64}(molecule.charges.coords.xf), precision_code(precision), sr, lr))
+    return EwaldSum(sr[1], lr[1])
+end
+
+function total_electrostatic_potential_energy(crystal::Crystal, molecules::Array{Molecule{Frac}, 1},
+                                              eparams::EwaldParams, eikr::Eikr;
+                                              precision::Symbol=:fp64)
+    total = EwaldSum(0.0, 0.0)
+    for m in molecules
+        total += electrostatic_potential_energy(crystal, m, eparams, eikr; precision=precision)
+    end
+    return total
+end
+
+const UNIV_GAS_CONST = 8.3144598e-5
+const K_TO_KJ_PER_MOL = 8.3144598e-3
+const N_BLOCKS = 5
+
+"""
+    henry_coefficient(crystal, molecule, temperature, ljff; insertions_per_volume=200,
+                      verbose=true, ewald_precision=1e-6, autosave=true,
+                      filename_comment="", seed=0, precision=:mixed)
+
+`src/henry.jl:34-181` with the insertions on the device: same insertion count,
+blocks, replication, statistics and 16-key results `Dict`.  Insertions come
+from the Philox stream keyed by `seed`, each with a fresh Arvo rotation.
+"""
+function henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Float64,
+                           ljff::LJForceField; insertions_per_volume::Union{Int, Float64}=200,
+                           verbose::Bool=true, ewald_precision::Float64=1e-6,
+                           autosave::Bool=true, filename_comment::AbstractString="",
+                           accessibility_grid=nothing, seed::Integer=0,
+                           precision::Symbol=:mixed)
+    assert_P1_symmetry(crystal)
+    isnothing(accessibility_grid) ||
+        error("accessibility-grid blocking is not supported by the device path")
+    time_start = time()
+    nb_insertions = max(N_BLOCKS, ceil(Int, insertions_per_volume * crystal.box.Ω))
+    nb_per_block = ceil(Int, nb_insertions / N_BLOCKS)
+    repfactors = replication_factors(crystal.box, ljff)
+    xtal_rep = replicate(crystal, repfactors)
+    charged_system = has_charges(crystal) & has_charges(molecule)
+    ρ = crystal_density(xtal_rep)
+    fw = framework_handle(crystal, ljff, repfactors)
+    ff = forcefield_handle(ljff)
+    ew = C_NULL
+    ew_h = nothing
+    if charged_system
+        eparams = setup_Ewald_sum(xtal_rep.box, sqrt(ljff.r²_cutoff), ϵ=ewald_precision,
+                                  verbose=verbose)
+        ew_h = ewald_handle(fw, eparams)
+        ew = ew_h.ptr
+    end
+    pr = probe_handle(molecule, ljff)
+    sums = zeros(3, N_BLOCKS)
+    check(ccall((:pm_widom, libpmhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, UInt64, Int32, Int64,
+                 Int32, Ptr{Float64}, Ptr{Float64}),
+                fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int32(N_BLOCKS),
+                Int64(nb_per_block), precision_code(precision), sums, C_NULL))
+    boltzmann_factor_sums = sums[1, :]
+    average_energies = [PotentialEnergy(sums[2, b] / sums[1, b], sums[3, b] / sums[1, b])
+                        for b in 1:N_BLOCKS]
+    henry_coefficients = boltzmann_factor_sums / (UNIV_GAS_CONST * temperature * nb_per_block)
+    err_kh = 2.0 * std(henry_coefficients) / sqrt(N_BLOCKS)
+    err_vdw = 2.0 * std([e.vdw for e in average_energies]) / sqrt(N_BLOCKS)
+    err_es = 2.0 * std([e.es for e in average_energies]) / sqrt(N_BLOCKS)
+    results = Dict{String, Any}()
+    results["henry coefficient [mol/(m³-bar)]"] = mean(henry_coefficients)
+    results["xtal"] = crystal.name
+    results["henry coefficient [mmol/(g-bar)]"] = results["henry coefficient [mol/(m³-bar)]"] / ρ
+    results["err henry coefficient [mmol/(g-bar)]"] = err_kh / ρ
+    results["henry coefficient [mol/(kg-Pa)]"] = results["henry coefficient [mmol/(g-bar)]"] / 100000.0
+    results["⟨U, vdw⟩ (K)"] = mean([e.vdw for e in average_energies])
+    results["⟨U, es⟩ (K)"] = mean([e.es for e in average_energies])
+    results["⟨U⟩ (K)"] = results["⟨U, vdw⟩ (K)"] + results["⟨U, es⟩ (K)"]
+    results["⟨U⟩ (kJ/mol)"] = results["⟨U⟩ (K)"] * K_TO_KJ_PER_MOL
+    results["⟨U, vdw⟩ (kJ/mol)"] = results["⟨U, vdw⟩ (K)"] * K_TO_KJ_PER_MOL
+    results["err ⟨U, vdw⟩ (kJ/mol)"] = err_vdw * K_TO_KJ_PER_MOL
+    results["⟨U, es⟩ (kJ/mol)"] = results["⟨U, es⟩ (K)"] * K_TO_KJ_PER_MOL
+    results["err ⟨U, es⟩ (kJ/mol)"] = err_es * K_TO_KJ_PER_MOL
+    results["Qst (kJ/mol)"] = -results["⟨U⟩ (kJ/mol)"] + temperature * K_TO_KJ_PER_MOL
+    results["err Qst (kJ/mol)"] = (err_vdw + err_es) * K_TO_KJ_PER_MOL
+    results["elapsed time (min)"] = (time() - time_start) / 60
+    if autosave
+        path = rc[:paths][:simulations]
+        isdir(path) || mkdir(path)
+        savename = joinpath(path, henry_result_savename(crystal, molecule, temperature, ljff,
+                                                        insertions_per_volume,
+                                                        comment=filename_comment))
+        PorousMaterials.JLD2.@save savename results
+    end
+    return results
+end
+
+"""
+    energy_grid(crystal, molecule, ljff; resolution=1.0, n_rotations=1000,
+                temperature=NaN, units=:kJ_mol, center=false, verbose=true,
+                precision=:mixed)
+
+`src/grid.jl:291-362` for single-site probes on the device; a single
+point-charge probe gives the per-point Ewald energy.
+"""
+function energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljforcefield::LJForceField;
+                     resolution::Union{Float64, Tuple{Int, Int, Int}}=1.0, n_rotations::Int=1000,
+                     temperature::Float64=NaN, units::Symbol=:kJ_mol, center::Bool=false,
+                     verbose::Bool=true, precision::Symbol=:mixed, ewald_precision::Float64=1e-6)
+    assert_P1_symmetry(crystal)
+    units in (:kJ_mol, :K) || error("Pass :kJ_mol or :K for units of kJ/mol or K, respectively.")
+    needs_rotations(molecule) &&
+        error("energy grids of multi-site probes are not supported by the device path")
+    n_pts = isa(resolution, Tuple{Int, Int, Int}) ? resolution : required_n_pts(crystal.box, resolution)
+    repfactors = replication_factors(crystal.box, ljforcefield)
+    fw = framework_handle(crystal, ljforcefield, repfactors)
+    ff = forcefield_handle(ljforcefield)
+    ew = C_NULL
+    ew_h = nothing
+    if has_charges(crystal) && has_charges(molecule)
+        xtal_rep = replicate(crystal, repfactors)
+        eparams = setup_Ewald_sum(xtal_rep.box, sqrt(ljforcefield.r²_cutoff), ϵ=ewald_precision)
+        ew_h = ewald_handle(fw, eparams)
+        ew = ew_h.ptr
+    end
+    pr = probe_handle(molecule, ljforcefield)
+    data = zeros(Float64, n_pts...)
+    check(ccall((:pm_energy_grid, libpmhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Float64}),
+                fw.ptr, ff.ptr, ew, pr.ptr, Int32[n_pts...], precision_code(precision), data))
+    if units == :kJ_mol
+        data .*= 8.314 / 1000.0
+    end
+    return Grid(crystal.box, n_pts, data, units,
+                center ? crystal.box.f_to_c * [-0.5, -0.5, -0.5] : [0.0, 0.0, 0.0])
+end
+
+"""
+    enable!()
+
+Route PorousMaterials' own `henry_coefficient` and `energy_grid` through the
+device (type-specific methods added to the reference's functions).
+"""
+function enable!()
+    @eval PorousMaterials.henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Float64,
+                                            ljff::LJForceField; kwargs...) =
+        PorousMaterialsHIP.henry_coefficient(crystal, molecule, temperature, ljff; kwargs...)
+    @eval PorousMaterials.energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljff::LJForceField;
+                                      kwargs...) =
+        PorousMaterialsHIP.energy_grid(crystal, molecule, ljff; kwargs...)
+    return nothing
+end
+
+end # module
