@@ -158,7 +158,7 @@ def main():
             "insertions_per_step": n_ins, "precision": "mixed"},
         "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "point_pair_kernel<true, float>", "kernel_ms": p_ms,
+                     "kernel": "point_pair_kernel<1, float>", "kernel_ms": p_ms,
                      "flop_per_launch": pair_flop,
                      "candidate_pairs_per_point": cand / n_points,
                      "pruning_factor": (n_points * xr.n_atoms) / cand},

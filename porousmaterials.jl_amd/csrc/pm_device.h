@@ -477,13 +477,16 @@ __global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_
     for (unsigned k = 0; k < ns; ++k) items[base + k] = make_uint2((unsigned)c, k);
 }
 
+constexpr int ERFC_SEG = 512;  // cubic segments of erfc(alpha r) on [0, r_sr] (FP32 path)
+
 struct PArgs {
     double f11, f12, f13, f22, f23, f33;  // replicated-box f_to_c (upper triangular)
     double repd[3];
     double r2_lj, r2_sr, alpha;
     double r_cmax;       // max(r_lj, r_sr)
     double chunk_r;      // bound on |point - chunk centre| (Cartesian)
-    int fast;            // image list covers r_cmax + chunk_r around every cell
+    double erfc_inv_h;   // ERFC_SEG / r_sr
+    int fast;           // image list covers r_cmax + chunk_r around every cell
     int mode;            // MODE_WIDOM (cells) or MODE_GRID (bricks)
     int nsite, n_types, n_img, n_atoms;
     unsigned site_lj_mask;
@@ -502,6 +505,7 @@ struct PArgs {
     const double4* atoms; // replicated atoms (fallback path)
     const int* species;
     const void* ljtab;    // [n_types][nsite] LJP<Real>
+    const float4* erfc_tab;  // ERFC_SEG cubics of erfc(alpha r) in t = r/h - i (FP32 path)
     double* out_lj;       // by point slot / grid index
     double* out_sr;
     unsigned long long* cand_pairs;  // evaluated candidate pairs (reporting)
@@ -511,12 +515,48 @@ __device__ inline double grid_coord(double idx, int n, double rep) {
     return (n > 1 ? idx / (double)(n - 1) : 0.0) / rep;
 }
 
-template <bool ES, typename Real>
+// q erfc(alpha r) / r with erfc(alpha r) from the cubic table: segment
+// i = floor(r/h), clamped to the table (r beyond r_sr is masked by the
+// caller), t = r/h - i in [0, 1).
+__device__ inline float coul_table(double r2, double q, float inv_h, const float4* tab) {
+    const float rf2 = (float)r2;
+    const float rinv = __builtin_amdgcn_rsqf(rf2);
+    const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
+    const int i = (int)x;
+    const float t = x - (float)i;
+    const float4 c = tab[i];
+    const float e = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+    return (float)q * e * rinv;
+}
+
+// One candidate pair: LJ into plj, real-space Ewald into psr.
+// ESM: 0 no electrostatics, 1 LJ and Ewald share the cutoff, 2 separate cutoffs.
+template <int ESM, typename Real>
+__device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool is_lj, double r2lj,
+                                 double r2sr, double alpha, float inv_h, const float4* tab, bool& ovl,
+                                 Real& plj, Real& psr) {
+    const bool in = r2 < r2lj;
+    ovl |= in & (r2 < R2_OVERLAP) & is_lj;
+    const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
+    plj += in ? e : (Real)0;
+    if constexpr (ESM != 0) {
+        const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (q != 0.0);
+        Real cv;
+        if constexpr (sizeof(Real) == 4) cv = coul_table(r2, q, inv_h, tab);
+        else cv = PairMath<Real>::coul(r2, q, alpha);
+        psr += ins ? cv : (Real)0;
+    }
+}
+
+template <int ESM, typename Real>
 __global__ __launch_bounds__(WG) void point_pair_kernel(PArgs a) {
+    constexpr bool ES = ESM != 0;
+    constexpr int ERFC_BYTES = (ES && sizeof(Real) == 4) ? ERFC_SEG * (int)sizeof(float4) : 0;
     extern __shared__ __align__(16) unsigned char smem[];
     double4* s_c = reinterpret_cast<double4*>(smem);
     int* s_t = reinterpret_cast<int*>(smem + TILE * sizeof(double4));
-    LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + TILE * (sizeof(double4) + sizeof(int)));
+    float4* s_erfc = reinterpret_cast<float4*>(smem + TILE * (sizeof(double4) + sizeof(int)));
+    LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + TILE * (sizeof(double4) + sizeof(int)) + ERFC_BYTES);
     __shared__ int s_wcnt[WG / 64];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -558,11 +598,17 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(PArgs a) {
     }
     const LJP<Real>* gtab = reinterpret_cast<const LJP<Real>*>(a.ljtab);
     for (int t = tid; t < a.n_types * a.nsite; t += WG) s_lj[t] = gtab[t];
+    if constexpr (ERFC_BYTES != 0)
+        for (int t = tid; t < ERFC_SEG; t += WG) s_erfc[t] = a.erfc_tab[t];
 
     const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
     const double r2lj = a.r2_lj, r2sr = a.r2_sr, alpha = a.alpha;
+    const float inv_h = (float)a.erfc_inv_h;
     const bool is_lj = (a.site_lj_mask >> site) & 1u;
     const int nsite = a.nsite;
+    // a wave with no valid point skips the pair loops; it still takes part in
+    // the tile scans and barriers
+    const bool wave_live = __ballot(valid) != 0ull;
     double acc_lj = 0.0, acc_sr = 0.0;
     bool ovl = false;
     unsigned long long n_cand_total = 0;
@@ -592,7 +638,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(PArgs a) {
                 const double cz = f33 * ez;
                 keep = cx * cx + cy * cy + cz * cz < R2;
                 rec = make_double4(cx, cy, cz, im.w);
-                ty = a.img_type[t];
+                ty = a.img_type[t] * nsite;
             }
             const unsigned long long m = __ballot(keep);
             const int before = __popcll(m & ((1ull << lane) - 1ull));
@@ -609,28 +655,21 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(PArgs a) {
             }
             __syncthreads();
             n_cand_total += (unsigned long long)woff[WG / 64];
-            // candidates of each 64-image group accumulate in Real, flushed to FP64
+            if (wave_live) {
+                // candidates of each 64-image group accumulate in Real, flushed to FP64
 #pragma unroll
-            for (int g = 0; g < WG / 64; ++g) {
-                Real plj = (Real)0, psr = (Real)0;
-                for (int j = woff[g]; j < woff[g + 1]; ++j) {
-                    const double4 c = s_c[j];
-                    const int cty = s_t[j];
-                    const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
-                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const LJP<Real> p = s_lj[cty * nsite + site];
-                    const bool in = r2 < r2lj;
-                    ovl |= in & (r2 < R2_OVERLAP) & is_lj;
-                    const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
-                    plj += in ? e : (Real)0;
-                    if (ES) {
-                        const bool ins = (r2 < r2sr) & (c.w != 0.0);
-                        const Real cv = PairMath<Real>::coul(r2, c.w, alpha);
-                        psr += ins ? cv : (Real)0;
+                for (int g = 0; g < WG / 64; ++g) {
+                    Real plj = (Real)0, psr = (Real)0;
+                    for (int j = woff[g]; j < woff[g + 1]; ++j) {
+                        const double4 c = s_c[j];
+                        const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
+                        const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        pair_term<ESM, Real>(r2, c.w, s_lj[s_t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
+                                             s_erfc, ovl, plj, psr);
                     }
+                    acc_lj += (double)plj;
+                    acc_sr += (double)psr;
                 }
-                acc_lj += (double)plj;
-                acc_sr += (double)psr;
             }
         }
     } else {
@@ -640,43 +679,36 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(PArgs a) {
             __syncthreads();
             if (tid < nt) {
                 s_c[tid] = a.atoms[base + tid];
-                s_t[tid] = a.species[base + tid];
+                s_t[tid] = a.species[base + tid] * nsite;
             }
             __syncthreads();
             n_cand_total += (unsigned long long)nt;
-            for (int g = 0; g < nt; g += 64) {
-                Real plj = (Real)0, psr = (Real)0;
-                const int ge = min(nt, g + 64);
-                for (int j = g; j < ge; ++j) {
-                    const double4 at = s_c[j];
-                    const int cty = s_t[j];
-                    double ex = at.x - px, ey = at.y - py, ez = at.z - pz;
-                    ex -= rint(ex);
-                    ey -= rint(ey);
-                    ez -= rint(ez);
-                    const double cx = f11 * ex + f12 * ey + f13 * ez;
-                    const double cy = f22 * ey + f23 * ez;
-                    const double cz = f33 * ez;
-                    const double r2 = cx * cx + cy * cy + cz * cz;
-                    const LJP<Real> p = s_lj[cty * nsite + site];
-                    const bool in = r2 < r2lj;
-                    ovl |= in & (r2 < R2_OVERLAP) & is_lj;
-                    const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
-                    plj += in ? e : (Real)0;
-                    if (ES) {
-                        const bool ins = (r2 < r2sr) & (at.w != 0.0);
-                        const Real cv = PairMath<Real>::coul(r2, at.w, alpha);
-                        psr += ins ? cv : (Real)0;
+            if (wave_live) {
+                for (int g = 0; g < nt; g += 64) {
+                    Real plj = (Real)0, psr = (Real)0;
+                    const int ge = min(nt, g + 64);
+                    for (int j = g; j < ge; ++j) {
+                        const double4 at = s_c[j];
+                        double ex = at.x - px, ey = at.y - py, ez = at.z - pz;
+                        ex -= rint(ex);
+                        ey -= rint(ey);
+                        ez -= rint(ez);
+                        const double cx = f11 * ex + f12 * ey + f13 * ez;
+                        const double cy = f22 * ey + f23 * ez;
+                        const double cz = f33 * ez;
+                        const double r2 = cx * cx + cy * cy + cz * cz;
+                        pair_term<ESM, Real>(r2, at.w, s_lj[s_t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
+                                             s_erfc, ovl, plj, psr);
                     }
+                    acc_lj += (double)plj;
+                    acc_sr += (double)psr;
                 }
-                acc_lj += (double)plj;
-                acc_sr += (double)psr;
             }
         }
     }
     if (valid) {
         a.out_lj[pid] = ovl ? INFINITY : acc_lj;
-        if (ES) a.out_sr[pid] = acc_sr;
+        if constexpr (ES) a.out_sr[pid] = acc_sr;
     }
     if (a.cand_pairs != nullptr) {
         const unsigned long long m = __ballot(valid);

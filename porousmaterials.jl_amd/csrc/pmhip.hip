@@ -172,6 +172,7 @@ struct pm_ewald {
     KVec* d_k = nullptr;
     int Ma = 0, Mb = 0, Mc = 0;
     double2* d_box = nullptr;
+    float4* d_erfc = nullptr;  // ERFC_SEG cubics of erfc(alpha r) on [0, sr_cutoff]
 };
 
 struct pm_probe {
@@ -386,6 +387,8 @@ void fill_pargs_common(PArgs& a, const pm_framework* fw, const pm_forcefield* ff
     a.r2_sr = es ? ew->sr_cutoff * ew->sr_cutoff : 0.0;
     a.alpha = es ? ew->alpha : 1.0;
     a.r_cmax = std::sqrt(std::max(a.r2_lj, a.r2_sr));
+    a.erfc_tab = es ? ew->d_erfc : nullptr;
+    a.erfc_inv_h = es ? ERFC_SEG / ew->sr_cutoff : 1.0;
     a.nsite = nsite;
     a.n_types = (int)fw->species_global.size();
     a.n_atoms = (int)fw->n_rep;
@@ -410,24 +413,27 @@ int choose_path(pm_framework* fw, PArgs& a) {
     return PM_OK;
 }
 
-size_t point_lds_bytes(int n_types, int ns, bool fp64) {
-    return (size_t)TILE * (sizeof(double4) + sizeof(int)) + (size_t)std::max(n_types, 1) * ns * (fp64 ? 16 : 8);
+size_t point_lds_bytes(int n_types, int ns, bool fp64, bool es) {
+    return (size_t)TILE * (sizeof(double4) + sizeof(int)) + ((es && !fp64) ? ERFC_SEG * sizeof(float4) : 0) +
+           (size_t)std::max(n_types, 1) * ns * (fp64 ? 16 : 8);
 }
 
-template <bool ES, typename Real>
+template <int ESM, typename Real>
 void point_pair_launch_t(const PArgs& a, int n_wg) {
-    const size_t shm = point_lds_bytes(a.n_types, a.nsite, sizeof(Real) == 8);
-    hipLaunchKernelGGL((point_pair_kernel<ES, Real>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
+    const size_t shm = point_lds_bytes(a.n_types, a.nsite, sizeof(Real) == 8, ESM != 0);
+    hipLaunchKernelGGL((point_pair_kernel<ESM, Real>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
+}
+
+template <typename Real>
+void point_pair_launch_r(const PArgs& a, int n_wg, bool es) {
+    if (!es) point_pair_launch_t<0, Real>(a, n_wg);
+    else if (a.r2_lj == a.r2_sr) point_pair_launch_t<1, Real>(a, n_wg);
+    else point_pair_launch_t<2, Real>(a, n_wg);
 }
 
 void point_pair_launch(const PArgs& a, int n_wg, bool es, int precision) {
-    if (precision == PM_FP64) {
-        if (es) point_pair_launch_t<true, double>(a, n_wg);
-        else point_pair_launch_t<false, double>(a, n_wg);
-    } else {
-        if (es) point_pair_launch_t<true, float>(a, n_wg);
-        else point_pair_launch_t<false, float>(a, n_wg);
-    }
+    if (precision == PM_FP64) point_pair_launch_r<double>(a, n_wg, es);
+    else point_pair_launch_r<float>(a, n_wg, es);
 }
 
 void fill_fargs(FArgs& f, const pm_framework* fw, const pm_ewald* ew) {
@@ -675,10 +681,28 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
     int* d_m = nullptr;
     double* d_w = nullptr;
     auto cleanup = [&]() {
-        (void)hipFree(ew->d_k); (void)hipFree(ew->d_box); (void)hipFree(d_m); (void)hipFree(d_w);
+        (void)hipFree(ew->d_k); (void)hipFree(ew->d_box); (void)hipFree(ew->d_erfc);
+        (void)hipFree(d_m); (void)hipFree(d_w);
         delete ew;
     };
-    if (hipMalloc(&ew->d_box, box_n * sizeof(double2)) != hipSuccess ||
+    // Cubic Hermite segments of erfc(alpha r) on [0, sr_cutoff], h = sr_cutoff /
+    // ERFC_SEG, in t = r/h - i: c0 + c1 t + c2 t^2 + c3 t^3 matches erfc and its
+    // derivative at both ends (interpolation error <= h^4 max|f''''| / 384).
+    std::vector<float4> etab(ERFC_SEG);
+    {
+        const double h = sr_cutoff / ERFC_SEG;
+        const double dscale = -2.0 * alpha / std::sqrt(M_PI);
+        for (int i = 0; i < ERFC_SEG; ++i) {
+            const double r0 = i * h, r1 = (i + 1) * h;
+            const double f0 = std::erfc(alpha * r0), f1 = std::erfc(alpha * r1);
+            const double d0 = h * dscale * std::exp(-alpha * alpha * r0 * r0);
+            const double d1 = h * dscale * std::exp(-alpha * alpha * r1 * r1);
+            etab[i] = make_float4((float)f0, (float)d0, (float)(3.0 * (f1 - f0) - 2.0 * d0 - d1),
+                                  (float)(2.0 * (f0 - f1) + d0 + d1));
+        }
+    }
+    if (hipMalloc(&ew->d_erfc, ERFC_SEG * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&ew->d_box, box_n * sizeof(double2)) != hipSuccess ||
         hipMalloc(&ew->d_k, std::max<size_t>(ew->nk, 1) * sizeof(KVec)) != hipSuccess ||
         hipMalloc(&d_m, std::max<size_t>(m.size(), 1) * sizeof(int)) != hipSuccess ||
         hipMalloc(&d_w, std::max<size_t>(w.size(), 1) * sizeof(double)) != hipSuccess) {
@@ -686,6 +710,9 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
         return fail(PM_ERR_OOM, "pm_ewald_create: device allocation failed");
     }
     hipError_t e = hipMemsetAsync(ew->d_box, 0, box_n * sizeof(double2), g_ctx.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ew->d_erfc, etab.data(), ERFC_SEG * sizeof(float4), hipMemcpyHostToDevice,
+                           g_ctx.stream);
     if (e == hipSuccess && ew->nk > 0) {
         e = hipMemcpyAsync(d_m, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, g_ctx.stream);
         if (e == hipSuccess)
@@ -721,6 +748,7 @@ int pm_ewald_destroy(pm_ewald* ew) {
     if (!ew) return PM_OK;
     (void)hipFree(ew->d_k);
     (void)hipFree(ew->d_box);
+    (void)hipFree(ew->d_erfc);
     delete ew;
     return PM_OK;
 }
