@@ -85,6 +85,59 @@ def cpu_baseline(pm, case, xr, seed, target_s):
     }
 
 
+def screening_leg(pm, steps, n_per_block=20000):
+    """256 synthetic frameworks x (CH4, CO2) x 10^5 insertions in one
+    pm_widom_batch call per step (mixed precision)."""
+    t0 = time.perf_counter()
+    cases = pm.synthetic.screening_cases()
+    runs, flop_in, n_ins = [], 0.0, 0
+    for i, c in enumerate(cases):
+        x, ff = c.crystal, c.ljff
+        rep = pm.replication_factors(x.box, ff)
+        xr = pm.replicate(x, rep)
+        fw = pm.device.framework(xr, ff)
+        ffh = pm.device.forcefield(ff)
+        ep = pm.setup_Ewald_sum(xr.box, math.sqrt(ff.r2_cutoff), eps=c.ewald_precision, verbose=False)
+        ew = pm.device.ewald(xr, fw, ep)
+        r_lj, r_sr = math.sqrt(ff.r2_cutoff), ep.sr_cutoff_r
+        for name in ("CH4", "CO2"):
+            m = c.probes[name]
+            runs.append((fw, ffh, ew if m.n_charges else None, pm.device.probe(m, ff), 298.0, 4242 + i, 5,
+                         n_per_block))
+            n = 5 * n_per_block
+            n_ins += n
+            flop_in += n * (m.n_atoms * in_cutoff(xr.n_atoms, xr.box.Omega, r_lj) * C_LJ
+                            + (m.n_charges * in_cutoff(xr.n_atoms, xr.box.Omega, r_sr) * C_SR
+                               if m.n_charges else 0.0))
+    setup_s = time.perf_counter() - t0
+    pm.device.widom_batch(runs, "mixed")  # warmup (image lists, scratch)
+    pipe, pair, cand = [], [], []
+    t = time.perf_counter()
+    for _ in range(steps):
+        pm.device.widom_batch(runs, "mixed")
+        pipe.append(pm.device.last_kernel_stats()[0])
+        pk, pc = pm.device.last_pair_kernel_stats()
+        pair.append(pk)
+        cand.append(pc)
+    dt = (time.perf_counter() - t) / steps
+    p_ms = float(np.mean(pair))
+    flop = C_GEOM * float(np.mean(cand)) + flop_in
+    t = time.perf_counter()
+    pm.device.widom_batch(runs, "fp64")
+    dt64 = time.perf_counter() - t
+    print(f"screening: {len(runs)} runs, {n_ins} insertions, {dt * 1e3:.1f} ms/call, pair kernel "
+          f"{p_ms:.1f} ms, setup {setup_s:.1f} s", file=sys.stderr, flush=True)
+    return {
+        "workload": (f"screening: {len(cases)} synthetic triclinic frameworks (N ~ logU[50,3000] capped at "
+                     f"0.05/A^3, lengths U[8,40] A, angles U[70,110] deg, 5 charged species) x (CH4, CO2) x "
+                     f"{5 * n_per_block} insertions, one pm_widom_batch call"),
+        "runs": len(runs), "insertions": n_ins, "value": n_ins / dt, "unit": "insertions/s",
+        "ms_per_call": dt * 1e3, "device_pipeline_ms": float(np.mean(pipe)), "pair_kernel_ms": p_ms,
+        "pair_kernel_tflops": flop / (p_ms * 1e-3) / 1e12,
+        "fp64_value": n_ins / dt64, "host_setup_s": setup_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +147,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-screening", action="store_true", help="skip the batched screening leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -181,6 +235,8 @@ def main():
         out["mixed_over_fp64"] = value / (n_ins / dt64)
         out["kh_rel_diff_mixed_vs_fp64"] = abs(r32[kh] - r64[kh]) / abs(r64[kh])
         out["u_rel_diff_mixed_vs_fp64"] = abs(r32["⟨U⟩ (K)"] - r64["⟨U⟩ (K)"]) / abs(r64["⟨U⟩ (K)"])
+    if not args.no_screening:
+        out["screening"] = screening_leg(pm, max(1, min(args.steps, 3)))
     if not args.no_cpu:
         cb = cpu_baseline(pm, case, xr, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb

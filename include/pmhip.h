@@ -163,6 +163,51 @@ int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewa
                    const pm_probe* pr, const int32_t n_pts[3], int32_t precision,
                    double* out);
 
+#define PM_MAX_BATCH_RUNS 65536  /* runs per pm_widom_batch / pm_energy_grid_batch call */
+
+/* One Widom run of a batch: the arguments of pm_widom. */
+typedef struct pm_widom_run {
+    const pm_framework* framework;
+    const pm_forcefield* forcefield;
+    const pm_ewald* ewald;          /* NULL when the framework or the probe is uncharged */
+    const pm_probe* probe;
+    double temperature;
+    uint64_t seed;
+    int64_t n_per_block;
+    int32_t n_blocks;
+    int32_t reserved;               /* 0 */
+} pm_widom_run;
+
+/*
+ * Screening: many Widom runs (frameworks x adsorbates) in one call.  The
+ * reference runs `henry_coefficient` (src/henry.jl:34-181) once per
+ * (crystal, molecule) pair from a user loop; this is that loop with every
+ * run's insertions in the same launches (one per pipeline stage, over a work
+ * list of all runs' insertions, most expensive runs first).  Each run gives
+ * exactly what pm_widom gives for it; out_block_sums holds run r's n_blocks x 3
+ * sums after those of runs 0..r-1.
+ */
+int pm_widom_batch(int32_t n_runs, const pm_widom_run* runs, int32_t precision,
+                   double* out_block_sums);
+
+/* One grid of a batch: the arguments of pm_energy_grid. */
+typedef struct pm_grid_run {
+    const pm_framework* framework;
+    const pm_forcefield* forcefield;
+    const pm_ewald* ewald;
+    const pm_probe* probe;
+    int32_t n_pts[3];
+    int32_t reserved;               /* 0 */
+    double* out;                    /* n_pts[0]*n_pts[1]*n_pts[2], i fastest */
+} pm_grid_run;
+
+/*
+ * Many energy grids (src/grid.jl:291-362, one call per framework in the
+ * reference) in one call: every grid's points in the same launches.  Each
+ * grid equals what pm_energy_grid gives for it.
+ */
+int pm_energy_grid_batch(int32_t n_runs, const pm_grid_run* runs, int32_t precision);
+
 /* Timing of the last pm_widom / pm_energy_grid / pm_vdw_energy /
  * pm_electrostatic_energy call (HIP events on the library stream): the whole
  * device pipeline of the call in ms, and the site x replicated-atom pair

@@ -34,6 +34,22 @@ D = ctypes.POINTER(ctypes.c_double)
 I32 = ctypes.POINTER(ctypes.c_int32)
 I64 = ctypes.POINTER(ctypes.c_int64)
 
+
+
+class WidomRun(ctypes.Structure):
+    """pm_widom_run: one Widom run of pm_widom_batch."""
+    _fields_ = [("framework", P), ("forcefield", P), ("ewald", P), ("probe", P),
+                ("temperature", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("n_per_block", ctypes.c_int64), ("n_blocks", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class GridRun(ctypes.Structure):
+    """pm_grid_run: one grid of pm_energy_grid_batch."""
+    _fields_ = [("framework", P), ("forcefield", P), ("ewald", P), ("probe", P),
+                ("n_pts", ctypes.c_int32 * 3), ("reserved", ctypes.c_int32), ("out", D)]
+
+
 _SIGS = {
     "pm_version": (ctypes.c_char_p, []),
     "pm_last_error": (ctypes.c_char_p, []),
@@ -54,6 +70,8 @@ _SIGS = {
     "pm_widom": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
                                 ctypes.c_int64, ctypes.c_int32, D, D]),
     "pm_energy_grid": (ctypes.c_int, [P, P, P, P, I32, ctypes.c_int32, D]),
+    "pm_widom_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(WidomRun), ctypes.c_int32, D]),
+    "pm_energy_grid_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GridRun), ctypes.c_int32]),
     "pm_last_kernel_stats": (ctypes.c_int, [D, D]),
     "pm_last_pair_kernel_stats": (ctypes.c_int, [D, D]),
 }

@@ -1,0 +1,679 @@
+// pm_points.h -- the point engine of libpmhip.so (gfx950, wave64).
+//
+// Every probe site is an independent point.  A call runs a work list of runs
+// (Widom runs: framework, probe, insertion stream and block layout; or energy
+// grids), cut into segments that fill launch groups of at most a fixed number
+// of point slots.  Each stage is one launch over every segment of a group:
+//
+//   widom_gen_kernel        poses from the counter stream, sites wrapped into
+//                           [0,1)^3 and binned into the run's cells
+//   scan_*_kernel,
+//   scatter_points_kernel,
+//   slice_items_kernel      counting sort by cell; work items (cell, slice)
+//   point_pair_kernel       LJ + real-space Ewald of a chunk of points (one
+//                           slice of a cell, or one brick of a grid) against
+//                           the framework images near the chunk
+//   point_fourier_kernel    FP64 Fourier part per charged point
+//   widom_combine_kernel    per insertion energy, Boltzmann weight, partials
+//   grid_combine_kernel     per grid point energy
+//   block_sums_kernel       fixed-order per-block sums of every run
+//
+// Per-run parameters live in device memory (RunArgs); a workgroup finds its
+// segment (and with it its run) by a binary search that is uniform across the
+// workgroup.  A point's energy is a function of the point and its run alone:
+// pair vectors are differences of absolute Cartesian coordinates, and the
+// FP32 partial sums follow the image list, which is fixed by the framework and
+// the cutoff radius.  The cell layout, the segment and launch group a point
+// lands in and the tuning of chunk sizes therefore never change a result.
+#pragma once
+
+#include "pm_device.h"
+
+namespace pmk {
+
+constexpr int ERFC_SEG = 512;      // cubic segments of erfc(alpha r) on [0, r_sr] (FP32 path)
+constexpr int SCAN_WG = 1024;      // work-items of the scan kernels
+constexpr int SCAN_PER = 4;        // cells per work-item in the scan kernels
+constexpr int SCAN_SPAN = SCAN_WG * SCAN_PER;
+
+// One Widom run or grid of a call (device memory).
+struct RunArgs {
+    // replicated box
+    double f11, f12, f13, f22, f23, f33;  // f_to_c (upper triangular)
+    double c2f[9];                        // c_to_f, column-major
+    double repd[3];
+    // pair terms
+    double r2_lj, r2_sr, alpha, r_cmax, erfc_inv_h;
+    double chunk_r;          // bound on |point - chunk centre| (Cartesian)
+    double inv4pe;
+    int esm;                 // 0 no electrostatics, 1 LJ and Ewald share r_c, 2 separate cutoffs
+    int fast;                // the image list covers r_cmax + chunk_r around every chunk
+    int nsite, n_types, n_img, n_atoms;
+    unsigned site_lj_mask, site_q_mask;
+    const double4* img;      // images near the box: Cartesian xyz (A) + charge
+    const int* img_type;
+    const double4* atoms;    // replicated atoms (fallback path)
+    const int* species;
+    const void* ljtab;       // [n_types][nsite] LJP<Real>
+    const float4* erfc_tab;  // ERFC_SEG cubics of erfc(alpha r) in t = r/h - i
+    // Fourier box: ma in [0, Ma], mb in [-Mb, Mb], mc in [-Mc, Mc]
+    int Ma, Mb, Mc;
+    const double2* A;        // wt * N_R * S(m), zeros off the kept set
+    // Widom
+    unsigned long long seed;
+    long long n_per_block;
+    int rotate, n_blocks, chunks_per_block, block_base;
+    int ncell[3];
+    double off[PM_MAX_SITES][3];  // site offsets from the COM (Cartesian)
+    double site_q[PM_MAX_SITES];
+    double temperature;
+    double* partials;        // per insertion workgroup of the run: (S, W_vdw, W_es)
+    double* out_e;           // nullable: (E_vdw, E_es) per insertion
+    // grid
+    int grid_n[3], brick[3], nbrick[3];
+    double grid_q;
+    double* grid_d;          // FP64 output, or
+    float* grid_f;           // FP32 output
+};
+
+// A contiguous piece of one run inside a launch group.
+struct SegArgs {
+    int run;
+    int w0;               // Widom: first insertion workgroup of the run in this segment
+    int wg_base;          // first gen/combine workgroup (Widom) or brick (grid) of the segment
+    unsigned cell_base;   // Widom: first cell of the segment
+    long long slot_base;  // first point slot of the segment (a multiple of WG)
+    long long n_slots;    // point slots of the segment (grid: the grid's points)
+};
+
+struct GroupArgs {
+    const RunArgs* runs;
+    const SegArgs* segs;
+    int n_seg, mode;
+    double4* pos;            // Widom: per slot, fractional xyz in [0,1)^3, w = 1 (valid) or 0
+    unsigned* cell;          // Widom: per slot, cell in the group or 0xffffffff
+    unsigned* count;         // per cell
+    const unsigned* point_off;
+    const uint4* items;      // (cell, slice, segment, 0)
+    const unsigned* n_items;
+    const unsigned* order;   // slots sorted by cell
+    double* out_lj;          // per slot
+    double* out_sr;
+    double* out_lr;
+    unsigned long long* cand_pairs;  // evaluated candidate pairs (reporting)
+};
+
+enum { SEG_BY_WG = 0, SEG_BY_CELL = 1, SEG_BY_SLOT = 2 };
+
+// Last segment whose first workgroup / cell / slot is <= v.
+template <int KEY>
+__device__ inline int find_segment(const SegArgs* s, int n, long long v) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const long long k = KEY == SEG_BY_WG ? (long long)s[mid].wg_base
+                            : KEY == SEG_BY_CELL ? (long long)s[mid].cell_base
+                                                 : s[mid].slot_base;
+        if (k <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline double grid_coord(double idx, int n, double rep) {
+    return (n > 1 ? idx / (double)(n - 1) : 0.0) / rep;
+}
+
+// ------------------------------------------------------------ Widom gen ----
+// One work-item per insertion slot: pose from the stream, sites wrapped into
+// [0,1)^3 (integer shifts change neither the nearest-image pairs nor the
+// Fourier phases m.(R o x)), binned into the run's cells.
+__global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
+    const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
+    const RunArgs& a = g.runs[sg.run];
+    const int lw = (int)blockIdx.x - sg.wg_base;  // workgroup within the segment
+    const int w = sg.w0 + lw;                      // insertion workgroup within the run
+    const int b = w / a.chunks_per_block, c = w - b * a.chunks_per_block;
+    const long long i = (long long)c * WG + threadIdx.x;
+    const bool valid = i < a.n_per_block;
+    const int ns = a.nsite;
+    const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
+    if (!valid) {
+        for (int s = 0; s < ns; ++s) {
+            g.pos[p0 + s] = make_double4(0.0, 0.0, 0.0, 0.0);
+            g.cell[p0 + s] = 0xffffffffu;
+        }
+        return;
+    }
+    const unsigned long long gi = (unsigned long long)b * a.n_per_block + i;
+    double u[6];
+    insertion_uniforms(a.seed, gi, u);
+    double R[9];
+    if (a.rotate) arvo_rotation(&u[3], R);
+    const int n0 = a.ncell[0], n1 = a.ncell[1], n2 = a.ncell[2];
+    for (int s = 0; s < ns; ++s) {
+        double y0 = a.off[s][0], y1 = a.off[s][1], y2 = a.off[s][2];
+        if (a.rotate) {
+            const double t0 = R[0] * y0 + R[3] * y1 + R[6] * y2;
+            const double t1 = R[1] * y0 + R[4] * y1 + R[7] * y2;
+            const double t2 = R[2] * y0 + R[5] * y1 + R[8] * y2;
+            y0 = t0; y1 = t1; y2 = t2;
+        }
+        double x = u[0] + (a.c2f[0] * y0 + a.c2f[3] * y1 + a.c2f[6] * y2);
+        double y = u[1] + (a.c2f[1] * y0 + a.c2f[4] * y1 + a.c2f[7] * y2);
+        double z = u[2] + (a.c2f[2] * y0 + a.c2f[5] * y1 + a.c2f[8] * y2);
+        x -= floor(x);
+        y -= floor(y);
+        z -= floor(z);
+        const int ia = min((int)(x * n0), n0 - 1);
+        const int ib = min((int)(y * n1), n1 - 1);
+        const int ic = min((int)(z * n2), n2 - 1);
+        const unsigned cid = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
+        g.pos[p0 + s] = make_double4(x, y, z, 1.0);
+        g.cell[p0 + s] = cid;
+        atomicAdd(&g.count[cid], 1u);
+    }
+}
+
+__global__ void fill_u32_kernel(unsigned* p, long long n, unsigned v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- scan ----
+// Exclusive scans over cells of the point counts and of the slice counts
+// (ceil(count / WG)) in three launches: per-span totals, a scan of the span
+// totals, and the per-span scans.
+__device__ inline void scan_pair_1024(unsigned& p, unsigned& s, unsigned& tot_p, unsigned& tot_s) {
+    // exclusive scan of (p, s) over the 1024 work-items; also the totals
+    __shared__ unsigned w_p[SCAN_WG / 64], w_s[SCAN_WG / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned ip = p, is = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned tp = __shfl_up(ip, off, 64), ts = __shfl_up(is, off, 64);
+        if (lane >= off) { ip += tp; is += ts; }
+    }
+    __syncthreads();
+    if (lane == 63) { w_p[wave] = ip; w_s[wave] = is; }
+    __syncthreads();
+    unsigned bp = 0, bs = 0;
+    tot_p = 0;
+    tot_s = 0;
+    for (int w = 0; w < SCAN_WG / 64; ++w) {
+        if (w < wave) { bp += w_p[w]; bs += w_s[w]; }
+        tot_p += w_p[w];
+        tot_s += w_s[w];
+    }
+    p = bp + ip - p;
+    s = bs + is - s;
+}
+
+__global__ __launch_bounds__(SCAN_WG) void scan_reduce_kernel(const unsigned* count, int n_cells,
+                                                              unsigned* span_p, unsigned* span_s) {
+    const long long c0 = (long long)blockIdx.x * SCAN_SPAN + (long long)threadIdx.x * SCAN_PER;
+    unsigned p = 0, s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const unsigned v = c0 + k < n_cells ? count[c0 + k] : 0u;
+        p += v;
+        s += (v + WG - 1) / WG;
+    }
+    unsigned tp, ts;
+    scan_pair_1024(p, s, tp, ts);
+    if (threadIdx.x == 0) { span_p[blockIdx.x] = tp; span_s[blockIdx.x] = ts; }
+}
+
+// One workgroup: exclusive scan of the span totals in place; the grand totals
+// go to point_off[n_cells], slice_off[n_cells] and *n_slices.
+__global__ __launch_bounds__(SCAN_WG) void scan_spans_kernel(unsigned* span_p, unsigned* span_s, int n_spans,
+                                                             unsigned* point_off, unsigned* slice_off,
+                                                             int n_cells, unsigned* n_slices) {
+    unsigned run_p = 0, run_s = 0;
+    for (int base = 0; base < n_spans; base += SCAN_WG) {
+        const int i = base + threadIdx.x;
+        unsigned p = i < n_spans ? span_p[i] : 0u, s = i < n_spans ? span_s[i] : 0u;
+        unsigned tp, ts;
+        scan_pair_1024(p, s, tp, ts);
+        if (i < n_spans) { span_p[i] = run_p + p; span_s[i] = run_s + s; }
+        run_p += tp;
+        run_s += ts;
+    }
+    if (threadIdx.x == 0) {
+        point_off[n_cells] = run_p;
+        slice_off[n_cells] = run_s;
+        *n_slices = run_s;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_WG) void scan_down_kernel(const unsigned* count, int n_cells,
+                                                            const unsigned* span_p, const unsigned* span_s,
+                                                            unsigned* point_off, unsigned* slice_off) {
+    const long long c0 = (long long)blockIdx.x * SCAN_SPAN + (long long)threadIdx.x * SCAN_PER;
+    unsigned v[SCAN_PER];
+    unsigned p = 0, s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        v[k] = c0 + k < n_cells ? count[c0 + k] : 0u;
+        p += v[k];
+        s += (v[k] + WG - 1) / WG;
+    }
+    unsigned tp, ts;
+    scan_pair_1024(p, s, tp, ts);
+    p += span_p[blockIdx.x];
+    s += span_s[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        if (c0 + k < n_cells) {
+            point_off[c0 + k] = p;
+            slice_off[c0 + k] = s;
+        }
+        p += v[k];
+        s += (v[k] + WG - 1) / WG;
+    }
+}
+
+__global__ void scatter_points_kernel(const unsigned* cell, long long n_pts, const unsigned* point_off,
+                                      unsigned* cursor, unsigned* order) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pts) return;
+    unsigned c = cell[p];
+    if (c == 0xffffffffu) return;
+    unsigned slot = point_off[c] + atomicAdd(&cursor[c], 1u);
+    order[slot] = (unsigned)p;
+}
+
+__global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_off, const SegArgs* segs,
+                                   int n_seg, uint4* items, int n_cells) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    const int si = find_segment<SEG_BY_CELL>(segs, n_seg, c);
+    unsigned ns = (count[c] + WG - 1) / WG, base = slice_off[c];
+    for (unsigned k = 0; k < ns; ++k) items[base + k] = make_uint4((unsigned)c, k, (unsigned)si, 0u);
+}
+
+// ---------------------------------------------------------------- pairs ----
+// q erfc(alpha r) / r with erfc(alpha r) from the cubic table: segment
+// i = floor(r/h), clamped to the table (r beyond r_sr is masked by the
+// caller), t = r/h - i in [0, 1).
+__device__ inline float coul_table(double r2, double q, float inv_h, const float4* tab) {
+    const float rf2 = (float)r2;
+    const float rinv = __builtin_amdgcn_rsqf(rf2);
+    const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
+    const int i = (int)x;
+    const float t = x - (float)i;
+    const float4 c = tab[i];
+    const float e = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+    return (float)q * e * rinv;
+}
+
+// One candidate pair: LJ into plj, real-space Ewald into psr.
+template <int ESM, typename Real>
+__device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool is_lj, double r2lj,
+                                 double r2sr, double alpha, float inv_h, const float4* tab, bool& ovl,
+                                 Real& plj, Real& psr) {
+    const bool in = r2 < r2lj;
+    ovl |= in & (r2 < R2_OVERLAP) & is_lj;
+    const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
+    plj += in ? e : (Real)0;
+    if constexpr (ESM != 0) {
+        const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (q != 0.0);
+        Real cv;
+        if constexpr (sizeof(Real) == 4) cv = coul_table(r2, q, inv_h, tab);
+        else cv = PairMath<Real>::coul(r2, q, alpha);
+        psr += ins ? cv : (Real)0;
+    }
+}
+
+struct PairLds {
+    double4* c;     // candidate records (Cartesian relative to the chunk centre, charge)
+    int* t;         // candidate type * nsite
+    const float4* erfc;
+    int* wcnt;      // per-wave candidate counts
+};
+
+// The pair loops of one workgroup's chunk.  Fast path: scan the image list,
+// keep the images within r_cmax + chunk_r of the chunk centre by a
+// wave-ballot compaction into LDS, evaluate every lane's point against them.
+// Slow path: every replicated atom, nearest image per pair.
+template <int ESM, typename Real>
+__device__ inline void chunk_pairs(const RunArgs& a, bool valid, double px, double py, double pz, double cf0,
+                                   double cf1, double cf2, int site, const LJP<Real>* s_lj, PairLds L,
+                                   double& acc_lj, double& acc_sr, bool& ovl, unsigned long long& n_cand) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
+    const double r2lj = a.r2_lj, r2sr = a.r2_sr, alpha = a.alpha;
+    const float inv_h = (float)a.erfc_inv_h;
+    const bool is_lj = (a.site_lj_mask >> site) & 1u;
+    const int nsite = a.nsite;
+    // a wave with no valid point skips the pair loops; it still takes part in
+    // the tile scans and barriers
+    const bool wave_live = __ballot(valid) != 0ull;
+    if (a.fast) {
+        // The point and the chunk centre in Cartesian coordinates (FP64).  The
+        // images are stored in Cartesian coordinates, so a pair's r^2 depends
+        // on the point and the image alone, not on the chunk it was found from.
+        const double prx = f11 * px + f12 * py + f13 * pz;
+        const double pry = f22 * py + f23 * pz;
+        const double prz = f33 * pz;
+        const double ccx = f11 * cf0 + f12 * cf1 + f13 * cf2;
+        const double ccy = f22 * cf1 + f23 * cf2;
+        const double ccz = f33 * cf2;
+        const double Rc = a.r_cmax + a.chunk_r + 1e-3;
+        const double R2 = Rc * Rc;
+        const int n_img = a.n_img;
+        for (int base = 0; base < n_img; base += TILE) {
+            const int t = base + tid;
+            bool keep = false;
+            double4 rec = make_double4(0.0, 0.0, 0.0, 0.0);
+            int ty = 0;
+            if (t < n_img) {
+                rec = a.img[t];
+                const double cx = rec.x - ccx, cy = rec.y - ccy, cz = rec.z - ccz;
+                keep = cx * cx + cy * cy + cz * cz < R2;
+                ty = a.img_type[t] * nsite;
+            }
+            const unsigned long long m = __ballot(keep);
+            const int before = __popcll(m & ((1ull << lane) - 1ull));
+            __syncthreads();  // the previous tile's candidates are consumed
+            if (lane == 0) L.wcnt[wave] = __popcll(m);
+            __syncthreads();
+            int woff[WG / 64 + 1];
+            woff[0] = 0;
+#pragma unroll
+            for (int w = 0; w < WG / 64; ++w) woff[w + 1] = woff[w] + L.wcnt[w];
+            if (keep) {
+                L.c[woff[wave] + before] = rec;
+                L.t[woff[wave] + before] = ty;
+            }
+            __syncthreads();
+            n_cand += (unsigned long long)woff[WG / 64];
+            if (wave_live) {
+                // candidates of each 64-image group accumulate in Real, flushed to FP64
+#pragma unroll
+                for (int gi = 0; gi < WG / 64; ++gi) {
+                    Real plj = (Real)0, psr = (Real)0;
+                    for (int j = woff[gi]; j < woff[gi + 1]; ++j) {
+                        const double4 c = L.c[j];
+                        const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
+                        const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        pair_term<ESM, Real>(r2, c.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
+                                             L.erfc, ovl, plj, psr);
+                    }
+                    acc_lj += (double)plj;
+                    acc_sr += (double)psr;
+                }
+            }
+        }
+    } else {
+        const int n_atoms = a.n_atoms;
+        for (int base = 0; base < n_atoms; base += TILE) {
+            const int nt = min(TILE, n_atoms - base);
+            __syncthreads();
+            if (tid < nt) {
+                L.c[tid] = a.atoms[base + tid];
+                L.t[tid] = a.species[base + tid] * nsite;
+            }
+            __syncthreads();
+            n_cand += (unsigned long long)nt;
+            if (wave_live) {
+                for (int gi = 0; gi < nt; gi += 64) {
+                    Real plj = (Real)0, psr = (Real)0;
+                    const int ge = min(nt, gi + 64);
+                    for (int j = gi; j < ge; ++j) {
+                        const double4 at = L.c[j];
+                        double ex = at.x - px, ey = at.y - py, ez = at.z - pz;
+                        ex -= rint(ex);
+                        ey -= rint(ey);
+                        ez -= rint(ez);
+                        const double cx = f11 * ex + f12 * ey + f13 * ez;
+                        const double cy = f22 * ey + f23 * ez;
+                        const double cz = f33 * ez;
+                        const double r2 = cx * cx + cy * cy + cz * cz;
+                        pair_term<ESM, Real>(r2, at.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
+                                             L.erfc, ovl, plj, psr);
+                    }
+                    acc_lj += (double)plj;
+                    acc_sr += (double)psr;
+                }
+            }
+        }
+    }
+}
+
+// LDS: TILE candidate records and types, the erfc table (FP32 kernels), the
+// LJ table of the run.
+template <typename Real>
+constexpr int pair_lds_fixed() {
+    return TILE * (int)(sizeof(double4) + sizeof(int)) + (sizeof(Real) == 4 ? ERFC_SEG * (int)sizeof(float4) : 0);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_wcnt[WG / 64];
+    PairLds L;
+    L.c = reinterpret_cast<double4*>(smem);
+    L.t = reinterpret_cast<int*>(smem + TILE * sizeof(double4));
+    float4* s_erfc = reinterpret_cast<float4*>(smem + TILE * (sizeof(double4) + sizeof(int)));
+    L.erfc = s_erfc;
+    L.wcnt = s_wcnt;
+    LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + pair_lds_fixed<Real>());
+
+    const int tid = threadIdx.x;
+    int si;
+    uint4 it = make_uint4(0u, 0u, 0u, 0u);
+    if (g.mode == MODE_WIDOM) {
+        if (blockIdx.x >= *g.n_items) return;  // the whole workgroup leaves
+        it = g.items[blockIdx.x];
+        si = (int)it.z;
+    } else {
+        si = find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x);
+    }
+    const SegArgs sg = g.segs[si];
+    const RunArgs& a = g.runs[sg.run];
+
+    double cf0, cf1, cf2;  // chunk centre (fractional)
+    bool valid;
+    long long pid;
+    double px, py, pz;
+    int site;
+    if (g.mode == MODE_WIDOM) {
+        const int c = (int)(it.x - sg.cell_base);
+        const int n0 = a.ncell[0], n1 = a.ncell[1];
+        const int ia = c % n0, ib = (c / n0) % n1, ic = c / (n0 * n1);
+        cf0 = (ia + 0.5) / n0;
+        cf1 = (ib + 0.5) / n1;
+        cf2 = (ic + 0.5) / a.ncell[2];
+        const unsigned k = g.point_off[it.x] + it.y * WG + tid;
+        valid = k < g.point_off[it.x + 1];
+        pid = valid ? (long long)g.order[k] : sg.slot_base;
+        const double4 p = valid ? g.pos[pid] : make_double4(cf0, cf1, cf2, 0.0);
+        px = p.x; py = p.y; pz = p.z;
+        site = (int)((pid - sg.slot_base) % a.nsite);
+    } else {
+        const int lb = (int)blockIdx.x - sg.wg_base;
+        const int nb0 = a.nbrick[0], nb1 = a.nbrick[1];
+        const int bi = lb % nb0, bj = (lb / nb0) % nb1, bk = lb / (nb0 * nb1);
+        const int i0 = bi * a.brick[0], j0 = bj * a.brick[1], k0 = bk * a.brick[2];
+        const int ti = tid % a.brick[0], tj = (tid / a.brick[0]) % a.brick[1], tk = tid / (a.brick[0] * a.brick[1]);
+        const int i = i0 + ti, j = j0 + tj, k = k0 + tk;
+        const int n1 = a.grid_n[0], n2 = a.grid_n[1], n3 = a.grid_n[2];
+        valid = i < n1 && j < n2 && k < n3;
+        pid = sg.slot_base + (valid ? (long long)i + (long long)n1 * ((long long)j + (long long)n2 * k) : 0);
+        px = grid_coord(i, n1, a.repd[0]);
+        py = grid_coord(j, n2, a.repd[1]);
+        pz = grid_coord(k, n3, a.repd[2]);
+        cf0 = grid_coord(i0 + 0.5 * (a.brick[0] - 1), n1, a.repd[0]);
+        cf1 = grid_coord(j0 + 0.5 * (a.brick[1] - 1), n2, a.repd[1]);
+        cf2 = grid_coord(k0 + 0.5 * (a.brick[2] - 1), n3, a.repd[2]);
+        site = 0;
+    }
+    const LJP<Real>* gtab = reinterpret_cast<const LJP<Real>*>(a.ljtab);
+    for (int t = tid; t < a.n_types * a.nsite; t += WG) s_lj[t] = gtab[t];
+    if constexpr (sizeof(Real) == 4)
+        if (a.esm != 0)
+            for (int t = tid; t < ERFC_SEG; t += WG) s_erfc[t] = a.erfc_tab[t];
+
+    double acc_lj = 0.0, acc_sr = 0.0;
+    bool ovl = false;
+    unsigned long long n_cand = 0;
+    if (a.esm == 0)
+        chunk_pairs<0, Real>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, L, acc_lj, acc_sr, ovl, n_cand);
+    else if (a.esm == 1)
+        chunk_pairs<1, Real>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, L, acc_lj, acc_sr, ovl, n_cand);
+    else
+        chunk_pairs<2, Real>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, L, acc_lj, acc_sr, ovl, n_cand);
+    if (valid) {
+        g.out_lj[pid] = ovl ? INFINITY : acc_lj;
+        g.out_sr[pid] = acc_sr;
+    }
+    if (g.cand_pairs != nullptr) {
+        const unsigned long long m = __ballot(valid);
+        if ((tid & 63) == 0 && m != 0ull)
+            atomicAdd(g.cand_pairs, (unsigned long long)__popcll(m) * n_cand);
+    }
+}
+
+// -------------------------------------------------------------- Fourier ----
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Fourier part per point: sum_m Re(A_m e^{-i 2 pi m.u}), u = R o x (unit-cell
+// fractional), by complex recurrences along each axis of the box (FP64).
+// Point slots of a segment start at a multiple of WG, so a workgroup lies in
+// one segment.
+__global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
+    __shared__ double2 s_a[FT];
+    const int tid = threadIdx.x;
+    const long long p0 = (long long)blockIdx.x * WG;
+    const SegArgs sg = g.segs[find_segment<SEG_BY_SLOT>(g.segs, g.n_seg, p0)];
+    const RunArgs& a = g.runs[sg.run];
+    if (a.esm == 0) return;  // uniform: the whole workgroup leaves
+    const long long pid = p0 + tid, lp = pid - sg.slot_base;
+    bool valid = lp < sg.n_slots;
+    double x = 0.0, y = 0.0, z = 0.0;
+    int site = 0;
+    if (g.mode == MODE_WIDOM) {
+        const double4 p = valid ? g.pos[pid] : make_double4(0.0, 0.0, 0.0, 0.0);
+        valid = valid && p.w != 0.0;
+        x = p.x; y = p.y; z = p.z;
+        site = (int)(lp % a.nsite);
+    } else if (valid) {
+        const int n1 = a.grid_n[0], n2 = a.grid_n[1], n3 = a.grid_n[2];
+        const long long i = lp % n1, j = (lp / n1) % n2, k = lp / ((long long)n1 * n2);
+        x = grid_coord((double)i, n1, a.repd[0]);
+        y = grid_coord((double)j, n2, a.repd[1]);
+        z = grid_coord((double)k, n3, a.repd[2]);
+    }
+    const bool charged = valid && ((a.site_q_mask >> site) & 1u);
+    double t0 = x * a.repd[0], t1 = y * a.repd[1], t2 = z * a.repd[2];
+    t0 -= rint(t0);
+    t1 -= rint(t1);
+    t2 -= rint(t2);
+    double s, c;
+    sincospi(2.0 * t0, &s, &c);
+    const double2 Ea = make_double2(c, -s);
+    sincospi(2.0 * t1, &s, &c);
+    const double2 Eb = make_double2(c, -s);
+    sincospi(2.0 * t2, &s, &c);
+    const double2 Ec = make_double2(c, -s);
+    const int Ma = a.Ma, Mb = a.Mb, Mc = a.Mc;
+    double2 Eb_neg = make_double2(1.0, 0.0), Ec_neg = make_double2(1.0, 0.0);
+    const double2 Eb_c = make_double2(Eb.x, -Eb.y), Ec_c = make_double2(Ec.x, -Ec.y);
+    for (int q = 0; q < Mb; ++q) Eb_neg = cmul(Eb_neg, Eb_c);
+    for (int q = 0; q < Mc; ++q) Ec_neg = cmul(Ec_neg, Ec_c);
+
+    const int rowlen = 2 * Mc + 1, nb = 2 * Mb + 1;
+    const int n_rows = (Ma + 1) * nb;
+    const int rows_per_tile = max(1, FT / rowlen);
+    double2 Pa = make_double2(1.0, 0.0);
+    double2 Pb = Eb_neg;
+    double acc = 0.0;
+    for (int r0 = 0; r0 < n_rows; r0 += rows_per_tile) {
+        const int r1 = min(n_rows, r0 + rows_per_tile);
+        const int n_ent = (r1 - r0) * rowlen;
+        __syncthreads();
+        for (int e = tid; e < n_ent; e += WG) s_a[e] = a.A[(long long)r0 * rowlen + e];
+        __syncthreads();
+        for (int r = r0; r < r1; ++r) {
+            if (r > 0 && (r % nb) == 0) {  // next m_a
+                Pa = cmul(Pa, Ea);
+                Pb = cmul(Pa, Eb_neg);
+            }
+            double2 P = cmul(Pb, Ec_neg);
+            const double2* row = s_a + (r - r0) * rowlen;
+            for (int mc = 0; mc < rowlen; ++mc) {
+                const double2 A = row[mc];
+                acc += A.x * P.x - A.y * P.y;
+                P = cmul(P, Ec);
+            }
+            Pb = cmul(Pb, Eb);
+        }
+    }
+    if (valid) g.out_lr[pid] = charged ? acc : 0.0;
+}
+
+// -------------------------------------------------------------- combine ----
+// Per insertion: E_vdw = sum_s lj_s, E_es = sum_s q_s (sr_s / (4 pi eps0) + lr_s),
+// Boltzmann weight and the workgroup's fixed-order partial sums.
+__global__ __launch_bounds__(WG) void widom_combine_kernel(GroupArgs g) {
+    const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
+    const RunArgs& a = g.runs[sg.run];
+    const int lw = (int)blockIdx.x - sg.wg_base, w = sg.w0 + lw;
+    const int b = w / a.chunks_per_block, c = w - b * a.chunks_per_block;
+    const long long i = (long long)c * WG + threadIdx.x;
+    const bool valid = i < a.n_per_block;
+    const int ns = a.nsite;
+    const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
+    double vdw = 0.0, es = 0.0;
+    if (valid) {
+        for (int s = 0; s < ns; ++s) {
+            vdw += g.out_lj[p0 + s];
+            if (a.esm != 0) es += a.site_q[s] * (g.out_sr[p0 + s] * a.inv4pe + g.out_lr[p0 + s]);
+        }
+    }
+    const double wgt = exp(-(vdw + es) / a.temperature);
+    const bool fin = !isinf(vdw) && !isinf(es);
+    if (a.out_e != nullptr && valid) {
+        const long long gi = (long long)b * a.n_per_block + i;
+        a.out_e[2 * gi] = vdw;
+        a.out_e[2 * gi + 1] = es;
+    }
+    block_sum3(valid ? wgt : 0.0, (valid && fin) ? wgt * vdw : 0.0, (valid && fin) ? wgt * es : 0.0,
+               a.partials + 3LL * w);
+}
+
+__global__ __launch_bounds__(WG) void grid_combine_kernel(GroupArgs g) {
+    const long long p0 = (long long)blockIdx.x * WG;
+    const SegArgs sg = g.segs[find_segment<SEG_BY_SLOT>(g.segs, g.n_seg, p0)];
+    const RunArgs& a = g.runs[sg.run];
+    const long long p = p0 + threadIdx.x, lp = p - sg.slot_base;
+    if (lp >= sg.n_slots) return;
+    double e = g.out_lj[p];
+    if (a.esm != 0) e += a.grid_q * (g.out_sr[p] * a.inv4pe + g.out_lr[p]);
+    if (a.grid_f != nullptr) a.grid_f[lp] = (float)e;
+    else a.grid_d[lp] = e;
+}
+
+// Fixed-order per-block reduction of the workgroup partials: one workgroup
+// per Widom block of every run, strided partial sums then a shuffle tree.
+__global__ __launch_bounds__(WG) void block_sums_kernel(const RunArgs* runs, int n_runs, double* out) {
+    int lo = 0, hi = n_runs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (runs[mid].block_base <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const RunArgs& a = runs[lo];
+    const int b = (int)blockIdx.x - a.block_base, chunks = a.chunks_per_block;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int c = threadIdx.x; c < chunks; c += WG) {
+        const double* p = a.partials + 3LL * ((long long)b * chunks + c);
+        t0 += p[0]; t1 += p[1]; t2 += p[2];
+    }
+    block_sum3(t0, t1, t2, out + 3LL * blockIdx.x);
+}
+
+}  // namespace pmk

@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "pm_device.h"
+#include "pm_points.h"
 #include "pmhip.h"
 
 #define PM_VERSION "pmhip 0.2.0 (gfx950)"
@@ -25,8 +25,11 @@ namespace {
 // Point-engine tunables.
 constexpr double CHUNK_R_MAX = 3.0;       // A: largest chunk radius served by the image list
 constexpr double CELL_POINTS = 640.0;     // target points per Widom cell
-constexpr long long MAX_POINT_SLOTS = 1LL << 24;  // point slots per Widom batch
-constexpr long long MAX_IMAGES = 1LL << 26;
+constexpr long long MAX_POINT_SLOTS = 1LL << 24;  // point slots per launch group
+constexpr long long MAX_IMAGES = 1LL << 26;       // images per list
+constexpr size_t MAX_IMAGE_LISTS = 8;             // image lists (radii) per framework
+constexpr double MAX_RUN_CELLS = 1 << 22;         // cells of one Widom run
+constexpr long long MAX_GROUP_CELLS = 1LL << 24;  // cells of one launch group
 
 thread_local std::string g_err;
 std::mutex g_mu;
@@ -153,11 +156,15 @@ struct pm_framework {
     double4* d_atoms = nullptr;
     int32_t* d_species = nullptr;
     double4* d_unit_q = nullptr;
-    // image list covering every image within r_img of any point of the box
-    double r_img = -1.0;
-    int64_t n_img = 0;
-    double4* d_img = nullptr;
-    int32_t* d_img_type = nullptr;
+    // Image lists, one per cutoff radius r: every periodic image within r of
+    // some point of the replicated box, Cartesian xyz + charge, and its type.
+    struct Images {
+        double r = 0.0;
+        int64_t n = 0;
+        double4* d_img = nullptr;
+        int32_t* d_type = nullptr;
+    };
+    std::vector<Images> images;
 };
 
 struct pm_forcefield {
@@ -332,12 +339,19 @@ int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* e
 }
 
 // ---------------------------------------------------- point engine ----
-// Image list of the framework: every periodic image (x_unit + t) / rep whose
+// Image list for radius r: every periodic image (x_unit + t) / rep whose
 // replicated-box fractional coordinates lie within [-m_a, 1 + m_a] on each
 // axis, m_a = r / width_a, so that every image within r of any point of the
-// box [0,1)^3 is present.  Built once per framework and radius.
-int ensure_images(pm_framework* fw, double r) {
-    if (fw->r_img >= r && fw->d_img != nullptr) return PM_OK;
+// box [0,1)^3 is present; stored in Cartesian coordinates.  Lists are cached
+// per framework and radius, so a run's list -- and with it the order of its
+// FP32 partial sums -- depends on the framework and the radius alone.
+int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
+    for (const auto& im : fw->images)
+        if (im.r == r) {
+            *out = &im;
+            return PM_OK;
+        }
+    if (fw->images.size() >= MAX_IMAGE_LISTS) return fail(PM_ERR_ARG, "too many image-list radii");
     double m[3];
     for (int i = 0; i < 3; ++i) m[i] = r / fw->width[i];
     int64_t est = fw->n_unit;
@@ -347,6 +361,7 @@ int ensure_images(pm_framework* fw, double r) {
     std::vector<int32_t> ty;
     img.reserve((size_t)est);
     ty.reserve((size_t)est);
+    const double* F = fw->F;
     for (int64_t u = 0; u < fw->n_unit; ++u) {
         const double* x = &fw->unit_xf[3 * u];
         int lo[3], hi[3];
@@ -357,31 +372,41 @@ int ensure_images(pm_framework* fw, double r) {
         for (int ta = lo[0]; ta <= hi[0]; ++ta)
             for (int tb = lo[1]; tb <= hi[1]; ++tb)
                 for (int tc = lo[2]; tc <= hi[2]; ++tc) {
-                    img.push_back(make_double4((x[0] + ta) / fw->rep[0], (x[1] + tb) / fw->rep[1],
-                                               (x[2] + tc) / fw->rep[2], fw->unit_q[u]));
+                    const double fa = (x[0] + ta) / fw->rep[0], fb = (x[1] + tb) / fw->rep[1],
+                                 fc = (x[2] + tc) / fw->rep[2];
+                    img.push_back(make_double4(F[0] * fa + F[3] * fb + F[6] * fc, F[4] * fb + F[7] * fc,
+                                               F[8] * fc, fw->unit_q[u]));
                     ty.push_back(fw->unit_type[u]);
                 }
     }
-    PM_HIP(hipStreamSynchronize(g_ctx.stream));
-    if (fw->d_img) { PM_HIP(hipFree(fw->d_img)); fw->d_img = nullptr; }
-    if (fw->d_img_type) { PM_HIP(hipFree(fw->d_img_type)); fw->d_img_type = nullptr; }
-    fw->r_img = -1.0;
-    fw->n_img = (int64_t)img.size();
-    size_t n = std::max<size_t>(img.size(), 1);
-    PM_HIP(hipMalloc(&fw->d_img, n * sizeof(double4)));
-    PM_HIP(hipMalloc(&fw->d_img_type, n * sizeof(int32_t)));
-    if (!img.empty()) {
-        PM_HIP(hipMemcpy(fw->d_img, img.data(), img.size() * sizeof(double4), hipMemcpyHostToDevice));
-        PM_HIP(hipMemcpy(fw->d_img_type, ty.data(), ty.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    pm_framework::Images im;
+    im.r = r;
+    im.n = (int64_t)img.size();
+    if (im.n > (int64_t)INT32_MAX) return fail(PM_ERR_ARG, "image list too large");
+    const size_t n = std::max<size_t>(img.size(), 1);
+    PM_HIP(hipMalloc(&im.d_img, n * sizeof(double4)));
+    hipError_t e = hipMalloc(&im.d_type, n * sizeof(int32_t));
+    if (e == hipSuccess && !img.empty()) {
+        e = hipMemcpy(im.d_img, img.data(), img.size() * sizeof(double4), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(im.d_type, ty.data(), ty.size() * sizeof(int32_t), hipMemcpyHostToDevice);
     }
-    fw->r_img = r;
+    if (e != hipSuccess) {
+        (void)hipFree(im.d_img);
+        (void)hipFree(im.d_type);
+        return fail(PM_ERR_DEVICE, std::string("image list upload: ") + hipGetErrorString(e));
+    }
+    fw->images.push_back(im);
+    *out = &fw->images.back();
     return PM_OK;
 }
 
-void fill_pargs_common(PArgs& a, const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
-                       bool es, int nsite, const Site* site) {
+// Pair and Fourier parameters of a run.
+void fill_run_pair(RunArgs& a, const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, bool es,
+                   int nsite, const Site* site) {
     a.f11 = fw->F[0]; a.f12 = fw->F[3]; a.f13 = fw->F[6];
     a.f22 = fw->F[4]; a.f23 = fw->F[7]; a.f33 = fw->F[8];
+    for (int i = 0; i < 9; ++i) a.c2f[i] = fw->c2f[i];
     for (int i = 0; i < 3; ++i) a.repd[i] = (double)fw->rep[i];
     a.r2_lj = ff ? ff->r2_cutoff : 0.0;
     a.r2_sr = es ? ew->sr_cutoff * ew->sr_cutoff : 0.0;
@@ -389,70 +414,44 @@ void fill_pargs_common(PArgs& a, const pm_framework* fw, const pm_forcefield* ff
     a.r_cmax = std::sqrt(std::max(a.r2_lj, a.r2_sr));
     a.erfc_tab = es ? ew->d_erfc : nullptr;
     a.erfc_inv_h = es ? ERFC_SEG / ew->sr_cutoff : 1.0;
+    a.inv4pe = INV_4PI_EPS0;
+    a.esm = !es ? 0 : (a.r2_lj == a.r2_sr ? 1 : 2);
     a.nsite = nsite;
     a.n_types = (int)fw->species_global.size();
     a.n_atoms = (int)fw->n_rep;
     a.atoms = fw->d_atoms;
     a.species = fw->d_species;
     a.site_lj_mask = 0;
-    for (int s = 0; s < nsite; ++s)
+    a.site_q_mask = 0;
+    for (int s = 0; s < nsite; ++s) {
         if (site[s].lj) a.site_lj_mask |= 1u << s;
-}
-
-// Uses the image list (fast path) when it can cover r_cmax + chunk_r.
-int choose_path(pm_framework* fw, PArgs& a) {
-    a.fast = 0;
-    if (a.chunk_r <= CHUNK_R_MAX) {
-        int rc = ensure_images(fw, a.r_cmax + CHUNK_R_MAX + 0.01);
-        if (rc == PM_OK) a.fast = 1;
-        else if (rc != PM_ERR_ARG) return rc;
+        if (site[s].q != 0.0) a.site_q_mask |= 1u << s;
     }
-    a.n_img = (int)fw->n_img;
-    a.img = fw->d_img;
-    a.img_type = fw->d_img_type;
-    return PM_OK;
+    if (es) {
+        a.Ma = ew->Ma;
+        a.Mb = ew->Mb;
+        a.Mc = ew->Mc;
+        a.A = ew->d_box;
+    }
 }
 
-size_t point_lds_bytes(int n_types, int ns, bool fp64, bool es) {
-    return (size_t)TILE * (sizeof(double4) + sizeof(int)) + ((es && !fp64) ? ERFC_SEG * sizeof(float4) : 0) +
-           (size_t)std::max(n_types, 1) * ns * (fp64 ? 16 : 8);
-}
-
-template <int ESM, typename Real>
-void point_pair_launch_t(const PArgs& a, int n_wg) {
-    const size_t shm = point_lds_bytes(a.n_types, a.nsite, sizeof(Real) == 8, ESM != 0);
-    hipLaunchKernelGGL((point_pair_kernel<ESM, Real>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
-}
-
-template <typename Real>
-void point_pair_launch_r(const PArgs& a, int n_wg, bool es) {
-    if (!es) point_pair_launch_t<0, Real>(a, n_wg);
-    else if (a.r2_lj == a.r2_sr) point_pair_launch_t<1, Real>(a, n_wg);
-    else point_pair_launch_t<2, Real>(a, n_wg);
-}
-
-void point_pair_launch(const PArgs& a, int n_wg, bool es, int precision) {
-    if (precision == PM_FP64) point_pair_launch_r<double>(a, n_wg, es);
-    else point_pair_launch_r<float>(a, n_wg, es);
-}
-
-void fill_fargs(FArgs& f, const pm_framework* fw, const pm_ewald* ew) {
-    for (int i = 0; i < 3; ++i) f.repd[i] = (double)fw->rep[i];
-    f.Ma = ew->Ma;
-    f.Mb = ew->Mb;
-    f.Mc = ew->Mc;
-    f.A = ew->d_box;
-}
-
-// Cells of the Widom counting sort: about CELL_POINTS points each, half
-// diagonal <= CHUNK_R_MAX where the point density allows.
+// Cells of a Widom run's counting sort: about CELL_POINTS points each, half
+// diagonal <= CHUNK_R_MAX where the point density allows, at most
+// MAX_RUN_CELLS cells.
 void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double* cell_r) {
     double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
     double h = std::cbrt(vol * CELL_POINTS / std::max(n_points, 1.0));
     double scale = 1.0;
+    int prev[3] = {1, 1, 1};
+    double prev_r = 0.0;
     for (int iter = 0; iter < 64; ++iter) {
         for (int i = 0; i < 3; ++i)
             ncell[i] = (int)std::min(1024.0, std::max(1.0, std::round(fw->width[i] / h * scale)));
+        if (iter > 0 && (double)ncell[0] * ncell[1] * ncell[2] > (double)MAX_RUN_CELLS) {
+            for (int i = 0; i < 3; ++i) ncell[i] = prev[i];
+            *cell_r = prev_r;
+            return;
+        }
         double ea[3], eb[3], ec[3];
         for (int i = 0; i < 3; ++i) {
             ea[i] = fw->F[i] / ncell[0];
@@ -460,9 +459,425 @@ void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double*
             ec[i] = fw->F[6 + i] / ncell[2];
         }
         *cell_r = half_diagonal(ea, eb, ec);
-        if (*cell_r <= CHUNK_R_MAX || (ncell[0] == 1024 && ncell[1] == 1024 && ncell[2] == 1024)) break;
+        if (*cell_r <= CHUNK_R_MAX) return;
+        for (int i = 0; i < 3; ++i) prev[i] = ncell[i];
+        prev_r = *cell_r;
         scale *= 1.25;
     }
+}
+
+// One validated run of a point-engine call.
+struct PointRun {
+    pm_framework* fw = nullptr;
+    const pm_forcefield* ff = nullptr;
+    const pm_ewald* ew = nullptr;
+    const pm_probe* pr = nullptr;
+    bool es = false;
+    int ns = 0;
+    int64_t n_wg = 0;      // Widom: insertion workgroups; grid: bricks
+    int64_t n_points = 0;  // grid points (grid runs)
+    int64_t n_cells = 0;   // Widom cells
+    double cost = 0.0;     // scheduling estimate per point
+    double* host_out = nullptr;
+    size_t tab_off = 0;
+    std::vector<unsigned char> tab;
+    RunArgs ra{};
+};
+
+int widom_prepare(const pm_widom_run& in, PointRun& r) {
+    pm_framework* fw = const_cast<pm_framework*>(in.framework);  // the image-list cache only
+    const pm_forcefield* ff = in.forcefield;
+    const pm_ewald* ew = in.ewald;
+    const pm_probe* pr = in.probe;
+    if (!fw || !ff || !pr || in.n_blocks < 1 || in.n_per_block < 1)
+        return fail(PM_ERR_ARG, "pm_widom: null handle or empty block layout");
+    if (!(in.temperature > 0.0) || !std::isfinite(in.temperature))
+        return fail(PM_ERR_ARG, "pm_widom: temperature must be positive and finite");
+    if (int rc = check_species(fw, ff)) return rc;
+    for (int s = 0; s < pr->ns; ++s)
+        if (pr->site[s].lj && pr->lj_species[s] >= ff->n)
+            return fail(PM_ERR_ARG, "pm_widom: probe species outside the force field");
+    const bool es = fw->charged && pr->charged;  // src/henry.jl:93
+    if (es && !ew) return fail(PM_ERR_ARG, "pm_widom: charged framework and probe need an Ewald handle");
+    const int ns = pr->ns;
+    const int64_t chunks = (in.n_per_block + WG - 1) / WG;
+    const int64_t n_wg = chunks * in.n_blocks;
+    if (n_wg > (int64_t)INT32_MAX / 2 || (int64_t)in.n_blocks * in.n_per_block > ((int64_t)1 << 53))
+        return fail(PM_ERR_ARG, "pm_widom: too many insertions for one run");
+    r.fw = fw;
+    r.ff = ff;
+    r.ew = es ? ew : nullptr;
+    r.pr = pr;
+    r.es = es;
+    r.ns = ns;
+    r.n_wg = n_wg;
+    RunArgs& a = r.ra;
+    fill_run_pair(a, fw, ff, es ? ew : nullptr, es, ns, pr->site);
+    a.seed = in.seed;
+    a.n_per_block = in.n_per_block;
+    a.rotate = pr->rotate;
+    a.n_blocks = in.n_blocks;
+    a.chunks_per_block = (int)chunks;
+    a.temperature = in.temperature;
+    for (int s = 0; s < ns; ++s) {
+        a.off[s][0] = pr->site[s].ox;
+        a.off[s][1] = pr->site[s].oy;
+        a.off[s][2] = pr->site[s].oz;
+        a.site_q[s] = pr->site[s].q;
+    }
+    const double pts = std::min((double)in.n_blocks * (double)in.n_per_block * ns, (double)MAX_POINT_SLOTS);
+    choose_cells(fw, pts, a.ncell, &a.chunk_r);
+    r.n_cells = (int64_t)a.ncell[0] * a.ncell[1] * a.ncell[2];
+    const double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
+    r.cost = (double)fw->n_rep / vol * std::pow(a.r_cmax + a.chunk_r, 3);
+    return PM_OK;
+}
+
+int grid_prepare(const pm_grid_run& in, PointRun& r) {
+    pm_framework* fw = const_cast<pm_framework*>(in.framework);  // the image-list cache only
+    const pm_forcefield* ff = in.forcefield;
+    const pm_ewald* ew = in.ewald;
+    const pm_probe* pr = in.probe;
+    if (!fw || !pr || !in.out) return fail(PM_ERR_ARG, "pm_energy_grid: null handle or pointer");
+    if (pr->ns != 1) return fail(PM_ERR_ARG, "pm_energy_grid: probe must have exactly one site");
+    if (pr->site[0].ox != 0.0 || pr->site[0].oy != 0.0 || pr->site[0].oz != 0.0)
+        return fail(PM_ERR_ARG, "pm_energy_grid: single-site probe must sit at its centre of mass");
+    for (int i = 0; i < 3; ++i)
+        if (in.n_pts[i] < 1 || in.n_pts[i] > 65536) return fail(PM_ERR_ARG, "pm_energy_grid: n_pts out of range");
+    if (pr->site[0].lj) {
+        if (!ff) return fail(PM_ERR_ARG, "pm_energy_grid: LJ probe needs a force field");
+        if (int rc = check_species(fw, ff)) return rc;
+        if (pr->lj_species[0] >= ff->n)
+            return fail(PM_ERR_ARG, "pm_energy_grid: probe species outside the force field");
+    }
+    const bool es = fw->charged && pr->charged;
+    if (es && !ew) return fail(PM_ERR_ARG, "pm_energy_grid: charged probe needs an Ewald handle");
+    const int brick[3] = {8, 8, 4};
+    int nbrick[3];
+    for (int i = 0; i < 3; ++i) nbrick[i] = (in.n_pts[i] + brick[i] - 1) / brick[i];
+    const long long n_bricks = (long long)nbrick[0] * nbrick[1] * nbrick[2];
+    const long long n_pts = (long long)in.n_pts[0] * in.n_pts[1] * in.n_pts[2];
+    if (n_bricks > INT32_MAX / 2 || n_pts > (1LL << 40))
+        return fail(PM_ERR_ARG, "pm_energy_grid: grid too large");
+    r.fw = fw;
+    r.ff = ff;
+    r.ew = es ? ew : nullptr;
+    r.pr = pr;
+    r.es = es;
+    r.ns = 1;
+    r.n_wg = n_bricks;
+    r.n_points = n_pts;
+    r.host_out = in.out;
+    RunArgs& a = r.ra;
+    fill_run_pair(a, fw, pr->site[0].lj ? ff : nullptr, es ? ew : nullptr, es, 1, pr->site);
+    for (int i = 0; i < 3; ++i) {
+        a.grid_n[i] = in.n_pts[i];
+        a.brick[i] = brick[i];
+        a.nbrick[i] = nbrick[i];
+    }
+    a.grid_q = pr->site[0].q;
+    // Cartesian span of one brick and its half diagonal
+    double ed[3][3];
+    for (int d = 0; d < 3; ++d) {
+        const double step =
+            in.n_pts[d] > 1 ? (double)(brick[d] - 1) / (double)(in.n_pts[d] - 1) / fw->rep[d] : 0.0;
+        for (int i = 0; i < 3; ++i) ed[d][i] = fw->F[3 * d + i] * step;
+    }
+    a.chunk_r = half_diagonal(ed[0], ed[1], ed[2]);
+    const double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
+    r.cost = (double)fw->n_rep / vol * std::pow(a.r_cmax + a.chunk_r, 3);
+    return PM_OK;
+}
+
+template <typename Real>
+void pair_launch_t(const GroupArgs& g, int n_wg, size_t shm) {
+    hipLaunchKernelGGL(point_pair_kernel<Real>, dim3(n_wg), dim3(WG), shm, g_ctx.stream, g);
+}
+
+unsigned grid1d(long long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// Runs every run of a call through the point engine.  Widom: out_sums gets
+// n_blocks x (S, W_vdw, W_es) per run in input order.  Grids: each run's
+// host_out gets its grid (i fastest).
+int execute_points(std::vector<PointRun>& runs, int mode, int precision, double* out_sums) {
+    const bool fp64 = precision == PM_FP64;
+    const int n_runs = (int)runs.size();
+    // image lists and paths
+    for (auto& r : runs) {
+        RunArgs& a = r.ra;
+        a.fast = 0;
+        a.n_img = 0;
+        a.img = nullptr;
+        a.img_type = nullptr;
+        if (a.chunk_r <= CHUNK_R_MAX) {
+            const pm_framework::Images* im = nullptr;
+            const int rc = get_images(r.fw, a.r_cmax + 0.01, &im);
+            if (rc == PM_OK) {
+                a.fast = 1;
+                a.img = im->d_img;
+                a.img_type = im->d_type;
+                a.n_img = (int)im->n;
+            } else if (rc != PM_ERR_ARG) {
+                return rc;
+            }
+        }
+    }
+    // scheduling order: the most expensive points first (results do not depend on it)
+    std::vector<int> order(n_runs);
+    for (int i = 0; i < n_runs; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return runs[x].cost > runs[y].cost; });
+
+    // segments and launch groups
+    struct Group {
+        size_t seg0 = 0;
+        int n_seg = 0;
+        long long n_wg = 0, n_slots = 0, n_cells = 0;
+        bool es = false;
+    };
+    std::vector<SegArgs> segs;
+    std::vector<Group> groups;
+    Group cur;
+    auto close_group = [&]() {
+        if (cur.n_seg > 0) groups.push_back(cur);
+        cur = Group();
+        cur.seg0 = segs.size();
+    };
+    for (int oi : order) {
+        PointRun& r = runs[oi];
+        if (mode == MODE_WIDOM) {
+            const long long per_wg = (long long)WG * r.ns;
+            int64_t w = 0;
+            while (w < r.n_wg) {
+                const long long room = (MAX_POINT_SLOTS - cur.n_slots) / per_wg;
+                if (room <= 0 || (cur.n_seg > 0 && cur.n_cells + r.n_cells > MAX_GROUP_CELLS)) {
+                    close_group();
+                    continue;
+                }
+                const int64_t take = std::min<int64_t>(room, r.n_wg - w);
+                SegArgs s{};
+                s.run = oi;
+                s.w0 = (int)w;
+                s.wg_base = (int)cur.n_wg;
+                s.cell_base = (unsigned)cur.n_cells;
+                s.slot_base = cur.n_slots;
+                s.n_slots = take * per_wg;
+                segs.push_back(s);
+                cur.n_seg++;
+                cur.n_wg += take;
+                cur.n_slots += take * per_wg;
+                cur.n_cells += r.n_cells;
+                cur.es = cur.es || r.es;
+                w += take;
+            }
+        } else {
+            const long long padded = (r.n_points + WG - 1) / WG * WG;
+            if (cur.n_seg > 0 && (cur.n_slots + padded > MAX_POINT_SLOTS || cur.n_wg + r.n_wg > INT32_MAX / 2))
+                close_group();
+            SegArgs s{};
+            s.run = oi;
+            s.wg_base = (int)cur.n_wg;
+            s.slot_base = cur.n_slots;
+            s.n_slots = r.n_points;
+            segs.push_back(s);
+            cur.n_seg++;
+            cur.n_wg += r.n_wg;
+            cur.n_slots += padded;
+            cur.es = cur.es || r.es;
+        }
+    }
+    close_group();
+    long long max_slots = 0, max_cells = 0, max_items = 0;
+    for (const Group& gr : groups) {
+        max_slots = std::max(max_slots, gr.n_slots);
+        max_cells = std::max(max_cells, gr.n_cells);
+        max_items = std::max(max_items, gr.n_cells + (gr.n_slots + WG - 1) / WG + 1);
+    }
+
+    // LJ tables (host), per-run output offsets
+    PM_HIP(hipStreamSynchronize(g_ctx.stream));  // the staging buffer may still be read by a copy
+    size_t tab_total = 0, lj_lds = 0;
+    long long part_total = 0, blocks_total = 0, grid_total = 0;
+    std::vector<long long> part_off(n_runs), grid_off(n_runs);
+    for (int i = 0; i < n_runs; ++i) {
+        PointRun& r = runs[i];
+        if (fp64) build_ljtab<double>(r.fw, r.ff, r.ns, r.pr->site, r.pr->lj_species, r.tab);
+        else build_ljtab<float>(r.fw, r.ff, r.ns, r.pr->site, r.pr->lj_species, r.tab);
+        r.tab_off = tab_total;
+        tab_total += align256(r.tab.size());
+        lj_lds = std::max(lj_lds, r.tab.size());
+        if (mode == MODE_WIDOM) {
+            part_off[i] = part_total;
+            part_total += r.n_wg * 3;
+            r.ra.block_base = (int)blocks_total;
+            blocks_total += r.ra.n_blocks;
+        } else {
+            grid_off[i] = grid_total;
+            grid_total += (r.n_points * (fp64 ? 8 : 4) + 255) / 256 * 256;
+        }
+    }
+    if (blocks_total > INT32_MAX / 2) return fail(PM_ERR_ARG, "too many Widom blocks in one call");
+    Arena ar;
+    const size_t o_runs = ar.take((size_t)n_runs * sizeof(RunArgs));
+    const size_t o_segs = ar.take(segs.size() * sizeof(SegArgs));
+    const size_t o_tabs = ar.take(tab_total);
+    const size_t stage_bytes = ar.used;
+    const bool widom = mode == MODE_WIDOM;
+    const size_t o_part = ar.take(widom ? (size_t)part_total * sizeof(double) : 0);
+    const size_t o_sums = ar.take(widom ? (size_t)blocks_total * 3 * sizeof(double) : 0);
+    const size_t o_grid = ar.take(widom ? 0 : (size_t)grid_total);
+    const size_t o_pos = ar.take(widom ? (size_t)max_slots * sizeof(double4) : 0);
+    const size_t o_cell = ar.take(widom ? (size_t)max_slots * sizeof(unsigned) : 0);
+    const size_t o_order = ar.take(widom ? (size_t)max_slots * sizeof(unsigned) : 0);
+    const size_t o_lj = ar.take((size_t)max_slots * sizeof(double));
+    const size_t o_sr = ar.take((size_t)max_slots * sizeof(double));
+    const size_t o_lr = ar.take((size_t)max_slots * sizeof(double));
+    const long long max_spans = max_cells / SCAN_SPAN + 1;
+    const size_t o_count = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
+    const size_t o_cursor = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
+    const size_t o_poff = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
+    const size_t o_soff = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
+    const size_t o_span = ar.take(widom ? (size_t)max_spans * 2 * sizeof(unsigned) : 0);
+    const size_t o_items = ar.take(widom ? (size_t)max_items * sizeof(uint4) : 0);
+    const size_t o_nitems = ar.take(sizeof(unsigned) * 4);
+    const size_t o_cand = ar.take(sizeof(unsigned long long));
+    if (int rc = ensure_scratch(ar.used)) return rc;
+    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
+
+    // device pointers of each run, then one staged upload of runs, segments, tables
+    for (int i = 0; i < n_runs; ++i) {
+        RunArgs& a = runs[i].ra;
+        a.ljtab = base + o_tabs + runs[i].tab_off;
+        if (widom) {
+            a.partials = reinterpret_cast<double*>(base + o_part) + part_off[i];
+        } else {
+            a.grid_d = fp64 ? reinterpret_cast<double*>(base + o_grid + grid_off[i]) : nullptr;
+            a.grid_f = fp64 ? nullptr : reinterpret_cast<float*>(base + o_grid + grid_off[i]);
+        }
+    }
+    std::vector<unsigned char>& stage = g_ctx.host_tab;
+    stage.assign(stage_bytes, 0);
+    for (int i = 0; i < n_runs; ++i) {
+        std::memcpy(stage.data() + o_runs + i * sizeof(RunArgs), &runs[i].ra, sizeof(RunArgs));
+        if (!runs[i].tab.empty())
+            std::memcpy(stage.data() + o_tabs + runs[i].tab_off, runs[i].tab.data(), runs[i].tab.size());
+    }
+    if (!segs.empty()) std::memcpy(stage.data() + o_segs, segs.data(), segs.size() * sizeof(SegArgs));
+    PM_HIP(hipMemcpyAsync(base, stage.data(), stage_bytes, hipMemcpyHostToDevice, g_ctx.stream));
+
+    const RunArgs* d_runs = reinterpret_cast<const RunArgs*>(base + o_runs);
+    const SegArgs* d_segs = reinterpret_cast<const SegArgs*>(base + o_segs);
+    unsigned* d_count = reinterpret_cast<unsigned*>(base + o_count);
+    unsigned* d_cursor = reinterpret_cast<unsigned*>(base + o_cursor);
+    unsigned* d_poff = reinterpret_cast<unsigned*>(base + o_poff);
+    unsigned* d_soff = reinterpret_cast<unsigned*>(base + o_soff);
+    unsigned* d_span_p = reinterpret_cast<unsigned*>(base + o_span);
+    unsigned* d_span_s = d_span_p + max_spans;
+    unsigned* d_nitems = reinterpret_cast<unsigned*>(base + o_nitems);
+    unsigned long long* d_cand = reinterpret_cast<unsigned long long*>(base + o_cand);
+    double* d_sums = reinterpret_cast<double*>(base + o_sums);
+
+    GroupArgs g{};
+    g.runs = d_runs;
+    g.mode = mode;
+    g.pos = reinterpret_cast<double4*>(base + o_pos);
+    g.cell = reinterpret_cast<unsigned*>(base + o_cell);
+    g.count = d_count;
+    g.point_off = d_poff;
+    g.items = reinterpret_cast<const uint4*>(base + o_items);
+    g.n_items = d_nitems;
+    g.order = reinterpret_cast<const unsigned*>(base + o_order);
+    g.out_lj = reinterpret_cast<double*>(base + o_lj);
+    g.out_sr = reinterpret_cast<double*>(base + o_sr);
+    g.out_lr = reinterpret_cast<double*>(base + o_lr);
+    g.cand_pairs = d_cand;
+    const size_t shm = (fp64 ? pair_lds_fixed<double>() : pair_lds_fixed<float>()) + std::max<size_t>(lj_lds, 16);
+
+    PM_HIP(hipMemsetAsync(d_cand, 0, sizeof(unsigned long long), g_ctx.stream));
+    PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
+    float pair_ms = 0.f;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const Group& gr = groups[gi];
+        g.segs = d_segs + gr.seg0;
+        g.n_seg = gr.n_seg;
+        long long pair_wg = gr.n_wg;
+        if (widom) {
+            const long long n_cells = gr.n_cells;
+            const long long n_spans = (n_cells + SCAN_SPAN - 1) / SCAN_SPAN;
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_cells + 1, 256)), dim3(256), 0, g_ctx.stream, d_count,
+                               n_cells + 1, 0u);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_cells + 1, 256)), dim3(256), 0, g_ctx.stream, d_cursor,
+                               n_cells + 1, 0u);
+            hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
+            hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)n_spans), dim3(SCAN_WG), 0, g_ctx.stream, d_count,
+                               (int)n_cells, d_span_p, d_span_s);
+            hipLaunchKernelGGL(scan_spans_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_span_p, d_span_s,
+                               (int)n_spans, d_poff, d_soff, (int)n_cells, d_nitems);
+            hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)n_spans), dim3(SCAN_WG), 0, g_ctx.stream, d_count,
+                               (int)n_cells, d_span_p, d_span_s, d_poff, d_soff);
+            hipLaunchKernelGGL(scatter_points_kernel, dim3(grid1d(gr.n_slots, 256)), dim3(256), 0, g_ctx.stream,
+                               g.cell, gr.n_slots, d_poff, d_cursor, const_cast<unsigned*>(g.order));
+            hipLaunchKernelGGL(slice_items_kernel, dim3(grid1d(n_cells, 256)), dim3(256), 0, g_ctx.stream, d_count,
+                               d_soff, g.segs, gr.n_seg, const_cast<uint4*>(g.items), (int)n_cells);
+            pair_wg = n_cells + (gr.n_slots + WG - 1) / WG;  // >= the slices the scan counts
+        }
+        PM_HIP(hipEventRecord(g_ctx.evp0, g_ctx.stream));
+        if (fp64) pair_launch_t<double>(g, (int)pair_wg, shm);
+        else pair_launch_t<float>(g, (int)pair_wg, shm);
+        PM_HIP(hipEventRecord(g_ctx.evp1, g_ctx.stream));
+        const long long slot_wg = (gr.n_slots + WG - 1) / WG;
+        if (gr.es) hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
+        if (widom) hipLaunchKernelGGL(widom_combine_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
+        else hipLaunchKernelGGL(grid_combine_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
+        PM_HIP(hipGetLastError());
+        if (gi + 1 < groups.size()) {
+            // the next group reuses the events: read this group's pair-kernel time now
+            PM_HIP(hipEventSynchronize(g_ctx.evp1));
+            float ms = 0.f;
+            PM_HIP(hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1));
+            pair_ms += ms;
+        }
+    }
+    if (widom) {
+        hipLaunchKernelGGL(block_sums_kernel, dim3((unsigned)blocks_total), dim3(WG), 0, g_ctx.stream, d_runs, n_runs,
+                           d_sums);
+        PM_HIP(hipGetLastError());
+    }
+    PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
+    unsigned long long cand = 0;
+    PM_HIP(hipMemcpyAsync(&cand, d_cand, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream));
+    std::vector<std::vector<float>> tmp;
+    double pairs = 0.0;
+    if (widom) {
+        PM_HIP(hipMemcpyAsync(out_sums, d_sums, (size_t)blocks_total * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                              g_ctx.stream));
+        for (const PointRun& r : runs) pairs += (double)r.ra.n_blocks * r.ra.n_per_block * r.ns * r.fw->n_rep;
+    } else {
+        tmp.resize(fp64 ? 0 : n_runs);
+        for (int i = 0; i < n_runs; ++i) {
+            const PointRun& r = runs[i];
+            pairs += (double)r.n_points * r.fw->n_rep;
+            if (fp64) {
+                PM_HIP(hipMemcpyAsync(r.host_out, base + o_grid + grid_off[i], (size_t)r.n_points * sizeof(double),
+                                      hipMemcpyDeviceToHost, g_ctx.stream));
+            } else {
+                tmp[i].resize((size_t)r.n_points);
+                PM_HIP(hipMemcpyAsync(tmp[i].data(), base + o_grid + grid_off[i], (size_t)r.n_points * sizeof(float),
+                                      hipMemcpyDeviceToHost, g_ctx.stream));
+            }
+        }
+    }
+    if (int rc = finish_timing(pairs)) return rc;
+    {
+        float ms = 0.f;
+        PM_HIP(hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1));
+        pair_ms += ms;
+    }
+    g_ctx.last_pair_ms = pair_ms;
+    g_ctx.last_cand = (double)cand;
+    if (!widom && !fp64)
+        for (int i = 0; i < n_runs; ++i)
+            for (long long p = 0; p < runs[i].n_points; ++p) runs[i].host_out[p] = (double)tmp[i][p];
+    return PM_OK;
 }
 
 }  // namespace
@@ -602,8 +1017,10 @@ int pm_framework_destroy(pm_framework* fw) {
     (void)hipFree(fw->d_atoms);
     (void)hipFree(fw->d_species);
     (void)hipFree(fw->d_unit_q);
-    (void)hipFree(fw->d_img);
-    (void)hipFree(fw->d_img_type);
+    for (auto& im : fw->images) {
+        (void)hipFree(im.d_img);
+        (void)hipFree(im.d_type);
+    }
     delete fw;
     return PM_OK;
 }
@@ -863,326 +1280,90 @@ int pm_electrostatic_energy(const pm_framework* fw, const pm_ewald* ew, int32_t 
     return PM_OK;
 }
 
-int pm_widom(const pm_framework* fw_c, const pm_forcefield* ff, const pm_ewald* ew,
+int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
              const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
              int64_t n_per_block, int32_t precision, double* out_block_sums,
              double* out_energies) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = require_init()) return rc;
-    pm_framework* fw = const_cast<pm_framework*>(fw_c);  // the image-list cache only
-    if (!fw || !ff || !pr || !out_block_sums || n_blocks < 1 || n_per_block < 1)
-        return fail(PM_ERR_ARG, "pm_widom: null handle/pointer or empty block layout");
+    if (!out_block_sums) return fail(PM_ERR_ARG, "pm_widom: null output pointer");
     if (precision != PM_FP64 && precision != PM_MIXED)
         return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
-    if (!(temperature > 0.0) || !std::isfinite(temperature))
-        return fail(PM_ERR_ARG, "pm_widom: temperature must be positive and finite");
-    if (int rc = check_species(fw, ff)) return rc;
-    for (int s = 0; s < pr->ns; ++s)
-        if (pr->site[s].lj && pr->lj_species[s] >= ff->n)
-            return fail(PM_ERR_ARG, "pm_widom: probe species outside the force field");
-    const bool es = fw->charged && pr->charged;  // src/henry.jl:93
-    if (es && !ew) return fail(PM_ERR_ARG, "pm_widom: charged framework and probe need an Ewald handle");
-    const int ns = pr->ns;
-    const int64_t chunks = (n_per_block + WG - 1) / WG;
-    const int64_t n_wg = chunks * n_blocks;
-    if (n_wg > (int64_t)INT32_MAX || (int64_t)n_blocks * n_per_block > ((int64_t)1 << 53))
-        return fail(PM_ERR_ARG, "pm_widom: too many insertions for one call");
-    const int64_t batch_wg = std::max<int64_t>(1, MAX_POINT_SLOTS / ((int64_t)WG * ns));
-    const int64_t bw_max = std::min(batch_wg, n_wg);
-    const int64_t slots_max = bw_max * WG * ns;
-
-    int ncell[3];
-    double cell_r = 0.0;
-    choose_cells(fw, (double)std::min<int64_t>(n_per_block * n_blocks, bw_max * WG) * ns, ncell, &cell_r);
-    const int64_t n_cells = (int64_t)ncell[0] * ncell[1] * ncell[2];
-    PArgs pa{};
-    fill_pargs_common(pa, fw, ff, ew, es, ns, pr->site);
-    pa.mode = MODE_WIDOM;
-    pa.chunk_r = cell_r;
-    if (int rc = choose_path(fw, pa)) return rc;
-    for (int i = 0; i < 3; ++i) pa.ncell[i] = ncell[i];
-
-    size_t tab_bytes = 0;
-    if (int rc = stage_ljtab(fw, ff, ns, pr->site, pr->lj_species, precision, &tab_bytes)) return rc;
-    const int64_t items_max = n_cells + (slots_max + WG - 1) / WG + 1;
-    Arena ar;
-    const size_t o_tab = ar.take(tab_bytes);
-    const size_t o_part = ar.take((size_t)n_wg * 3 * sizeof(double));
-    const size_t o_sums = ar.take((size_t)n_blocks * 3 * sizeof(double));
-    const size_t o_pos = ar.take((size_t)slots_max * sizeof(double4));
-    const size_t o_cell = ar.take((size_t)slots_max * sizeof(unsigned));
-    const size_t o_order = ar.take((size_t)slots_max * sizeof(unsigned));
-    const size_t o_lj = ar.take((size_t)slots_max * sizeof(double));
-    const size_t o_sr = ar.take((size_t)slots_max * sizeof(double));
-    const size_t o_lr = ar.take((size_t)slots_max * sizeof(double));
-    const size_t o_count = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
-    const size_t o_cursor = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
-    const size_t o_poff = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
-    const size_t o_soff = ar.take((size_t)(n_cells + 1) * sizeof(unsigned));
-    const size_t o_nitems = ar.take(sizeof(unsigned) * 4);
-    const size_t o_cand = ar.take(sizeof(unsigned long long));
-    const size_t o_items = ar.take((size_t)items_max * sizeof(uint2));
-    if (int rc = ensure_scratch(ar.used)) return rc;
-    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
-    if (int rc = copy_ljtab(base + o_tab)) return rc;
-    double* d_part = reinterpret_cast<double*>(base + o_part);
-    double* d_sums = reinterpret_cast<double*>(base + o_sums);
-    double4* d_pos = reinterpret_cast<double4*>(base + o_pos);
-    unsigned* d_cell = reinterpret_cast<unsigned*>(base + o_cell);
-    unsigned* d_order = reinterpret_cast<unsigned*>(base + o_order);
-    double* d_lj = reinterpret_cast<double*>(base + o_lj);
-    double* d_sr = reinterpret_cast<double*>(base + o_sr);
-    double* d_lr = reinterpret_cast<double*>(base + o_lr);
-    unsigned* d_count = reinterpret_cast<unsigned*>(base + o_count);
-    unsigned* d_cursor = reinterpret_cast<unsigned*>(base + o_cursor);
-    unsigned* d_poff = reinterpret_cast<unsigned*>(base + o_poff);
-    unsigned* d_soff = reinterpret_cast<unsigned*>(base + o_soff);
-    unsigned* d_nitems = reinterpret_cast<unsigned*>(base + o_nitems);
-    unsigned long long* d_cand = reinterpret_cast<unsigned long long*>(base + o_cand);
-    uint2* d_items = reinterpret_cast<uint2*>(base + o_items);
-    pa.ljtab = base + o_tab;
-    pa.point_off = d_poff;
-    pa.items = d_items;
-    pa.n_items = d_nitems;
-    pa.order = d_order;
-    pa.pos = d_pos;
-    pa.out_lj = d_lj;
-    pa.out_sr = d_sr;
-    pa.cand_pairs = d_cand;
-
+    pm_widom_run in{};
+    in.framework = fw;
+    in.forcefield = ff;
+    in.ewald = ew;
+    in.probe = pr;
+    in.temperature = temperature;
+    in.seed = seed;
+    in.n_per_block = n_per_block;
+    in.n_blocks = n_blocks;
+    std::vector<PointRun> runs(1);
+    if (int rc = widom_prepare(in, runs[0])) return rc;
     double* d_energies = nullptr;
+    const size_t e_bytes = (size_t)n_blocks * (size_t)n_per_block * 2 * sizeof(double);
     if (out_energies) {
-        hipError_t e = hipMalloc(&d_energies, (size_t)n_blocks * n_per_block * 2 * sizeof(double));
-        if (e != hipSuccess) return fail(PM_ERR_OOM, "pm_widom: energy buffer allocation failed");
+        if (hipMalloc(&d_energies, e_bytes) != hipSuccess)
+            return fail(PM_ERR_OOM, "pm_widom: energy buffer allocation failed");
+        runs[0].ra.out_e = d_energies;
     }
-    auto bail = [&](int rc) {
+    int rc = execute_points(runs, MODE_WIDOM, precision, out_block_sums);
+    if (rc == PM_OK && out_energies) {
+        hipError_t e = hipMemcpy(out_energies, d_energies, e_bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(PM_ERR_DEVICE, std::string("pm_widom: copy: ") + hipGetErrorString(e));
+    }
+    if (d_energies) {
         (void)hipStreamSynchronize(g_ctx.stream);
         (void)hipFree(d_energies);
-        return rc;
-    };
-
-    GenArgs ga{};
-    ga.seed = seed;
-    ga.rotate = pr->rotate;
-    ga.nsite = ns;
-    ga.n_blocks = n_blocks;
-    ga.chunks_per_block = (int)chunks;
-    ga.n_per_block = n_per_block;
-    for (int i = 0; i < 3; ++i) ga.ncell[i] = ncell[i];
-    for (int s = 0; s < ns; ++s) {
-        ga.off[s][0] = pr->site[s].ox;
-        ga.off[s][1] = pr->site[s].oy;
-        ga.off[s][2] = pr->site[s].oz;
     }
-    for (int i = 0; i < 9; ++i) ga.c2f[i] = fw->c2f[i];
-    ga.pos = d_pos;
-    ga.cell = d_cell;
-    ga.count = d_count;
-
-    FArgs fa{};
-    fa.mode = MODE_WIDOM;
-    fa.nsite = ns;
-    fa.pos = d_pos;
-    fa.out_lr = d_lr;
-    for (int s = 0; s < ns; ++s)
-        if (pr->site[s].q != 0.0) fa.site_q_mask |= 1u << s;
-    if (es) fill_fargs(fa, fw, ew);
-
-    CArgs ca{};
-    ca.nsite = ns;
-    ca.es = es ? 1 : 0;
-    ca.n_blocks = n_blocks;
-    ca.chunks_per_block = (int)chunks;
-    ca.n_per_block = n_per_block;
-    for (int s = 0; s < ns; ++s) ca.site_q[s] = pr->site[s].q;
-    ca.inv4pe = INV_4PI_EPS0;
-    ca.temperature = temperature;
-    ca.lj = d_lj;
-    ca.sr = d_sr;
-    ca.lr = d_lr;
-    ca.out_e = d_energies;
-    ca.partials = d_part;
-
-    if (hipMemsetAsync(d_cand, 0, sizeof(unsigned long long), g_ctx.stream) != hipSuccess)
-        return bail(fail(PM_ERR_DEVICE, "pm_widom: memset failed"));
-    if (hipEventRecord(g_ctx.ev0, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
-    float pair_ms = 0.f;
-    for (int64_t w0 = 0; w0 < n_wg; w0 += batch_wg) {
-        const int64_t bw = std::min(batch_wg, n_wg - w0);
-        const long long n_slots = bw * WG * ns;
-        const long long fill_n = n_cells + 1;
-        hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((fill_n + 255) / 256)), dim3(256), 0, g_ctx.stream,
-                           d_count, fill_n, 0u);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((fill_n + 255) / 256)), dim3(256), 0, g_ctx.stream,
-                           d_cursor, fill_n, 0u);
-        ga.w0 = (int)w0;
-        hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)bw), dim3(WG), 0, g_ctx.stream, ga);
-        hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, g_ctx.stream, d_count, d_poff, d_soff,
-                           d_nitems, (int)n_cells);
-        hipLaunchKernelGGL(scatter_points_kernel, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0,
-                           g_ctx.stream, d_cell, n_slots, d_poff, d_cursor, d_order);
-        hipLaunchKernelGGL(slice_items_kernel, dim3((unsigned)((n_cells + 255) / 256)), dim3(256), 0,
-                           g_ctx.stream, d_count, d_soff, d_items, (int)n_cells);
-        const int64_t n_items_bound = n_cells + (n_slots + WG - 1) / WG;
-        if (hipEventRecord(g_ctx.evp0, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
-        point_pair_launch(pa, (int)n_items_bound, es, precision);
-        if (hipEventRecord(g_ctx.evp1, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
-        if (es) {
-            fa.n_slots = n_slots;
-            hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)((n_slots + WG - 1) / WG)), dim3(WG), 0,
-                               g_ctx.stream, fa);
-        }
-        ca.w0 = (int)w0;
-        hipLaunchKernelGGL(widom_combine_kernel, dim3((unsigned)bw), dim3(WG), 0, g_ctx.stream, ca);
-        hipError_t le = hipGetLastError();
-        if (le != hipSuccess)
-            return bail(fail(PM_ERR_DEVICE, std::string("pm_widom: launch: ") + hipGetErrorString(le)));
-        if (w0 + batch_wg < n_wg) {
-            // the next batch reuses the events: read this batch's pair-kernel time now
-            if (hipEventSynchronize(g_ctx.evp1) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event sync"));
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1);
-            pair_ms += ms;
-        }
-    }
-    hipLaunchKernelGGL(block_sums_kernel, dim3((unsigned)n_blocks), dim3(WG), 0, g_ctx.stream, d_part,
-                       (int)chunks, d_sums);
-    if (hipGetLastError() != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: block sums launch"));
-    if (hipEventRecord(g_ctx.ev1, g_ctx.stream) != hipSuccess) return bail(fail(PM_ERR_DEVICE, "pm_widom: event"));
-    unsigned long long cand = 0;
-    hipError_t ce = hipMemcpyAsync(out_block_sums, d_sums, (size_t)n_blocks * 3 * sizeof(double),
-                                   hipMemcpyDeviceToHost, g_ctx.stream);
-    if (ce == hipSuccess) ce = hipMemcpyAsync(&cand, d_cand, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream);
-    if (ce == hipSuccess && out_energies)
-        ce = hipMemcpyAsync(out_energies, d_energies, (size_t)n_blocks * n_per_block * 2 * sizeof(double),
-                            hipMemcpyDeviceToHost, g_ctx.stream);
-    int frc = finish_timing((double)n_blocks * n_per_block * ns * fw->n_rep);
-    {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1) == hipSuccess) pair_ms += ms;
-    }
-    g_ctx.last_pair_ms = pair_ms;
-    g_ctx.last_cand = (double)cand;
-    (void)hipFree(d_energies);
-    if (ce != hipSuccess) return fail(PM_ERR_DEVICE, std::string("pm_widom: copy: ") + hipGetErrorString(ce));
-    return frc;
+    return rc;
 }
 
-int pm_energy_grid(const pm_framework* fw_c, const pm_forcefield* ff, const pm_ewald* ew,
+int pm_widom_batch(int32_t n_runs, const pm_widom_run* runs_in, int32_t precision, double* out_block_sums) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (n_runs < 1 || n_runs > PM_MAX_BATCH_RUNS || !runs_in || !out_block_sums)
+        return fail(PM_ERR_ARG, "pm_widom_batch: null pointer or run count outside [1, PM_MAX_BATCH_RUNS]");
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
+    std::vector<PointRun> runs((size_t)n_runs);
+    for (int32_t i = 0; i < n_runs; ++i)
+        if (int rc = widom_prepare(runs_in[i], runs[i]))
+            return fail(rc, "pm_widom_batch: run " + std::to_string(i) + ": " + g_err);
+    return execute_points(runs, MODE_WIDOM, precision, out_block_sums);
+}
+
+int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
                    const pm_probe* pr, const int32_t n_pts[3], int32_t precision, double* out) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = require_init()) return rc;
-    pm_framework* fw = const_cast<pm_framework*>(fw_c);  // the image-list cache only
-    if (!fw || !pr || !n_pts || !out)
-        return fail(PM_ERR_ARG, "pm_energy_grid: null handle or pointer");
+    if (!n_pts) return fail(PM_ERR_ARG, "pm_energy_grid: null n_pts");
     if (precision != PM_FP64 && precision != PM_MIXED)
         return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
-    if (pr->ns != 1) return fail(PM_ERR_ARG, "pm_energy_grid: probe must have exactly one site");
-    if (pr->site[0].ox != 0.0 || pr->site[0].oy != 0.0 || pr->site[0].oz != 0.0)
-        return fail(PM_ERR_ARG, "pm_energy_grid: single-site probe must sit at its centre of mass");
-    for (int i = 0; i < 3; ++i)
-        if (n_pts[i] < 1 || n_pts[i] > 65536) return fail(PM_ERR_ARG, "pm_energy_grid: n_pts out of range");
-    if (pr->site[0].lj) {
-        if (!ff) return fail(PM_ERR_ARG, "pm_energy_grid: LJ probe needs a force field");
-        if (int rc = check_species(fw, ff)) return rc;
-        if (pr->lj_species[0] >= ff->n) return fail(PM_ERR_ARG, "pm_energy_grid: probe species outside the force field");
-    }
-    const bool es = fw->charged && pr->charged;
-    if (es && !ew) return fail(PM_ERR_ARG, "pm_energy_grid: charged probe needs an Ewald handle");
-    const long long n_items = (long long)n_pts[0] * n_pts[1] * n_pts[2];
-    const int brick[3] = {8, 8, 4};
-    int nbrick[3];
-    for (int i = 0; i < 3; ++i) nbrick[i] = (n_pts[i] + brick[i] - 1) / brick[i];
-    const long long n_bricks = (long long)nbrick[0] * nbrick[1] * nbrick[2];
-    if (n_bricks > INT32_MAX || (n_items + WG - 1) / WG > INT32_MAX)
-        return fail(PM_ERR_ARG, "pm_energy_grid: grid too large");
-    // Cartesian span of one brick and its half diagonal
-    double ed[3][3];
-    for (int d = 0; d < 3; ++d) {
-        double step = n_pts[d] > 1 ? (double)(brick[d] - 1) / (double)(n_pts[d] - 1) / fw->rep[d] : 0.0;
-        for (int i = 0; i < 3; ++i) ed[d][i] = fw->F[3 * d + i] * step;
-    }
-    PArgs pa{};
-    fill_pargs_common(pa, fw, ff, ew, es, 1, pr->site);
-    pa.mode = MODE_GRID;
-    pa.chunk_r = half_diagonal(ed[0], ed[1], ed[2]);
-    if (int rc = choose_path(fw, pa)) return rc;
-    for (int i = 0; i < 3; ++i) {
-        pa.grid_n[i] = n_pts[i];
-        pa.brick[i] = brick[i];
-        pa.nbrick[i] = nbrick[i];
-    }
-    size_t tab_bytes = 0;
-    if (int rc = stage_ljtab(fw, ff, 1, pr->site, pr->lj_species, precision, &tab_bytes)) return rc;
-    const bool fp64 = precision == PM_FP64;
-    Arena ar;
-    const size_t o_tab = ar.take(tab_bytes);
-    const size_t o_lj = ar.take((size_t)n_items * sizeof(double));
-    const size_t o_sr = ar.take((size_t)n_items * sizeof(double));
-    const size_t o_lr = ar.take((size_t)n_items * sizeof(double));
-    const size_t o_grid = ar.take((size_t)n_items * (fp64 ? sizeof(double) : sizeof(float)));
-    const size_t o_cand = ar.take(sizeof(unsigned long long));
-    if (int rc = ensure_scratch(ar.used)) return rc;
-    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
-    if (int rc = copy_ljtab(base + o_tab)) return rc;
-    pa.ljtab = base + o_tab;
-    pa.out_lj = reinterpret_cast<double*>(base + o_lj);
-    pa.out_sr = reinterpret_cast<double*>(base + o_sr);
-    pa.cand_pairs = reinterpret_cast<unsigned long long*>(base + o_cand);
-    double* d_lr = reinterpret_cast<double*>(base + o_lr);
-    PM_HIP(hipMemsetAsync(pa.cand_pairs, 0, sizeof(unsigned long long), g_ctx.stream));
+    pm_grid_run in{};
+    in.framework = fw;
+    in.forcefield = ff;
+    in.ewald = ew;
+    in.probe = pr;
+    for (int i = 0; i < 3; ++i) in.n_pts[i] = n_pts[i];
+    in.out = out;
+    std::vector<PointRun> runs(1);
+    if (int rc = grid_prepare(in, runs[0])) return rc;
+    return execute_points(runs, MODE_GRID, precision, nullptr);
+}
 
-    PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
-    PM_HIP(hipEventRecord(g_ctx.evp0, g_ctx.stream));
-    point_pair_launch(pa, (int)n_bricks, es, precision);
-    PM_HIP(hipGetLastError());
-    PM_HIP(hipEventRecord(g_ctx.evp1, g_ctx.stream));
-    if (es) {
-        FArgs fa{};
-        fa.mode = MODE_GRID;
-        fa.nsite = 1;
-        fa.n_slots = n_items;
-        for (int i = 0; i < 3; ++i) fa.grid_n[i] = n_pts[i];
-        fa.site_q_mask = 1u;
-        fa.out_lr = d_lr;
-        fill_fargs(fa, fw, ew);
-        hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)((n_items + WG - 1) / WG)), dim3(WG), 0,
-                           g_ctx.stream, fa);
-        PM_HIP(hipGetLastError());
-    }
-    GCArgs gc{};
-    gc.n = n_items;
-    gc.q = pr->site[0].q;
-    gc.inv4pe = INV_4PI_EPS0;
-    gc.es = es ? 1 : 0;
-    gc.lj = pa.out_lj;
-    gc.sr = pa.out_sr;
-    gc.lr = d_lr;
-    gc.out_d = fp64 ? reinterpret_cast<double*>(base + o_grid) : nullptr;
-    gc.out_f = fp64 ? nullptr : reinterpret_cast<float*>(base + o_grid);
-    hipLaunchKernelGGL(grid_combine_kernel, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0,
-                       g_ctx.stream, gc);
-    PM_HIP(hipGetLastError());
-    PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
-    unsigned long long cand = 0;
-    PM_HIP(hipMemcpyAsync(&cand, pa.cand_pairs, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream));
-    std::vector<float> tmp;
-    if (fp64) {
-        PM_HIP(hipMemcpyAsync(out, base + o_grid, (size_t)n_items * sizeof(double), hipMemcpyDeviceToHost,
-                              g_ctx.stream));
-    } else {
-        tmp.resize((size_t)n_items);
-        PM_HIP(hipMemcpyAsync(tmp.data(), base + o_grid, (size_t)n_items * sizeof(float), hipMemcpyDeviceToHost,
-                              g_ctx.stream));
-    }
-    if (int rc = finish_timing((double)n_items * fw->n_rep)) return rc;
-    float ms = 0.f;
-    PM_HIP(hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1));
-    g_ctx.last_pair_ms = ms;
-    g_ctx.last_cand = (double)cand;
-    if (!fp64)
-        for (long long i = 0; i < n_items; ++i) out[i] = (double)tmp[i];
-    return PM_OK;
+int pm_energy_grid_batch(int32_t n_runs, const pm_grid_run* runs_in, int32_t precision) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (n_runs < 1 || n_runs > PM_MAX_BATCH_RUNS || !runs_in)
+        return fail(PM_ERR_ARG, "pm_energy_grid_batch: null pointer or run count outside [1, PM_MAX_BATCH_RUNS]");
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
+    std::vector<PointRun> runs((size_t)n_runs);
+    for (int32_t i = 0; i < n_runs; ++i)
+        if (int rc = grid_prepare(runs_in[i], runs[i]))
+            return fail(rc, "pm_energy_grid_batch: grid " + std::to_string(i) + ": " + g_err);
+    return execute_points(runs, MODE_GRID, precision, nullptr);
 }
 
 int pm_last_kernel_stats(double* kernel_ms, double* pair_evals) {

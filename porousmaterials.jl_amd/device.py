@@ -103,12 +103,13 @@ def forcefield(ljff) -> _Handle:
 
 
 def ewald(crystal, fw: _Handle, eparams) -> _Handle:
-    key = ("ew", id(eparams), fw.ptr.value)
+    kidx = np.ascontiguousarray(eparams.kidx, dtype=np.int32)
+    wt = np.ascontiguousarray(eparams.wt, dtype=np.float64)
+    # keyed by content (an id() of a collected EwaldParams can be reused)
+    key = ("ew", float(eparams.alpha), float(eparams.sr_cutoff_r), kidx.tobytes(), wt.tobytes(), fw.ptr.value)
     h = crystal._device_cache.get(key)
     if h is not None:
         return h
-    kidx = np.ascontiguousarray(eparams.kidx, dtype=np.int32)
-    wt = np.ascontiguousarray(eparams.wt, dtype=np.float64)
     out = ctypes.c_void_p()
     check(_lib.load().pm_ewald_create(fw.ptr, float(eparams.alpha), float(eparams.sr_cutoff_r),
                                       int(wt.size), iptr(kidx), dptr(wt), ctypes.byref(out)))
@@ -237,3 +238,61 @@ def energy_grid(fw, ff, ew, pr, n_pts, precision) -> np.ndarray:
                              ew.ptr if ew is not None else None, pr.ptr, iptr(n),
                              precision_code(precision), dptr(out)))
     return out
+
+
+def widom_batch(runs, precision):
+    """Block sums of many Widom runs in one call (pm_widom_batch).
+
+    ``runs``: sequence of (fw, ff, ew, pr, temperature, seed, n_blocks,
+    n_per_block), handles as for :func:`widom`.  Returns a list of
+    (n_blocks, 3) arrays in the order of ``runs``."""
+    lib = _lib.load()
+    arr = (_lib.WidomRun * len(runs))()
+    keep = []
+    total = 0
+    for i, (fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block) in enumerate(runs):
+        ff = _ff_handle(ff)
+        keep.append(ff)
+        r = arr[i]
+        r.framework = fw.ptr
+        r.forcefield = ff.ptr
+        r.ewald = ew.ptr if ew is not None else None
+        r.probe = pr.ptr
+        r.temperature = float(temperature)
+        r.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        r.n_per_block = int(n_per_block)
+        r.n_blocks = int(n_blocks)
+        total += int(n_blocks)
+    sums = np.zeros(3 * total)
+    check(lib.pm_widom_batch(len(runs), arr, precision_code(precision), dptr(sums)))
+    out, k = [], 0
+    for run in runs:
+        nb = int(run[6])
+        out.append(sums[3 * k:3 * (k + nb)].reshape(nb, 3))
+        k += nb
+    return out
+
+
+def energy_grid_batch(runs, precision):
+    """Many energy grids in one call (pm_energy_grid_batch).
+
+    ``runs``: sequence of (fw, ff, ew, pr, n_pts).  Returns a list of flat
+    i-fastest arrays in the order of ``runs``."""
+    lib = _lib.load()
+    arr = (_lib.GridRun * len(runs))()
+    outs, keep = [], []
+    for i, (fw, ff, ew, pr, n_pts) in enumerate(runs):
+        ff = _ff_handle(ff)
+        keep.append(ff)
+        out = np.zeros(int(np.prod(n_pts)))
+        outs.append(out)
+        r = arr[i]
+        r.framework = fw.ptr
+        r.forcefield = ff.ptr if ff is not None else None
+        r.ewald = ew.ptr if ew is not None else None
+        r.probe = pr.ptr
+        for d in range(3):
+            r.n_pts[d] = int(n_pts[d])
+        r.out = dptr(out)
+    check(lib.pm_energy_grid_batch(len(runs), arr, precision_code(precision)))
+    return outs

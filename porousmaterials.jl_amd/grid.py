@@ -43,9 +43,7 @@ def required_n_pts(box: Box, dx: float):
     return tuple(int(math.ceil(n / dx)) + 1 for n in norms)
 
 
-def energy_grid(crystal, molecule, ljff, resolution=1.0, n_rotations=1000, temperature=math.nan,
-                units="kJ_mol", center=False, verbose=True, precision="mixed",
-                ewald_precision=1e-6) -> Grid:
+def _grid_setup(crystal, molecule, ljff, resolution, units, center, verbose, ewald_precision):
     assert_P1_symmetry(crystal)
     if units not in ("kJ_mol", "K"):
         raise ValueError("Pass kJ_mol or K for units of kJ/mol or K, respectively.")
@@ -71,7 +69,33 @@ def energy_grid(crystal, molecule, ljff, resolution=1.0, n_rotations=1000, tempe
         eparams = setup_Ewald_sum(xtal.box, math.sqrt(ljff.r2_cutoff), eps=ewald_precision)
         ew = device.ewald(xtal, fw, eparams)
     pr = device.probe(molecule, ljff)
-    data = device.energy_grid(fw, ff, ew, pr, n_pts, precision).reshape(n_pts, order="F")
+    return (fw, ff, ew, pr, n_pts), grid_origin
+
+
+def _to_grid(crystal, n_pts, flat, units, origin):
+    data = flat.reshape(n_pts, order="F")
     if units == "kJ_mol":
         data = data * 8.314 / 1000.0
-    return Grid(crystal.box, n_pts, data, units, grid_origin)
+    return Grid(crystal.box, n_pts, data, units, origin)
+
+
+def energy_grid(crystal, molecule, ljff, resolution=1.0, n_rotations=1000, temperature=math.nan,
+                units="kJ_mol", center=False, verbose=True, precision="mixed",
+                ewald_precision=1e-6) -> Grid:
+    run, origin = _grid_setup(crystal, molecule, ljff, resolution, units, center, verbose,
+                              ewald_precision)
+    flat = device.energy_grid(*run, precision)
+    return _to_grid(crystal, run[4], flat, units, origin)
+
+
+def energy_grids(crystals, molecule, ljff, resolution=1.0, units="kJ_mol", center=False,
+                 verbose=False, precision="mixed", ewald_precision=1e-6):
+    """``energy_grid`` of ``molecule`` in every crystal of ``crystals``, all
+    grids in one device call (pm_energy_grid_batch).  Each grid equals what
+    ``energy_grid`` returns for that crystal."""
+    crystals = list(crystals)
+    setups = [_grid_setup(x, molecule, ljff, resolution, units, center, verbose, ewald_precision)
+              for x in crystals]
+    flats = device.energy_grid_batch([run for run, _ in setups], precision)
+    return [_to_grid(x, run[4], flat, units, origin)
+            for x, (run, origin), flat in zip(crystals, setups, flats)]

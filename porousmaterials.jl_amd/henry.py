@@ -76,16 +76,12 @@ def block_statistics(block_sums, nb_per_block, temperature, rho):
     return r
 
 
-def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volume=200,
-                      verbose=True, ewald_precision=1e-6, autosave=True, filename_comment="",
-                      accessibility_grid=None, seed=0, precision="mixed", return_block_sums=False):
-    """K_H and <U> of ``molecule`` (Cartesian template) in ``crystal``."""
+def _widom_setup(crystal, molecule, temperature, ljff, insertions_per_volume, ewald_precision, verbose):
+    """Insertion count, replication and device handles of one Henry run
+    (src/henry.jl:41-109)."""
     assert_P1_symmetry(crystal)
-    if accessibility_grid is not None:
-        raise NotImplementedError("accessibility-grid blocking is not supported by the device path")
     if molecule.frac:
         raise ValueError("henry_coefficient expects a Cartesian molecule template")
-    t0 = time.time()
     nb_insertions = max(N_BLOCKS, math.ceil(insertions_per_volume * crystal.box.Omega))
     nb_per_block = math.ceil(nb_insertions / N_BLOCKS)
     repfactors = replication_factors(crystal.box, ljff)
@@ -107,8 +103,11 @@ def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volum
                                   verbose=verbose)
         ew = device.ewald(xtal, fw, eparams)
     pr = device.probe(molecule, ljff)
-    sums = device.widom(fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block, precision)
-    results = block_statistics(sums, nb_per_block, temperature, rho)
+    return fw, ff, ew, pr, nb_per_block, rho
+
+
+def _finish(results, crystal, molecule, temperature, ljff, insertions_per_volume, sums, nb_per_block,
+            rho, t0, verbose, autosave, filename_comment):
     results["xtal"] = crystal.name
     results["elapsed time (min)"] = (time.time() - t0) / 60.0
     if verbose:
@@ -129,6 +128,51 @@ def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volum
             json.dump(results, f, indent=1, ensure_ascii=False)
         if verbose:
             print("\tResults saved in: ", name)
+    return results
+
+
+def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volume=200,
+                      verbose=True, ewald_precision=1e-6, autosave=True, filename_comment="",
+                      accessibility_grid=None, seed=0, precision="mixed", return_block_sums=False):
+    """K_H and <U> of ``molecule`` (Cartesian template) in ``crystal``."""
+    if accessibility_grid is not None:
+        raise NotImplementedError("accessibility-grid blocking is not supported by the device path")
+    t0 = time.time()
+    fw, ff, ew, pr, nb_per_block, rho = _widom_setup(crystal, molecule, temperature, ljff,
+                                                     insertions_per_volume, ewald_precision, verbose)
+    sums = device.widom(fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block, precision)
+    results = block_statistics(sums, nb_per_block, temperature, rho)
+    _finish(results, crystal, molecule, temperature, ljff, insertions_per_volume, sums, nb_per_block,
+            rho, t0, verbose, autosave, filename_comment)
     if return_block_sums:
         return results, sums
     return results
+
+
+def henry_coefficients(pairs, temperature, ljff, insertions_per_volume=200, verbose=False,
+                       ewald_precision=1e-6, autosave=False, filename_comment="", seeds=None,
+                       precision="mixed", return_block_sums=False):
+    """Screening: ``henry_coefficient`` for every (crystal, molecule) in
+    ``pairs``, all runs in one device call (pm_widom_batch).  Each result is
+    what ``henry_coefficient`` returns for that pair with the same seed
+    (``seeds[i]``, default 0).  Returns the list of results dicts."""
+    t0 = time.time()
+    pairs = list(pairs)
+    seeds = [0] * len(pairs) if seeds is None else list(seeds)
+    if len(seeds) != len(pairs):
+        raise ValueError("one seed per (crystal, molecule) pair")
+    setups, runs = [], []
+    for (crystal, molecule), seed in zip(pairs, seeds):
+        fw, ff, ew, pr, nb_per_block, rho = _widom_setup(crystal, molecule, temperature, ljff,
+                                                         insertions_per_volume, ewald_precision,
+                                                         verbose)
+        setups.append((nb_per_block, rho))
+        runs.append((fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block))
+    all_sums = device.widom_batch(runs, precision)
+    out = []
+    for (crystal, molecule), (nb_per_block, rho), sums in zip(pairs, setups, all_sums):
+        results = block_statistics(sums, nb_per_block, temperature, rho)
+        _finish(results, crystal, molecule, temperature, ljff, insertions_per_volume, sums,
+                nb_per_block, rho, t0, verbose, autosave, filename_comment)
+        out.append((results, sums) if return_block_sums else results)
+    return out
