@@ -247,6 +247,40 @@ const K_TO_KJ_PER_MOL = 8.3144598e-3
 const N_BLOCKS = 5
 
 """
+    widom_results(sums, nb_per_block, temperature, ρ, crystal)
+
+Block statistics and the results `Dict` of `src/henry.jl:112-159` from the
+device's per-block sums (3 x N_BLOCKS: Σw, Σw E_vdw, Σw E_es).
+"""
+function widom_results(sums::Matrix{Float64}, nb_per_block::Int, temperature::Float64, ρ::Float64,
+                       crystal::Crystal)
+    boltzmann_factor_sums = sums[1, :]
+    average_energies = [PotentialEnergy(sums[2, b] / sums[1, b], sums[3, b] / sums[1, b])
+                        for b in 1:N_BLOCKS]
+    henry_coefficients = boltzmann_factor_sums / (UNIV_GAS_CONST * temperature * nb_per_block)
+    err_kh = 2.0 * std(henry_coefficients) / sqrt(N_BLOCKS)
+    err_vdw = 2.0 * std([e.vdw for e in average_energies]) / sqrt(N_BLOCKS)
+    err_es = 2.0 * std([e.es for e in average_energies]) / sqrt(N_BLOCKS)
+    results = Dict{String, Any}()
+    results["henry coefficient [mol/(m³-bar)]"] = mean(henry_coefficients)
+    results["xtal"] = crystal.name
+    results["henry coefficient [mmol/(g-bar)]"] = results["henry coefficient [mol/(m³-bar)]"] / ρ
+    results["err henry coefficient [mmol/(g-bar)]"] = err_kh / ρ
+    results["henry coefficient [mol/(kg-Pa)]"] = results["henry coefficient [mmol/(g-bar)]"] / 100000.0
+    results["⟨U, vdw⟩ (K)"] = mean([e.vdw for e in average_energies])
+    results["⟨U, es⟩ (K)"] = mean([e.es for e in average_energies])
+    results["⟨U⟩ (K)"] = results["⟨U, vdw⟩ (K)"] + results["⟨U, es⟩ (K)"]
+    results["⟨U⟩ (kJ/mol)"] = results["⟨U⟩ (K)"] * K_TO_KJ_PER_MOL
+    results["⟨U, vdw⟩ (kJ/mol)"] = results["⟨U, vdw⟩ (K)"] * K_TO_KJ_PER_MOL
+    results["err ⟨U, vdw⟩ (kJ/mol)"] = err_vdw * K_TO_KJ_PER_MOL
+    results["⟨U, es⟩ (kJ/mol)"] = results["⟨U, es⟩ (K)"] * K_TO_KJ_PER_MOL
+    results["err ⟨U, es⟩ (kJ/mol)"] = err_es * K_TO_KJ_PER_MOL
+    results["Qst (kJ/mol)"] = -results["⟨U⟩ (kJ/mol)"] + temperature * K_TO_KJ_PER_MOL
+    results["err Qst (kJ/mol)"] = (err_vdw + err_es) * K_TO_KJ_PER_MOL
+    return results
+end
+
+"""
     henry_coefficient(crystal, molecule, temperature, ljff; insertions_per_volume=200,
                       verbose=true, ewald_precision=1e-6, autosave=true,
                       filename_comment="", seed=0, precision=:mixed)
@@ -288,29 +322,7 @@ function henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Fl
                  Int32, Ptr{Float64}, Ptr{Float64}),
                 fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int32(N_BLOCKS),
                 Int64(nb_per_block), precision_code(precision), sums, C_NULL))
-    boltzmann_factor_sums = sums[1, :]
-    average_energies = [PotentialEnergy(sums[2, b] / sums[1, b], sums[3, b] / sums[1, b])
-                        for b in 1:N_BLOCKS]
-    henry_coefficients = boltzmann_factor_sums / (UNIV_GAS_CONST * temperature * nb_per_block)
-    err_kh = 2.0 * std(henry_coefficients) / sqrt(N_BLOCKS)
-    err_vdw = 2.0 * std([e.vdw for e in average_energies]) / sqrt(N_BLOCKS)
-    err_es = 2.0 * std([e.es for e in average_energies]) / sqrt(N_BLOCKS)
-    results = Dict{String, Any}()
-    results["henry coefficient [mol/(m³-bar)]"] = mean(henry_coefficients)
-    results["xtal"] = crystal.name
-    results["henry coefficient [mmol/(g-bar)]"] = results["henry coefficient [mol/(m³-bar)]"] / ρ
-    results["err henry coefficient [mmol/(g-bar)]"] = err_kh / ρ
-    results["henry coefficient [mol/(kg-Pa)]"] = results["henry coefficient [mmol/(g-bar)]"] / 100000.0
-    results["⟨U, vdw⟩ (K)"] = mean([e.vdw for e in average_energies])
-    results["⟨U, es⟩ (K)"] = mean([e.es for e in average_energies])
-    results["⟨U⟩ (K)"] = results["⟨U, vdw⟩ (K)"] + results["⟨U, es⟩ (K)"]
-    results["⟨U⟩ (kJ/mol)"] = results["⟨U⟩ (K)"] * K_TO_KJ_PER_MOL
-    results["⟨U, vdw⟩ (kJ/mol)"] = results["⟨U, vdw⟩ (K)"] * K_TO_KJ_PER_MOL
-    results["err ⟨U, vdw⟩ (kJ/mol)"] = err_vdw * K_TO_KJ_PER_MOL
-    results["⟨U, es⟩ (kJ/mol)"] = results["⟨U, es⟩ (K)"] * K_TO_KJ_PER_MOL
-    results["err ⟨U, es⟩ (kJ/mol)"] = err_es * K_TO_KJ_PER_MOL
-    results["Qst (kJ/mol)"] = -results["⟨U⟩ (kJ/mol)"] + temperature * K_TO_KJ_PER_MOL
-    results["err Qst (kJ/mol)"] = (err_vdw + err_es) * K_TO_KJ_PER_MOL
+    results = widom_results(sums, nb_per_block, temperature, ρ, crystal)
     results["elapsed time (min)"] = (time() - time_start) / 60
     if autosave
         path = rc[:paths][:simulations]
@@ -321,6 +333,75 @@ function henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Fl
         PorousMaterials.JLD2.@save savename results
     end
     return results
+end
+
+"""
+Mirror of `pm_widom_run` (include/pmhip.h): one run of `pm_widom_batch`.
+"""
+struct WidomRun
+    framework::Ptr{Cvoid}
+    forcefield::Ptr{Cvoid}
+    ewald::Ptr{Cvoid}
+    probe::Ptr{Cvoid}
+    temperature::Float64
+    seed::UInt64
+    n_per_block::Int64
+    n_blocks::Int32
+    reserved::Int32
+end
+
+"""
+    henry_coefficients(pairs, temperature, ljff; insertions_per_volume=200,
+                       ewald_precision=1e-6, seeds=zeros(Int, length(pairs)),
+                       precision=:mixed)
+
+Screening: `henry_coefficient` for every `(crystal, molecule)` in `pairs`, all
+runs in one `pm_widom_batch` call.  Each results `Dict` equals the one
+`henry_coefficient` returns for that pair and seed.
+"""
+function henry_coefficients(pairs::Vector{<:Tuple{Crystal, Molecule}}, temperature::Float64,
+                            ljff::LJForceField; insertions_per_volume::Union{Int, Float64}=200,
+                            ewald_precision::Float64=1e-6, seeds=zeros(Int, length(pairs)),
+                            precision::Symbol=:mixed)
+    time_start = time()
+    handles = Any[]   # keeps every handle alive until the call returns
+    runs = WidomRun[]
+    setups = Tuple{Int, Float64}[]
+    ff = forcefield_handle(ljff)
+    for ((crystal, molecule), seed) in zip(pairs, seeds)
+        assert_P1_symmetry(crystal)
+        nb_insertions = max(N_BLOCKS, ceil(Int, insertions_per_volume * crystal.box.Ω))
+        nb_per_block = ceil(Int, nb_insertions / N_BLOCKS)
+        repfactors = replication_factors(crystal.box, ljff)
+        xtal_rep = replicate(crystal, repfactors)
+        fw = framework_handle(crystal, ljff, repfactors)
+        ew = C_NULL
+        if has_charges(crystal) & has_charges(molecule)
+            eparams = setup_Ewald_sum(xtal_rep.box, sqrt(ljff.r²_cutoff), ϵ=ewald_precision, verbose=false)
+            ew_h = ewald_handle(fw, eparams)
+            push!(handles, ew_h)
+            ew = ew_h.ptr
+        end
+        pr = probe_handle(molecule, ljff)
+        push!(handles, fw, pr)
+        push!(runs, WidomRun(fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int64(nb_per_block),
+                             Int32(N_BLOCKS), Int32(0)))
+        push!(setups, (nb_per_block, crystal_density(xtal_rep)))
+    end
+    sums = zeros(3, N_BLOCKS * length(runs))
+    GC.@preserve handles ff begin
+        check(ccall((:pm_widom_batch, libpmhip), Cint, (Int32, Ptr{WidomRun}, Int32, Ptr{Float64}),
+                    Int32(length(runs)), runs, precision_code(precision), sums))
+    end
+    out = Dict{String, Any}[]
+    for (i, (crystal, molecule)) in enumerate(pairs)
+        nb_per_block, ρ = setups[i]
+        results = widom_results(sums[:, (i - 1) * N_BLOCKS + 1:i * N_BLOCKS], nb_per_block, temperature, ρ,
+                                crystal)
+        results["elapsed time (min)"] = (time() - time_start) / 60
+        push!(out, results)
+    end
+    return out
 end
 
 """
