@@ -184,3 +184,30 @@ def test_probe_merge_and_limits(pm):
     sp9 = np.zeros(9, dtype=np.int32)
     assert _lib.load().pm_probe_create(9, _lib.dptr(many), _lib.iptr(sp9), 0, None, None,
                                        ctypes.byref(out)) == _lib.PM_ERR_ARG
+
+
+def test_batch_structs_match_the_c_layout(tmp_path):
+    """ctypes mirrors of pm_widom_run / pm_grid_run have the C compiler's
+    sizes and field offsets."""
+    import shutil
+    import subprocess
+    from porousmaterials_jl_amd import _lib
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    fields = {"pm_widom_run": [f for f, _ in _lib.WidomRun._fields_],
+              "pm_grid_run": [f for f, _ in _lib.GridRun._fields_]}
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "pmhip.h"', "int main(void) {"]
+    for st, fs in fields.items():
+        src.append(f'printf("%zu\\n", sizeof({st}));')
+        src += [f'printf("%zu\\n", offsetof({st}, {f}));' for f in fs]
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(_lib.HEADER), str(c), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = []
+    for cls in (_lib.WidomRun, _lib.GridRun):
+        want.append(ctypes.sizeof(cls))
+        want += [getattr(cls, f).offset for f, _ in cls._fields_]
+    assert got == want
