@@ -45,6 +45,7 @@ enum { PM_FP64 = 0, PM_MIXED = 1 };
 
 #define PM_MAX_SITES 8          /* sites (LJ spheres + point charges) per probe */
 #define PM_MAX_FW_SPECIES 64    /* distinct LJ species within one framework     */
+#define PM_MAX_GRAD_TYPES 8     /* framework species of a pm_widom_gradient run  */
 
 typedef struct pm_framework pm_framework;
 typedef struct pm_forcefield pm_forcefield;
@@ -162,6 +163,28 @@ int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew
 int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
                    const pm_probe* pr, const int32_t n_pts[3], int32_t precision,
                    double* out);
+
+/*
+ * Adjoint of pm_widom: the same run (out_block_sums bit for bit as pm_widom
+ * gives them) plus, per block b and force-field species t present in the
+ * framework, the Boltzmann-weighted derivative sums
+ *   out_block_grads[(b*n_ff + t)*3 + {0,1,2}] = sum_i w_i dE_i/d{eps_t, sigma_t, q_t}
+ * with w_i = e^{-E_i/T} (Inf energies excluded), n_ff = the force field's species
+ * count, zeros for species absent from the framework.  eps_t, sigma_t are the
+ * framework-side pure parameters (the probe's held fixed, also where a probe
+ * site shares the species); q_t is a uniform shift of the charges of every
+ * framework atom of species t.  jac[(i*n_ff + j)*4 + k] is the mixing rule's
+ * Jacobian for framework species i and probe species j: d eps_ij/d eps_i,
+ * d eps_ij/d sigma_i, d sigma_ij/d eps_i, d sigma_ij/d sigma_i.  Then
+ * d ln K_H / d theta = -(1/T) sum_b G_b(theta) / sum_b S_b (src/henry.jl:112-135
+ * differentiated).  The reference has no derivative of henry_coefficient;
+ * this is the backward pass of a ChainRulesCore rrule around it.
+ * At most PM_MAX_GRAD_TYPES framework species.
+ */
+int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+                      const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
+                      int64_t n_per_block, int32_t precision, const double* jac,
+                      double* out_block_sums, double* out_block_grads);
 
 #define PM_MAX_BATCH_RUNS 65536  /* runs per pm_widom_batch / pm_energy_grid_batch call */
 

@@ -17,7 +17,7 @@ from .ewald import (EPS0, FOUR_PI_EPS0, Eikr, EwaldParams, EwaldSum, determine_e
 from .forcefield import LJForceField, forcefield_coverage, lennard_jones, replication_factors
 from .grid import Grid, energy_grid, energy_grids, required_n_pts
 from .henry import (K_TO_KJ_PER_MOL, N_BLOCKS, UNIV_GAS_CONST, block_statistics, henry_coefficient,
-                    henry_coefficients,
+                    henry_coefficient_gradient, henry_coefficients,
                     henry_result_savename)
 from .molecule import (Cart, Frac, Molecule, arvo_rotation_matrix, center_of_mass, ion, lj_sphere,
                        needs_rotations, point_charge, site_offsets, translate_to)

@@ -42,7 +42,8 @@ constexpr double R2_OVERLAP = 0.1;  // src/vdw_energetics.jl:2
 enum { MODE_WIDOM = 0, MODE_GRID = 2 };
 
 struct KVec {
-    double ma, mb, mc, are, aim, pad;
+    double ma, mb, mc, are, aim;
+    double wnr;  // wt * N_R (the factor of the structure factor in A)
 };
 
 struct Site {
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(64) void structure_factor_kernel(const double4* uq,
         kv.ma = ma; kv.mb = mb; kv.mc = mc;
         kv.are = wt[k] * nr * sre;
         kv.aim = wt[k] * nr * sim;
-        kv.pad = 0.0;
+        kv.wnr = wt[k] * nr;
         out[k] = kv;
     }
 }

@@ -35,6 +35,9 @@ constexpr int ERFC_SEG = 512;      // cubic segments of erfc(alpha r) on [0, r_s
 constexpr int SCAN_WG = 1024;      // work-items of the scan kernels
 constexpr int SCAN_PER = 4;        // cells per work-item in the scan kernels
 constexpr int SCAN_SPAN = SCAN_WG * SCAN_PER;
+constexpr int GT = PM_MAX_GRAD_TYPES;  // framework types of a gradient run (fixed stride)
+constexpr int GQ = 4;                  // per-type sums per point: d/d eps, d/d sigma, erfc/r, Fourier
+constexpr int GK = 3 + 3 * GT;         // per-workgroup partials of a gradient run
 
 // One Widom run or grid of a call (device memory).
 struct RunArgs {
@@ -74,6 +77,14 @@ struct RunArgs {
     double grid_q;
     double* grid_d;          // FP64 output, or
     float* grid_f;           // FP32 output
+    // parameter gradients (adjoint Widom runs)
+    int n_gt;                // framework types (<= GT)
+    int pstride;             // doubles per workgroup partial: 3, or GK for a gradient run
+    const void* gtab;        // [n_types][nsite] Real: 4 eps_ts / sigma_ts (d LJ / d sigma_ts factor)
+    const double4* jac;      // [n_types][nsite]: d eps_ts/d eps_t, d eps_ts/d sigma_t, d sigma_ts/d eps_t,
+                             // d sigma_ts/d sigma_t of the mixing rule
+    const double2* A_t;      // per-type Fourier boxes: GT complex coefficients per box entry
+    double* gsums;           // per block: 3 GT sums of w dE/d(eps_t, sigma_t, q_t)
 };
 
 // A contiguous piece of one run inside a launch group.
@@ -101,6 +112,7 @@ struct GroupArgs {
     double* out_sr;
     double* out_lr;
     unsigned long long* cand_pairs;  // evaluated candidate pairs (reporting)
+    double* out_g;           // gradient runs: per slot GQ x GT per-type sums
 };
 
 enum { SEG_BY_WG = 0, SEG_BY_CELL = 1, SEG_BY_SLOT = 2 };
@@ -325,6 +337,52 @@ __device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool i
     }
 }
 
+// pair_term with the per-type derivative sums of a gradient run: cE += u^2 - u
+// (d LJ / d eps_ts = 4 (u^2 - u), u = (sigma_ts^2 / r^2)^3), cS += d LJ / d sigma_ts,
+// cQ += erfc(alpha r) / r, all for in-cutoff pairs.  The energy terms are
+// computed exactly as in pair_term.
+template <int ESM, typename Real>
+__device__ inline void pair_term_grad(double r2, double q, const LJP<Real>& p, Real csig, bool is_lj, double r2lj,
+                                      double r2sr, double alpha, float inv_h, const float4* tab, bool& ovl,
+                                      Real& plj, Real& psr, double& cE, double& cS, double& cQ) {
+    const bool in = r2 < r2lj;
+    ovl |= in & (r2 < R2_OVERLAP) & is_lj;
+    const bool lj_in = in & is_lj;
+    if constexpr (sizeof(Real) == 4) {
+        const float s = p.sig2 * __builtin_amdgcn_rcpf((float)r2);
+        const float u = s * s * s;
+        const float w = fmaf(u, u, -u);
+        plj += in ? p.eps4 * w : 0.0f;
+        cE += lj_in ? (double)w : 0.0;
+        cS += lj_in ? (double)(csig * u * fmaf(12.0f, u, -6.0f)) : 0.0;
+    } else {
+        const double s = p.sig2 / r2;
+        const double u = s * s * s;
+        plj += in ? p.eps4 * u * (u - 1.0) : 0.0;
+        cE += lj_in ? u * (u - 1.0) : 0.0;
+        cS += lj_in ? csig * u * (12.0 * u - 6.0) : 0.0;
+    }
+    if constexpr (ESM != 0) {
+        const bool ins = (ESM == 1) ? in : (r2 < r2sr);
+        if constexpr (sizeof(Real) == 4) {
+            const float rf2 = (float)r2;
+            const float rinv = __builtin_amdgcn_rsqf(rf2);
+            const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
+            const int i = (int)x;
+            const float t = x - (float)i;
+            const float4 c = tab[i];
+            const float e = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+            psr += (ins & (q != 0.0)) ? (float)q * e * rinv : 0.0f;
+            cQ += ins ? (double)(e * rinv) : 0.0;
+        } else {
+            const double r = sqrt(r2);
+            const double ec = erfc(r * alpha);
+            psr += (ins & (q != 0.0)) ? q / r * ec : 0.0;
+            cQ += ins ? ec / r : 0.0;
+        }
+    }
+}
+
 struct PairLds {
     double4* c;     // candidate records (Cartesian relative to the chunk centre, charge)
     int* t;         // candidate type * nsite
@@ -332,14 +390,52 @@ struct PairLds {
     int* wcnt;      // per-wave candidate counts
 };
 
+// Per-type derivative sums of a gradient run's point: the running sums of the
+// current framework type (cE, cS, cQ) are added to g[] when the type changes.
+// The type is the same for every lane of the wave (all lanes read the same
+// candidate), so the flush is a uniform branch; image lists are sorted by type,
+// so the types of a point's candidate stream never decrease and it runs at
+// most once per type.
+struct TypeSums {
+    int cur = -1;  // current type * nsite
+    double cE = 0.0, cS = 0.0, cQ = 0.0;
+    double g[3 * GT];
+    __device__ TypeSums() {
+#pragma unroll
+        for (int k = 0; k < 3 * GT; ++k) g[k] = 0.0;
+    }
+    __device__ __attribute__((always_inline)) inline void flush(int nsite) {
+        if (cur >= 0) {
+            const int t = cur / nsite;
+            // every entry is updated (a conditional update of one entry would be
+            // folded into a dynamically indexed array held in scratch memory)
+#pragma unroll
+            for (int k = 0; k < GT; ++k) {
+                g[k] += (k == t) ? cE : 0.0;
+                g[GT + k] += (k == t) ? cS : 0.0;
+                g[2 * GT + k] += (k == t) ? cQ : 0.0;
+            }
+        }
+        cE = cS = cQ = 0.0;
+    }
+    __device__ __attribute__((always_inline)) inline void type(int tt, int nsite) {
+        if (tt != cur) {
+            flush(nsite);
+            cur = tt;
+        }
+    }
+};
+
 // The pair loops of one workgroup's chunk.  Fast path: scan the image list,
 // keep the images within r_cmax + chunk_r of the chunk centre by a
 // wave-ballot compaction into LDS, evaluate every lane's point against them.
-// Slow path: every replicated atom, nearest image per pair.
-template <int ESM, typename Real>
-__device__ inline void chunk_pairs(const RunArgs& a, bool valid, double px, double py, double pz, double cf0,
-                                   double cf1, double cf2, int site, const LJP<Real>* s_lj, PairLds L,
-                                   double& acc_lj, double& acc_sr, bool& ovl, unsigned long long& n_cand) {
+// Slow path: every replicated atom, nearest image per pair.  GRAD adds the
+// per-type derivative sums of a gradient run.
+template <int ESM, typename Real, bool GRAD>
+__device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs& a, bool valid, double px, double py, double pz, double cf0,
+                                   double cf1, double cf2, int site, const LJP<Real>* s_lj, const Real* s_gc,
+                                   PairLds L, double& acc_lj, double& acc_sr, bool& ovl, unsigned long long& n_cand,
+                                   TypeSums& ts) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
     const double r2lj = a.r2_lj, r2sr = a.r2_sr, alpha = a.alpha;
@@ -397,8 +493,15 @@ __device__ inline void chunk_pairs(const RunArgs& a, bool valid, double px, doub
                         const double4 c = L.c[j];
                         const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
                         const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        pair_term<ESM, Real>(r2, c.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
-                                             L.erfc, ovl, plj, psr);
+                        if constexpr (GRAD) {
+                            const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
+                            ts.type(tt, nsite);
+                            pair_term_grad<ESM, Real>(r2, c.w, s_lj[tt + site], s_gc[tt + site], is_lj, r2lj, r2sr,
+                                                      alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
+                        } else {
+                            pair_term<ESM, Real>(r2, c.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
+                                                 L.erfc, ovl, plj, psr);
+                        }
                     }
                     acc_lj += (double)plj;
                     acc_sr += (double)psr;
@@ -430,8 +533,15 @@ __device__ inline void chunk_pairs(const RunArgs& a, bool valid, double px, doub
                         const double cy = f22 * ey + f23 * ez;
                         const double cz = f33 * ez;
                         const double r2 = cx * cx + cy * cy + cz * cz;
-                        pair_term<ESM, Real>(r2, at.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
-                                             L.erfc, ovl, plj, psr);
+                        if constexpr (GRAD) {
+                            const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
+                            ts.type(tt, nsite);
+                            pair_term_grad<ESM, Real>(r2, at.w, s_lj[tt + site], s_gc[tt + site], is_lj, r2lj, r2sr,
+                                                      alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
+                        } else {
+                            pair_term<ESM, Real>(r2, at.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
+                                                 L.erfc, ovl, plj, psr);
+                        }
                     }
                     acc_lj += (double)plj;
                     acc_sr += (double)psr;
@@ -439,16 +549,17 @@ __device__ inline void chunk_pairs(const RunArgs& a, bool valid, double px, doub
             }
         }
     }
+    if constexpr (GRAD) ts.flush(nsite);
 }
 
-// LDS: TILE candidate records and types, the erfc table (FP32 kernels), the
-// LJ table of the run.
+// LDS: TILE candidate records and types, the erfc table (FP32 kernels), then
+// the LJ table of the run (and for a gradient run its d/d sigma factors).
 template <typename Real>
 constexpr int pair_lds_fixed() {
     return TILE * (int)(sizeof(double4) + sizeof(int)) + (sizeof(Real) == 4 ? ERFC_SEG * (int)sizeof(float4) : 0);
 }
 
-template <typename Real>
+template <typename Real, bool GRAD>
 __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_wcnt[WG / 64];
@@ -472,6 +583,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     }
     const SegArgs sg = g.segs[si];
     const RunArgs& a = g.runs[sg.run];
+    Real* s_gc = reinterpret_cast<Real*>(s_lj + a.n_types * a.nsite);
 
     double cf0, cf1, cf2;  // chunk centre (fractional)
     bool valid;
@@ -509,8 +621,13 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
         cf2 = grid_coord(k0 + 0.5 * (a.brick[2] - 1), n3, a.repd[2]);
         site = 0;
     }
+    const int n_tab = a.n_types * a.nsite;
     const LJP<Real>* gtab = reinterpret_cast<const LJP<Real>*>(a.ljtab);
-    for (int t = tid; t < a.n_types * a.nsite; t += WG) s_lj[t] = gtab[t];
+    for (int t = tid; t < n_tab; t += WG) s_lj[t] = gtab[t];
+    if constexpr (GRAD) {
+        const Real* gc = reinterpret_cast<const Real*>(a.gtab);
+        for (int t = tid; t < n_tab; t += WG) s_gc[t] = gc[t];
+    }
     if constexpr (sizeof(Real) == 4)
         if (a.esm != 0)
             for (int t = tid; t < ERFC_SEG; t += WG) s_erfc[t] = a.erfc_tab[t];
@@ -518,15 +635,24 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     double acc_lj = 0.0, acc_sr = 0.0;
     bool ovl = false;
     unsigned long long n_cand = 0;
+    TypeSums ts;
     if (a.esm == 0)
-        chunk_pairs<0, Real>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, L, acc_lj, acc_sr, ovl, n_cand);
+        chunk_pairs<0, Real, GRAD>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+                                   n_cand, ts);
     else if (a.esm == 1)
-        chunk_pairs<1, Real>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, L, acc_lj, acc_sr, ovl, n_cand);
+        chunk_pairs<1, Real, GRAD>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+                                   n_cand, ts);
     else
-        chunk_pairs<2, Real>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, L, acc_lj, acc_sr, ovl, n_cand);
+        chunk_pairs<2, Real, GRAD>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+                                   n_cand, ts);
     if (valid) {
         g.out_lj[pid] = ovl ? INFINITY : acc_lj;
         g.out_sr[pid] = acc_sr;
+        if constexpr (GRAD) {
+            double* o = g.out_g + pid * (GQ * GT);
+#pragma unroll
+            for (int k = 0; k < 3 * GT; ++k) o[k] = ts.g[k];
+        }
     }
     if (g.cand_pairs != nullptr) {
         const unsigned long long m = __ballot(valid);
@@ -657,6 +783,26 @@ __global__ __launch_bounds__(WG) void grid_combine_kernel(GroupArgs g) {
     else a.grid_d[lp] = e;
 }
 
+// Fixed-order workgroup sum of K values; thread k < K writes the k-th sum to
+// out_a[k] (k < 3) or out_b[k - 3].  For k < 3 the arithmetic is block_sum3's.
+template <int K>
+__device__ inline void block_sum_k(const double* v, double* out_a, double* out_b) {
+    __shared__ double red[K][WG / 64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double sk = wave_sum(v[k]);
+        if (lane == 0) red[k][wave] = sk;
+    }
+    __syncthreads();
+    if (tid < K) {
+        double t = 0.0;
+        for (int w = 0; w < WG / 64; ++w) t += red[tid][w];
+        if (tid < 3) out_a[tid] = t;
+        else out_b[tid - 3] = t;
+    }
+}
+
 // Fixed-order per-block reduction of the workgroup partials: one workgroup
 // per Widom block of every run, strided partial sums then a shuffle tree.
 __global__ __launch_bounds__(WG) void block_sums_kernel(const RunArgs* runs, int n_runs, double* out) {
@@ -668,12 +814,187 @@ __global__ __launch_bounds__(WG) void block_sums_kernel(const RunArgs* runs, int
     }
     const RunArgs& a = runs[lo];
     const int b = (int)blockIdx.x - a.block_base, chunks = a.chunks_per_block;
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    for (int c = threadIdx.x; c < chunks; c += WG) {
-        const double* p = a.partials + 3LL * ((long long)b * chunks + c);
-        t0 += p[0]; t1 += p[1]; t2 += p[2];
+    if (a.pstride == 3) {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int c = threadIdx.x; c < chunks; c += WG) {
+            const double* p = a.partials + 3LL * ((long long)b * chunks + c);
+            t0 += p[0]; t1 += p[1]; t2 += p[2];
+        }
+        block_sum3(t0, t1, t2, out + 3LL * blockIdx.x);
+    } else {
+        double t[GK];
+#pragma unroll
+        for (int k = 0; k < GK; ++k) t[k] = 0.0;
+        for (int c = threadIdx.x; c < chunks; c += WG) {
+            const double* p = a.partials + (long long)GK * ((long long)b * chunks + c);
+#pragma unroll
+            for (int k = 0; k < GK; ++k) t[k] += p[k];
+        }
+        block_sum_k<GK>(t, out + 3LL * blockIdx.x, a.gsums + (long long)(3 * GT) * b);
     }
-    block_sum3(t0, t1, t2, out + 3LL * blockIdx.x);
+}
+
+// ------------------------------------------------------------- adjoint ----
+// Per-type Fourier coefficients for the charge derivatives: one wave per kept
+// k-vector, lanes over the unit-cell atoms (xyz, w = compact type), unit
+// charges.  box_t[entry * GT + t] = wt N_R sum_{atoms of type t} e^{i 2 pi m.x}.
+__global__ __launch_bounds__(64) void type_structure_factor_kernel(const double4* ua, long long n, const KVec* kv,
+                                                                   int Mb, int Mc, double2* box_t) {
+    const int k = blockIdx.x, lane = threadIdx.x;
+    const KVec v = kv[k];
+    double sre[GT], sim[GT];
+#pragma unroll
+    for (int t = 0; t < GT; ++t) { sre[t] = 0.0; sim[t] = 0.0; }
+    for (long long c = lane; c < n; c += 64) {
+        const double4 p = ua[c];
+        double ph = v.ma * p.x + v.mb * p.y + v.mc * p.z;
+        ph -= rint(ph);
+        double sn, cs;
+        sincospi(2.0 * ph, &sn, &cs);
+        const int ty = (int)p.w;
+#pragma unroll
+        for (int t = 0; t < GT; ++t) {
+            sre[t] += (ty == t) ? cs : 0.0;
+            sim[t] += (ty == t) ? sn : 0.0;
+        }
+    }
+    const long long idx =
+        ((long long)v.ma * (2 * Mb + 1) + (long long)v.mb + Mb) * (2 * Mc + 1) + (long long)v.mc + Mc;
+#pragma unroll
+    for (int t = 0; t < GT; ++t) {
+        const double r = wave_sum(sre[t]), i = wave_sum(sim[t]);
+        if (lane == 0) box_t[idx * GT + t] = make_double2(v.wnr * r, v.wnr * i);
+    }
+}
+
+// Per-type Fourier sums of every charged point of a gradient run (FP64,
+// the recurrences of point_fourier_kernel over the per-type boxes).
+__global__ __launch_bounds__(WG) void point_fourier_grad_kernel(GroupArgs g) {
+    __shared__ double2 s_a[FT];
+    const int tid = threadIdx.x;
+    const long long p0 = (long long)blockIdx.x * WG;
+    const SegArgs sg = g.segs[find_segment<SEG_BY_SLOT>(g.segs, g.n_seg, p0)];
+    const RunArgs& a = g.runs[sg.run];
+    if (a.esm == 0) return;  // uniform: the whole workgroup leaves
+    const long long pid = p0 + tid, lp = pid - sg.slot_base;
+    bool valid = lp < sg.n_slots;
+    const double4 p = valid ? g.pos[pid] : make_double4(0.0, 0.0, 0.0, 0.0);
+    valid = valid && p.w != 0.0;
+    const int site = (int)(lp % a.nsite);
+    const bool charged = valid && ((a.site_q_mask >> site) & 1u);
+    double t0 = p.x * a.repd[0], t1 = p.y * a.repd[1], t2 = p.z * a.repd[2];
+    t0 -= rint(t0);
+    t1 -= rint(t1);
+    t2 -= rint(t2);
+    double s, c;
+    sincospi(2.0 * t0, &s, &c);
+    const double2 Ea = make_double2(c, -s);
+    sincospi(2.0 * t1, &s, &c);
+    const double2 Eb = make_double2(c, -s);
+    sincospi(2.0 * t2, &s, &c);
+    const double2 Ec = make_double2(c, -s);
+    const int Ma = a.Ma, Mb = a.Mb, Mc = a.Mc;
+    double2 Eb_neg = make_double2(1.0, 0.0), Ec_neg = make_double2(1.0, 0.0);
+    const double2 Eb_c = make_double2(Eb.x, -Eb.y), Ec_c = make_double2(Ec.x, -Ec.y);
+    for (int q = 0; q < Mb; ++q) Eb_neg = cmul(Eb_neg, Eb_c);
+    for (int q = 0; q < Mc; ++q) Ec_neg = cmul(Ec_neg, Ec_c);
+
+    const int rowlen = 2 * Mc + 1, nb = 2 * Mb + 1;
+    const int n_rows = (Ma + 1) * nb;
+    const int rows_per_tile = max(1, (FT / GT) / rowlen);
+    double2 Pa = make_double2(1.0, 0.0);
+    double2 Pb = Eb_neg;
+    double acc[GT];
+#pragma unroll
+    for (int t = 0; t < GT; ++t) acc[t] = 0.0;
+    for (int r0 = 0; r0 < n_rows; r0 += rows_per_tile) {
+        const int r1 = min(n_rows, r0 + rows_per_tile);
+        const int n_ent = (r1 - r0) * rowlen * GT;
+        __syncthreads();
+        for (int e = tid; e < n_ent; e += WG) s_a[e] = a.A_t[(long long)r0 * rowlen * GT + e];
+        __syncthreads();
+        for (int r = r0; r < r1; ++r) {
+            if (r > 0 && (r % nb) == 0) {  // next m_a
+                Pa = cmul(Pa, Ea);
+                Pb = cmul(Pa, Eb_neg);
+            }
+            double2 P = cmul(Pb, Ec_neg);
+            const double2* row = s_a + (r - r0) * rowlen * GT;
+            for (int mc = 0; mc < rowlen; ++mc) {
+#pragma unroll
+                for (int t = 0; t < GT; ++t) {
+                    const double2 A = row[mc * GT + t];
+                    acc[t] += A.x * P.x - A.y * P.y;
+                }
+                P = cmul(P, Ec);
+            }
+            Pb = cmul(Pb, Eb);
+        }
+    }
+    if (valid) {
+        double* o = g.out_g + pid * (GQ * GT) + 3 * GT;
+#pragma unroll
+        for (int t = 0; t < GT; ++t) o[t] = charged ? acc[t] : 0.0;
+    }
+}
+
+// Per insertion of a gradient run: the energies and weight of
+// widom_combine_kernel, and dE/d(eps_t, sigma_t, q_t) from the per-type sums
+// through the mixing-rule Jacobian:
+//   dE/d eps_t   = sum_s 4 RE_st d eps_ts/d eps_t   + RS_st d sigma_ts/d eps_t
+//   dE/d sigma_t = sum_s 4 RE_st d eps_ts/d sigma_t + RS_st d sigma_ts/d sigma_t
+//   dE/d q_t     = sum_s q_s (RQ_st / (4 pi eps0) + LR_st)
+// Workgroup partials: (S, W_vdw, W_es, then w dE/d theta for every type).
+__global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
+    const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
+    const RunArgs& a = g.runs[sg.run];
+    const int lw = (int)blockIdx.x - sg.wg_base, w = sg.w0 + lw;
+    const int b = w / a.chunks_per_block, c = w - b * a.chunks_per_block;
+    const long long i = (long long)c * WG + threadIdx.x;
+    const bool valid = i < a.n_per_block;
+    const int ns = a.nsite;
+    const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
+    double vdw = 0.0, es = 0.0;
+    double de[GT], dsg[GT], dq[GT];
+#pragma unroll
+    for (int t = 0; t < GT; ++t) { de[t] = 0.0; dsg[t] = 0.0; dq[t] = 0.0; }
+    if (valid) {
+        for (int s = 0; s < ns; ++s) {
+            vdw += g.out_lj[p0 + s];
+            if (a.esm != 0) es += a.site_q[s] * (g.out_sr[p0 + s] * a.inv4pe + g.out_lr[p0 + s]);
+            const double* gs = g.out_g + (p0 + s) * (GQ * GT);
+#pragma unroll
+            for (int t = 0; t < GT; ++t) {
+                if (t < a.n_gt) {
+                    const double4 J = a.jac[t * ns + s];
+                    const double re = 4.0 * gs[t], rs = gs[GT + t];
+                    de[t] += re * J.x + rs * J.z;
+                    dsg[t] += re * J.y + rs * J.w;
+                    if (a.esm != 0) dq[t] += a.site_q[s] * (gs[2 * GT + t] * a.inv4pe + gs[3 * GT + t]);
+                }
+            }
+        }
+    }
+    const double wgt = exp(-(vdw + es) / a.temperature);
+    const bool fin = !isinf(vdw) && !isinf(es);
+    const bool use = valid && fin;
+    if (a.out_e != nullptr && valid) {
+        const long long gi = (long long)b * a.n_per_block + i;
+        a.out_e[2 * gi] = vdw;
+        a.out_e[2 * gi + 1] = es;
+    }
+    double v[GK];
+    v[0] = valid ? wgt : 0.0;
+    v[1] = use ? wgt * vdw : 0.0;
+    v[2] = use ? wgt * es : 0.0;
+#pragma unroll
+    for (int t = 0; t < GT; ++t) {
+        v[3 + 3 * t] = use ? wgt * de[t] : 0.0;
+        v[4 + 3 * t] = use ? wgt * dsg[t] : 0.0;
+        v[5 + 3 * t] = use ? wgt * dq[t] : 0.0;
+    }
+    double* part = a.partials + (long long)GK * w;
+    block_sum_k<GK>(v, part, part + 3);
 }
 
 }  // namespace pmk

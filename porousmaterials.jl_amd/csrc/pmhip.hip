@@ -180,6 +180,7 @@ struct pm_ewald {
     int Ma = 0, Mb = 0, Mc = 0;
     double2* d_box = nullptr;
     float4* d_erfc = nullptr;  // ERFC_SEG cubics of erfc(alpha r) on [0, sr_cutoff]
+    double2* d_box_t = nullptr;  // per-type boxes (gradient runs), built on first use
 };
 
 struct pm_probe {
@@ -362,7 +363,13 @@ int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
     img.reserve((size_t)est);
     ty.reserve((size_t)est);
     const double* F = fw->F;
-    for (int64_t u = 0; u < fw->n_unit; ++u) {
+    // sorted by species (then unit-cell atom): a gradient run's per-type sums
+    // change type about once per tile
+    std::vector<int64_t> by_type((size_t)fw->n_unit);
+    for (int64_t u = 0; u < fw->n_unit; ++u) by_type[(size_t)u] = u;
+    std::stable_sort(by_type.begin(), by_type.end(),
+                     [&](int64_t x, int64_t y) { return fw->unit_type[x] < fw->unit_type[y]; });
+    for (int64_t u : by_type) {
         const double* x = &fw->unit_xf[3 * u];
         int lo[3], hi[3];
         for (int a = 0; a < 3; ++a) {
@@ -481,6 +488,10 @@ struct PointRun {
     double* host_out = nullptr;
     size_t tab_off = 0;
     std::vector<unsigned char> tab;
+    // gradient runs: d LJ / d sigma_ts factors (Real) and mixing-rule Jacobian per (type, site)
+    size_t gtab_off = 0, jac_off = 0;
+    std::vector<unsigned char> gtab;
+    std::vector<double4> jac;
     RunArgs ra{};
 };
 
@@ -589,17 +600,20 @@ int grid_prepare(const pm_grid_run& in, PointRun& r) {
     return PM_OK;
 }
 
-template <typename Real>
+template <typename Real, bool GRAD>
 void pair_launch_t(const GroupArgs& g, int n_wg, size_t shm) {
-    hipLaunchKernelGGL(point_pair_kernel<Real>, dim3(n_wg), dim3(WG), shm, g_ctx.stream, g);
+    hipLaunchKernelGGL((point_pair_kernel<Real, GRAD>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, g);
 }
 
 unsigned grid1d(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // Runs every run of a call through the point engine.  Widom: out_sums gets
-// n_blocks x (S, W_vdw, W_es) per run in input order.  Grids: each run's
-// host_out gets its grid (i fastest).
-int execute_points(std::vector<PointRun>& runs, int mode, int precision, double* out_sums) {
+// n_blocks x (S, W_vdw, W_es) per run in input order; with grad (gradient
+// runs, their gtab/jac filled) out_gsums gets n_blocks x 3 GT sums of
+// w dE/d(eps_t, sigma_t, q_t).  Grids: each run's host_out gets its grid
+// (i fastest).
+int execute_points(std::vector<PointRun>& runs, int mode, int precision, double* out_sums, bool grad = false,
+                   double* out_gsums = nullptr) {
     const bool fp64 = precision == PM_FP64;
     const int n_runs = (int)runs.size();
     // image lists and paths
@@ -704,10 +718,17 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         else build_ljtab<float>(r.fw, r.ff, r.ns, r.pr->site, r.pr->lj_species, r.tab);
         r.tab_off = tab_total;
         tab_total += align256(r.tab.size());
-        lj_lds = std::max(lj_lds, r.tab.size());
+        lj_lds = std::max(lj_lds, r.tab.size() + (grad ? r.gtab.size() : 0));
+        if (grad) {
+            r.gtab_off = tab_total;
+            tab_total += align256(r.gtab.size());
+            r.jac_off = tab_total;
+            tab_total += align256(r.jac.size() * sizeof(double4));
+        }
+        r.ra.pstride = grad ? GK : 3;
         if (mode == MODE_WIDOM) {
             part_off[i] = part_total;
-            part_total += r.n_wg * 3;
+            part_total += r.n_wg * r.ra.pstride;
             r.ra.block_base = (int)blocks_total;
             blocks_total += r.ra.n_blocks;
         } else {
@@ -740,6 +761,8 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const size_t o_items = ar.take(widom ? (size_t)max_items * sizeof(uint4) : 0);
     const size_t o_nitems = ar.take(sizeof(unsigned) * 4);
     const size_t o_cand = ar.take(sizeof(unsigned long long));
+    const size_t o_g = ar.take(grad ? (size_t)max_slots * GQ * GT * sizeof(double) : 0);
+    const size_t o_gsums = ar.take(grad ? (size_t)blocks_total * 3 * GT * sizeof(double) : 0);
     if (int rc = ensure_scratch(ar.used)) return rc;
     unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
 
@@ -747,6 +770,11 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     for (int i = 0; i < n_runs; ++i) {
         RunArgs& a = runs[i].ra;
         a.ljtab = base + o_tabs + runs[i].tab_off;
+        if (grad) {
+            a.gtab = base + o_tabs + runs[i].gtab_off;
+            a.jac = reinterpret_cast<const double4*>(base + o_tabs + runs[i].jac_off);
+            a.gsums = reinterpret_cast<double*>(base + o_gsums) + (long long)a.block_base * 3 * GT;
+        }
         if (widom) {
             a.partials = reinterpret_cast<double*>(base + o_part) + part_off[i];
         } else {
@@ -760,6 +788,13 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         std::memcpy(stage.data() + o_runs + i * sizeof(RunArgs), &runs[i].ra, sizeof(RunArgs));
         if (!runs[i].tab.empty())
             std::memcpy(stage.data() + o_tabs + runs[i].tab_off, runs[i].tab.data(), runs[i].tab.size());
+        if (grad) {
+            if (!runs[i].gtab.empty())
+                std::memcpy(stage.data() + o_tabs + runs[i].gtab_off, runs[i].gtab.data(), runs[i].gtab.size());
+            if (!runs[i].jac.empty())
+                std::memcpy(stage.data() + o_tabs + runs[i].jac_off, runs[i].jac.data(),
+                            runs[i].jac.size() * sizeof(double4));
+        }
     }
     if (!segs.empty()) std::memcpy(stage.data() + o_segs, segs.data(), segs.size() * sizeof(SegArgs));
     PM_HIP(hipMemcpyAsync(base, stage.data(), stage_bytes, hipMemcpyHostToDevice, g_ctx.stream));
@@ -790,6 +825,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     g.out_sr = reinterpret_cast<double*>(base + o_sr);
     g.out_lr = reinterpret_cast<double*>(base + o_lr);
     g.cand_pairs = d_cand;
+    g.out_g = grad ? reinterpret_cast<double*>(base + o_g) : nullptr;
     const size_t shm = (fp64 ? pair_lds_fixed<double>() : pair_lds_fixed<float>()) + std::max<size_t>(lj_lds, 16);
 
     PM_HIP(hipMemsetAsync(d_cand, 0, sizeof(unsigned long long), g_ctx.stream));
@@ -821,12 +857,22 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             pair_wg = n_cells + (gr.n_slots + WG - 1) / WG;  // >= the slices the scan counts
         }
         PM_HIP(hipEventRecord(g_ctx.evp0, g_ctx.stream));
-        if (fp64) pair_launch_t<double>(g, (int)pair_wg, shm);
-        else pair_launch_t<float>(g, (int)pair_wg, shm);
+        if (grad) {
+            if (fp64) pair_launch_t<double, true>(g, (int)pair_wg, shm);
+            else pair_launch_t<float, true>(g, (int)pair_wg, shm);
+        } else {
+            if (fp64) pair_launch_t<double, false>(g, (int)pair_wg, shm);
+            else pair_launch_t<float, false>(g, (int)pair_wg, shm);
+        }
         PM_HIP(hipEventRecord(g_ctx.evp1, g_ctx.stream));
         const long long slot_wg = (gr.n_slots + WG - 1) / WG;
         if (gr.es) hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
-        if (widom) hipLaunchKernelGGL(widom_combine_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
+        if (grad && gr.es)
+            hipLaunchKernelGGL(point_fourier_grad_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
+        if (widom && grad)
+            hipLaunchKernelGGL(widom_combine_grad_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
+        else if (widom)
+            hipLaunchKernelGGL(widom_combine_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
         else hipLaunchKernelGGL(grid_combine_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
         PM_HIP(hipGetLastError());
         if (gi + 1 < groups.size()) {
@@ -850,6 +896,9 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     if (widom) {
         PM_HIP(hipMemcpyAsync(out_sums, d_sums, (size_t)blocks_total * 3 * sizeof(double), hipMemcpyDeviceToHost,
                               g_ctx.stream));
+        if (grad)
+            PM_HIP(hipMemcpyAsync(out_gsums, base + o_gsums, (size_t)blocks_total * 3 * GT * sizeof(double),
+                                  hipMemcpyDeviceToHost, g_ctx.stream));
         for (const PointRun& r : runs) pairs += (double)r.ra.n_blocks * r.ra.n_per_block * r.ns * r.fw->n_rep;
     } else {
         tmp.resize(fp64 ? 0 : n_runs);
@@ -878,6 +927,73 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         for (int i = 0; i < n_runs; ++i)
             for (long long p = 0; p < runs[i].n_points; ++p) runs[i].host_out[p] = (double)tmp[i][p];
     return PM_OK;
+}
+
+// Per-type Fourier boxes of an Ewald handle (gradient runs), built once: the
+// structure factor of each framework species with unit charges on every atom
+// of that species, on the handle's kept k-vectors.
+int ensure_type_boxes(pm_ewald* ew, const pm_framework* fw) {
+    if (ew->d_box_t) return PM_OK;
+    if ((int)fw->species_global.size() > GT)
+        return fail(PM_ERR_ARG, "gradient runs support at most PM_MAX_GRAD_TYPES framework species");
+    if (2 * ew->Mc + 1 > FT / GT) return fail(PM_ERR_ARG, "k-vector box too wide for the gradient pass");
+    const size_t box_n = (size_t)(ew->Ma + 1) * (2 * ew->Mb + 1) * (2 * ew->Mc + 1);
+    std::vector<double4> ua((size_t)std::max<int64_t>(fw->n_unit, 1));
+    for (int64_t u = 0; u < fw->n_unit; ++u)
+        ua[(size_t)u] = make_double4(fw->unit_xf[3 * u], fw->unit_xf[3 * u + 1], fw->unit_xf[3 * u + 2],
+                                     (double)fw->unit_type[u]);
+    double4* d_ua = nullptr;
+    double2* d_bt = nullptr;
+    PM_HIP(hipStreamSynchronize(g_ctx.stream));
+    hipError_t e = hipMalloc(&d_bt, box_n * GT * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc(&d_ua, ua.size() * sizeof(double4));
+    if (e == hipSuccess) e = hipMemsetAsync(d_bt, 0, box_n * GT * sizeof(double2), g_ctx.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_ua, ua.data(), ua.size() * sizeof(double4), hipMemcpyHostToDevice, g_ctx.stream);
+    if (e == hipSuccess && ew->nk > 0 && fw->n_unit > 0) {
+        hipLaunchKernelGGL(type_structure_factor_kernel, dim3((unsigned)ew->nk), dim3(64), 0, g_ctx.stream, d_ua,
+                           (long long)fw->n_unit, ew->d_k, ew->Mb, ew->Mc, d_bt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.stream);
+    (void)hipFree(d_ua);
+    if (e != hipSuccess) {
+        (void)hipFree(d_bt);
+        return fail(PM_ERR_DEVICE, std::string("per-type structure factors: ") + hipGetErrorString(e));
+    }
+    ew->d_box_t = d_bt;
+    return PM_OK;
+}
+
+// Gradient tables of a prepared Widom run: per (compact type t, site s) the
+// factor 4 eps_ts / sigma_ts of d LJ / d sigma_ts (in the pair precision) and
+// the mixing-rule Jacobian from jac[(i * n + j) * 4 + k] (i framework
+// species, j probe species, k: d eps_ij/d eps_i, d eps_ij/d sigma_i,
+// d sigma_ij/d eps_i, d sigma_ij/d sigma_i).
+void fill_gradient_tables(PointRun& r, const double* jac, bool fp64) {
+    const pm_framework* fw = r.fw;
+    const pm_forcefield* ff = r.ff;
+    const int nt = (int)fw->species_global.size(), ns = r.ns, n = ff->n;
+    r.jac.assign((size_t)std::max(nt, 1) * ns, make_double4(0.0, 0.0, 0.0, 0.0));
+    std::vector<double> gc((size_t)std::max(nt, 1) * ns, 0.0);
+    for (int t = 0; t < nt; ++t)
+        for (int s = 0; s < ns; ++s) {
+            if (!r.pr->site[s].lj) continue;
+            const int i = fw->species_global[t], j = r.pr->lj_species[s];
+            const size_t idx = (size_t)i * n + j;
+            const double sg = std::sqrt(ff->sig2[idx]);
+            gc[(size_t)t * ns + s] = sg > 0.0 ? 4.0 * ff->eps[idx] / sg : 0.0;
+            const double* J = jac + 4 * idx;
+            r.jac[(size_t)t * ns + s] = make_double4(J[0], J[1], J[2], J[3]);
+        }
+    if (fp64) {
+        r.gtab.resize(gc.size() * sizeof(double));
+        std::memcpy(r.gtab.data(), gc.data(), r.gtab.size());
+    } else {
+        std::vector<float> f(gc.begin(), gc.end());
+        r.gtab.resize(f.size() * sizeof(float));
+        std::memcpy(r.gtab.data(), f.data(), r.gtab.size());
+    }
 }
 
 }  // namespace
@@ -1166,6 +1282,7 @@ int pm_ewald_destroy(pm_ewald* ew) {
     (void)hipFree(ew->d_k);
     (void)hipFree(ew->d_box);
     (void)hipFree(ew->d_erfc);
+    (void)hipFree(ew->d_box_t);
     delete ew;
     return PM_OK;
 }
@@ -1317,6 +1434,48 @@ int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew
         (void)hipFree(d_energies);
     }
     return rc;
+}
+
+int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+                      const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
+                      int64_t n_per_block, int32_t precision, const double* jac,
+                      double* out_block_sums, double* out_block_grads) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (!out_block_sums || !out_block_grads || !jac) return fail(PM_ERR_ARG, "pm_widom_gradient: null pointer");
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
+    pm_widom_run in{};
+    in.framework = fw;
+    in.forcefield = ff;
+    in.ewald = ew;
+    in.probe = pr;
+    in.temperature = temperature;
+    in.seed = seed;
+    in.n_per_block = n_per_block;
+    in.n_blocks = n_blocks;
+    std::vector<PointRun> runs(1);
+    if (int rc = widom_prepare(in, runs[0])) return rc;
+    PointRun& r = runs[0];
+    const int nt = (int)r.fw->species_global.size();
+    if (nt > GT) return fail(PM_ERR_ARG, "pm_widom_gradient: more than PM_MAX_GRAD_TYPES framework species");
+    for (int64_t k = 0; k < (int64_t)ff->n * ff->n * 4; ++k)
+        if (!std::isfinite(jac[k])) return fail(PM_ERR_ARG, "pm_widom_gradient: non-finite Jacobian entry");
+    if (r.es) {
+        if (int rc = ensure_type_boxes(const_cast<pm_ewald*>(r.ew), r.fw)) return rc;
+        r.ra.A_t = r.ew->d_box_t;
+    }
+    r.ra.n_gt = nt;
+    fill_gradient_tables(r, jac, precision == PM_FP64);
+    std::vector<double> gs((size_t)n_blocks * 3 * GT, 0.0);
+    if (int rc = execute_points(runs, MODE_WIDOM, precision, out_block_sums, true, gs.data())) return rc;
+    for (int64_t k = 0; k < (int64_t)n_blocks * ff->n * 3; ++k) out_block_grads[k] = 0.0;
+    for (int b = 0; b < n_blocks; ++b)
+        for (int t = 0; t < nt; ++t)
+            for (int q = 0; q < 3; ++q)
+                out_block_grads[((size_t)b * ff->n + r.fw->species_global[t]) * 3 + q] =
+                    gs[(size_t)b * 3 * GT + 3 * t + q];
+    return PM_OK;
 }
 
 int pm_widom_batch(int32_t n_runs, const pm_widom_run* runs_in, int32_t precision, double* out_block_sums) {

@@ -296,3 +296,21 @@ def energy_grid_batch(runs, precision):
         r.out = dptr(out)
     check(lib.pm_energy_grid_batch(len(runs), arr, precision_code(precision)))
     return outs
+
+
+def widom_gradient(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision, ljff):
+    """Block sums (n_blocks, 3) and Boltzmann-weighted parameter-derivative
+    sums (n_blocks, n_species, 3) of a Widom run (pm_widom_gradient):
+    grads[b, t] = sum_i w_i dE_i/d(eps_t, sigma_t, q_t) for framework species t
+    of ``ljff`` (its mixing rule supplies the Jacobian)."""
+    lib = _lib.load()
+    ffh = _ff_handle(ff if ff is not None else ljff)
+    n = len(ljff.species)
+    jac = np.ascontiguousarray(ljff.mixing_jacobian(), dtype=np.float64)
+    sums = np.zeros(3 * n_blocks)
+    grads = np.zeros(n_blocks * n * 3)
+    check(lib.pm_widom_gradient(fw.ptr, ffh.ptr, ew.ptr if ew is not None else None, pr.ptr,
+                                float(temperature), int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_blocks),
+                                int(n_per_block), precision_code(precision), dptr(jac), dptr(sums),
+                                dptr(grads)))
+    return sums.reshape(n_blocks, 3), grads.reshape(n_blocks, n, 3)

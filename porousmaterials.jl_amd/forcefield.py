@@ -9,6 +9,7 @@ The dense per-pair tables handed to the device library are built here.
 from __future__ import annotations
 
 import csv
+import cmath
 import math
 import os
 
@@ -34,23 +35,8 @@ class LJForceField:
         sig2 = np.zeros((n, n))
         for i, a in enumerate(self.species):
             for j, b in enumerate(self.species):
-                ea, eb = self.pure_epsilon[a], self.pure_epsilon[b]
-                sa, sb = self.pure_sigma[a], self.pure_sigma[b]
-                if rule == "lorentz-berthelot":
-                    e = math.sqrt(ea * eb)
-                    s2 = ((sa + sb) / 2.0) ** 2
-                elif rule == "kong":
-                    es6_a = ea * sa ** 6
-                    es6_b = eb * sb ** 6
-                    es12_a = es6_a * sa ** 6
-                    es12_b = es6_b * sb ** 6
-                    es6 = math.sqrt(es6_a * es6_b)
-                    es12 = ((es12_a ** (1 / 13) + es12_b ** (1 / 13)) / 2) ** 13
-                    e = es6 ** 2 / es12
-                    s2 = (es12 / es6) ** (1 / 3)
-                else:
-                    e = math.sqrt(ea * eb)
-                    s2 = sa * sb
+                e, s2 = _mix(rule, self.pure_epsilon[a], self.pure_sigma[a], self.pure_epsilon[b],
+                             self.pure_sigma[b])
                 eps[i, j] = e
                 sig2[i, j] = s2
         self.epsilon_table = eps
@@ -72,6 +58,29 @@ class LJForceField:
         name = os.path.splitext(os.path.basename(path))[0]
         return cls(name, sig, eps, r_cutoff, mixing_rules)
 
+    def mixing_jacobian(self) -> np.ndarray:
+        """(n, n, 4) derivatives of the mixed pair parameters with respect to
+        the first species' pure parameters, the second species' held fixed:
+        d eps_ij/d eps_i, d eps_ij/d sigma_i, d sigma_ij/d eps_i,
+        d sigma_ij/d sigma_i (sigma_ij = sqrt(sigma2_ij)).  Complex-step
+        differentiation of the mixing formulas (exact to rounding: every rule
+        is analytic in the pure parameters)."""
+        rule = self.mixing_rules.lower()
+        n = len(self.species)
+        jac = np.zeros((n, n, 4))
+        h = 1e-30
+        for i, a in enumerate(self.species):
+            ea, sa = self.pure_epsilon[a], self.pure_sigma[a]
+            for j, b in enumerate(self.species):
+                eb, sb = self.pure_epsilon[b], self.pure_sigma[b]
+                e1, s21 = _mix(rule, complex(ea, h), sa, eb, sb)
+                e2, s22 = _mix(rule, ea, complex(sa, h), eb, sb)
+                jac[i, j, 0] = e1.imag / h
+                jac[i, j, 1] = e2.imag / h
+                jac[i, j, 2] = cmath.sqrt(s21).imag / h
+                jac[i, j, 3] = cmath.sqrt(s22).imag / h
+        return jac
+
     def index(self, species: str) -> int:
         try:
             return self._index[species]
@@ -87,6 +96,23 @@ class LJForceField:
     def __repr__(self) -> str:
         return (f"LJForceField({self.name!r}, {len(self.species)} species, "
                 f"r_cutoff={math.sqrt(self.r2_cutoff):.4f} A, {self.mixing_rules})")
+
+
+def _mix(rule, ea, sa, eb, sb):
+    """(epsilon_ij, sigma2_ij) of src/forcefield.jl:75-92; complex-safe."""
+    if rule == "lorentz-berthelot":
+        return cmath.sqrt(ea * eb) if isinstance(ea * eb, complex) else math.sqrt(ea * eb), ((sa + sb) / 2.0) ** 2
+    if rule == "kong":
+        es6_a = ea * sa ** 6
+        es6_b = eb * sb ** 6
+        es12_a = es6_a * sa ** 6
+        es12_b = es6_b * sb ** 6
+        root = cmath.sqrt if isinstance(es6_a * es6_b, complex) else math.sqrt
+        es6 = root(es6_a * es6_b)
+        es12 = ((es12_a ** (1 / 13) + es12_b ** (1 / 13)) / 2) ** 13
+        return es6 ** 2 / es12, (es12 / es6) ** (1 / 3)
+    root = cmath.sqrt if isinstance(ea * eb, complex) else math.sqrt
+    return root(ea * eb), sa * sb
 
 
 def forcefield_coverage(species, ljff: LJForceField) -> bool:

@@ -176,3 +176,34 @@ def henry_coefficients(pairs, temperature, ljff, insertions_per_volume=200, verb
                 nb_per_block, rho, t0, verbose, autosave, filename_comment)
         out.append((results, sums) if return_block_sums else results)
     return out
+
+
+def henry_coefficient_gradient(crystal, molecule, temperature, ljff, insertions_per_volume=200,
+                               ewald_precision=1e-6, seed=0, precision="mixed", verbose=False):
+    """K_H (the ``henry_coefficient`` results dict) and its logarithmic
+    derivatives with respect to the framework species' pure LJ parameters and
+    a uniform shift of their charges, from one adjoint device pass
+    (pm_widom_gradient):
+
+        d ln K_H / d theta = -(1/T) sum_b sum_{i in b} w_i dE_i/d theta / sum_b S_b
+
+    (K_H = mean_b S_b / (R T n_b), ``src/henry.jl:112-135``, differentiated).
+    Returns (results, grad) with grad[species] = {"epsilon": ..., "sigma": ...,
+    "charge": ...} for every species in the framework; epsilon and sigma are
+    the framework side's parameters, the probe's held fixed."""
+    t0 = time.time()
+    fw, ff, ew, pr, nb_per_block, rho = _widom_setup(crystal, molecule, temperature, ljff,
+                                                     insertions_per_volume, ewald_precision, verbose)
+    sums, grads = device.widom_gradient(fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block,
+                                        precision, ljff)
+    results = block_statistics(sums, nb_per_block, temperature, rho)
+    results["xtal"] = crystal.name
+    results["elapsed time (min)"] = (time.time() - t0) / 60.0
+    s_tot = float(np.sum(sums[:, 0]))
+    g_tot = grads.sum(axis=0)
+    grad = {}
+    for sp in sorted(set(crystal.species), key=ljff.index):
+        k = ljff.index(sp)
+        d = -g_tot[k] / (temperature * s_tot)
+        grad[sp] = {"epsilon": float(d[0]), "sigma": float(d[1]), "charge": float(d[2])}
+    return results, grad

@@ -1,0 +1,129 @@
+"""Adjoint Widom pass (pm_widom_gradient) on the CO2 workload's framework.
+
+d ln K_H / d theta for theta = (eps_t, sigma_t, q_t) of the 5 framework
+species is checked against central finite differences (h = 1e-4 relative) of
+the oracle's FP64 ln K_H on the same insertion stream, to 1e-4 relative
+(floor: 1e-6 of the largest component).  q_t is a uniform shift of every
+atom of species t (h = 1e-4 of their mean |q|).  The pass's block sums must
+equal pm_widom's bit for bit."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+T = 298.0
+N_BLOCKS, NB = 5, 400
+SEED = 2024
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def adj(pm):
+    case = pm.synthetic.co2_widom_case()
+    x, ff = case.crystal, case.ljff
+    rep = pm.replication_factors(x.box, ff)
+    xr = pm.replicate(x, rep)
+    ep = pm.setup_Ewald_sum(xr.box, math.sqrt(ff.r2_cutoff), eps=case.ewald_precision, verbose=False)
+    fw = pm.device.framework(xr, ff)
+    ew = pm.device.ewald(xr, fw, ep)
+    pr = pm.device.probe(case.probes["CO2"], ff)
+    return case, fw, ew, pr
+
+
+def _oracle_lnS(orc, pm, case, ljff, q_atoms):
+    x, co2 = case.crystal, case.probes["CO2"]
+    lj_off, q_off = pm.site_offsets(co2)
+    o = orc.OracleSystem((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
+                         [ljff.index(s) for s in x.species], q_atoms, ljff.epsilon_table, ljff.sigma2_table,
+                         math.sqrt(ljff.r2_cutoff), lj_off.T, [ljff.index(s) for s in co2.atom_species], q_off.T,
+                         co2.charges_q, ewald_precision=case.ewald_precision)
+    v, e = o.widom_energies(SEED, 0, N_BLOCKS * NB)
+    E = v + e
+    fin = np.isfinite(E)
+    return math.log(np.sum(np.exp(-E[fin] / T)))
+
+
+def _perturbed_ff(pm, ff, species, which, factor):
+    eps = dict(ff.pure_epsilon)
+    sig = dict(ff.pure_sigma)
+    if which == "epsilon":
+        eps[species] *= factor
+    else:
+        sig[species] *= factor
+    return pm.LJForceField(ff.name, sig, eps, math.sqrt(ff.r2_cutoff), ff.mixing_rules)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gradient_pass_block_sums_equal_pm_widom(pm, adj, precision):
+    say(f"case: pm_widom_gradient block sums vs pm_widom, CO2 5 x 3000 ({precision})")
+    case, fw, ew, pr = adj
+    s_g, _ = pm.device.widom_gradient(fw, case.ljff, ew, pr, T, 7, 5, 3000, precision, case.ljff)
+    s_w = pm.device.widom(fw, case.ljff, ew, pr, T, 7, 5, 3000, precision)
+    assert np.array_equal(s_g, s_w)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dlnKH_matches_oracle_finite_differences(pm, orc, adj, precision):
+    say(f"case: d ln K_H / d(eps_t, sigma_t, q_t), CO2 5 x {NB} insertions, vs oracle central FD ({precision})")
+    case, fw, ew, pr = adj
+    ff, x = case.ljff, case.crystal
+    sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, T, SEED, N_BLOCKS, NB, precision, ff)
+    S = sums[:, 0].sum()
+    assert S > 0.0
+    g_dev = -grads.sum(axis=0) / (T * S)  # (n_species, 3)
+    species = sorted(set(x.species), key=ff.index)
+    h = 1e-4
+    q0 = np.array(x.atom_q, dtype=float)
+    fd, dev = [], []
+    for sp in species:
+        k = ff.index(sp)
+        for j, which in enumerate(("epsilon", "sigma")):
+            p0 = ff.pure_epsilon[sp] if which == "epsilon" else ff.pure_sigma[sp]
+            lp = _oracle_lnS(orc, pm, case, _perturbed_ff(pm, ff, sp, which, 1 + h), q0)
+            lm = _oracle_lnS(orc, pm, case, _perturbed_ff(pm, ff, sp, which, 1 - h), q0)
+            fd.append((lp - lm) / (2 * h * p0))
+            dev.append(g_dev[k, j])
+        mask = np.array([s == sp for s in x.species])
+        hq = h * float(np.mean(np.abs(q0[mask])))
+        lp = _oracle_lnS(orc, pm, case, ff, q0 + hq * mask)
+        lm = _oracle_lnS(orc, pm, case, ff, q0 - hq * mask)
+        fd.append((lp - lm) / (2 * hq))
+        dev.append(g_dev[k, 2])
+    fd, dev = np.array(fd), np.array(dev)
+    floor = 1e-6 * np.max(np.abs(fd))
+    err = np.abs(dev - fd) / np.maximum(np.abs(fd), floor)
+    for sp_i, sp in enumerate(species):
+        say(f"   {sp}: d/d eps {dev[3 * sp_i]:+.6e} (FD {fd[3 * sp_i]:+.6e}), d/d sigma {dev[3 * sp_i + 1]:+.6e} "
+            f"(FD {fd[3 * sp_i + 1]:+.6e}), d/d q {dev[3 * sp_i + 2]:+.6e} (FD {fd[3 * sp_i + 2]:+.6e})")
+    say(f"   max relative deviation from FD: {err.max():.2e}")
+    assert err.max() <= 1e-4
+
+
+def test_henry_coefficient_gradient_api(pm):
+    """The host call surface on a real CIF: ZIF-71 (bogus charges) with
+    CO2-EPM2 (test/electrostatics.jl:140-153 setting), mixed vs FP64 within
+    1e-4 for every component larger than 1e-3 of the largest."""
+    from conftest import golden
+    say("case: henry_coefficient_gradient, ZIF-71 / CO2-EPM2, mixed vs FP64")
+    x = pm.read_cif(golden("crystals", "zif71_bogus_charges.cif"))
+    pm.strip_numbers_from_atom_labels(x)
+    ff = pm.LJForceField.from_csv(golden("forcefields", "Greg_bogus_ZIF71.csv"), r_cutoff=12.8)
+    m = pm.Molecule.from_dir(golden("molecules", "CO2EPM2"))
+    r64, g64 = pm.henry_coefficient_gradient(x, m, 298.0, ff, insertions_per_volume=20, seed=3, precision="fp64")
+    r32, g32 = pm.henry_coefficient_gradient(x, m, 298.0, ff, insertions_per_volume=20, seed=3, precision="mixed")
+    want = pm.henry_coefficient(x, m, 298.0, ff, insertions_per_volume=20, seed=3, precision="fp64",
+                                verbose=False, autosave=False)
+    k = "henry coefficient [mol/(kg-Pa)]"
+    assert r64[k] == want[k]
+    a = np.array([[g64[s][p] for p in ("epsilon", "sigma", "charge")] for s in g64])
+    b = np.array([[g32[s][p] for p in ("epsilon", "sigma", "charge")] for s in g32])
+    big = np.abs(a) > 1e-3 * np.abs(a).max()
+    say(f"   species {list(g64)}; max mixed-vs-FP64 relative deviation {np.max(np.abs(a - b)[big] / np.abs(a)[big]):.2e}")
+    assert np.all(np.abs(a - b)[big] <= 1e-4 * np.abs(a)[big])

@@ -211,3 +211,28 @@ def test_batch_structs_match_the_c_layout(tmp_path):
         want.append(ctypes.sizeof(cls))
         want += [getattr(cls, f).offset for f, _ in cls._fields_]
     assert got == want
+
+
+@pytest.mark.parametrize("rule", ["Lorentz-Berthelot", "geometric", "Kong"])
+def test_mixing_jacobian_matches_finite_differences(pm, rule):
+    """LJForceField.mixing_jacobian (the adjoint's chain rule through the
+    mixing rule) vs central differences of the mixed tables."""
+    sig = {"A": 3.1, "B": 2.7, "C": 3.6}
+    eps = {"A": 40.0, "B": 70.0, "C": 25.0}
+    ff = pm.LJForceField("t", sig, eps, 12.0, rule)
+    J = ff.mixing_jacobian()
+    h = 1e-6
+    for i, a in enumerate(ff.species):
+        for k, (which, p0) in enumerate((("eps", eps[a]), ("sig", sig[a]))):
+            tabs = []
+            for f in (1 + h, 1 - h):
+                e2, s2 = dict(eps), dict(sig)
+                (e2 if which == "eps" else s2)[a] = p0 * f
+                g = pm.LJForceField("t", s2, e2, 12.0, rule)
+                tabs.append((g.epsilon_table[i], np.sqrt(g.sigma2_table[i])))
+            d_eps = (tabs[0][0] - tabs[1][0]) / (2 * h * p0)
+            d_sig = (tabs[0][1] - tabs[1][1]) / (2 * h * p0)
+            # the diagonal pair moves with both of its species: compare off-diagonal pairs
+            off = np.arange(len(ff.species)) != i
+            np.testing.assert_allclose(J[i, off, k], d_eps[off], rtol=1e-7, atol=1e-9)
+            np.testing.assert_allclose(J[i, off, 2 + k], d_sig[off], rtol=1e-7, atol=1e-9)
