@@ -138,6 +138,20 @@ int pm_electrostatic_energy(const pm_framework* fw, const pm_ewald* ew, int32_t 
                             int32_t precision, double* out_sr, double* out_lr);
 
 /*
+ * Energies and forces of n_poses rigid poses: per pose out_e = (vdw, sr, lr)
+ * as pm_vdw_energy / pm_electrostatic_energy give them, and out_grad
+ * (n_poses x n_sites x 3) = dE/d(site Cartesian xyz) in K/A of the pose's
+ * total energy E = vdw + sr + lr.  Sites carry an LJ species (-1: none) and a
+ * charge (0: none); site_xf as for pm_vdw_energy.  Replaces the derivative-free
+ * objective of `find_energy_minimum` (src/energy_min.jl:19-58, Nelder-Mead on
+ * `vdw_energy`), so a gradient-based minimiser can use LJ + Ewald forces.
+ * Gradients are undefined for poses whose energy is +Inf (overlap).
+ */
+int pm_pose_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, int32_t n_sites,
+                     const int32_t* site_species, const double* site_charges, int64_t n_poses,
+                     const double* site_xf, int32_t precision, double* out_e, double* out_grad);
+
+/*
  * Widom insertions (src/henry.jl:185-265): n_blocks x n_per_block insertions
  * of the probe, insertion g = b*n_per_block + i drawing its centre of mass
  * (uniform in the replicated box) and an absolute Arvo rotation from the

@@ -15,6 +15,7 @@ from .energetics import (electrostatic_energies, electrostatic_potential_energy,
 from .ewald import (EPS0, FOUR_PI_EPS0, Eikr, EwaldParams, EwaldSum, determine_ewald_params,
                     make_eikr, precompute_kvec_wts, required_kreps, setup_Ewald_sum, total)
 from .forcefield import LJForceField, forcefield_coverage, lennard_jones, replication_factors
+from .energy_min import find_energy_minima, find_energy_minimum, find_energy_minimum_gridsearch
 from .grid import Grid, energy_grid, energy_grids, required_n_pts
 from .henry import (K_TO_KJ_PER_MOL, N_BLOCKS, UNIV_GAS_CONST, block_statistics, henry_coefficient,
                     henry_coefficient_gradient, henry_coefficients,

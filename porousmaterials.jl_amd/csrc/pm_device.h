@@ -189,6 +189,7 @@ struct PoseArgs {
     int ns, n_atoms, n_types, nk;
     long long n_items;
     double f11, f12, f13, f22, f23, f33;  // replicated-box f_to_c (upper triangular)
+    double c2f[9];                        // replicated-box c_to_f, column-major (forces)
     double repd[3];                       // replication factors (unit-cell phases)
     double r2_lj, r2_sr, alpha, inv4pe;
     Site site[PM_MAX_SITES];
@@ -198,9 +199,16 @@ struct PoseArgs {
     const KVec* kv;
     const double* pose_xf; // n_items x ns x 3
     double* out_e;         // n_items x 3: (vdw, sr, lr)
+    double* out_g;         // GRAD: n_items x ns x 3, dE/d(site Cartesian xyz) in K/A
 };
 
-template <int NS, bool ES, typename Real>
+// One work-item per pose.  GRAD adds the gradient of the pose's energy with
+// respect to every site's Cartesian position:
+//   LJ      d/dx_s = 6 eps4 (2u^2 - u) / r^2 * d,   u = (sigma^2 / r^2)^3
+//   real    d/dx_s = q_s q_a [erfc(a r) / r + 2a/sqrt(pi) e^{-a^2 r^2}] / r^2 * d / (4 pi eps0)
+//   Fourier d/dx_s = q_s 2 pi C^T sum_k (-A_re sin th + A_im cos th) (m o rep),  th = 2 pi m.(rep o x)
+// with d = x_atom - x_s (nearest image, Cartesian) and C the replicated box's c_to_f.
+template <int NS, bool ES, typename Real, bool GRAD>
 __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int ATOM_BYTES = TILE * (int)sizeof(double4);
@@ -226,11 +234,16 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
     }
 
     double acc_lj[NS], acc_sr[NS];
+    double gx[NS], gy[NS], gz[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) { acc_lj[s] = 0.0; acc_sr[s] = 0.0; }
+    for (int s = 0; s < NS; ++s) {
+        acc_lj[s] = 0.0; acc_sr[s] = 0.0;
+        gx[s] = 0.0; gy[s] = 0.0; gz[s] = 0.0;
+    }
     bool ovl = false;
     const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
     const double r2lj = a.r2_lj, r2sr = a.r2_sr, alpha = a.alpha;
+    const double two_a_rpi = 2.0 * alpha * 0.56418958354775628;  // 2 alpha / sqrt(pi)
     for (int base = 0; base < a.n_atoms; base += TILE) {
         int nt = min(TILE, a.n_atoms - base);
         __syncthreads();
@@ -257,18 +270,38 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
                 ovl |= in & (r2 < R2_OVERLAP) & (a.site[s].lj != 0);
                 Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
                 acc_lj[s] += in ? (double)e : 0.0;
+                if (GRAD) {
+                    // 6 eps4 (2u^2 - u) / r^2
+                    const double rr = 1.0 / r2;
+                    const double sg = (double)p.sig2 * rr;
+                    const double u = sg * sg * sg;
+                    const double f = in ? 6.0 * (double)p.eps4 * u * (2.0 * u - 1.0) * rr : 0.0;
+                    gx[s] += f * cx;
+                    gy[s] += f * cy;
+                    gz[s] += f * cz;
+                }
                 if (ES) {
                     bool ins = (r2 < r2sr) & (at.w != 0.0);
                     Real c = PairMath<Real>::coul(r2, at.w, alpha);
                     acc_sr[s] += ins ? (double)c : 0.0;
+                    if (GRAD) {
+                        const double r = sqrt(r2);
+                        const double f = ins ? at.w * (erfc(alpha * r) / r + two_a_rpi * exp(-alpha * alpha * r2)) /
+                                                   r2 * a.site[s].q * a.inv4pe
+                                             : 0.0;
+                        gx[s] += f * cx;
+                        gy[s] += f * cy;
+                        gz[s] += f * cz;
+                    }
                 }
             }
         }
     }
 
     double acc_lr[NS];
+    double hx[NS], hy[NS], hz[NS];  // Fourier gradient in (m o rep) space
 #pragma unroll
-    for (int s = 0; s < NS; ++s) acc_lr[s] = 0.0;
+    for (int s = 0; s < NS; ++s) { acc_lr[s] = 0.0; hx[s] = 0.0; hy[s] = 0.0; hz[s] = 0.0; }
     if (ES) {
         double ux[NS], uy[NS], uz[NS];
 #pragma unroll
@@ -292,6 +325,12 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
                         double sn, cs;
                         sincospi(2.0 * t, &sn, &cs);
                         acc_lr[s] += kv.are * cs + kv.aim * sn;
+                        if (GRAD) {
+                            const double d = kv.aim * cs - kv.are * sn;
+                            hx[s] += d * kv.ma;
+                            hy[s] += d * kv.mb;
+                            hz[s] += d * kv.mc;
+                        }
                     }
                 }
             }
@@ -310,6 +349,19 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
         a.out_e[3 * item + 0] = vdw;
         a.out_e[3 * item + 1] = sr * a.inv4pe;
         a.out_e[3 * item + 2] = lr;
+        if (GRAD) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                // Fourier: 2 pi q_s C^T (h o rep)
+                const double q2p = 2.0 * M_PI * a.site[s].q;
+                const double vx = hx[s] * a.repd[0] * q2p, vy = hy[s] * a.repd[1] * q2p, vz = hz[s] * a.repd[2] * q2p;
+                const double* C = a.c2f;
+                double* o = a.out_g + (item * NS + s) * 3;
+                o[0] = gx[s] + C[0] * vx + C[1] * vy + C[2] * vz;
+                o[1] = gy[s] + C[3] * vx + C[4] * vy + C[5] * vz;
+                o[2] = gz[s] + C[6] * vx + C[7] * vy + C[8] * vz;
+            }
+        }
     }
 }
 

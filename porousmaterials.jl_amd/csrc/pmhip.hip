@@ -259,37 +259,45 @@ size_t pose_lds_bytes(int n_types, int ns, bool fp64) {
     return region + TILE * sizeof(int) + (size_t)std::max(n_types, 1) * ns * (fp64 ? 16 : 8);
 }
 
-template <int NS, bool ES, typename Real>
+template <int NS, bool ES, typename Real, bool GRAD>
 void pose_launch_t(const PoseArgs& a, int n_wg, size_t shm) {
-    hipLaunchKernelGGL((pose_kernel<NS, ES, Real>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
+    hipLaunchKernelGGL((pose_kernel<NS, ES, Real, GRAD>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, a);
 }
 
-template <bool ES, typename Real>
+template <bool ES, typename Real, bool GRAD>
 int pose_launch_ns(const PoseArgs& a, int n_wg, size_t shm) {
     switch (a.ns) {
-        case 1: pose_launch_t<1, ES, Real>(a, n_wg, shm); break;
-        case 2: pose_launch_t<2, ES, Real>(a, n_wg, shm); break;
-        case 3: pose_launch_t<3, ES, Real>(a, n_wg, shm); break;
-        case 4: pose_launch_t<4, ES, Real>(a, n_wg, shm); break;
-        case 5: pose_launch_t<5, ES, Real>(a, n_wg, shm); break;
-        case 6: pose_launch_t<6, ES, Real>(a, n_wg, shm); break;
-        case 8: pose_launch_t<8, ES, Real>(a, n_wg, shm); break;
+        case 1: pose_launch_t<1, ES, Real, GRAD>(a, n_wg, shm); break;
+        case 2: pose_launch_t<2, ES, Real, GRAD>(a, n_wg, shm); break;
+        case 3: pose_launch_t<3, ES, Real, GRAD>(a, n_wg, shm); break;
+        case 4: pose_launch_t<4, ES, Real, GRAD>(a, n_wg, shm); break;
+        case 5: pose_launch_t<5, ES, Real, GRAD>(a, n_wg, shm); break;
+        case 6: pose_launch_t<6, ES, Real, GRAD>(a, n_wg, shm); break;
+        case 8: pose_launch_t<8, ES, Real, GRAD>(a, n_wg, shm); break;
         default: return fail(PM_ERR_ARG, "unsupported site count " + std::to_string(a.ns));
     }
     return PM_OK;
 }
 
-// Explicit-pose driver behind pm_vdw_energy and pm_electrostatic_energy:
-// out3 receives (vdw, sr, lr) per pose.
+template <bool GRAD>
+int pose_launch(const PoseArgs& a, int n_wg, size_t shm, bool fp64, bool es) {
+    if (fp64) return es ? pose_launch_ns<true, double, GRAD>(a, n_wg, shm) : pose_launch_ns<false, double, GRAD>(a, n_wg, shm);
+    return es ? pose_launch_ns<true, float, GRAD>(a, n_wg, shm) : pose_launch_ns<false, float, GRAD>(a, n_wg, shm);
+}
+
+// Explicit-pose driver behind pm_vdw_energy, pm_electrostatic_energy and
+// pm_pose_gradient: out3 receives (vdw, sr, lr) per pose; out_grad (if not
+// null) n_poses x ns x 3 site gradients.
 int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, PoseArgs a,
               const int32_t* site_species, int precision, bool es, const double* site_xf,
-              std::vector<double>& out3) {
+              std::vector<double>& out3, double* out_grad = nullptr) {
     if (precision != PM_FP64 && precision != PM_MIXED)
         return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
     const int64_t n = a.n_items;
     a.f11 = fw->F[0]; a.f12 = fw->F[3]; a.f13 = fw->F[6];
     a.f22 = fw->F[4]; a.f23 = fw->F[7]; a.f33 = fw->F[8];
     for (int i = 0; i < 3; ++i) a.repd[i] = (double)fw->rep[i];
+    for (int i = 0; i < 9; ++i) a.c2f[i] = fw->c2f[i];
     a.n_atoms = (int)fw->n_rep;
     a.n_types = (int)fw->species_global.size();
     a.atoms = fw->d_atoms;
@@ -313,6 +321,7 @@ int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* e
     size_t o_tab = ar.take(tab_bytes);
     size_t o_pose = ar.take((size_t)n * a.ns * 3 * sizeof(double));
     size_t o_out = ar.take((size_t)n * 3 * sizeof(double));
+    size_t o_grad = ar.take(out_grad ? (size_t)n * a.ns * 3 * sizeof(double) : 0);
     if (int rc = ensure_scratch(ar.used)) return rc;
     unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
     if (int rc = copy_ljtab(base + o_tab)) return rc;
@@ -323,19 +332,21 @@ int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* e
                           g_ctx.stream));
     a.pose_xf = d_pose;
     a.out_e = d_out;
+    a.out_g = out_grad ? reinterpret_cast<double*>(base + o_grad) : nullptr;
     const bool fp64 = precision == PM_FP64;
     const size_t shm = pose_lds_bytes(a.n_types, a.ns, fp64);
     const int n_wg = (int)((n + WG - 1) / WG);
     PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
-    int rc;
-    if (fp64) rc = es ? pose_launch_ns<true, double>(a, n_wg, shm) : pose_launch_ns<false, double>(a, n_wg, shm);
-    else rc = es ? pose_launch_ns<true, float>(a, n_wg, shm) : pose_launch_ns<false, float>(a, n_wg, shm);
+    const int rc = out_grad ? pose_launch<true>(a, n_wg, shm, fp64, es) : pose_launch<false>(a, n_wg, shm, fp64, es);
     if (rc != PM_OK) return rc;
     PM_HIP(hipGetLastError());
     PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
     out3.resize((size_t)n * 3);
     PM_HIP(hipMemcpyAsync(out3.data(), d_out, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost,
                           g_ctx.stream));
+    if (out_grad)
+        PM_HIP(hipMemcpyAsync(out_grad, a.out_g, (size_t)n * a.ns * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                              g_ctx.stream));
     return finish_timing((double)n * a.ns * fw->n_rep);
 }
 
@@ -1394,6 +1405,48 @@ int pm_electrostatic_energy(const pm_framework* fw, const pm_ewald* ew, int32_t 
         out_sr[i] = out3[3 * i + 1];
         out_lr[i] = out3[3 * i + 2];
     }
+    return PM_OK;
+}
+
+int pm_pose_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, int32_t n_sites,
+                     const int32_t* site_species, const double* site_charges, int64_t n_poses,
+                     const double* site_xf, int32_t precision, double* out_e, double* out_grad) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (!fw || !out_e || !out_grad || n_poses < 0 || n_sites < 1 || n_sites > PM_MAX_SITES || n_sites == 7 ||
+        !site_species || !site_charges || (n_poses > 0 && !site_xf))
+        return fail(PM_ERR_ARG, "pm_pose_gradient: bad handle, pointer or size (sites in 1..6 or 8)");
+    bool any_lj = false, any_q = false;
+    for (int s = 0; s < n_sites; ++s) {
+        if (site_species[s] >= 0) any_lj = true;
+        if (!std::isfinite(site_charges[s])) return fail(PM_ERR_ARG, "pm_pose_gradient: non-finite charge");
+        if (site_charges[s] != 0.0) any_q = true;
+    }
+    if (any_lj) {
+        if (!ff) return fail(PM_ERR_ARG, "pm_pose_gradient: LJ sites need a force field");
+        if (int rc = check_species(fw, ff)) return rc;
+        for (int s = 0; s < n_sites; ++s)
+            if (site_species[s] >= ff->n)
+                return fail(PM_ERR_ARG, "pm_pose_gradient: site species outside the force field");
+    }
+    const bool es = fw->charged && any_q;
+    if (es && !ew) return fail(PM_ERR_ARG, "pm_pose_gradient: charged sites need an Ewald handle");
+    if (n_poses == 0) return PM_OK;
+    if (n_poses > (int64_t)INT32_MAX - WG) return fail(PM_ERR_ARG, "pm_pose_gradient: too many poses");
+    PoseArgs a{};
+    a.ns = n_sites;
+    a.n_items = n_poses;
+    std::vector<int32_t> sp(n_sites, 0);
+    for (int s = 0; s < n_sites; ++s) {
+        a.site[s].lj = site_species[s] >= 0 ? 1 : 0;
+        a.site[s].q = site_charges[s];
+        sp[s] = site_species[s] >= 0 ? site_species[s] : 0;
+    }
+    std::vector<double> out3;
+    if (int rc = run_poses(fw, any_lj ? ff : nullptr, es ? ew : nullptr, a, sp.data(), precision, es, site_xf, out3,
+                           out_grad))
+        return rc;
+    std::memcpy(out_e, out3.data(), out3.size() * sizeof(double));
     return PM_OK;
 }
 

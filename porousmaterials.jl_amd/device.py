@@ -314,3 +314,29 @@ def widom_gradient(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, pre
                                 int(n_per_block), precision_code(precision), dptr(jac), dptr(sums),
                                 dptr(grads)))
     return sums.reshape(n_blocks, 3), grads.reshape(n_blocks, n, 3)
+
+
+def pose_gradients(fw, ff, ew, site_species, site_charges, site_xf, precision):
+    """Energies (n_poses, 3: vdw, sr, lr) and dE/d(site Cartesian xyz)
+    (n_poses, n_sites, 3) in K/A of rigid poses (pm_pose_gradient).
+    ``site_species``: force-field index per site or -1 (no LJ sphere);
+    ``site_charges``: charge per site (0: none); ``site_xf``: (n_poses,
+    n_sites, 3) replicated-box fractional coordinates."""
+    lib = _lib.load()
+    ffh = _ff_handle(ff)
+    site_xf = np.asarray(site_xf, dtype=np.float64)
+    n_poses, n_sites = site_xf.shape[0], site_xf.shape[1]
+    energies = np.zeros((n_poses, 3))
+    grads = np.zeros((n_poses, n_sites, 3))
+    for a, b in _chunks(n_sites):
+        x = np.ascontiguousarray(site_xf[:, a:b, :])
+        sp = np.ascontiguousarray(np.asarray(site_species[a:b], dtype=np.int32))
+        q = np.ascontiguousarray(np.asarray(site_charges[a:b], dtype=np.float64))
+        e = np.zeros(3 * n_poses)
+        g = np.zeros(n_poses * (b - a) * 3)
+        check(lib.pm_pose_gradient(fw.ptr, ffh.ptr if ffh is not None else None,
+                                   ew.ptr if ew is not None else None, b - a, iptr(sp), dptr(q), n_poses,
+                                   dptr(x), precision_code(precision), dptr(e), dptr(g)))
+        energies += e.reshape(n_poses, 3)
+        grads[:, a:b, :] = g.reshape(n_poses, b - a, 3)
+    return energies, grads

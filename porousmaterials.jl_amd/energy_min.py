@@ -1,0 +1,231 @@
+"""Minimum-energy position of a rigid guest, on device forces.
+
+Mirrors ``find_energy_minimum`` (``src/energy_min.jl:19-58``) and
+``find_energy_minimum_gridsearch`` (``:75-85``): the guest-host energy of a
+rigid molecule is minimised over its centre of mass (the orientation stays
+as given, as in the reference), energies are returned in kJ/mol
+(x 8.314/1000, ``:48``), the molecule comes back in unit-cell fractional
+coordinates with its centre of mass wrapped into [0, 1) (``:51-54``).
+
+Differences, by design:
+
+* the reference runs Nelder-Mead (Optim) on ``vdw_energy``; here an L-BFGS
+  iteration uses the analytic forces of ``pm_pose_gradient``, and many start
+  poses advance together (``find_energy_minima``: one device call per
+  iteration for all of them);
+* when both the crystal and the molecule carry charges the Ewald energy and
+  its forces are included (the reference raises an error there).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import device
+from .crystal import has_charges as xtal_has_charges, replicate
+from .ewald import setup_Ewald_sum
+from .forcefield import replication_factors
+from .molecule import Frac, has_charges as mol_has_charges, needs_rotations, translate_to
+
+K_TO_KJ = 8.314 / 1000.0  # src/energy_min.jl:48
+
+
+class _Objective:
+    """Total energy (K) and COM gradient (K/A) of P rigid poses of one
+    molecule in a replicated crystal."""
+
+    def __init__(self, xtal_rep, molecule_cart, ljff, ewald_precision, precision):
+        self.box = xtal_rep.box
+        charged = xtal_has_charges(xtal_rep) and mol_has_charges(molecule_cart)
+        self.fw = device.framework(xtal_rep, ljff)
+        self.ff = device.forcefield(ljff)
+        self.ew = None
+        if charged:
+            ep = setup_Ewald_sum(xtal_rep.box, math.sqrt(ljff.r2_cutoff), eps=ewald_precision, verbose=False)
+            self.ew = device.ewald(xtal_rep, self.fw, ep)
+        m = molecule_cart
+        lj_off = m.atoms_x - m.com[:, None]
+        q_off = m.charges_x - m.com[:, None] if charged else np.zeros((3, 0))
+        self.offsets = np.concatenate([lj_off, q_off], axis=1).T  # (n_sites, 3) Cartesian
+        self.species = [ljff.index(s) for s in m.atom_species] + [-1] * q_off.shape[1]
+        self.charges = [0.0] * m.n_atoms + (list(m.charges_q) if charged else [])
+        self.precision = precision
+        self.n_evals = 0
+
+    def __call__(self, com_cart):
+        """com_cart: (P, 3) Cartesian centres of mass."""
+        sites = com_cart[:, None, :] + self.offsets[None, :, :]
+        xf = sites @ self.box.c_to_f.T
+        e, g = device.pose_gradients(self.fw, self.ff, self.ew, self.species, self.charges, xf, self.precision)
+        self.n_evals += 1
+        return e.sum(axis=1), g.sum(axis=1)
+
+
+def _lbfgs(obj, x0, m=10, max_iter=500, gtol=1e-4, max_step=0.5):
+    """Batched L-BFGS with backtracking (Armijo) over P independent
+    3-dimensional problems; returns (x, f, g, iterations, converged)."""
+    x = np.array(x0, dtype=np.float64)
+    P = x.shape[0]
+    f, g = obj(x)
+    S = np.zeros((m, P, 3))
+    Y = np.zeros((m, P, 3))
+    R = np.zeros((m, P))
+    n_hist = np.zeros(P, dtype=int)
+    head = 0
+    gam = np.ones(P)  # initial inverse-Hessian scale s.y / y.y of each pose's latest pair
+    done = ~np.isfinite(f) | (np.linalg.norm(g, axis=1) < gtol)
+    it = 0
+    for it in range(1, max_iter + 1):
+        if done.all():
+            break
+        # two-loop recursion over the ring buffer (entries older than a pose's
+        # history are masked out by R = 0)
+        q = g.copy()
+        alpha = np.zeros((m, P))
+        order = [(head - 1 - i) % m for i in range(m)]
+        for i in order:
+            alpha[i] = R[i] * np.einsum("pk,pk->p", S[i], q)
+            q -= alpha[i][:, None] * Y[i]
+        r = np.where(n_hist > 0, gam, 1.0)[:, None] * q
+        for i in reversed(order):
+            b = R[i] * np.einsum("pk,pk->p", Y[i], r)
+            r += S[i] * (alpha[i] - b)[:, None]
+        d = -r
+        gd = np.einsum("pk,pk->p", g, d)
+        bad = ~(gd < 0)
+        d[bad] = -g[bad]
+        gd[bad] = -np.einsum("pk,pk->p", g[bad], g[bad])
+        # first step of a fresh history: at most 0.1 A; never more than max_step
+        dn = np.linalg.norm(d, axis=1)
+        t = np.where(n_hist == 0, np.minimum(1.0, 0.1 / np.maximum(dn, 1e-300)), 1.0)
+        t = np.minimum(t, max_step / np.maximum(dn, 1e-300))
+        accepted = done.copy()
+        x_new, f_new, g_new = x.copy(), f.copy(), g.copy()
+        for _ in range(40):
+            trial = ~accepted
+            if not trial.any():
+                break
+            xt = x.copy()
+            xt[trial] = x[trial] + t[trial, None] * d[trial]
+            ft, gt = obj(xt)
+            # Armijo, with slack for the rounding of f near the minimum
+            ok = trial & np.isfinite(ft) & (ft <= f + 1e-4 * t * gd + 1e-14 * np.abs(f))
+            x_new[ok], f_new[ok], g_new[ok] = xt[ok], ft[ok], gt[ok]
+            accepted |= ok
+            t = np.where(accepted, t, 0.5 * t)
+        stuck = ~accepted
+        moved = accepted & ~done
+        s = x_new - x
+        y = g_new - g
+        sy = np.einsum("pk,pk->p", s, y)
+        keep = moved & (sy > 1e-12 * np.linalg.norm(s, axis=1) * np.linalg.norm(y, axis=1))
+        S[head][keep], Y[head][keep] = s[keep], y[keep]
+        R[head] = np.where(keep, 1.0 / np.where(keep, sy, 1.0), 0.0)
+        yy = np.einsum("pk,pk->p", y, y)
+        gam = np.where(keep, sy / np.where(keep, yy, 1.0), gam)
+        n_hist = np.where(keep, np.minimum(n_hist + 1, m), n_hist)
+        head = (head + 1) % m
+        x, f, g = x_new, f_new, g_new
+        gn = np.linalg.norm(g, axis=1)
+        # a pose whose line search failed restarts from steepest descent once;
+        # failing again with a small gradient ends it
+        restart = stuck & (n_hist > 0)
+        n_hist[restart] = 0
+        R[:, restart] = 0.0
+        done |= (gn < gtol) | (stuck & ~restart)
+    gn = np.linalg.norm(g, axis=1)
+    return x, f, g, it, gn < gtol
+
+
+_PROBE_DIRS = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]]
+                       + [[a, b, c] for a in (1, -1) for b in (1, -1) for c in (1, -1)], dtype=np.float64)
+_PROBE_DIRS /= np.linalg.norm(_PROBE_DIRS, axis=1)[:, None]
+
+
+def _minimise(obj, x0, gtol, max_iter, probe=0.05, rounds=4):
+    """L-BFGS to a stationary point, then a second-order check: the energy
+    at 14 points probe A away (axes and diagonals) must not be lower; a pose
+    that sits on a saddle or maximum (a zero gradient by symmetry, say)
+    resumes from its lowest probe."""
+    x, f, g, it, conv = _lbfgs(obj, x0, max_iter=max_iter, gtol=gtol)
+    total = it
+    for _ in range(rounds):
+        P = x.shape[0]
+        trial = (x[:, None, :] + probe * _PROBE_DIRS[None, :, :]).reshape(-1, 3)
+        ft, _ = obj(trial)
+        ft = ft.reshape(P, -1)
+        ft = np.where(np.isfinite(ft), ft, np.inf)
+        best = np.argmin(ft, axis=1)
+        fb = ft[np.arange(P), best]
+        lower = np.isfinite(f) & (fb < f - 1e-10 * (np.abs(f) + 1.0))
+        if not lower.any():
+            break
+        x1 = x.copy()
+        x1[lower] = trial.reshape(P, -1, 3)[lower, best[lower]]
+        sub = np.nonzero(lower)[0]
+        xs, fs, gs, its, cs = _lbfgs(obj, x1[sub], max_iter=max_iter, gtol=gtol)  # poses are independent
+        x[sub], f[sub], g[sub], conv[sub] = xs, fs, gs, cs
+        total += its
+    return x, f, g, total, conv
+
+
+def _start_template(molecule, box):
+    """Cartesian template of ``molecule`` (given Cartesian, or fractional in ``box``)."""
+    m = molecule if not molecule.frac else molecule.copy()
+    if molecule.frac:
+        m.atoms_x = box.f_to_c @ molecule.atoms_x
+        m.charges_x = box.f_to_c @ molecule.charges_x
+        m.com = box.f_to_c @ molecule.com
+        m.frac = False
+    return m
+
+
+def find_energy_minima(crystal, molecule, ljff, starts_xf, ewald_precision=1e-6, precision="fp64",
+                       gtol=1e-4, max_iter=500):
+    """Minimise the energy of ``molecule`` (its orientation as given) from
+    every centre-of-mass start in ``starts_xf`` ((P, 3) unit-cell fractional)
+    at once.  Returns (xf_min (P, 3) unit-cell fractional in [0, 1),
+    E_min (P,) kJ/mol, info dict)."""
+    rep = replication_factors(crystal.box, ljff)
+    xtal = replicate(crystal, rep)
+    tmpl = _start_template(molecule, crystal.box)
+    obj = _Objective(xtal, tmpl, ljff, ewald_precision, precision)
+    starts = np.atleast_2d(np.asarray(starts_xf, dtype=np.float64))
+    x0 = (starts / np.array(rep)[None, :]) @ xtal.box.f_to_c.T
+    x, f, g, it, conv = _minimise(obj, x0, gtol, max_iter)
+    xf_rep = x @ xtal.box.c_to_f.T
+    xf = np.mod(np.mod(xf_rep, 1.0) * np.array(rep)[None, :], 1.0)
+    info = {"iterations": it, "device_calls": obj.n_evals, "converged": conv,
+            "grad_norm_K_per_A": np.linalg.norm(g, axis=1)}
+    return xf, f * K_TO_KJ, info
+
+
+def find_energy_minimum(crystal, molecule, ljff, ewald_precision=1e-6, precision="fp64", gtol=1e-4,
+                        max_iter=500):
+    """``src/energy_min.jl:19-58``: the molecule (Cartesian, or fractional in
+    the unit cell) at a good initial position; returns (molecule at the
+    minimum, fractional in the unit cell, and the minimum energy in kJ/mol)."""
+    if needs_rotations(molecule):
+        print("needs rotations. does not optimize over configurations, only over center of mass")
+    com_xf = molecule.com if molecule.frac else crystal.box.c_to_f @ molecule.com
+    xf, e, _ = find_energy_minima(crystal, molecule, ljff, com_xf[None, :], ewald_precision, precision, gtol,
+                                  max_iter)
+    out = Frac(molecule, crystal.box) if not molecule.frac else molecule.copy()
+    translate_to(out, xf[0])
+    return out, float(e[0])
+
+
+def find_energy_minimum_gridsearch(crystal, molecule, ljff, resolution=1.0, precision="fp64"):
+    """``src/energy_min.jl:75-85``: energy grid (kJ/mol) over the unit cell,
+    the molecule moved to the grid point of lowest energy (first in Julia's
+    column-major order among ties) and that energy."""
+    from .grid import energy_grid
+    grid = energy_grid(crystal, molecule, ljff, resolution=resolution, verbose=True, precision=precision)
+    flat = grid.data.ravel(order="F")
+    k = int(np.argmin(flat))
+    idx = np.unravel_index(k, grid.n_pts, order="F")
+    xf = np.array([i / (n - 1) if n > 1 else 0.0 for i, n in zip(idx, grid.n_pts)])
+    out = Frac(molecule, crystal.box)
+    translate_to(out, xf)
+    return out, float(flat[k])

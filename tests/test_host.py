@@ -236,3 +236,33 @@ def test_mixing_jacobian_matches_finite_differences(pm, rule):
             off = np.arange(len(ff.species)) != i
             np.testing.assert_allclose(J[i, off, k], d_eps[off], rtol=1e-7, atol=1e-9)
             np.testing.assert_allclose(J[i, off, 2 + k], d_sig[off], rtol=1e-7, atol=1e-9)
+
+
+def test_batched_lbfgs_on_quadratics_and_a_ridge(pm):
+    """The minimiser's batched L-BFGS (host logic): P independent 3-d problems
+    -- anisotropic quadratics and a curved valley -- reach their minima."""
+    from porousmaterials_jl_amd.energy_min import _lbfgs
+    rng = np.random.Generator(np.random.PCG64(3))
+    P = 12
+    A = []
+    for _ in range(P):
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        A.append(q @ np.diag(rng.uniform(0.5, 200.0, 3)) @ q.T)
+    A = np.array(A)
+    c = rng.standard_normal((P, 3))
+
+    class Obj:
+        def __call__(self, x):
+            d = x - c
+            f = 0.5 * np.einsum("pi,pij,pj->p", d, A, d)
+            # a quartic valley term on the last two problems
+            f[-2:] += (x[-2:, 0] ** 2 - x[-2:, 1]) ** 2
+            g = np.einsum("pij,pj->pi", A, d)
+            t = 2 * (x[-2:, 0] ** 2 - x[-2:, 1])
+            g[-2:, 0] += t * 2 * x[-2:, 0]
+            g[-2:, 1] -= t
+            return f, g
+
+    x, f, g, it, conv = _lbfgs(Obj(), np.zeros((P, 3)), gtol=1e-7, max_iter=500)
+    assert conv.all(), (it, np.linalg.norm(g, axis=1))
+    np.testing.assert_allclose(x[:-2], c[:-2], atol=1e-8)
