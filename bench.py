@@ -138,6 +138,52 @@ def screening_leg(pm, steps, n_per_block=20000):
     }
 
 
+def adjoint_leg(pm, case, fw, ff, ew, pr, xr, n_per_block, seed):
+    """configs[3] on the CO2 workload: (b) one adjoint Widom pass over the
+    full 10^6 insertions (pm_widom_gradient, mixed), (a) forces of 4096 CO2
+    poses (pm_pose_gradient, mixed) and their batched minimisation (FP64)."""
+    co2 = case.probes["CO2"]
+    pm.device.widom_gradient(fw, ff, ew, pr, case.temperature, seed, 5, n_per_block, "mixed", case.ljff)
+    t = time.perf_counter()
+    sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, case.temperature, seed, 5, n_per_block, "mixed",
+                                           case.ljff)
+    dt_b = time.perf_counter() - t
+    k_ms = pm.device.last_kernel_stats()[0]
+    S = float(sums[:, 0].sum())
+    g = -grads.sum(axis=0) / (case.temperature * S)
+    lj_off, _ = pm.site_offsets(co2)
+    rng = np.random.Generator(np.random.PCG64(4096))
+    com = rng.random((4096, 3)) @ xr.box.f_to_c.T
+    poses = []
+    for c in com:
+        R = pm.arvo_rotation_matrix(*rng.random(3))
+        poses.append((c[None, :] + lj_off.T @ R.T) @ xr.box.c_to_f.T)
+    poses = np.array(poses)
+    species = [case.ljff.index(s) for s in co2.atom_species]
+    pm.device.pose_gradients(fw, ff, ew, species, list(co2.charges_q), poses, "mixed")
+    t = time.perf_counter()
+    pm.device.pose_gradients(fw, ff, ew, species, list(co2.charges_q), poses, "mixed")
+    dt_a = time.perf_counter() - t
+    a_ms = pm.device.last_kernel_stats()[0]
+    rep = np.array(pm.replication_factors(case.crystal.box, case.ljff))
+    starts = np.mod((com @ xr.box.c_to_f.T) * rep[None, :], 1.0)
+    t = time.perf_counter()
+    _, e_min, info = pm.find_energy_minima(case.crystal, co2, case.ljff, starts, max_iter=200)
+    dt_m = time.perf_counter() - t
+    print(f"adjoint: pass {dt_b * 1e3:.1f} ms, 4096 pose forces {dt_a * 1e3:.1f} ms, 4096 minimisations "
+          f"{dt_m:.1f} s", file=sys.stderr, flush=True)
+    return {
+        "widom_gradient": {"insertions": 5 * n_per_block, "value": 5 * n_per_block / dt_b,
+                           "unit": "insertions/s", "device_pipeline_ms": k_ms,
+                           "parameters": "d ln K_H / d(eps_t, sigma_t, q_t), 5 framework species",
+                           "dlnKH_deps_T1": float(g[case.ljff.index("T1"), 0])},
+        "pose_forces": {"poses": 4096, "value": 4096 / dt_a, "unit": "poses/s", "device_ms": a_ms},
+        "minimise_4096": {"seconds": dt_m, "iterations": int(info["iterations"]),
+                          "device_calls": int(info["device_calls"]),
+                          "converged": int(np.sum(info["converged"]))},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +194,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-screening", action="store_true", help="skip the batched screening leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (gradient) leg")
+    ap.add_argument("--cell-points", type=float, default=None,
+                    help="chunk tuning: points per Widom cell (default: autotuned on the workload)")
+    ap.add_argument("--chunk-r-max", type=float, default=None,
+                    help="chunk tuning: largest chunk radius in A (default: autotuned on the workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -173,6 +224,15 @@ def main():
     def run(precision):
         return pm.device.widom(fw, ff, ew, pr, case.temperature, seed, 5, nb, precision)
 
+    # chunk shapes: autotuned on this workload (untimed) unless given; they
+    # change the schedule only, never a result
+    if args.cell_points is None and args.chunk_r_max is None:
+        tuning = pm.device.autotune_widom(fw, ff, ew, pr, case.temperature, seed, 5, nb, "mixed")
+        tuning["source"] = "pm_autotune_widom on this workload"
+    else:
+        pm.device.set_tuning(args.cell_points, args.chunk_r_max)
+        tuning = dict(pm.device.get_tuning(), source="command line")
+    print(f"tuning: {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         run("mixed")
     pipe_ms, pair_ms, cands = [], [], []
@@ -209,7 +269,8 @@ def main():
                          f"synthetic triclinic 600-atom framework replicated {rep} to "
                          f"{xr.n_atoms} atoms, {n_k} unit-cell k-vectors, "
                          f"{n_ins} insertions = 5 blocks x {nb}"),
-            "insertions_per_step": n_ins, "precision": "mixed"},
+            "insertions_per_step": n_ins, "precision": "mixed",
+            "tuning": {k: tuning[k] for k in ("cell_points", "chunk_r_max", "source")}},
         "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "point_pair_kernel<1, float>", "kernel_ms": p_ms,
@@ -235,6 +296,8 @@ def main():
         out["mixed_over_fp64"] = value / (n_ins / dt64)
         out["kh_rel_diff_mixed_vs_fp64"] = abs(r32[kh] - r64[kh]) / abs(r64[kh])
         out["u_rel_diff_mixed_vs_fp64"] = abs(r32["⟨U⟩ (K)"] - r64["⟨U⟩ (K)"]) / abs(r64["⟨U⟩ (K)"])
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(pm, case, fw, ff, ew, pr, xr, nb, seed)
     if not args.no_screening:
         out["screening"] = screening_leg(pm, max(1, min(args.steps, 3)))
     if not args.no_cpu:

@@ -17,6 +17,7 @@ PM_OK, PM_ERR_ARG, PM_ERR_DEVICE, PM_ERR_OOM, PM_ERR_STATE = 0, 1, 2, 3, 4
 PM_FP64, PM_MIXED = 0, 1
 PM_MAX_SITES = 8
 PM_MAX_GRAD_TYPES = 8
+PM_TUNE_CELL_POINTS, PM_TUNE_CHUNK_R_MAX = 0, 1
 
 _ERR_NAMES = {1: "PM_ERR_ARG", 2: "PM_ERR_DEVICE", 3: "PM_ERR_OOM", 4: "PM_ERR_STATE"}
 
@@ -76,6 +77,10 @@ _SIGS = {
                                          ctypes.c_int64, ctypes.c_int32, D, D, D]),
     "pm_widom_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(WidomRun), ctypes.c_int32, D]),
     "pm_energy_grid_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GridRun), ctypes.c_int32]),
+    "pm_set_tuning": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double]),
+    "pm_get_tuning": (ctypes.c_int, [ctypes.c_int32, D]),
+    "pm_autotune_widom": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int64,
+                                         ctypes.c_int32, D, D, D]),
     "pm_last_kernel_stats": (ctypes.c_int, [D, D]),
     "pm_last_pair_kernel_stats": (ctypes.c_int, [D, D]),
 }

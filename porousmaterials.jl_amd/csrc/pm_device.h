@@ -202,8 +202,10 @@ struct PoseArgs {
     double* out_g;         // GRAD: n_items x ns x 3, dE/d(site Cartesian xyz) in K/A
 };
 
-// One work-item per pose.  GRAD adds the gradient of the pose's energy with
-// respect to every site's Cartesian position:
+// One wave per pose: the workgroup's four poses share LDS tiles of framework
+// atoms (and of k-vectors), the wave's lanes stride over a tile, and FP64 lane
+// partials are reduced by a fixed shuffle tree.  GRAD adds the gradient of the
+// pose's energy with respect to every site's Cartesian position:
 //   LJ      d/dx_s = 6 eps4 (2u^2 - u) / r^2 * d,   u = (sigma^2 / r^2)^3
 //   real    d/dx_s = q_s q_a [erfc(a r) / r + 2a/sqrt(pi) e^{-a^2 r^2}] / r^2 * d / (4 pi eps0)
 //   Fourier d/dx_s = q_s 2 pi C^T sum_k (-A_re sin th + A_im cos th) (m o rep),  th = 2 pi m.(rep o x)
@@ -219,11 +221,11 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
     int* s_type = reinterpret_cast<int*>(smem + REGION);
     LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + REGION + TILE * sizeof(int));
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const LJP<Real>* gtab = reinterpret_cast<const LJP<Real>*>(a.ljtab);
     for (int i = tid; i < a.n_types * NS; i += WG) s_lj[i] = gtab[i];
 
-    const long long item = (long long)blockIdx.x * WG + tid;
+    const long long item = (long long)blockIdx.x * (WG / 64) + wave;
     const bool valid = item < a.n_items;
     const long long it = valid ? item : 0;
     double sx[NS], sy[NS], sz[NS];
@@ -245,14 +247,15 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
     const double r2lj = a.r2_lj, r2sr = a.r2_sr, alpha = a.alpha;
     const double two_a_rpi = 2.0 * alpha * 0.56418958354775628;  // 2 alpha / sqrt(pi)
     for (int base = 0; base < a.n_atoms; base += TILE) {
-        int nt = min(TILE, a.n_atoms - base);
+        const int nt = min(TILE, a.n_atoms - base);
         __syncthreads();
         if (tid < nt) {
             s_atoms[tid] = a.atoms[base + tid];
             s_type[tid] = a.species[base + tid];
         }
         __syncthreads();
-        for (int j = 0; j < nt; ++j) {
+        if (!valid) continue;
+        for (int j = lane; j < nt; j += 64) {
             const double4 at = s_atoms[j];
             const int t = s_type[j];
 #pragma unroll
@@ -261,14 +264,14 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
                 dx -= rint(dx);
                 dy -= rint(dy);
                 dz -= rint(dz);
-                double cx = f11 * dx + f12 * dy + f13 * dz;
-                double cy = f22 * dy + f23 * dz;
-                double cz = f33 * dz;
-                double r2 = cx * cx + cy * cy + cz * cz;
+                const double cx = f11 * dx + f12 * dy + f13 * dz;
+                const double cy = f22 * dy + f23 * dz;
+                const double cz = f33 * dz;
+                const double r2 = cx * cx + cy * cy + cz * cz;
                 const LJP<Real> p = s_lj[t * NS + s];
-                bool in = r2 < r2lj;
+                const bool in = r2 < r2lj;
                 ovl |= in & (r2 < R2_OVERLAP) & (a.site[s].lj != 0);
-                Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
+                const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
                 acc_lj[s] += in ? (double)e : 0.0;
                 if (GRAD) {
                     // 6 eps4 (2u^2 - u) / r^2
@@ -281,8 +284,8 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
                     gz[s] += f * cz;
                 }
                 if (ES) {
-                    bool ins = (r2 < r2sr) & (at.w != 0.0);
-                    Real c = PairMath<Real>::coul(r2, at.w, alpha);
+                    const bool ins = (r2 < r2sr) & (at.w != 0.0);
+                    const Real c = PairMath<Real>::coul(r2, at.w, alpha);
                     acc_sr[s] += ins ? (double)c : 0.0;
                     if (GRAD) {
                         const double r = sqrt(r2);
@@ -311,11 +314,12 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
             uz[s] = sz[s] * a.repd[2];
         }
         for (int base = 0; base < a.nk; base += KTILE) {
-            int nt = min(KTILE, a.nk - base);
+            const int nt = min(KTILE, a.nk - base);
             __syncthreads();
             if (tid < nt) s_k[tid] = a.kv[base + tid];
             __syncthreads();
-            for (int k = 0; k < nt; ++k) {
+            if (!valid) continue;
+            for (int k = lane; k < nt; k += 64) {
                 const KVec kv = s_k[k];
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
@@ -337,15 +341,32 @@ __global__ __launch_bounds__(WG) void pose_kernel(PoseArgs a) {
         }
     }
 
+    // fixed-order reductions over the wave's lanes
     double vdw = 0.0, sr = 0.0, lr = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        vdw += acc_lj[s];
-        sr += a.site[s].q * acc_sr[s];
-        lr += a.site[s].q * acc_lr[s];
+        vdw += wave_sum(acc_lj[s]);
+        if (ES) {
+            sr += a.site[s].q * wave_sum(acc_sr[s]);
+            lr += a.site[s].q * wave_sum(acc_lr[s]);
+        }
     }
+    ovl = __ballot(ovl) != 0ull;
     if (ovl) vdw = INFINITY;
-    if (valid) {
+    if (GRAD) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            gx[s] = wave_sum(gx[s]);
+            gy[s] = wave_sum(gy[s]);
+            gz[s] = wave_sum(gz[s]);
+            if (ES) {
+                hx[s] = wave_sum(hx[s]);
+                hy[s] = wave_sum(hy[s]);
+                hz[s] = wave_sum(hz[s]);
+            }
+        }
+    }
+    if (valid && lane == 0) {
         a.out_e[3 * item + 0] = vdw;
         a.out_e[3 * item + 1] = sr * a.inv4pe;
         a.out_e[3 * item + 2] = lr;

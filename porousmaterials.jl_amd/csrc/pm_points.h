@@ -62,6 +62,7 @@ struct RunArgs {
     // Fourier box: ma in [0, Ma], mb in [-Mb, Mb], mc in [-Mc, Mc]
     int Ma, Mb, Mc;
     const double2* A;        // wt * N_R * S(m), zeros off the kept set
+    const int2* krows;       // per (ma, mb) row: kept mc range (lo > hi: empty row)
     // Widom
     unsigned long long seed;
     long long n_per_block;
@@ -667,9 +668,12 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 }
 
 // Fourier part per point: sum_m Re(A_m e^{-i 2 pi m.u}), u = R o x (unit-cell
-// fractional), by complex recurrences along each axis of the box (FP64).
-// Point slots of a segment start at a multiple of WG, so a workgroup lies in
-// one segment.
+// fractional), FP64.  With z_a = e^{-i 2 pi u_a}: for every (ma, mb) row the
+// sum over the row's kept mc range is evaluated as two polynomials by Horner,
+// sum_{mc >= 0} A_mc z_c^mc and z_c^* sum_{j >= 1} A_{-j} (z_c^*)^{j-1}, and
+// multiplied by z_a^ma z_b^mb, carried along the rows by recurrence.  Point
+// slots of a segment start at a multiple of WG, so a workgroup lies in one
+// segment.
 __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
     __shared__ double2 s_a[FT];
     const int tid = threadIdx.x;
@@ -704,18 +708,17 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
     sincospi(2.0 * t1, &s, &c);
     const double2 Eb = make_double2(c, -s);
     sincospi(2.0 * t2, &s, &c);
-    const double2 Ec = make_double2(c, -s);
+    const double zr = c, zi = -s;  // z_c
     const int Ma = a.Ma, Mb = a.Mb, Mc = a.Mc;
-    double2 Eb_neg = make_double2(1.0, 0.0), Ec_neg = make_double2(1.0, 0.0);
-    const double2 Eb_c = make_double2(Eb.x, -Eb.y), Ec_c = make_double2(Ec.x, -Ec.y);
+    double2 Eb_neg = make_double2(1.0, 0.0);
+    const double2 Eb_c = make_double2(Eb.x, -Eb.y);
     for (int q = 0; q < Mb; ++q) Eb_neg = cmul(Eb_neg, Eb_c);
-    for (int q = 0; q < Mc; ++q) Ec_neg = cmul(Ec_neg, Ec_c);
 
     const int rowlen = 2 * Mc + 1, nb = 2 * Mb + 1;
     const int n_rows = (Ma + 1) * nb;
     const int rows_per_tile = max(1, FT / rowlen);
     double2 Pa = make_double2(1.0, 0.0);
-    double2 Pb = Eb_neg;
+    double2 Pb = Eb_neg;  // z_a^ma z_b^mb of the current row
     double acc = 0.0;
     for (int r0 = 0; r0 < n_rows; r0 += rows_per_tile) {
         const int r1 = min(n_rows, r0 + rows_per_tile);
@@ -728,12 +731,27 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
                 Pa = cmul(Pa, Ea);
                 Pb = cmul(Pa, Eb_neg);
             }
-            double2 P = cmul(Pb, Ec_neg);
-            const double2* row = s_a + (r - r0) * rowlen;
-            for (int mc = 0; mc < rowlen; ++mc) {
-                const double2 A = row[mc];
-                acc += A.x * P.x - A.y * P.y;
-                P = cmul(P, Ec);
+            const int2 lh = a.krows[r];
+            if (lh.x <= lh.y) {
+                const double2* row = s_a + (r - r0) * rowlen + Mc;  // row[mc], mc in [-Mc, Mc]
+                double qr = 0.0, qi = 0.0;                           // sum_{mc >= 0} A_mc z^mc
+                for (int mc = lh.y; mc >= 0; --mc) {
+                    const double2 A = row[mc];
+                    const double nr = fma(qr, zr, fma(-qi, zi, A.x));
+                    qi = fma(qr, zi, fma(qi, zr, A.y));
+                    qr = nr;
+                }
+                double nr_ = 0.0, ni_ = 0.0;                         // sum_{j >= 1} A_{-j} (z*)^{j-1}
+                for (int j = -lh.x; j >= 1; --j) {
+                    const double2 A = row[-j];
+                    const double tr = fma(nr_, zr, fma(ni_, zi, A.x));
+                    ni_ = fma(ni_, zr, fma(-nr_, zi, A.y));
+                    nr_ = tr;
+                }
+                // R = q + z* n;  acc += Re(Pb R)
+                const double Rr = qr + (zr * nr_ + zi * ni_);
+                const double Ri = qi + (zr * ni_ - zi * nr_);
+                acc += Pb.x * Rr - Pb.y * Ri;
             }
             Pb = cmul(Pb, Eb);
         }

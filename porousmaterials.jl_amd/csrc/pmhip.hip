@@ -23,8 +23,9 @@ using namespace pmk;
 namespace {
 
 // Point-engine tunables.
-constexpr double CHUNK_R_MAX = 3.0;       // A: largest chunk radius served by the image list
-constexpr double CELL_POINTS = 640.0;     // target points per Widom cell
+// Point-engine tunables (defaults; pm_set_tuning / pm_autotune_widom change them).
+constexpr double DEFAULT_CHUNK_R_MAX = 3.0;   // A: largest chunk radius of the image-list path
+constexpr double DEFAULT_CELL_POINTS = 640.0; // target points per Widom cell
 constexpr long long MAX_POINT_SLOTS = 1LL << 24;  // point slots per launch group
 constexpr long long MAX_IMAGES = 1LL << 26;       // images per list
 constexpr size_t MAX_IMAGE_LISTS = 8;             // image lists (radii) per framework
@@ -43,6 +44,8 @@ struct Ctx {
     size_t scratch_bytes = 0;
     // host staging for the per-call LJ table: outlives the asynchronous copy
     std::vector<unsigned char> host_tab;
+    double cell_points = DEFAULT_CELL_POINTS;
+    double chunk_r_max = DEFAULT_CHUNK_R_MAX;
 };
 Ctx g_ctx;
 
@@ -179,6 +182,7 @@ struct pm_ewald {
     KVec* d_k = nullptr;
     int Ma = 0, Mb = 0, Mc = 0;
     double2* d_box = nullptr;
+    int2* d_rows = nullptr;    // per (ma, mb) row of the box: kept mc range (lo > hi: empty)
     float4* d_erfc = nullptr;  // ERFC_SEG cubics of erfc(alpha r) on [0, sr_cutoff]
     double2* d_box_t = nullptr;  // per-type boxes (gradient runs), built on first use
 };
@@ -335,7 +339,7 @@ int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* e
     a.out_g = out_grad ? reinterpret_cast<double*>(base + o_grad) : nullptr;
     const bool fp64 = precision == PM_FP64;
     const size_t shm = pose_lds_bytes(a.n_types, a.ns, fp64);
-    const int n_wg = (int)((n + WG - 1) / WG);
+    const int n_wg = (int)((n + WG / 64 - 1) / (WG / 64));  // one wave per pose
     PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
     const int rc = out_grad ? pose_launch<true>(a, n_wg, shm, fp64, es) : pose_launch<false>(a, n_wg, shm, fp64, es);
     if (rc != PM_OK) return rc;
@@ -450,15 +454,16 @@ void fill_run_pair(RunArgs& a, const pm_framework* fw, const pm_forcefield* ff, 
         a.Mb = ew->Mb;
         a.Mc = ew->Mc;
         a.A = ew->d_box;
+        a.krows = ew->d_rows;
     }
 }
 
-// Cells of a Widom run's counting sort: about CELL_POINTS points each, half
-// diagonal <= CHUNK_R_MAX where the point density allows, at most
+// Cells of a Widom run's counting sort: about cell_points points each, half
+// diagonal <= chunk_r_max where the point density allows, at most
 // MAX_RUN_CELLS cells.
 void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double* cell_r) {
     double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
-    double h = std::cbrt(vol * CELL_POINTS / std::max(n_points, 1.0));
+    double h = std::cbrt(vol * g_ctx.cell_points / std::max(n_points, 1.0));
     double scale = 1.0;
     int prev[3] = {1, 1, 1};
     double prev_r = 0.0;
@@ -477,7 +482,7 @@ void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double*
             ec[i] = fw->F[6 + i] / ncell[2];
         }
         *cell_r = half_diagonal(ea, eb, ec);
-        if (*cell_r <= CHUNK_R_MAX) return;
+        if (*cell_r <= g_ctx.chunk_r_max) return;
         for (int i = 0; i < 3; ++i) prev[i] = ncell[i];
         prev_r = *cell_r;
         scale *= 1.25;
@@ -634,7 +639,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         a.n_img = 0;
         a.img = nullptr;
         a.img_type = nullptr;
-        if (a.chunk_r <= CHUNK_R_MAX) {
+        if (a.chunk_r <= g_ctx.chunk_r_max) {
             const pm_framework::Images* im = nullptr;
             const int rc = get_images(r.fw, a.r_cmax + 0.01, &im);
             if (rc == PM_OK) {
@@ -1225,7 +1230,7 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
     int* d_m = nullptr;
     double* d_w = nullptr;
     auto cleanup = [&]() {
-        (void)hipFree(ew->d_k); (void)hipFree(ew->d_box); (void)hipFree(ew->d_erfc);
+        (void)hipFree(ew->d_k); (void)hipFree(ew->d_box); (void)hipFree(ew->d_erfc); (void)hipFree(ew->d_rows);
         (void)hipFree(d_m); (void)hipFree(d_w);
         delete ew;
     };
@@ -1245,7 +1250,17 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
                                   (float)(2.0 * (f0 - f1) + d0 + d1));
         }
     }
-    if (hipMalloc(&ew->d_erfc, ERFC_SEG * sizeof(float4)) != hipSuccess ||
+    // kept mc range of every (ma, mb) row of the box
+    const size_t n_rows = (size_t)(Ma + 1) * (2 * Mb + 1);
+    std::vector<int2> rows(n_rows, make_int2(1, 0));
+    for (size_t k = 0; k < w.size(); ++k) {
+        int2& r = rows[(size_t)m[3 * k] * (2 * Mb + 1) + (size_t)(m[3 * k + 1] + Mb)];
+        const int mc = m[3 * k + 2];
+        if (r.x > r.y) r = make_int2(mc, mc);
+        else r = make_int2(std::min(r.x, mc), std::max(r.y, mc));
+    }
+    if (hipMalloc(&ew->d_rows, n_rows * sizeof(int2)) != hipSuccess ||
+        hipMalloc(&ew->d_erfc, ERFC_SEG * sizeof(float4)) != hipSuccess ||
         hipMalloc(&ew->d_box, box_n * sizeof(double2)) != hipSuccess ||
         hipMalloc(&ew->d_k, std::max<size_t>(ew->nk, 1) * sizeof(KVec)) != hipSuccess ||
         hipMalloc(&d_m, std::max<size_t>(m.size(), 1) * sizeof(int)) != hipSuccess ||
@@ -1257,6 +1272,8 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
     if (e == hipSuccess)
         e = hipMemcpyAsync(ew->d_erfc, etab.data(), ERFC_SEG * sizeof(float4), hipMemcpyHostToDevice,
                            g_ctx.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ew->d_rows, rows.data(), n_rows * sizeof(int2), hipMemcpyHostToDevice, g_ctx.stream);
     if (e == hipSuccess && ew->nk > 0) {
         e = hipMemcpyAsync(d_m, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, g_ctx.stream);
         if (e == hipSuccess)
@@ -1294,6 +1311,7 @@ int pm_ewald_destroy(pm_ewald* ew) {
     (void)hipFree(ew->d_box);
     (void)hipFree(ew->d_erfc);
     (void)hipFree(ew->d_box_t);
+    (void)hipFree(ew->d_rows);
     delete ew;
     return PM_OK;
 }
@@ -1576,6 +1594,88 @@ int pm_energy_grid_batch(int32_t n_runs, const pm_grid_run* runs_in, int32_t pre
         if (int rc = grid_prepare(runs_in[i], runs[i]))
             return fail(rc, "pm_energy_grid_batch: grid " + std::to_string(i) + ": " + g_err);
     return execute_points(runs, MODE_GRID, precision, nullptr);
+}
+
+int pm_set_tuning(int32_t key, double value) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!std::isfinite(value)) return fail(PM_ERR_ARG, "pm_set_tuning: non-finite value");
+    switch (key) {
+        case PM_TUNE_CELL_POINTS:
+            if (value < 16.0 || value > 65536.0) return fail(PM_ERR_ARG, "pm_set_tuning: cell points outside [16, 65536]");
+            g_ctx.cell_points = value;
+            return PM_OK;
+        case PM_TUNE_CHUNK_R_MAX:
+            if (value < 0.5 || value > 20.0) return fail(PM_ERR_ARG, "pm_set_tuning: chunk radius outside [0.5, 20] A");
+            g_ctx.chunk_r_max = value;
+            return PM_OK;
+        default:
+            return fail(PM_ERR_ARG, "pm_set_tuning: unknown key");
+    }
+}
+
+int pm_get_tuning(int32_t key, double* value) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!value) return fail(PM_ERR_ARG, "pm_get_tuning: null pointer");
+    switch (key) {
+        case PM_TUNE_CELL_POINTS: *value = g_ctx.cell_points; return PM_OK;
+        case PM_TUNE_CHUNK_R_MAX: *value = g_ctx.chunk_r_max; return PM_OK;
+        default: return fail(PM_ERR_ARG, "pm_get_tuning: unknown key");
+    }
+}
+
+int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, const pm_probe* pr,
+                      double temperature, uint64_t seed, int32_t n_blocks, int64_t n_per_block, int32_t precision,
+                      double* out_cell_points, double* out_chunk_r_max, double* out_ms) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
+    pm_widom_run in{};
+    in.framework = fw;
+    in.forcefield = ff;
+    in.ewald = ew;
+    in.probe = pr;
+    in.temperature = temperature;
+    in.seed = seed;
+    in.n_per_block = n_per_block;
+    in.n_blocks = n_blocks;
+    {
+        std::vector<PointRun> probe_run(1);
+        if (int rc = widom_prepare(in, probe_run[0])) return rc;  // validates the arguments
+    }
+    const double cps[] = {192.0, 256.0, 384.0, 512.0, 640.0, 896.0, 1280.0};
+    const double crs[] = {3.0, 4.5, 6.0};
+    const double keep_cp = g_ctx.cell_points, keep_cr = g_ctx.chunk_r_max;
+    double best_ms = INFINITY, best_cp = keep_cp, best_cr = keep_cr;
+    std::vector<double> sums((size_t)n_blocks * 3);
+    for (double cr : crs)
+        for (double cp : cps) {
+            g_ctx.cell_points = cp;
+            g_ctx.chunk_r_max = cr;
+            double ms = INFINITY;
+            for (int rep = 0; rep < 2; ++rep) {
+                std::vector<PointRun> runs(1);
+                int rc = widom_prepare(in, runs[0]);
+                if (rc == PM_OK) rc = execute_points(runs, MODE_WIDOM, precision, sums.data());
+                if (rc != PM_OK) {
+                    g_ctx.cell_points = keep_cp;
+                    g_ctx.chunk_r_max = keep_cr;
+                    return rc;
+                }
+                ms = std::min(ms, g_ctx.last_ms);
+            }
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_cp = cp;
+                best_cr = cr;
+            }
+        }
+    g_ctx.cell_points = best_cp;
+    g_ctx.chunk_r_max = best_cr;
+    if (out_cell_points) *out_cell_points = best_cp;
+    if (out_chunk_r_max) *out_chunk_r_max = best_cr;
+    if (out_ms) *out_ms = best_ms;
+    return PM_OK;
 }
 
 int pm_last_kernel_stats(double* kernel_ms, double* pair_evals) {

@@ -340,3 +340,34 @@ def pose_gradients(fw, ff, ew, site_species, site_charges, site_xf, precision):
         energies += e.reshape(n_poses, 3)
         grads[:, a:b, :] = g.reshape(n_poses, b - a, 3)
     return energies, grads
+
+
+def set_tuning(cell_points=None, chunk_r_max=None):
+    """Process-wide point-engine tuning (no effect on results)."""
+    lib = _lib.load()
+    init()
+    if cell_points is not None:
+        check(lib.pm_set_tuning(_lib.PM_TUNE_CELL_POINTS, float(cell_points)))
+    if chunk_r_max is not None:
+        check(lib.pm_set_tuning(_lib.PM_TUNE_CHUNK_R_MAX, float(chunk_r_max)))
+
+
+def get_tuning():
+    lib = _lib.load()
+    v = ctypes.c_double()
+    check(lib.pm_get_tuning(_lib.PM_TUNE_CELL_POINTS, ctypes.byref(v)))
+    cp = v.value
+    check(lib.pm_get_tuning(_lib.PM_TUNE_CHUNK_R_MAX, ctypes.byref(v)))
+    return {"cell_points": cp, "chunk_r_max": v.value}
+
+
+def autotune_widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision):
+    """Time the run for a grid of chunk shapes, keep the fastest
+    (pm_autotune_widom); returns {cell_points, chunk_r_max, ms}."""
+    lib = _lib.load()
+    ffh = _ff_handle(ff)
+    cp, cr, ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib.pm_autotune_widom(fw.ptr, ffh.ptr, ew.ptr if ew is not None else None, pr.ptr, float(temperature),
+                                int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_blocks), int(n_per_block),
+                                precision_code(precision), ctypes.byref(cp), ctypes.byref(cr), ctypes.byref(ms)))
+    return {"cell_points": cp.value, "chunk_r_max": cr.value, "ms": ms.value}

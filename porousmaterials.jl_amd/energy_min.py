@@ -53,8 +53,9 @@ class _Objective:
         self.precision = precision
         self.n_evals = 0
 
-    def __call__(self, com_cart):
-        """com_cart: (P, 3) Cartesian centres of mass."""
+    def __call__(self, com_cart, idx=None):
+        """com_cart: (P, 3) Cartesian centres of mass (``idx``, the poses'
+        indices, does not matter: every pose has the same energy function)."""
         sites = com_cart[:, None, :] + self.offsets[None, :, :]
         xf = sites @ self.box.c_to_f.T
         e, g = device.pose_gradients(self.fw, self.ff, self.ew, self.species, self.charges, xf, self.precision)
@@ -64,10 +65,11 @@ class _Objective:
 
 def _lbfgs(obj, x0, m=10, max_iter=500, gtol=1e-4, max_step=0.5):
     """Batched L-BFGS with backtracking (Armijo) over P independent
-    3-dimensional problems; returns (x, f, g, iterations, converged)."""
+    3-dimensional problems; ``obj(x, idx)`` evaluates the problems ``idx`` at
+    ``x`` (rows).  Returns (x, f, g, iterations, converged)."""
     x = np.array(x0, dtype=np.float64)
     P = x.shape[0]
-    f, g = obj(x)
+    f, g = obj(x, np.arange(P))
     S = np.zeros((m, P, 3))
     Y = np.zeros((m, P, 3))
     R = np.zeros((m, P))
@@ -106,12 +108,15 @@ def _lbfgs(obj, x0, m=10, max_iter=500, gtol=1e-4, max_step=0.5):
             trial = ~accepted
             if not trial.any():
                 break
-            xt = x.copy()
-            xt[trial] = x[trial] + t[trial, None] * d[trial]
-            ft, gt = obj(xt)
+            # only the poses still searching are evaluated (poses are independent)
+            idx = np.nonzero(trial)[0]
+            xt = x[idx] + t[idx, None] * d[idx]
+            ft, gt = obj(xt, idx)
             # Armijo, with slack for the rounding of f near the minimum
-            ok = trial & np.isfinite(ft) & (ft <= f + 1e-4 * t * gd + 1e-14 * np.abs(f))
-            x_new[ok], f_new[ok], g_new[ok] = xt[ok], ft[ok], gt[ok]
+            ok_s = np.isfinite(ft) & (ft <= f[idx] + 1e-4 * t[idx] * gd[idx] + 1e-14 * np.abs(f[idx]))
+            ok = np.zeros(P, dtype=bool)
+            ok[idx[ok_s]] = True
+            x_new[idx[ok_s]], f_new[idx[ok_s]], g_new[idx[ok_s]] = xt[ok_s], ft[ok_s], gt[ok_s]
             accepted |= ok
             t = np.where(accepted, t, 0.5 * t)
         stuck = ~accepted
@@ -153,18 +158,19 @@ def _minimise(obj, x0, gtol, max_iter, probe=0.05, rounds=4):
     for _ in range(rounds):
         P = x.shape[0]
         trial = (x[:, None, :] + probe * _PROBE_DIRS[None, :, :]).reshape(-1, 3)
-        ft, _ = obj(trial)
+        ft, _ = obj(trial, np.repeat(np.arange(P), len(_PROBE_DIRS)))
         ft = ft.reshape(P, -1)
         ft = np.where(np.isfinite(ft), ft, np.inf)
         best = np.argmin(ft, axis=1)
         fb = ft[np.arange(P), best]
-        lower = np.isfinite(f) & (fb < f - 1e-10 * (np.abs(f) + 1.0))
+        with np.errstate(invalid="ignore"):
+            lower = np.isfinite(f) & (fb < f - 1e-10 * (np.abs(f) + 1.0))
         if not lower.any():
             break
         x1 = x.copy()
         x1[lower] = trial.reshape(P, -1, 3)[lower, best[lower]]
         sub = np.nonzero(lower)[0]
-        xs, fs, gs, its, cs = _lbfgs(obj, x1[sub], max_iter=max_iter, gtol=gtol)  # poses are independent
+        xs, fs, gs, its, cs = _lbfgs(lambda xx, ii: obj(xx, sub[ii]), x1[sub], max_iter=max_iter, gtol=gtol)
         x[sub], f[sub], g[sub], conv[sub] = xs, fs, gs, cs
         total += its
     return x, f, g, total, conv

@@ -140,3 +140,25 @@ def test_batch_rejects_bad_runs(pm):
         pm.device.widom_batch([(fw, ff, ew, pr, 298.0, 1, 5, 10), (fw, ff, ew, pr, -1.0, 1, 5, 10)], "mixed")
     with pytest.raises(PMError, match="Ewald"):
         pm.device.widom_batch([(fw, ff, None, pr, 298.0, 1, 5, 10)], "mixed")
+
+
+def test_chunk_tuning_changes_no_result(pm):
+    """Per-point energies do not depend on the cell layout: pm_widom gives the
+    same block sums bit for bit under any (cell points, chunk radius) the
+    image-list path serves, and after pm_autotune_widom."""
+    say("case: CO2 5 x 20000 under five chunk tunings and after the autotuner (mixed)")
+    case = pm.synthetic.co2_widom_case()
+    fw, ff, ew, pr, _ = _handles(pm, case, "CO2")
+    before = pm.device.get_tuning()
+    try:
+        pm.device.set_tuning(640, 3.0)
+        want = pm.device.widom(fw, ff, ew, pr, 298.0, 11, 5, 20000, "mixed")
+        for cp, cr in ((192, 3.0), (256, 3.0), (1280, 3.0), (1280, 6.0), (384, 4.5)):
+            pm.device.set_tuning(cp, cr)
+            got = pm.device.widom(fw, ff, ew, pr, 298.0, 11, 5, 20000, "mixed")
+            assert np.array_equal(got, want), (cp, cr)
+        best = pm.device.autotune_widom(fw, ff, ew, pr, 298.0, 11, 5, 20000, "mixed")
+        say(f"   autotuner: {best}")
+        assert np.array_equal(pm.device.widom(fw, ff, ew, pr, 298.0, 11, 5, 20000, "mixed"), want)
+    finally:
+        pm.device.set_tuning(before["cell_points"], before["chunk_r_max"])

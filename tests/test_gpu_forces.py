@@ -116,7 +116,7 @@ def test_find_energy_minimum_reference_doc_example(pm):
     0.02 A of each other; which one the reference prints follows Optim's
     Nelder-Mead path.  Pinned: the device energy at the printed minimum
     (-37.69376112588296 kJ/mol, to 1e-12), and the minimiser's end point
-    within 0.05 A of it at an energy within 2e-5."""
+    within 0.05 A of it (or of its mirror image) at an energy within 2e-5."""
     say("case: find_energy_minimum_gridsearch + find_energy_minimum, Xe / SBMOF-1 / UFF")
     x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
     ff = pm.LJForceField.from_csv(golden("forcefields", "UFF.csv"))
@@ -132,11 +132,16 @@ def test_find_energy_minimum_reference_doc_example(pm):
     m0, e0 = pm.find_energy_minimum_gridsearch(x, xe, ff, resolution=1.0)
     assert pm.required_n_pts(x.box, 1.0) == (13, 7, 24)
     m1, e1 = pm.find_energy_minimum(x, m0, ff)
-    d = np.asarray(m1.com) - ref_xf
-    d -= np.round(d)
-    dist = float(np.linalg.norm(x.box.f_to_c @ d))
+    # the grid search ends on SBMOF-1's inversion centre (a saddle, zero
+    # gradient), from which the minimum and its mirror image -xf (the same
+    # energy) are equally near; rounding decides which side a descent takes
+    dist = np.inf
+    for target in (ref_xf, -ref_xf):
+        d = np.asarray(m1.com) - target
+        d -= np.round(d)
+        dist = min(dist, float(np.linalg.norm(x.box.f_to_c @ d)))
     say(f"   grid minimum {e0:.6f} kJ/mol at {m0.com}; minimised {e1:.12f} kJ/mol at {m1.com}, "
-        f"{dist:.4f} A from the reference's minimum ({ref_e})")
+        f"{dist:.4f} A from the reference's minimum ({ref_e}) or its mirror image")
     assert e1 <= e0 + 1e-12 * abs(e0)
     assert dist <= 0.05
     assert abs(e1 - ref_e) <= 2e-5 * abs(ref_e)

@@ -252,15 +252,16 @@ def test_batched_lbfgs_on_quadratics_and_a_ridge(pm):
     c = rng.standard_normal((P, 3))
 
     class Obj:
-        def __call__(self, x):
-            d = x - c
-            f = 0.5 * np.einsum("pi,pij,pj->p", d, A, d)
+        def __call__(self, x, idx):
+            d = x - c[idx]
+            f = 0.5 * np.einsum("pi,pij,pj->p", d, A[idx], d)
+            g = np.einsum("pij,pj->pi", A[idx], d)
             # a quartic valley term on the last two problems
-            f[-2:] += (x[-2:, 0] ** 2 - x[-2:, 1]) ** 2
-            g = np.einsum("pij,pj->pi", A, d)
-            t = 2 * (x[-2:, 0] ** 2 - x[-2:, 1])
-            g[-2:, 0] += t * 2 * x[-2:, 0]
-            g[-2:, 1] -= t
+            v = idx >= P - 2
+            t = 2 * (x[v, 0] ** 2 - x[v, 1])
+            f[v] += (x[v, 0] ** 2 - x[v, 1]) ** 2
+            g[v, 0] += t * 2 * x[v, 0]
+            g[v, 1] -= t
             return f, g
 
     x, f, g, it, conv = _lbfgs(Obj(), np.zeros((P, 3)), gtol=1e-7, max_iter=500)
