@@ -15,6 +15,7 @@ bounded sample on this host's cores.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -184,6 +185,17 @@ def adjoint_leg(pm, case, fw, ff, ew, pr, xr, n_per_block, seed):
     }
 
 
+def _traffic_record():
+    """HBM bytes per launch of the pair kernel measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters (scaled by a plain copy of known size),
+    committed as profiles/pair_kernel_traffic.json; None if absent."""
+    path = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +207,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-screening", action="store_true", help="skip the batched screening leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (gradient) leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB device copy (pm_copy_probe) three times: the "
+                         "calibration launch for FETCH_SIZE / WRITE_SIZE counter runs")
     ap.add_argument("--cell-points", type=float, default=None,
                     help="chunk tuning: points per Widom cell (default: autotuned on the workload)")
     ap.add_argument("--chunk-r-max", type=float, default=None,
@@ -304,6 +319,16 @@ def main():
         cb = cpu_baseline(pm, case, xr, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+    if args.copy_probe:
+        ms = ctypes.c_double()
+        _lib = pm._lib
+        _lib.check(_lib.load().pm_copy_probe(256 << 20, 3, ctypes.byref(ms)))
+        out["copy_probe"] = {"bytes": 256 << 20, "ms": ms.value,
+                             "GB_per_s": 2 * (256 << 20) / (ms.value * 1e-3) / 1e9}
+    traffic = _traffic_record()
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     print(json.dumps(out), flush=True)
 
 

@@ -267,6 +267,14 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
                       double temperature, uint64_t seed, int32_t n_blocks, int64_t n_per_block, int32_t precision,
                       double* out_cell_points, double* out_chunk_r_max, double* out_ms);
 
+/*
+ * Measurement utility (no counterpart in the reference): copies n_bytes
+ * between two device buffers `repeats` times with a plain 16-byte-vector copy
+ * kernel and reports the fastest launch in ms.  It is the known-size launch
+ * that scales HBM traffic counters (FETCH_SIZE, WRITE_SIZE) of other kernels.
+ */
+int pm_copy_probe(int64_t n_bytes, int32_t repeats, double* out_ms);
+
 /* Timing of the last pm_widom / pm_energy_grid / pm_vdw_energy /
  * pm_electrostatic_energy call (HIP events on the library stream): the whole
  * device pipeline of the call in ms, and the site x replicated-atom pair

@@ -81,6 +81,7 @@ _SIGS = {
     "pm_get_tuning": (ctypes.c_int, [ctypes.c_int32, D]),
     "pm_autotune_widom": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int64,
                                          ctypes.c_int32, D, D, D]),
+    "pm_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, D]),
     "pm_last_kernel_stats": (ctypes.c_int, [D, D]),
     "pm_last_pair_kernel_stats": (ctypes.c_int, [D, D]),
 }

@@ -1015,4 +1015,13 @@ __global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
     block_sum_k<GK>(v, part, part + 3);
 }
 
+// ----------------------------------------------------------- measurement ----
+// Plain device copy (16-byte vector loads and stores, grid-stride): the
+// calibration launch for HBM traffic counters, n16 = bytes / 16.
+__global__ __launch_bounds__(WG) void copy_probe_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                         long long n16) {
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < n16; i += (long long)gridDim.x * WG)
+        dst[i] = src[i];
+}
+
 }  // namespace pmk

@@ -1678,6 +1678,40 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
     return PM_OK;
 }
 
+int pm_copy_probe(int64_t n_bytes, int32_t repeats, double* out_ms) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (n_bytes < 16 || n_bytes % 16 != 0 || n_bytes > (int64_t(1) << 34) || repeats < 1 || repeats > 100)
+        return fail(PM_ERR_ARG, "pm_copy_probe: bytes must be a multiple of 16 in [16, 2^34], repeats in [1, 100]");
+    double2* src = nullptr;
+    double2* dst = nullptr;
+    if (hipMalloc(&src, (size_t)n_bytes) != hipSuccess || hipMalloc(&dst, (size_t)n_bytes) != hipSuccess) {
+        (void)hipFree(src);
+        return fail(PM_ERR_OOM, "pm_copy_probe: allocation failed");
+    }
+    hipError_t e = hipMemsetAsync(src, 0, (size_t)n_bytes, g_ctx.stream);
+    const long long n16 = n_bytes / 16;
+    const unsigned grid = (unsigned)std::min<long long>((n16 + WG - 1) / WG, 65536);
+    float best = INFINITY;
+    for (int r = 0; r < repeats && e == hipSuccess; ++r) {
+        e = hipEventRecord(g_ctx.ev0, g_ctx.stream);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(WG), 0, g_ctx.stream, src, dst, n16);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(g_ctx.ev1, g_ctx.stream);
+        if (e == hipSuccess) e = hipEventSynchronize(g_ctx.ev1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
+        best = std::min(best, ms);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.stream);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return fail(PM_ERR_DEVICE, std::string("pm_copy_probe: ") + hipGetErrorString(e));
+    if (out_ms) *out_ms = best;
+    return PM_OK;
+}
+
 int pm_last_kernel_stats(double* kernel_ms, double* pair_evals) {
     if (kernel_ms) *kernel_ms = g_ctx.last_ms;
     if (pair_evals) *pair_evals = g_ctx.last_pairs;
