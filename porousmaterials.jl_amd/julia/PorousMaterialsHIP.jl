@@ -16,7 +16,9 @@ the device.
 module PorousMaterialsHIP
 
 using PorousMaterials
+using ChainRulesCore
 using LinearAlgebra
+using Optim
 using Printf
 using Statistics
 
@@ -442,6 +444,213 @@ function energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljforcefield::L
     end
     return Grid(crystal.box, n_pts, data, units,
                 center ? crystal.box.f_to_c * [-0.5, -0.5, -0.5] : [0.0, 0.0, 0.0])
+end
+
+# ------------------------------------------------------------ adjoint ----
+# (epsilon_ij, sigma_ij) of the reference's mixing rules (src/forcefield.jl:75-92),
+# generic in the number type so that a complex step differentiates it exactly
+function mix(rule::AbstractString, ea, sa, eb, sb)
+    r = lowercase(rule)
+    if r == "lorentz-berthelot"
+        return sqrt(ea * eb), (sa + sb) / 2
+    elseif r == "kong"
+        es6 = sqrt(ea * sa^6 * eb * sb^6)
+        es12 = (((ea * sa^12)^(1 / 13) + (eb * sb^12)^(1 / 13)) / 2)^13
+        return es6^2 / es12, (es12 / es6)^(1 / 6)
+    else
+        return sqrt(ea * eb), sqrt(sa * sb)
+    end
+end
+
+"""
+    mixing_jacobian(ljff, mixing_rules) -> jac (4 x n x n)
+
+d(eps_ij, sigma_ij)/d(eps_i, sigma_i) for the dense tables' species order (the
+zero-epsilon slot last, all zeros), by complex step: jac[:, j, i] = (d eps/d eps_i,
+d eps/d sigma_i, d sigma/d eps_i, d sigma/d sigma_i) for framework species i and
+probe species j, the layout pm_widom_gradient reads as jac[(i*n + j)*4 + k].
+"""
+function mixing_jacobian(ljff::LJForceField, mixing_rules::AbstractString)
+    sp = ff_species(ljff)
+    n = length(sp) + 1
+    jac = zeros(4, n, n)
+    h = 1e-30
+    for (i, a) in enumerate(sp), (j, b) in enumerate(sp)
+        ea, sa, eb, sb = ljff.pure_ϵ[a], ljff.pure_σ[a], ljff.pure_ϵ[b], ljff.pure_σ[b]
+        e1, s1 = mix(mixing_rules, complex(ea, h), sa, eb, sb)
+        e2, s2 = mix(mixing_rules, ea, complex(sa, h), eb, sb)
+        jac[:, j, i] .= (imag(e1) / h, imag(e2) / h, imag(s1) / h, imag(s2) / h)
+    end
+    return jac
+end
+
+"""
+    henry_coefficient_gradient(crystal, molecule, temperature, ljff;
+                               mixing_rules="Lorentz-Berthelot", kwargs...) -> (results, grad)
+
+One adjoint device pass (pm_widom_gradient): the `henry_coefficient` results
+`Dict` and d ln K_H / d theta for every framework species, theta = its pure
+epsilon, its pure sigma (the probe's held fixed) and a uniform shift of the
+charges of its atoms: `grad[species] = (epsilon=..., sigma=..., charge=...)`.
+"""
+function henry_coefficient_gradient(crystal::Crystal, molecule::Molecule, temperature::Float64,
+                                    ljff::LJForceField; mixing_rules::AbstractString="Lorentz-Berthelot",
+                                    insertions_per_volume::Union{Int, Float64}=200,
+                                    ewald_precision::Float64=1e-6, seed::Integer=0, precision::Symbol=:mixed)
+    nb_insertions = max(N_BLOCKS, ceil(Int, insertions_per_volume * crystal.box.Ω))
+    nb_per_block = ceil(Int, nb_insertions / N_BLOCKS)
+    repfactors = replication_factors(crystal.box, ljff)
+    xtal_rep = replicate(crystal, repfactors)
+    fw = framework_handle(crystal, ljff, repfactors)
+    ff = forcefield_handle(ljff)
+    ew = C_NULL
+    ew_h = nothing
+    if has_charges(crystal) & has_charges(molecule)
+        eparams = setup_Ewald_sum(xtal_rep.box, sqrt(ljff.r²_cutoff), ϵ=ewald_precision, verbose=false)
+        ew_h = ewald_handle(fw, eparams)
+        ew = ew_h.ptr
+    end
+    pr = probe_handle(molecule, ljff)
+    jac = mixing_jacobian(ljff, mixing_rules)
+    n = size(jac, 2)
+    sums = zeros(3, N_BLOCKS)
+    grads = zeros(3, n, N_BLOCKS)
+    check(ccall((:pm_widom_gradient, libpmhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, UInt64, Int32, Int64, Int32,
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int32(N_BLOCKS), Int64(nb_per_block),
+                precision_code(precision), jac, sums, grads))
+    results = widom_results(sums, nb_per_block, temperature, crystal_density(xtal_rep), crystal)
+    S = sum(sums[1, :])
+    G = dropdims(sum(grads, dims=3), dims=3)
+    sp = ff_species(ljff)
+    grad = Dict{Symbol, NamedTuple}()
+    for s in unique(crystal.atoms.species)
+        k = findfirst(==(s), sp)
+        d = -G[:, k] / (temperature * S)
+        grad[s] = (epsilon=d[1], sigma=d[2], charge=d[3])
+    end
+    return results, grad
+end
+
+"""
+    henry_kh(theta, crystal, molecule, temperature, ljff; species, mixing_rules, kwargs...)
+
+K_H (mol/(kg Pa)) as a function of theta = [eps_t; sigma_t; dq_t] for the
+framework species `species` (pure parameters, and a uniform shift of the
+charges of their atoms): the differentiable form of `henry_coefficient`,
+with the ChainRulesCore rrule below as its backward pass.
+"""
+function henry_kh(theta::AbstractVector, crystal::Crystal, molecule::Molecule, temperature::Float64,
+                  ljff::LJForceField; species::Vector{Symbol}, mixing_rules::AbstractString="Lorentz-Berthelot",
+                  kwargs...)
+    xtal, ff = apply_parameters(theta, crystal, ljff, species, mixing_rules)
+    return henry_coefficient(xtal, molecule, temperature, ff; verbose=false, autosave=false,
+                             kwargs...)["henry coefficient [mol/(kg-Pa)]"]
+end
+
+# The crystal and force field at parameters theta: pure (eps, sigma) of
+# `species` replaced, cross terms re-mixed, and the charges of the atoms of
+# each species shifted by dq_t.
+function apply_parameters(theta, crystal::Crystal, ljff::LJForceField, species::Vector{Symbol},
+                          mixing_rules::AbstractString)
+    nt = length(species)
+    pure_ϵ = copy(ljff.pure_ϵ)
+    pure_σ = copy(ljff.pure_σ)
+    for (t, s) in enumerate(species)
+        pure_ϵ[s] = theta[t]
+        pure_σ[s] = theta[nt + t]
+    end
+    ϵ = Dict{Symbol, Dict{Symbol, Float64}}()
+    σ² = Dict{Symbol, Dict{Symbol, Float64}}()
+    for a in keys(pure_σ)
+        ϵ[a] = Dict{Symbol, Float64}()
+        σ²[a] = Dict{Symbol, Float64}()
+        for b in keys(pure_σ)
+            e, sg = mix(mixing_rules, pure_ϵ[a], pure_σ[a], pure_ϵ[b], pure_σ[b])
+            ϵ[a][b] = e
+            σ²[a][b] = sg^2
+        end
+    end
+    ff = LJForceField(ljff.name, pure_σ, pure_ϵ, σ², ϵ, ljff.r²_cutoff)
+    q = copy(crystal.charges.q)
+    for c in 1:crystal.charges.n
+        x = crystal.charges.coords.xf[:, c]
+        k = findfirst(i -> crystal.atoms.coords.xf[:, i] == x, 1:crystal.atoms.n)
+        k === nothing && continue
+        t = findfirst(==(crystal.atoms.species[k]), species)
+        t === nothing || (q[c] += theta[2 * nt + t])
+    end
+    charges = Charges(q, Frac(crystal.charges.coords.xf))
+    xtal = Crystal(crystal.name, crystal.box, crystal.atoms, charges, crystal.bonds, crystal.symmetry)
+    return xtal, ff
+end
+
+function ChainRulesCore.rrule(::typeof(henry_kh), theta::AbstractVector, crystal::Crystal, molecule::Molecule,
+                              temperature::Float64, ljff::LJForceField; species::Vector{Symbol},
+                              mixing_rules::AbstractString="Lorentz-Berthelot", kwargs...)
+    xtal, ff = apply_parameters(theta, crystal, ljff, species, mixing_rules)
+    results, grad = henry_coefficient_gradient(xtal, molecule, temperature, ff; mixing_rules=mixing_rules,
+                                               kwargs...)
+    kh = results["henry coefficient [mol/(kg-Pa)]"]
+    dlnk = vcat([grad[s].epsilon for s in species], [grad[s].sigma for s in species],
+                [grad[s].charge for s in species])
+    henry_kh_pullback(ȳ) = (NoTangent(), ȳ * kh .* dlnk, NoTangent(), NoTangent(), NoTangent(), NoTangent())
+    return kh, henry_kh_pullback
+end
+
+# ------------------------------------------------------ forces, minimum ----
+"""
+    site_forces(crystal, molecule, ljff; eparams=nothing, precision=:fp64) -> (E, dE)
+
+Energy (K) and dE/d(site Cartesian xyz) (3 x n_sites, K/A) of a molecule pose
+in the (replicated) crystal through pm_pose_gradient: LJ spheres first, then
+charges.
+"""
+function site_forces(crystal::Crystal, molecule::Molecule{Frac}, ljff::LJForceField; eparams=nothing,
+                     precision::Symbol=:fp64, rep::Tuple{Int, Int, Int}=(1, 1, 1))
+    sp = ff_species(ljff)
+    idx = Dict(s => Int32(i - 1) for (i, s) in enumerate(sp))
+    charged = eparams !== nothing && has_charges(molecule)
+    species = vcat(Int32[idx[s] for s in molecule.atoms.species], fill(Int32(-1), charged ? molecule.charges.n : 0))
+    q = vcat(zeros(molecule.atoms.n), charged ? molecule.charges.q : Float64[])
+    xf = charged ? hcat(molecule.atoms.coords.xf, molecule.charges.coords.xf) : molecule.atoms.coords.xf
+    fw = framework_handle(crystal, ljff, rep)
+    ew_h = charged ? ewald_handle(fw, eparams) : nothing
+    e = zeros(3)
+    g = zeros(3, length(species))
+    check(ccall((:pm_pose_gradient, libpmhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Float64}, Int64, Ptr{Float64}, Int32,
+                 Ptr{Float64}, Ptr{Float64}),
+                fw.ptr, forcefield_handle(ljff).ptr, charged ? ew_h.ptr : C_NULL, Int32(length(species)), species,
+                q, Int64(1), Matrix{Float64}(xf), precision_code(precision), e, g))
+    return sum(e), g
+end
+
+"""
+    find_energy_minimum(xtal, molecule, ljff) -> (molecule, E_min kJ/mol)
+
+`src/energy_min.jl:19-58` with Optim's L-BFGS on device forces in place of
+Nelder-Mead on `vdw_energy`: centre of mass only, as the reference.
+"""
+function find_energy_minimum(xtal::Crystal, molecule::Molecule, ljff::LJForceField)
+    rep = replication_factors(xtal.box, ljff)
+    xtal_rep = replicate(xtal, rep)
+    m = isa(molecule, Molecule{Cart}) ? Frac(molecule, xtal_rep.box) :
+        (translate_to!(molecule, Frac(molecule.com.xf ./ rep)); molecule)
+    F = xtal_rep.box.f_to_c
+    function fg!(f, g, xf)
+        translate_to!(m, Frac(mod.(xf, 1.0)))
+        e, dE = site_forces(xtal, m, ljff; rep=rep)
+        g === nothing || (g .= transpose(F) * vec(sum(dE, dims=2)))  # d/d xf = F^T d/dx
+        return e * 8.314 / 1000
+    end
+    res = optimize(Optim.only_fg!((f, g, x) -> (g === nothing ? fg!(f, nothing, x) :
+                                                (v = fg!(f, g, x); g .*= 8.314 / 1000; v))),
+                   m.com.xf, LBFGS())
+    xf_min = mod.(mod.(Optim.minimizer(res), 1.0) .* rep, 1.0)
+    translate_to!(m, Frac(xf_min ./ rep))
+    return Frac(Cart(m, xtal_rep.box), xtal.box), Optim.minimum(res)
 end
 
 """
