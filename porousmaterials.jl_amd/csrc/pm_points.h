@@ -306,20 +306,6 @@ __global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_
 }
 
 // ---------------------------------------------------------------- pairs ----
-// q erfc(alpha r) / r with erfc(alpha r) from the cubic table: segment
-// i = floor(r/h), clamped to the table (r beyond r_sr is masked by the
-// caller), t = r/h - i in [0, 1).
-__device__ inline float coul_table(double r2, double q, float inv_h, const float4* tab) {
-    const float rf2 = (float)r2;
-    const float rinv = __builtin_amdgcn_rsqf(rf2);
-    const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
-    const int i = (int)x;
-    const float t = x - (float)i;
-    const float4 c = tab[i];
-    const float e = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-    return (float)q * e * rinv;
-}
-
 // One candidate pair: LJ into plj, real-space Ewald into psr.
 template <int ESM, typename Real>
 __device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool is_lj, double r2lj,
@@ -327,14 +313,27 @@ __device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool i
                                  Real& plj, Real& psr) {
     const bool in = r2 < r2lj;
     ovl |= in & (r2 < R2_OVERLAP) & is_lj;
-    const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
-    plj += in ? e : (Real)0;
-    if constexpr (ESM != 0) {
+    if constexpr (ESM != 0 && sizeof(Real) == 4) {
+        // one transcendental per pair: 1/r from v_rsq, 1/r^2 as its square
+        const float rf2 = (float)r2;
+        const float rinv = __builtin_amdgcn_rsqf(rf2);
+        const float s = p.sig2 * (rinv * rinv);
+        const float s3 = s * s * s;
+        plj += in ? p.eps4 * fmaf(s3, s3, -s3) : 0.0f;
         const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (q != 0.0);
-        Real cv;
-        if constexpr (sizeof(Real) == 4) cv = coul_table(r2, q, inv_h, tab);
-        else cv = PairMath<Real>::coul(r2, q, alpha);
-        psr += ins ? cv : (Real)0;
+        const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
+        const int i = (int)x;
+        const float t = x - (float)i;
+        const float4 c = tab[i];
+        const float ec = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+        psr += ins ? (float)q * ec * rinv : 0.0f;
+    } else {
+        const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
+        plj += in ? e : (Real)0;
+        if constexpr (ESM != 0) {
+            const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (q != 0.0);
+            psr += ins ? PairMath<Real>::coul(r2, q, alpha) : (Real)0;
+        }
     }
 }
 
@@ -350,7 +349,9 @@ __device__ inline void pair_term_grad(double r2, double q, const LJP<Real>& p, R
     ovl |= in & (r2 < R2_OVERLAP) & is_lj;
     const bool lj_in = in & is_lj;
     if constexpr (sizeof(Real) == 4) {
-        const float s = p.sig2 * __builtin_amdgcn_rcpf((float)r2);
+        // 1/r^2 exactly as pair_term computes it
+        const float s = p.sig2 * ((ESM != 0) ? __builtin_amdgcn_rsqf((float)r2) * __builtin_amdgcn_rsqf((float)r2)
+                                             : __builtin_amdgcn_rcpf((float)r2));
         const float u = s * s * s;
         const float w = fmaf(u, u, -u);
         plj += in ? p.eps4 * w : 0.0f;
