@@ -51,6 +51,7 @@ typedef struct pm_framework pm_framework;
 typedef struct pm_forcefield pm_forcefield;
 typedef struct pm_ewald pm_ewald;
 typedef struct pm_probe pm_probe;
+typedef struct pm_access pm_access;
 
 /* Library version string. */
 const char* pm_version(void);
@@ -116,6 +117,38 @@ int pm_probe_create(int32_t n_lj, const double* lj_xyz, const int32_t* lj_specie
 int pm_probe_destroy(pm_probe* pr);
 
 /*
+ * Accessibility grid (`Grid{Bool}` of `compute_accessibility_grid`,
+ * src/grid.jl:12-18,689-774) over the UNIT cell: n_pts[0]*n_pts[1]*n_pts[2]
+ * bytes, i fastest, nonzero = accessible.  A Widom run given one counts an
+ * insertion whose centre of mass is not `accessible(grid, xf, repfactors)`
+ * (src/grid.jl:778-836) as E_vdw = E_es = +Inf without computing its energy
+ * (src/henry.jl:225-232).  1 <= n_pts[d] <= 65536, at most 2^31 points.
+ */
+int pm_access_create(const int32_t n_pts[3], const uint8_t* data, pm_access** out);
+int pm_access_destroy(pm_access* acc);
+
+/*
+ * Pocket blocking of an energy grid: the steps of `compute_accessibility_grid`
+ * after `energy_grid` (src/grid.jl:704-764), on the host.
+ * data: n_pts[0]*n_pts[1]*n_pts[2] energies, i fastest, in the units of
+ * energy_tol.  block_pockets = 0: out_accessible = data < energy_tol
+ * (src/grid.jl:704-710).  block_pockets = 1: flood fill into segments
+ * (`_flood_fill!`, `_segment_grid`, src/grid.jl:383-456: a seed needs data <= energy_tol, a
+ * neighbour data < energy_tol, 26 neighbours, no periodic wrap), segment
+ * connections across the three unit-cell faces (`_note_connection!`,
+ * `_build_list_of_connections`, :485-530), every simple cycle of the segment / connection graph
+ * (`_translate_into_graph`, `_classify_segments`, :533-629; Johnson's
+ * algorithm), segments on a cycle with a nonzero net unit-cell translation
+ * accessible, all other segments blocked (:632-646, 750-751).
+ * out_accessible: 1 / 0 per point.  out_segments (nullable): the flood fill's
+ * labels per point (1..n_segments, -1 not occupiable), before blocking.
+ * out_counts (nullable): n_segments, segments blocked, simple cycles found.
+ * Fails with PM_ERR_ARG past max_cycles simple cycles (0: 10^8).
+ */
+int pm_block_pockets(const int32_t n_pts[3], const double* data, double energy_tol, int32_t block_pockets,
+                     int64_t max_cycles, uint8_t* out_accessible, int32_t* out_segments, int64_t out_counts[3]);
+
+/*
  * Guest-host LJ energy (K) of n_poses rigid poses: `vdw_energy(crystal,
  * molecule, ljff)` (src/vdw_energetics.jl:25-47,62-64) per pose.
  * site_xf: n_poses x n_sites x 3, replicated-box fractional coordinates.
@@ -160,11 +193,13 @@ int pm_pose_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_e
  * (sum e^{-E/T}, sum E_vdw e^{-E/T}, sum E_es e^{-E/T}) with Inf energies
  * excluded from the weighted sums (src/henry.jl:236-243).
  * out_energies (nullable): n_blocks*n_per_block x 2 (E_vdw, E_es) in K.
+ * access (nullable): the `accessibility_grid` argument of henry_coefficient
+ * (src/henry.jl:38,225), over the unit cell of the framework's replication.
  */
 int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
              const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
              int64_t n_per_block, int32_t precision, double* out_block_sums,
-             double* out_energies);
+             double* out_energies, const pm_access* access);
 
 /*
  * Energy grid (src/grid.jl:291-362, single-site probe): point (i,j,k) of an
@@ -193,12 +228,13 @@ int pm_energy_grid(const pm_framework* fw, const pm_forcefield* ff, const pm_ewa
  * d ln K_H / d theta = -(1/T) sum_b G_b(theta) / sum_b S_b (src/henry.jl:112-135
  * differentiated).  The reference has no derivative of henry_coefficient;
  * this is the backward pass of a ChainRulesCore rrule around it.
- * At most PM_MAX_GRAD_TYPES framework species.
+ * At most PM_MAX_GRAD_TYPES framework species.  access as for pm_widom
+ * (blocked insertions carry zero weight and zero derivative).
  */
 int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
                       const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
                       int64_t n_per_block, int32_t precision, const double* jac,
-                      double* out_block_sums, double* out_block_grads);
+                      double* out_block_sums, double* out_block_grads, const pm_access* access);
 
 #define PM_MAX_BATCH_RUNS 65536  /* runs per pm_widom_batch / pm_energy_grid_batch call */
 
@@ -213,6 +249,7 @@ typedef struct pm_widom_run {
     int64_t n_per_block;
     int32_t n_blocks;
     int32_t reserved;               /* 0 */
+    const pm_access* access;        /* NULL: no accessibility grid */
 } pm_widom_run;
 
 /*

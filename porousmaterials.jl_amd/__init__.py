@@ -7,6 +7,7 @@ the reference's interface: ``henry_coefficient``, ``energy_grid``,
 ``vdw_energy``, ``electrostatic_potential_energy``,
 ``total_electrostatic_potential_energy`` and the data types they take.
 """
+from .accessibility import accessible, compute_accessibility_grid
 from .box import Box, replicate_box, unit_cube
 from .crystal import (Crystal, assert_P1_symmetry, crystal_density, has_charges, net_charge,
                       read_cif, register_atomic_masses, replicate, strip_numbers_from_atom_labels)
@@ -16,7 +17,8 @@ from .ewald import (EPS0, FOUR_PI_EPS0, Eikr, EwaldParams, EwaldSum, determine_e
                     make_eikr, precompute_kvec_wts, required_kreps, setup_Ewald_sum, total)
 from .forcefield import LJForceField, forcefield_coverage, lennard_jones, replication_factors
 from .energy_min import find_energy_minima, find_energy_minimum, find_energy_minimum_gridsearch
-from .grid import Grid, energy_grid, energy_grids, required_n_pts
+from .grid import (Grid, box_isapprox, energy_grid, energy_grids, id_to_xf, read_cube, required_n_pts,
+                   write_cube, xf_to_id)
 from .henry import (K_TO_KJ_PER_MOL, N_BLOCKS, UNIV_GAS_CONST, block_statistics, henry_coefficient,
                     henry_coefficient_gradient, henry_coefficients,
                     henry_result_savename)

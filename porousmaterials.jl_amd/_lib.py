@@ -35,6 +35,7 @@ PP = ctypes.POINTER(ctypes.c_void_p)
 D = ctypes.POINTER(ctypes.c_double)
 I32 = ctypes.POINTER(ctypes.c_int32)
 I64 = ctypes.POINTER(ctypes.c_int64)
+U8 = ctypes.POINTER(ctypes.c_uint8)
 
 
 
@@ -43,7 +44,7 @@ class WidomRun(ctypes.Structure):
     _fields_ = [("framework", P), ("forcefield", P), ("ewald", P), ("probe", P),
                 ("temperature", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("n_per_block", ctypes.c_int64), ("n_blocks", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("access", P)]
 
 
 class GridRun(ctypes.Structure):
@@ -66,15 +67,18 @@ _SIGS = {
     "pm_ewald_info": (ctypes.c_int, [P, I64]),
     "pm_probe_create": (ctypes.c_int, [ctypes.c_int32, D, I32, ctypes.c_int32, D, D, PP]),
     "pm_probe_destroy": (ctypes.c_int, [P]),
+    "pm_access_create": (ctypes.c_int, [I32, U8, PP]),
+    "pm_access_destroy": (ctypes.c_int, [P]),
+    "pm_block_pockets": (ctypes.c_int, [I32, D, ctypes.c_double, ctypes.c_int32, ctypes.c_int64, U8, I32, I64]),
     "pm_vdw_energy": (ctypes.c_int, [P, P, ctypes.c_int32, I32, ctypes.c_int64, D, ctypes.c_int32, D]),
     "pm_electrostatic_energy": (ctypes.c_int, [P, P, ctypes.c_int32, D, ctypes.c_int64, D,
                                                ctypes.c_int32, D, D]),
     "pm_pose_gradient": (ctypes.c_int, [P, P, P, ctypes.c_int32, I32, D, ctypes.c_int64, D, ctypes.c_int32, D, D]),
     "pm_widom": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
-                                ctypes.c_int64, ctypes.c_int32, D, D]),
+                                ctypes.c_int64, ctypes.c_int32, D, D, P]),
     "pm_energy_grid": (ctypes.c_int, [P, P, P, P, I32, ctypes.c_int32, D]),
     "pm_widom_gradient": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
-                                         ctypes.c_int64, ctypes.c_int32, D, D, D]),
+                                         ctypes.c_int64, ctypes.c_int32, D, D, D, P]),
     "pm_widom_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(WidomRun), ctypes.c_int32, D]),
     "pm_energy_grid_batch": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(GridRun), ctypes.c_int32]),
     "pm_set_tuning": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double]),

@@ -130,10 +130,11 @@ def replicate(crystal: Crystal, repfactors) -> Crystal:
     out = Crystal(crystal.name, replicate_box(crystal.box, repfactors),
                   crystal.species * len(shifts), rep(crystal.xf),
                   np.tile(crystal.atom_q, len(shifts)), crystal.symmetry_p1)
-    # a replica of a replica holds the same atom set as one product replica of
-    # the unit cell (only the order differs), which is all the device needs
-    out._unit = crystal._unit if crystal._unit is not None else crystal
-    out._rep = tuple(int(a) * int(b) for a, b in zip(crystal._rep, (ra_n, rb_n, rc_n)))
+    # the device sees `crystal` as the unit cell and these factors: energy-grid
+    # points and accessibility lookups are fractional coordinates of the cell
+    # that was replicated, also when that cell is itself a replica
+    out._unit = crystal
+    out._rep = (ra_n, rb_n, rc_n)
     return out
 
 
@@ -209,8 +210,13 @@ def read_cif(path: str, species_col=("_atom_site_type_symbol", "_atom_site_label
     for headers, rows in loops:
         if "_atom_site_fract_x" in headers:
             atom_loop = (headers, rows)
-        if "_symmetry_equiv_pos_as_xyz" in headers or "_space_group_symop_operation_xyz" in headers:
-            ops = [" ".join(r).replace(" ", "").lower() for r in rows]
+        xyz_key = next((h for h in ("_symmetry_equiv_pos_as_xyz", "_space_group_symop_operation_xyz")
+                        if h in headers), None)
+        if xyz_key is not None:
+            # the operation column only (a loop may also carry a site id column)
+            k_op = headers.index(xyz_key)
+            ops = [(" ".join(r[k_op:]) if len(r) != len(headers) else r[k_op]).replace(" ", "").lower()
+                   for r in rows]
             if any(op not in ("x,y,z", "'x,y,z'") for op in ops):
                 p1 = False
     if atom_loop is None:

@@ -73,6 +73,8 @@ struct RunArgs {
     double temperature;
     double* partials;        // per insertion workgroup of the run: (S, W_vdw, W_es)
     double* out_e;           // nullable: (E_vdw, E_es) per insertion
+    const unsigned char* acc;  // nullable: accessibility grid over the unit cell, i fastest
+    int acc_n[3];
     // grid
     int grid_n[3], brick[3], nbrick[3];
     double grid_q;
@@ -137,10 +139,45 @@ __device__ inline double grid_coord(double idx, int n, double rep) {
     return (n > 1 ? idx / (double)(n - 1) : 0.0) / rep;
 }
 
+// Accessibility of an insertion's centre of mass u (replicated fractional):
+// unit-cell coordinates x = mod(u R, 1), the nearest grid point
+// round(x (n - 1)) (ties to even) and its 26 neighbours, index -1 -> n - 1 and
+// n -> 0; open if any of the 27 points is accessible (src/grid.jl:778-836).
+// Contraction is off so that u R and x (n - 1) round as two plain products.
+__device__ inline bool access_open(const RunArgs& a, const double* u) {
+#pragma clang fp contract(off)
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double x = u[d] * a.repd[d];
+        x = x - floor(x);
+        c[d] = (int)rint(x * (double)(a.acc_n[d] - 1));
+    }
+    const int n0 = a.acc_n[0], n1 = a.acc_n[1], n2 = a.acc_n[2];
+    bool open = false;
+    for (int dk = -1; dk <= 1; ++dk) {
+        int k = c[2] + dk;
+        k = k < 0 ? n2 - 1 : (k >= n2 ? 0 : k);
+        for (int dj = -1; dj <= 1; ++dj) {
+            int j = c[1] + dj;
+            j = j < 0 ? n1 - 1 : (j >= n1 ? 0 : j);
+            for (int di = -1; di <= 1; ++di) {
+                int i = c[0] + di;
+                i = i < 0 ? n0 - 1 : (i >= n0 ? 0 : i);
+                open |= a.acc[(long long)i + (long long)n0 * ((long long)j + (long long)n1 * k)] != 0;
+            }
+        }
+    }
+    return open;
+}
+
 // ------------------------------------------------------------ Widom gen ----
 // One work-item per insertion slot: pose from the stream, sites wrapped into
 // [0,1)^3 (integer shifts change neither the nearest-image pairs nor the
-// Fourier phases m.(R o x)), binned into the run's cells.
+// Fourier phases m.(R o x)), binned into the run's cells.  An insertion in a
+// blocked pocket of the run's accessibility grid emits no point (its slots
+// look like padding); the combine kernels give it E_vdw = E_es = +Inf
+// (src/henry.jl:225-232).
 __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
     const RunArgs& a = g.runs[sg.run];
@@ -151,16 +188,20 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     const bool valid = i < a.n_per_block;
     const int ns = a.nsite;
     const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
-    if (!valid) {
+    double u[6];
+    bool emit = valid;
+    if (valid) {
+        const unsigned long long gi = (unsigned long long)b * a.n_per_block + i;
+        insertion_uniforms(a.seed, gi, u);
+        if (a.acc != nullptr) emit = access_open(a, u);
+    }
+    if (!emit) {
         for (int s = 0; s < ns; ++s) {
             g.pos[p0 + s] = make_double4(0.0, 0.0, 0.0, 0.0);
             g.cell[p0 + s] = 0xffffffffu;
         }
         return;
     }
-    const unsigned long long gi = (unsigned long long)b * a.n_per_block + i;
-    double u[6];
-    insertion_uniforms(a.seed, gi, u);
     double R[9];
     if (a.rotate) arvo_rotation(&u[3], R);
     const int n0 = a.ncell[0], n1 = a.ncell[1], n2 = a.ncell[2];
@@ -772,8 +813,13 @@ __global__ __launch_bounds__(WG) void widom_combine_kernel(GroupArgs g) {
     const bool valid = i < a.n_per_block;
     const int ns = a.nsite;
     const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
+    // a blocked insertion emitted no point: its per-slot sums were never written
+    const bool blocked = valid && a.acc != nullptr && g.pos[p0].w == 0.0;
     double vdw = 0.0, es = 0.0;
-    if (valid) {
+    if (blocked) {
+        vdw = INFINITY;
+        es = INFINITY;
+    } else if (valid) {
         for (int s = 0; s < ns; ++s) {
             vdw += g.out_lj[p0 + s];
             if (a.esm != 0) es += a.site_q[s] * (g.out_sr[p0 + s] * a.inv4pe + g.out_lr[p0 + s]);
@@ -977,7 +1023,11 @@ __global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
     double de[GT], dsg[GT], dq[GT];
 #pragma unroll
     for (int t = 0; t < GT; ++t) { de[t] = 0.0; dsg[t] = 0.0; dq[t] = 0.0; }
-    if (valid) {
+    const bool blocked = valid && a.acc != nullptr && g.pos[p0].w == 0.0;
+    if (blocked) {
+        vdw = INFINITY;
+        es = INFINITY;
+    } else if (valid) {
         for (int s = 0; s < ns; ++s) {
             vdw += g.out_lj[p0 + s];
             if (a.esm != 0) es += a.site_q[s] * (g.out_sr[p0 + s] * a.inv4pe + g.out_lr[p0 + s]);

@@ -10,9 +10,11 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
+#include "pm_pockets.h"
 #include "pm_points.h"
 #include "pmhip.h"
 
@@ -193,6 +195,11 @@ struct pm_probe {
     Site site[PM_MAX_SITES];
     int32_t lj_species[PM_MAX_SITES];
     bool charged = false;
+};
+
+struct pm_access {
+    int32_t n[3] = {0, 0, 0};
+    unsigned char* d = nullptr;  // 0 / 1 per unit-cell grid point, i fastest
 };
 
 namespace {
@@ -543,6 +550,10 @@ int widom_prepare(const pm_widom_run& in, PointRun& r) {
     a.seed = in.seed;
     a.n_per_block = in.n_per_block;
     a.rotate = pr->rotate;
+    if (in.access) {
+        a.acc = in.access->d;
+        for (int d = 0; d < 3; ++d) a.acc_n[d] = in.access->n[d];
+    }
     a.n_blocks = in.n_blocks;
     a.chunks_per_block = (int)chunks;
     a.temperature = in.temperature;
@@ -1374,6 +1385,63 @@ int pm_probe_destroy(pm_probe* pr) {
     return PM_OK;
 }
 
+int pm_access_create(const int32_t n_pts[3], const uint8_t* data, pm_access** out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (!n_pts || !data || !out) return fail(PM_ERR_ARG, "pm_access_create: null pointer");
+    int64_t npts = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (n_pts[d] < 1 || n_pts[d] > 65536) return fail(PM_ERR_ARG, "pm_access_create: n_pts out of [1, 65536]");
+        npts *= n_pts[d];
+    }
+    if (npts > ((int64_t)1 << 31)) return fail(PM_ERR_ARG, "pm_access_create: more than 2^31 grid points");
+    std::vector<unsigned char> h((size_t)npts);
+    for (int64_t p = 0; p < npts; ++p) h[(size_t)p] = data[p] != 0 ? 1 : 0;
+    auto* acc = new pm_access();
+    for (int d = 0; d < 3; ++d) acc->n[d] = n_pts[d];
+    if (hipMalloc(&acc->d, (size_t)npts) != hipSuccess) {
+        delete acc;
+        return fail(PM_ERR_OOM, "pm_access_create: device allocation failed");
+    }
+    if (hipMemcpy(acc->d, h.data(), (size_t)npts, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(acc->d);
+        delete acc;
+        return fail(PM_ERR_DEVICE, "pm_access_create: host-to-device copy failed");
+    }
+    *out = acc;
+    return PM_OK;
+}
+
+int pm_access_destroy(pm_access* acc) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!acc) return PM_OK;
+    (void)hipFree(acc->d);
+    delete acc;
+    return PM_OK;
+}
+
+int pm_block_pockets(const int32_t n_pts[3], const double* data, double energy_tol, int32_t block_pockets,
+                     int64_t max_cycles, uint8_t* out_accessible, int32_t* out_segments, int64_t out_counts[3]) {
+    if (!n_pts || !data || !out_accessible) return fail(PM_ERR_ARG, "pm_block_pockets: null pointer");
+    int64_t npts = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (n_pts[d] < 1 || n_pts[d] > 65536) return fail(PM_ERR_ARG, "pm_block_pockets: n_pts out of [1, 65536]");
+        npts *= n_pts[d];
+    }
+    if (npts > ((int64_t)1 << 31)) return fail(PM_ERR_ARG, "pm_block_pockets: more than 2^31 grid points");
+    if (block_pockets != 0 && block_pockets != 1) return fail(PM_ERR_ARG, "pm_block_pockets: block_pockets must be 0 or 1");
+    if (max_cycles < 0) return fail(PM_ERR_ARG, "pm_block_pockets: negative max_cycles");
+    const int64_t cap = max_cycles == 0 ? (int64_t)100000000 : max_cycles;
+    try {
+        std::string err = pmhost::block_pockets(n_pts, data, energy_tol, block_pockets == 1, cap, out_accessible,
+                                                out_segments, out_counts);
+        if (!err.empty()) return fail(PM_ERR_ARG, err);
+    } catch (const std::bad_alloc&) {
+        return fail(PM_ERR_OOM, "pm_block_pockets: host allocation failed");
+    }
+    return PM_OK;
+}
+
 int pm_vdw_energy(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sites,
                   const int32_t* site_species, int64_t n_poses, const double* site_xf,
                   int32_t precision, double* out) {
@@ -1471,7 +1539,7 @@ int pm_pose_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_e
 int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
              const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
              int64_t n_per_block, int32_t precision, double* out_block_sums,
-             double* out_energies) {
+             double* out_energies, const pm_access* access) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = require_init()) return rc;
     if (!out_block_sums) return fail(PM_ERR_ARG, "pm_widom: null output pointer");
@@ -1486,6 +1554,7 @@ int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew
     in.seed = seed;
     in.n_per_block = n_per_block;
     in.n_blocks = n_blocks;
+    in.access = access;
     std::vector<PointRun> runs(1);
     if (int rc = widom_prepare(in, runs[0])) return rc;
     double* d_energies = nullptr;
@@ -1510,7 +1579,7 @@ int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew
 int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
                       const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
                       int64_t n_per_block, int32_t precision, const double* jac,
-                      double* out_block_sums, double* out_block_grads) {
+                      double* out_block_sums, double* out_block_grads, const pm_access* access) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = require_init()) return rc;
     if (!out_block_sums || !out_block_grads || !jac) return fail(PM_ERR_ARG, "pm_widom_gradient: null pointer");
@@ -1525,6 +1594,7 @@ int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_
     in.seed = seed;
     in.n_per_block = n_per_block;
     in.n_blocks = n_blocks;
+    in.access = access;
     std::vector<PointRun> runs(1);
     if (int rc = widom_prepare(in, runs[0])) return rc;
     PointRun& r = runs[0];

@@ -135,6 +135,49 @@ def probe(molecule, ljff) -> _Handle:
     return _Handle(out, "pm_probe_destroy")
 
 
+def access(acc_grid) -> _Handle:
+    """Upload an accessibility grid (``Grid`` with boolean ``data`` of shape
+    ``n_pts``, i fastest) for Widom runs; cached on the grid object."""
+    h = getattr(acc_grid, "_device_handle", None)
+    data = np.asarray(acc_grid.data)
+    n = np.array(data.shape, dtype=np.int32)
+    flat = np.ascontiguousarray(data.reshape(-1, order="F").astype(np.uint8))
+    key = (tuple(n), flat.tobytes())
+    if h is not None and h[0] == key:
+        return h[1]
+    init()
+    out = ctypes.c_void_p()
+    check(_lib.load().pm_access_create(iptr(n), flat.ctypes.data_as(_lib.U8), ctypes.byref(out)))
+    handle = _Handle(out, "pm_access_destroy")
+    try:
+        acc_grid._device_handle = (key, handle)
+    except AttributeError:
+        pass
+    return handle
+
+
+def block_pockets(data, energy_tol, block_pockets=True, max_cycles=0):
+    """Host pocket blocking of an energy grid (pm_block_pockets).
+
+    ``data``: (n1, n2, n3) energies in the units of ``energy_tol``.  Returns
+    (accessible bool (n1, n2, n3), segment labels int32 (n1, n2, n3) or None,
+    (n_segments, n_blocked, n_cycles))."""
+    data = np.asarray(data, dtype=np.float64)
+    n = np.array(data.shape, dtype=np.int32)
+    flat = np.ascontiguousarray(data.reshape(-1, order="F"))
+    acc = np.zeros(flat.size, dtype=np.uint8)
+    seg = np.zeros(flat.size, dtype=np.int32) if block_pockets else None
+    counts = np.zeros(3, dtype=np.int64)
+    check(_lib.load().pm_block_pockets(iptr(n), dptr(flat), float(energy_tol), 1 if block_pockets else 0,
+                                       int(max_cycles), acc.ctypes.data_as(_lib.U8),
+                                       iptr(seg) if seg is not None else None,
+                                       counts.ctypes.data_as(_lib.I64)))
+    shape = tuple(int(x) for x in n)
+    return (acc.reshape(shape, order="F").astype(bool),
+            seg.reshape(shape, order="F") if seg is not None else None,
+            tuple(int(c) for c in counts))
+
+
 def last_kernel_stats():
     ms = ctypes.c_double()
     pairs = ctypes.c_double()
@@ -212,9 +255,10 @@ def _ff_handle(ff):
 
 
 def widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision,
-          return_energies=False):
+          return_energies=False, acc=None):
     """Block sums (n_blocks, 3) of a Widom run; ``ff`` is a force-field
-    handle or an LJForceField."""
+    handle or an LJForceField; ``acc`` an accessibility-grid handle
+    (:func:`access`) or None."""
     lib = _lib.load()
     ff = _ff_handle(ff)
     sums = np.zeros(3 * n_blocks)
@@ -222,7 +266,8 @@ def widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision,
     check(lib.pm_widom(fw.ptr, ff.ptr, ew.ptr if ew is not None else None, pr.ptr,
                        float(temperature), int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_blocks),
                        int(n_per_block), precision_code(precision), dptr(sums),
-                       dptr(energies) if energies is not None else None))
+                       dptr(energies) if energies is not None else None,
+                       acc.ptr if acc is not None else None))
     sums = sums.reshape(n_blocks, 3)
     if return_energies:
         return sums, energies.reshape(n_blocks * n_per_block, 2)
@@ -244,13 +289,15 @@ def widom_batch(runs, precision):
     """Block sums of many Widom runs in one call (pm_widom_batch).
 
     ``runs``: sequence of (fw, ff, ew, pr, temperature, seed, n_blocks,
-    n_per_block), handles as for :func:`widom`.  Returns a list of
+    n_per_block[, acc]), handles as for :func:`widom`.  Returns a list of
     (n_blocks, 3) arrays in the order of ``runs``."""
     lib = _lib.load()
     arr = (_lib.WidomRun * len(runs))()
     keep = []
     total = 0
-    for i, (fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block) in enumerate(runs):
+    for i, run in enumerate(runs):
+        fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block = run[:8]
+        acc = run[8] if len(run) > 8 else None
         ff = _ff_handle(ff)
         keep.append(ff)
         r = arr[i]
@@ -262,6 +309,7 @@ def widom_batch(runs, precision):
         r.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         r.n_per_block = int(n_per_block)
         r.n_blocks = int(n_blocks)
+        r.access = acc.ptr if acc is not None else None
         total += int(n_blocks)
     sums = np.zeros(3 * total)
     check(lib.pm_widom_batch(len(runs), arr, precision_code(precision), dptr(sums)))
@@ -298,7 +346,7 @@ def energy_grid_batch(runs, precision):
     return outs
 
 
-def widom_gradient(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision, ljff):
+def widom_gradient(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision, ljff, acc=None):
     """Block sums (n_blocks, 3) and Boltzmann-weighted parameter-derivative
     sums (n_blocks, n_species, 3) of a Widom run (pm_widom_gradient):
     grads[b, t] = sum_i w_i dE_i/d(eps_t, sigma_t, q_t) for framework species t
@@ -312,7 +360,7 @@ def widom_gradient(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, pre
     check(lib.pm_widom_gradient(fw.ptr, ffh.ptr, ew.ptr if ew is not None else None, pr.ptr,
                                 float(temperature), int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_blocks),
                                 int(n_per_block), precision_code(precision), dptr(jac), dptr(sums),
-                                dptr(grads)))
+                                dptr(grads), acc.ptr if acc is not None else None))
     return sums.reshape(n_blocks, 3), grads.reshape(n_blocks, n, 3)
 
 

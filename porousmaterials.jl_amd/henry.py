@@ -16,6 +16,12 @@ Differences from the reference, by design:
   composes rotations cumulatively, ``:218-220``; both are uniform on SO(3)),
   so there is no bond-length drift to check (``:247-261``);
 * all blocks run in one device launch instead of a ``pmap`` over workers.
+
+With ``accessibility_grid`` (a ``Grid`` of booleans over the crystal's unit
+cell, from ``compute_accessibility_grid``) an insertion whose centre of mass
+is not ``accessible(grid, xf, repfactors)`` counts with E = +Inf and is not
+computed (``:225-232``); as in the reference, the grid's box is checked
+against the crystal's only when ``verbose`` (``:57-63``).
 """
 from __future__ import annotations
 
@@ -30,6 +36,7 @@ from . import device
 from .crystal import assert_P1_symmetry, crystal_density, has_charges as xtal_has_charges, replicate
 from .ewald import setup_Ewald_sum
 from .forcefield import replication_factors
+from .grid import box_isapprox
 from .molecule import has_charges as mol_has_charges
 
 UNIV_GAS_CONST = 8.3144598e-5    # m^3 bar / (K mol), src/PorousMaterials.jl:12
@@ -74,6 +81,16 @@ def block_statistics(block_sums, nb_per_block, temperature, rho):
     r["Qst (kJ/mol)"] = -r["⟨U⟩ (kJ/mol)"] + temperature * K_TO_KJ_PER_MOL
     r["err Qst (kJ/mol)"] = float((err_vdw + err_es) * K_TO_KJ_PER_MOL)
     return r
+
+
+def _access_handle(crystal, accessibility_grid, verbose):
+    if accessibility_grid is None:
+        return None
+    if verbose:
+        print("Using provided accessibility grid to block inaccessible pockets")
+        if not box_isapprox(accessibility_grid.box, crystal.box):
+            raise ValueError("accessibility grid box does not match box of %s.\n" % crystal.name)
+    return device.access(accessibility_grid)
 
 
 def _widom_setup(crystal, molecule, temperature, ljff, insertions_per_volume, ewald_precision, verbose):
@@ -135,12 +152,11 @@ def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volum
                       verbose=True, ewald_precision=1e-6, autosave=True, filename_comment="",
                       accessibility_grid=None, seed=0, precision="mixed", return_block_sums=False):
     """K_H and <U> of ``molecule`` (Cartesian template) in ``crystal``."""
-    if accessibility_grid is not None:
-        raise NotImplementedError("accessibility-grid blocking is not supported by the device path")
     t0 = time.time()
+    acc = _access_handle(crystal, accessibility_grid, verbose)
     fw, ff, ew, pr, nb_per_block, rho = _widom_setup(crystal, molecule, temperature, ljff,
                                                      insertions_per_volume, ewald_precision, verbose)
-    sums = device.widom(fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block, precision)
+    sums = device.widom(fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block, precision, acc=acc)
     results = block_statistics(sums, nb_per_block, temperature, rho)
     _finish(results, crystal, molecule, temperature, ljff, insertions_per_volume, sums, nb_per_block,
             rho, t0, verbose, autosave, filename_comment)
@@ -151,23 +167,27 @@ def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volum
 
 def henry_coefficients(pairs, temperature, ljff, insertions_per_volume=200, verbose=False,
                        ewald_precision=1e-6, autosave=False, filename_comment="", seeds=None,
-                       precision="mixed", return_block_sums=False):
+                       precision="mixed", return_block_sums=False, accessibility_grids=None):
     """Screening: ``henry_coefficient`` for every (crystal, molecule) in
     ``pairs``, all runs in one device call (pm_widom_batch).  Each result is
     what ``henry_coefficient`` returns for that pair with the same seed
-    (``seeds[i]``, default 0).  Returns the list of results dicts."""
+    (``seeds[i]``, default 0) and accessibility grid
+    (``accessibility_grids[i]``, default None).  Returns the list of results
+    dicts."""
     t0 = time.time()
     pairs = list(pairs)
     seeds = [0] * len(pairs) if seeds is None else list(seeds)
-    if len(seeds) != len(pairs):
-        raise ValueError("one seed per (crystal, molecule) pair")
+    grids = [None] * len(pairs) if accessibility_grids is None else list(accessibility_grids)
+    if len(seeds) != len(pairs) or len(grids) != len(pairs):
+        raise ValueError("one seed and one accessibility grid (or None) per (crystal, molecule) pair")
     setups, runs = [], []
-    for (crystal, molecule), seed in zip(pairs, seeds):
+    for (crystal, molecule), seed, agrid in zip(pairs, seeds, grids):
+        acc = _access_handle(crystal, agrid, verbose)
         fw, ff, ew, pr, nb_per_block, rho = _widom_setup(crystal, molecule, temperature, ljff,
                                                          insertions_per_volume, ewald_precision,
                                                          verbose)
         setups.append((nb_per_block, rho))
-        runs.append((fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block))
+        runs.append((fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block, acc))
     all_sums = device.widom_batch(runs, precision)
     out = []
     for (crystal, molecule), (nb_per_block, rho), sums in zip(pairs, setups, all_sums):
@@ -179,7 +199,8 @@ def henry_coefficients(pairs, temperature, ljff, insertions_per_volume=200, verb
 
 
 def henry_coefficient_gradient(crystal, molecule, temperature, ljff, insertions_per_volume=200,
-                               ewald_precision=1e-6, seed=0, precision="mixed", verbose=False):
+                               ewald_precision=1e-6, seed=0, precision="mixed", verbose=False,
+                               accessibility_grid=None):
     """K_H (the ``henry_coefficient`` results dict) and its logarithmic
     derivatives with respect to the framework species' pure LJ parameters and
     a uniform shift of their charges, from one adjoint device pass
@@ -192,10 +213,11 @@ def henry_coefficient_gradient(crystal, molecule, temperature, ljff, insertions_
     "charge": ...} for every species in the framework; epsilon and sigma are
     the framework side's parameters, the probe's held fixed."""
     t0 = time.time()
+    acc = _access_handle(crystal, accessibility_grid, verbose)
     fw, ff, ew, pr, nb_per_block, rho = _widom_setup(crystal, molecule, temperature, ljff,
                                                      insertions_per_volume, ewald_precision, verbose)
     sums, grads = device.widom_gradient(fw, ff, ew, pr, temperature, seed, N_BLOCKS, nb_per_block,
-                                        precision, ljff)
+                                        precision, ljff, acc=acc)
     results = block_statistics(sums, nb_per_block, temperature, rho)
     results["xtal"] = crystal.name
     results["elapsed time (min)"] = (time.time() - t0) / 60.0

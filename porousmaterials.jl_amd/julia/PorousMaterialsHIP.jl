@@ -62,6 +62,8 @@ mutable struct Handle
                     ccall((:pm_forcefield_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
                 elseif x.destroy === :pm_ewald_destroy
                     ccall((:pm_ewald_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
+                elseif x.destroy === :pm_access_destroy
+                    ccall((:pm_access_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
                 else
                     ccall((:pm_probe_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
                 end
@@ -295,11 +297,10 @@ function henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Fl
                            ljff::LJForceField; insertions_per_volume::Union{Int, Float64}=200,
                            verbose::Bool=true, ewald_precision::Float64=1e-6,
                            autosave::Bool=true, filename_comment::AbstractString="",
-                           accessibility_grid=nothing, seed::Integer=0,
+                           accessibility_grid::Union{Nothing, Grid{Bool}}=nothing, seed::Integer=0,
                            precision::Symbol=:mixed)
     assert_P1_symmetry(crystal)
-    isnothing(accessibility_grid) ||
-        error("accessibility-grid blocking is not supported by the device path")
+    acc = access_handle(crystal, accessibility_grid, verbose)
     time_start = time()
     nb_insertions = max(N_BLOCKS, ceil(Int, insertions_per_volume * crystal.box.Ω))
     nb_per_block = ceil(Int, nb_insertions / N_BLOCKS)
@@ -319,11 +320,13 @@ function henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Fl
     end
     pr = probe_handle(molecule, ljff)
     sums = zeros(3, N_BLOCKS)
-    check(ccall((:pm_widom, libpmhip), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, UInt64, Int32, Int64,
-                 Int32, Ptr{Float64}, Ptr{Float64}),
-                fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int32(N_BLOCKS),
-                Int64(nb_per_block), precision_code(precision), sums, C_NULL))
+    GC.@preserve acc begin
+        check(ccall((:pm_widom, libpmhip), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, UInt64, Int32, Int64,
+                     Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                    fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int32(N_BLOCKS),
+                    Int64(nb_per_block), precision_code(precision), sums, C_NULL, handle_ptr(acc)))
+    end
     results = widom_results(sums, nb_per_block, temperature, ρ, crystal)
     results["elapsed time (min)"] = (time() - time_start) / 60
     if autosave
@@ -350,6 +353,7 @@ struct WidomRun
     n_per_block::Int64
     n_blocks::Int32
     reserved::Int32
+    access::Ptr{Cvoid}
 end
 
 """
@@ -364,14 +368,17 @@ runs in one `pm_widom_batch` call.  Each results `Dict` equals the one
 function henry_coefficients(pairs::Vector{<:Tuple{Crystal, Molecule}}, temperature::Float64,
                             ljff::LJForceField; insertions_per_volume::Union{Int, Float64}=200,
                             ewald_precision::Float64=1e-6, seeds=zeros(Int, length(pairs)),
-                            precision::Symbol=:mixed)
+                            precision::Symbol=:mixed,
+                            accessibility_grids=fill(nothing, length(pairs)))
     time_start = time()
     handles = Any[]   # keeps every handle alive until the call returns
     runs = WidomRun[]
     setups = Tuple{Int, Float64}[]
     ff = forcefield_handle(ljff)
-    for ((crystal, molecule), seed) in zip(pairs, seeds)
+    for ((crystal, molecule), seed, agrid) in zip(pairs, seeds, accessibility_grids)
         assert_P1_symmetry(crystal)
+        acc = access_handle(crystal, agrid, false)
+        isnothing(acc) || push!(handles, acc)
         nb_insertions = max(N_BLOCKS, ceil(Int, insertions_per_volume * crystal.box.Ω))
         nb_per_block = ceil(Int, nb_insertions / N_BLOCKS)
         repfactors = replication_factors(crystal.box, ljff)
@@ -387,7 +394,7 @@ function henry_coefficients(pairs::Vector{<:Tuple{Crystal, Molecule}}, temperatu
         pr = probe_handle(molecule, ljff)
         push!(handles, fw, pr)
         push!(runs, WidomRun(fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int64(nb_per_block),
-                             Int32(N_BLOCKS), Int32(0)))
+                             Int32(N_BLOCKS), Int32(0), handle_ptr(acc)))
         push!(setups, (nb_per_block, crystal_density(xtal_rep)))
     end
     sums = zeros(3, N_BLOCKS * length(runs))
@@ -440,10 +447,91 @@ function energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljforcefield::L
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Float64}),
                 fw.ptr, ff.ptr, ew, pr.ptr, Int32[n_pts...], precision_code(precision), data))
     if units == :kJ_mol
-        data .*= 8.314 / 1000.0
+        data .= data .* 8.314 ./ 1000.0   # the reference's order of operations (src/grid.jl:358)
     end
     return Grid(crystal.box, n_pts, data, units,
                 center ? crystal.box.f_to_c * [-0.5, -0.5, -0.5] : [0.0, 0.0, 0.0])
+end
+
+# ------------------------------------------------------ accessibility ----
+handle_ptr(h::Nothing) = C_NULL
+handle_ptr(h::Handle) = h.ptr
+
+# Device copy of an accessibility grid (`Grid{Bool}`, i fastest like the
+# reference's Array{Bool,3}); the box check is the reference's, which it
+# makes only when verbose (src/henry.jl:57-63).
+function access_handle(crystal::Crystal, grid, verbose::Bool)
+    isnothing(grid) && return nothing
+    if verbose
+        println("Using provided accessibility grid to block inaccessible pockets")
+        isapprox(grid.box, crystal.box) ||
+            error(@sprintf("accessibility grid box does not match box of %s.\n", crystal.name))
+    end
+    data = UInt8.(vec(grid.data))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pm_access_create, libpmhip), Cint, (Ptr{Int32}, Ptr{UInt8}, Ref{Ptr{Cvoid}}),
+                Int32[grid.n_pts...], data, out))
+    return Handle(out[], :pm_access_destroy)
+end
+
+"""
+    accessible, segments, (n_segments, n_blocked, n_cycles) = block_pockets(grid, energy_tol;
+                                                                            block_pockets=true)
+
+The steps of `compute_accessibility_grid` after the energy grid
+(src/grid.jl:383-646, 704-764) in libpmhip.so's host code (pm_block_pockets).
+"""
+function block_pockets(grid::Grid, energy_tol::Float64; block_pockets::Bool=true, max_cycles::Integer=0)
+    n = length(grid.data)
+    acc = zeros(UInt8, grid.n_pts...)
+    seg = zeros(Int32, grid.n_pts...)
+    counts = zeros(Int64, 3)
+    check(ccall((:pm_block_pockets, libpmhip), Cint,
+                (Ptr{Int32}, Ptr{Float64}, Float64, Int32, Int64, Ptr{UInt8}, Ptr{Int32}, Ptr{Int64}),
+                Int32[grid.n_pts...], convert(Array{Float64, 3}, grid.data), energy_tol,
+                Int32(block_pockets), Int64(max_cycles), acc, seg, counts))
+    return acc .== 1, seg, (counts[1], counts[2], counts[3])
+end
+
+"""
+    compute_accessibility_grid(crystal, probe, forcefield; resolution=1.0, energy_tol=10.0,
+                               energy_units=:kJ_mol, verbose=true, write_b4_after_grids=true,
+                               block_inaccessible_pockets=true, precision=:fp64)
+
+`src/grid.jl:689-774` with the energy grid on the device (FP64 by default)
+and the flood fill / segment graph / cycles in `block_pockets`; same return
+values (grid, segments blocked, porosity).
+"""
+function compute_accessibility_grid(crystal::Crystal, probe::Molecule, forcefield::LJForceField;
+                                    resolution::Union{Float64, Tuple{Int, Int, Int}}=1.0,
+                                    energy_tol::Float64=10.0, energy_units::Symbol=:kJ_mol,
+                                    verbose::Bool=true, write_b4_after_grids::Bool=true,
+                                    block_inaccessible_pockets::Bool=true, precision::Symbol=:fp64)
+    assert_P1_symmetry(crystal)
+    grid = energy_grid(crystal, probe, forcefield, resolution=resolution, verbose=verbose,
+                       units=energy_units, precision=precision)
+    if !block_inaccessible_pockets
+        acc, _, _ = block_pockets(grid, energy_tol, block_pockets=false)
+        return Grid{Bool}(grid.box, grid.n_pts, acc, :accessibility, grid.origin), 0, sum(acc) / length(acc)
+    end
+    acc, seg, (n_seg, n_blocked, n_cycles) = block_pockets(grid, energy_tol)
+    stem = @sprintf("%s_in_%s_%s", probe.species, replace(replace(crystal.name, ".cif" => ""), ".cssr" => ""),
+                    replace(forcefield.name, ".csv" => ""))
+    if write_b4_after_grids
+        write_cube(Grid(grid.box, grid.n_pts, map(l -> l == -1 ? -1 : 1, seg), :Segment_No, grid.origin),
+                   stem * "_b4_pocket_blocking.cube")
+        write_cube(Grid(grid.box, grid.n_pts, map(a -> a ? 1 : -1, acc), :Segment_No, grid.origin),
+                   stem * "_after_pocket_blocking.cube")
+    end
+    any(acc) || @warn @sprintf("%s cannot enter the pores of %s with %f K energy tolerance.",
+                               probe.species, crystal.name, energy_tol)
+    porosity = Dict(:b4_blocking => sum(grid.data .< energy_tol) / length(grid.data),
+                    :after_blocking => sum(acc) / length(acc))
+    if n_blocked != 0
+        printstyled(@sprintf("\t%d pockets in %s were found to be inaccessible to %s and blocked.\n",
+                             n_blocked, crystal.name, probe.species), color=:yellow)
+    end
+    return Grid{Bool}(grid.box, grid.n_pts, acc, :accessibility, grid.origin), n_blocked, porosity
 end
 
 # ------------------------------------------------------------ adjoint ----
@@ -496,7 +584,9 @@ charges of its atoms: `grad[species] = (epsilon=..., sigma=..., charge=...)`.
 function henry_coefficient_gradient(crystal::Crystal, molecule::Molecule, temperature::Float64,
                                     ljff::LJForceField; mixing_rules::AbstractString="Lorentz-Berthelot",
                                     insertions_per_volume::Union{Int, Float64}=200,
-                                    ewald_precision::Float64=1e-6, seed::Integer=0, precision::Symbol=:mixed)
+                                    ewald_precision::Float64=1e-6, seed::Integer=0, precision::Symbol=:mixed,
+                                    accessibility_grid::Union{Nothing, Grid{Bool}}=nothing)
+    acc = access_handle(crystal, accessibility_grid, false)
     nb_insertions = max(N_BLOCKS, ceil(Int, insertions_per_volume * crystal.box.Ω))
     nb_per_block = ceil(Int, nb_insertions / N_BLOCKS)
     repfactors = replication_factors(crystal.box, ljff)
@@ -515,11 +605,13 @@ function henry_coefficient_gradient(crystal::Crystal, molecule::Molecule, temper
     n = size(jac, 2)
     sums = zeros(3, N_BLOCKS)
     grads = zeros(3, n, N_BLOCKS)
-    check(ccall((:pm_widom_gradient, libpmhip), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, UInt64, Int32, Int64, Int32,
-                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int32(N_BLOCKS), Int64(nb_per_block),
-                precision_code(precision), jac, sums, grads))
+    GC.@preserve acc begin
+        check(ccall((:pm_widom_gradient, libpmhip), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, UInt64, Int32, Int64, Int32,
+                     Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                    fw.ptr, ff.ptr, ew, pr.ptr, temperature, UInt64(seed), Int32(N_BLOCKS), Int64(nb_per_block),
+                    precision_code(precision), jac, sums, grads, handle_ptr(acc)))
+    end
     results = widom_results(sums, nb_per_block, temperature, crystal_density(xtal_rep), crystal)
     S = sum(sums[1, :])
     G = dropdims(sum(grads, dims=3), dims=3)

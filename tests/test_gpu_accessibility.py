@@ -1,0 +1,198 @@
+"""Accessibility grids and pocket-blocked Widom insertions on the device,
+and the reference's end-to-end Henry known answers.
+
+* compute_accessibility_grid (device FP64 energy grid + pm_block_pockets) on
+  the reference's cases (test/grid.jl:42-146): LTA blocks 8 segments, SOD
+  every segment, SBMOF-1 and CAXVII none; the LTA (2, 3, 1) corner identity;
+  and the grids equal the oracle's blocking of the oracle's energies.
+* Widom with an accessibility grid (src/henry.jl:225-232): every insertion
+  whose centre of mass is not accessible (by the oracle's restatement of
+  `accessible`, on the oracle's replay of the insertion stream) has
+  E_vdw = E_es = +Inf, every other matches the oracle's energy; the block sums
+  follow.  Batched and adjoint runs with a grid give the same block sums.
+* test/henry.jl:39-61: LTA with its blocked sodalite cages vs
+  LTA_manually_blocked.cif without a grid (same seed), K_H and <U> within 1%.
+* test/henry.jl:12-34: Xe in SBMOF-1 (K_H 0.00985348 mol/(kg Pa),
+  <U> -39.6811 kJ/mol) and CO2 in CAXVII_clean (2.88317e-5, -18.69582223),
+  Dreiding r_c 12.5 A, 500 insertions per A^3, rtol 0.025 (statistical).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def oacc():
+    from oracle import accessibility
+    return accessibility
+
+
+@pytest.fixture(scope="module")
+def uff(pm):
+    return pm.LJForceField.from_csv(golden("forcefields", "UFF.csv"))
+
+
+@pytest.fixture(scope="module")
+def ch4(pm):
+    return pm.Molecule.from_dir(golden("molecules", "CH4"))
+
+
+@pytest.fixture()
+def grids_dir(pm, tmp_path, monkeypatch):
+    import porousmaterials_jl_amd.grid as gridmod
+    monkeypatch.setitem(gridmod.PATHS, "grids", str(tmp_path))
+    return tmp_path
+
+
+def _oracle(orc, x, ff, species, rep=None):
+    return orc.OracleSystem((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
+                            [ff.index(s) for s in x.species], None, ff.epsilon_table, ff.sigma2_table,
+                            math.sqrt(ff.r2_cutoff), np.zeros((1, 3)), [ff.index(species)], np.zeros((0, 3)),
+                            np.zeros(0), rep=rep)
+
+
+def _oracle_grid_kj(orc, x, ff, n_pts):
+    return _oracle(orc, x, ff, "CH4").energy_grid(n_pts).reshape(n_pts, order="F") * 8.314 / 1000.0
+
+
+@pytest.mark.parametrize("name,res,tol,want_blocked",
+                         [("LTA", 2.0, 0.0, 8), ("SOD", 2.0, 0.0, "all"), ("SBMOF-1", 1.0, 5.0, 0),
+                          ("CAXVII_clean", 1.0, 5.0, 0)])
+def test_compute_accessibility_grid_kats(pm, orc, oacc, uff, ch4, grids_dir, name, res, tol, want_blocked):
+    say(f"case: compute_accessibility_grid {name}, CH4/UFF, resolution {res} A, energy_tol {tol} kJ/mol (fp64)")
+    x = pm.read_cif(golden("crystals", f"{name}.cif"))
+    g, n_bl, por = pm.compute_accessibility_grid(x, ch4, uff, resolution=res, energy_tol=tol, verbose=False,
+                                                 write_b4_after_grids=True)
+    e = _oracle_grid_kj(orc, x, uff, g.n_pts)
+    assert np.min(np.abs(e - tol)) > 1e-6  # no point within rounding of the threshold
+    acc, _, _, o_bl, _ = oacc.block(e, tol)
+    assert np.array_equal(g.data, acc)
+    assert n_bl == o_bl
+    if want_blocked == "all":
+        assert n_bl > 0 and not g.data.any()
+    else:
+        assert n_bl == want_blocked
+    if n_bl:
+        assert por["b4_blocking"] > por["after_blocking"]
+    assert pm.box_isapprox(x.box, g.box)
+    g_nb, n_nb, por_nb = pm.compute_accessibility_grid(x, ch4, uff, resolution=res, energy_tol=tol, verbose=False,
+                                                       write_b4_after_grids=False, block_inaccessible_pockets=False)
+    assert n_nb == 0 and math.isclose(por_nb, por["b4_blocking"])
+    if n_bl == 0:
+        assert g.isapprox(g_nb)
+    cubes = sorted(p.name for p in grids_dir.iterdir())
+    assert len(cubes) == 2 and all(c.startswith(f"CH4_in_{name}_UFF_") for c in cubes)
+    b4 = pm.read_cube(cubes[1])  # ..._b4_pocket_blocking.cube
+    assert set(np.unique(b4.data)) <= {-1.0, 1.0} and b4.n_pts == g.n_pts
+
+
+def test_lta_replicated_kat(pm, uff, ch4):
+    say("case: LTA and LTA x (2, 3, 1) accessibility grids (test/grid.jl:102-125)")
+    x = pm.read_cif(golden("crystals", "LTA.cif"))
+    g1, _, _ = pm.compute_accessibility_grid(x, ch4, uff, resolution=2.0, energy_tol=0.0, verbose=False,
+                                             write_b4_after_grids=False)
+    xr = pm.replicate(x, (2, 3, 1))
+    gr, n_bl, _ = pm.compute_accessibility_grid(xr, ch4, uff, resolution=2.0, energy_tol=0.0, verbose=False,
+                                                write_b4_after_grids=False)
+    assert np.array_equal(g1.data, gr.data[:7, :7, :7])
+    assert n_bl > 0
+    rng = np.random.default_rng(4)
+    assert all(pm.accessible(gr, xf) == pm.accessible(g1, xf, (2, 3, 1)) for xf in rng.random((10000, 3)))
+
+
+@pytest.fixture(scope="module")
+def lta_blocked(pm, uff, ch4):
+    x = pm.read_cif(golden("crystals", "LTA.cif"))
+    g, n_bl, _ = pm.compute_accessibility_grid(x, ch4, uff, resolution=(50, 50, 50), energy_tol=3.0 * 298.0,
+                                               verbose=False, write_b4_after_grids=False)
+    assert n_bl > 0
+    return x, g
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_widom_with_accessibility_matches_oracle(pm, orc, oacc, uff, ch4, lta_blocked, precision):
+    say(f"case: CH4 Widom in LTA with the 50^3 accessibility grid, 5 x 400 insertions vs oracle ({precision})")
+    x, g = lta_blocked
+    rep = pm.replication_factors(x.box, uff)
+    xr = pm.replicate(x, rep)
+    fw = pm.device.framework(xr, uff)
+    pr = pm.device.probe(ch4, uff)
+    acc = pm.device.access(g)
+    seed, nb, T = 21, 400, 298.0
+    sums, en = pm.device.widom(fw, uff, None, pr, T, seed, 5, nb, precision, return_energies=True, acc=acc)
+    o = _oracle(orc, x, uff, "CH4", rep=rep)
+    v, _ = o.widom_energies(seed, 0, 5 * nb)
+    blocked = np.array([not oacc.accessible(g.data, orc.insertion_uniforms(seed, i)[:3], rep)
+                        for i in range(5 * nb)])
+    say(f"   {blocked.sum()} of {5 * nb} insertions blocked")
+    assert 0 < blocked.sum() < 5 * nb
+    assert np.all(np.isposinf(en[blocked, 0])) and np.all(np.isposinf(en[blocked, 1]))
+    open_ = ~blocked
+    assert np.array_equal(np.isinf(en[open_, 0]), np.isinf(v[open_]))
+    fin = open_ & np.isfinite(v)
+    scale = np.array([o.vdw_scale(orc.insertion_uniforms(seed, i)[:3].reshape(1, 3)) for i in np.flatnonzero(fin)])
+    err = np.abs(en[fin, 0] - v[fin]) / (scale + 1.0)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    say(f"   max scaled energy error {err.max():.2e}")
+    assert err.max() <= tol
+    assert np.all(en[open_, 1] == 0.0)
+    if precision == "fp64":
+        ve = np.where(blocked, np.inf, v)
+        for b in range(5):
+            want = o.block_sums(ve[b * nb:(b + 1) * nb], np.where(blocked, np.inf, 0.0)[b * nb:(b + 1) * nb], T)
+            assert np.allclose(sums[b], want, rtol=1e-9, atol=0.0), (b, sums[b], want)
+    # the same run through the batch and the adjoint entry points
+    got = pm.device.widom_batch([(fw, uff, None, pr, T, seed, 5, nb, acc), (fw, uff, None, pr, T, seed, 5, nb)],
+                                precision)
+    assert np.array_equal(got[0], sums)
+    assert np.array_equal(got[1], pm.device.widom(fw, uff, None, pr, T, seed, 5, nb, precision))
+    gs, _ = pm.device.widom_gradient(fw, uff, None, pr, T, seed, 5, nb, precision, uff, acc=acc)
+    assert np.array_equal(gs, sums)
+
+
+def test_henry_blocked_vs_manually_blocked_kat(pm, uff, ch4, lta_blocked):
+    """test/henry.jl:39-61: blocking LTA's sodalite cages with the grid gives
+    the K_H and <U> of LTA_manually_blocked.cif (dummy 're' atoms, 5 A,
+    1e-4 K, mass 0, in the cages) within 1%."""
+    say("case: henry_coefficient LTA + accessibility grid vs LTA_manually_blocked, 500 insertions/A^3")
+    x, g = lta_blocked
+    pm.register_atomic_masses({"re": 0.0})
+    kw = dict(insertions_per_volume=500, verbose=False, autosave=False, seed=8)
+    r_free = pm.henry_coefficient(x, ch4, 298.0, uff, **kw)
+    r_blk = pm.henry_coefficient(x, ch4, 298.0, uff, accessibility_grid=g, **kw)
+    xm = pm.read_cif(golden("crystals", "LTA_manually_blocked.cif"))
+    r_man = pm.henry_coefficient(xm, ch4, 298.0, uff, **kw)
+    k = "henry coefficient [mmol/(g-bar)]"
+    say(f"   K_H free {r_free[k]:.6g}, blocked {r_blk[k]:.6g}, manual {r_man[k]:.6g} mmol/(g bar); "
+        f"<U> {r_free['⟨U⟩ (kJ/mol)']:.4f}, {r_blk['⟨U⟩ (kJ/mol)']:.4f}, {r_man['⟨U⟩ (kJ/mol)']:.4f} kJ/mol")
+    assert math.isclose(r_blk[k], r_man[k], rel_tol=0.01)
+    assert math.isclose(r_blk["⟨U⟩ (kJ/mol)"], r_man["⟨U⟩ (kJ/mol)"], rel_tol=0.01)
+    assert r_blk[k] <= r_free[k]  # same insertions, blocked ones with weight 0
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_henry_statistical_kats(pm, precision):
+    """test/henry.jl:12-34 (rtol 0.025, the reference's statistical bar)."""
+    ff = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
+    for xname, mname, kh, u in (("SBMOF-1", "Xe", 0.00985348, -39.6811),
+                                ("CAXVII_clean", "CO2", 2.88317e-05, -18.69582223)):
+        say(f"case: henry_coefficient {mname} in {xname}, Dreiding r_c 12.5, 500 insertions/A^3 ({precision})")
+        x = pm.read_cif(golden("crystals", f"{xname}.cif"))
+        m = pm.Molecule.from_dir(golden("molecules", mname))
+        r = pm.henry_coefficient(x, m, 298.0, ff, insertions_per_volume=500, verbose=False, autosave=False,
+                                 seed=2024, precision=precision)
+        say(f"   K_H {r['henry coefficient [mol/(kg-Pa)]']:.6g} mol/(kg Pa) (reference {kh}), "
+            f"<U> {r['⟨U⟩ (kJ/mol)']:.5f} kJ/mol (reference {u})")
+        assert math.isclose(r["henry coefficient [mol/(kg-Pa)]"], kh, rel_tol=0.025)
+        assert math.isclose(r["⟨U⟩ (kJ/mol)"], u, rel_tol=0.025)

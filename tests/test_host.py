@@ -23,6 +23,27 @@ def test_cif_reader(pm):
     pm.strip_numbers_from_atom_labels(z)
     assert set(z.species) == {"Zn", "N", "C", "Cl", "H"}
     assert np.all((z.xf >= 0.0) & (z.xf < 1.0))
+    # a symmetry loop with a site-id column before the operation
+    cx = pm.read_cif(golden("crystals", "CAXVII_clean.cif"))
+    assert cx.symmetry_p1 and cx.n_atoms == 54 and cx.species.count("Fe") == 6
+
+
+def test_cif_symmetry_operations(pm, tmp_path):
+    head = ("data_t\n_cell_length_a 10\n_cell_length_b 10\n_cell_length_c 10\n_cell_angle_alpha 90\n"
+            "_cell_angle_beta 90\n_cell_angle_gamma 90\n")
+    atoms = "loop_\n_atom_site_label\n_atom_site_fract_x\n_atom_site_fract_y\n_atom_site_fract_z\nC 0.1 0.2 0.3\n"
+    cases = {"loop_\n_symmetry_equiv_pos_site_id\n_symmetry_equiv_pos_as_xyz\n1 'x, y, z'\n": True,
+             "loop_\n_symmetry_equiv_pos_as_xyz\n'x, y, z'\n": True,
+             "loop_\n_symmetry_equiv_pos_site_id\n_symmetry_equiv_pos_as_xyz\n1 'x, y, z'\n2 '-x, -y, -z'\n": False,
+             "_symmetry_space_group_name_H-M 'P -1'\n": False}
+    for k, (sym, p1) in enumerate(cases.items()):
+        path = tmp_path / f"s{k}.cif"
+        path.write_text(head + sym + atoms)
+        x = pm.read_cif(str(path))
+        assert x.symmetry_p1 == p1, sym
+        if not p1:
+            with pytest.raises(ValueError, match="P1"):
+                pm.assert_P1_symmetry(x)
 
 
 def test_box_matches_oracle(pm, orc):
@@ -51,9 +72,14 @@ def test_replicate_crystal_matches_oracle(pm, orc):
     np.testing.assert_array_equal(xr.xf.T, orc.replicate(x.xf.T, (3, 6, 2)))
     assert xr._unit is x and xr._rep == (3, 6, 2)
     assert xr.species == x.species * 36
-    # a replica of a replica keeps the unit-cell provenance
-    xrr = pm.replicate(pm.replicate(x, (1, 2, 1)), (3, 3, 2))
-    assert xrr._unit is x and xrr._rep == (3, 6, 2)
+    # a replica of a replica is uploaded as its parent cell and the outer
+    # factors (grid points are fractions of the parent cell)
+    x2 = pm.replicate(x, (1, 2, 1))
+    xrr = pm.replicate(x2, (3, 3, 2))
+    assert xrr._unit is x2 and xrr._rep == (3, 3, 2)
+    # same atoms as one (3, 6, 2) replica, up to the rounding of two divisions
+    np.testing.assert_allclose(np.sort(xrr.xf, axis=1), np.sort(orc.replicate(x.xf.T, (3, 6, 2)).T, axis=1),
+                               rtol=0, atol=1e-14)
 
 
 def test_lorentz_berthelot_tables(pm):
