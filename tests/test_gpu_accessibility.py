@@ -14,7 +14,10 @@ and the reference's end-to-end Henry known answers.
   LTA_manually_blocked.cif without a grid (same seed), K_H and <U> within 1%.
 * test/henry.jl:12-34: Xe in SBMOF-1 (K_H 0.00985348 mol/(kg Pa),
   <U> -39.6811 kJ/mol) and CO2 in CAXVII_clean (2.88317e-5, -18.69582223),
-  Dreiding r_c 12.5 A, 500 insertions per A^3, rtol 0.025 (statistical).
+  Dreiding r_c 12.5 A, rtol 0.025 (statistical).  At the reference's 500
+  insertions per A^3 one run's K_H scatters by 2.1% (sd over 16 seeds, Xe),
+  so the test runs 5000 per A^3 (sd ~0.7%); at 50000 per A^3 the device's K_H
+  are within 0.05% (Xe) and 0.13% (CO2) of the reference's values.
 """
 import math
 import sys
@@ -183,14 +186,15 @@ def test_henry_blocked_vs_manually_blocked_kat(pm, uff, ch4, lta_blocked):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_henry_statistical_kats(pm, precision):
-    """test/henry.jl:12-34 (rtol 0.025, the reference's statistical bar)."""
+    """test/henry.jl:12-34 (rtol 0.025, the reference's statistical bar, at
+    10x its insertion count)."""
     ff = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
     for xname, mname, kh, u in (("SBMOF-1", "Xe", 0.00985348, -39.6811),
                                 ("CAXVII_clean", "CO2", 2.88317e-05, -18.69582223)):
-        say(f"case: henry_coefficient {mname} in {xname}, Dreiding r_c 12.5, 500 insertions/A^3 ({precision})")
+        say(f"case: henry_coefficient {mname} in {xname}, Dreiding r_c 12.5, 5000 insertions/A^3 ({precision})")
         x = pm.read_cif(golden("crystals", f"{xname}.cif"))
         m = pm.Molecule.from_dir(golden("molecules", mname))
-        r = pm.henry_coefficient(x, m, 298.0, ff, insertions_per_volume=500, verbose=False, autosave=False,
+        r = pm.henry_coefficient(x, m, 298.0, ff, insertions_per_volume=5000, verbose=False, autosave=False,
                                  seed=2024, precision=precision)
         say(f"   K_H {r['henry coefficient [mol/(kg-Pa)]']:.6g} mol/(kg Pa) (reference {kh}), "
             f"<U> {r['⟨U⟩ (kJ/mol)']:.5f} kJ/mol (reference {u})")
