@@ -139,8 +139,80 @@ def screening_leg(pm, steps, n_per_block=20000):
     }
 
 
+def grids_leg(pm, steps):
+    """Energy grids through pm_energy_grid (50^3
+    CH4 grid of a 424-atom framework; 200^3 point-charge and CH4 grids of a
+    3000-atom framework) and pm_energy_grid_batch (32 point-charge grids at
+    80^3).  Grid points/s over the device pipeline (the grid is then copied
+    to the caller's host buffer, which the wall-clock rate includes), and the
+    pair kernel's algorithmic TFLOP/s with the per-pair constants above."""
+    from porousmaterials_jl_amd.grid import _grid_setup
+
+    def pair_flop(case, probe, cand, npts):
+        x, ffl = case.crystal, case.ljff
+        rep = pm.replication_factors(x.box, ffl)
+        n_rep, omega = x.n_atoms * int(np.prod(rep)), x.box.Omega * float(np.prod(rep))
+        r_c = math.sqrt(ffl.r2_cutoff)
+        f = C_GEOM * cand
+        if probe.n_atoms:
+            f += npts * C_LJ * in_cutoff(n_rep, omega, r_c)
+        if probe.n_charges and pm.has_charges(x):
+            f += npts * C_SR * in_cutoff(n_rep, omega, r_c)
+        return f
+
+    def measure(case, probe_name, precision):
+        probe = case.probes[probe_name]
+        run, _ = _grid_setup(case.crystal, probe, case.ljff, case.meta["n_pts"], "K", False, False,
+                             case.ewald_precision)
+        pm.device.energy_grid(*run, precision)  # warmup: image list, scratch
+        pipe, pair, cand = [], [], []
+        t = time.perf_counter()
+        for _ in range(steps):
+            pm.device.energy_grid(*run, precision)
+            pipe.append(pm.device.last_kernel_stats()[0])
+            pk, pc = pm.device.last_pair_kernel_stats()
+            pair.append(pk)
+            cand.append(pc)
+        wall = (time.perf_counter() - t) / steps
+        npts = float(np.prod(case.meta["n_pts"]))
+        p_ms = float(np.mean(pair))
+        tf = pair_flop(case, probe, float(np.mean(cand)), npts) / (p_ms * 1e-3) / 1e12
+        peak = PEAK_FP32_TFLOPS if precision == "mixed" else PEAK_FP64_TFLOPS
+        return {"points": int(npts), "value": npts / (float(np.mean(pipe)) * 1e-3), "unit": "grid points/s",
+                "wall_value": npts / wall, "device_pipeline_ms": float(np.mean(pipe)), "pair_kernel_ms": p_ms,
+                "pair_kernel_tflops": tf, "frac": tf / peak, "candidates_per_point": float(np.mean(cand)) / npts}
+
+    out = {}
+    small = pm.synthetic.lj_grid_case()
+    large = pm.synthetic.large_grid_case()
+    for key, case, probe_name in (("ch4_50", small, "CH4"), ("ion_200", large, "ion"), ("ch4_200", large, "CH4")):
+        out[key] = {p: measure(case, probe_name, p) for p in ("mixed", "fp64")}
+        print(f"grid {key}: mixed {out[key]['mixed']['value']:.3e} points/s, fp64 "
+              f"{out[key]['fp64']['value']:.3e} points/s", file=sys.stderr, flush=True)
+    cases = pm.synthetic.small_grid_cases()
+    runs = [_grid_setup(c.crystal, c.probes["ion"], c.ljff, c.meta["n_pts"], "K", False, False,
+                        c.ewald_precision)[0] for c in cases]
+    pm.device.energy_grid_batch(runs, "mixed")
+    pipe = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        pm.device.energy_grid_batch(runs, "mixed")
+        pipe.append(pm.device.last_kernel_stats()[0])
+    wall = (time.perf_counter() - t) / steps
+    npts = float(sum(np.prod(c.meta["n_pts"]) for c in cases))
+    out["ion_80_x32_batch"] = {"points": int(npts), "value": npts / (float(np.mean(pipe)) * 1e-3),
+                               "unit": "grid points/s", "wall_value": npts / wall,
+                               "device_pipeline_ms": float(np.mean(pipe)), "precision": "mixed"}
+    out["workload"] = ("ch4_50: CH4 (LJ sphere) on a 50^3 grid of a cubic 25.8 A 424-atom framework (4 species); "
+                       "ion_200 / ch4_200: unit point charge / CH4 on a 200^3 grid of a 3000-atom triclinic "
+                       "framework (lengths 30-40 A, charged); ion_80_x32_batch: unit point charge on 80^3 grids "
+                       "of 32 frameworks of 200-1500 atoms in one pm_energy_grid_batch call; r_c 12.8 A, Ewald "
+                       "precision 1e-6")
+    return out
+
+
 def adjoint_leg(pm, case, fw, ff, ew, pr, xr, n_per_block, seed):
-    """configs[3] on the CO2 workload: (b) one adjoint Widom pass over the
+    """Gradients on the CO2 workload: (b) one adjoint Widom pass over the
     full 10^6 insertions (pm_widom_gradient, mixed), (a) forces of 4096 CO2
     poses (pm_pose_gradient, mixed) and their batched minimisation (FP64)."""
     co2 = case.probes["CO2"]
@@ -207,6 +279,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-screening", action="store_true", help="skip the batched screening leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (gradient) leg")
+    ap.add_argument("--no-grids", action="store_true", help="skip the energy-grid leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (pm_copy_probe) three times: the "
                          "calibration launch for FETCH_SIZE / WRITE_SIZE counter runs")
@@ -288,7 +361,7 @@ def main():
             "tuning": {k: tuning[k] for k in ("cell_points", "chunk_r_max", "source")}},
         "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "point_pair_kernel<1, float>", "kernel_ms": p_ms,
+                     "kernel": "point_pair_kernel<float, false>", "kernel_ms": p_ms,
                      "flop_per_launch": pair_flop,
                      "candidate_pairs_per_point": cand / n_points,
                      "pruning_factor": (n_points * xr.n_atoms) / cand},
@@ -315,6 +388,8 @@ def main():
         out["adjoint"] = adjoint_leg(pm, case, fw, ff, ew, pr, xr, nb, seed)
     if not args.no_screening:
         out["screening"] = screening_leg(pm, max(1, min(args.steps, 3)))
+    if not args.no_grids:
+        out["grids"] = grids_leg(pm, max(1, min(args.steps, 3)))
     if not args.no_cpu:
         cb = cpu_baseline(pm, case, xr, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb

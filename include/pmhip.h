@@ -323,6 +323,88 @@ int pm_last_kernel_stats(double* kernel_ms, double* pair_evals);
  * evaluated after spatial pruning. */
 int pm_last_pair_kernel_stats(double* pair_kernel_ms, double* candidate_pairs);
 
+/* ------------------------------------------------------------------ GCMC --
+ * Grand-canonical Monte Carlo (`μVT_sim`, src/gcmc.jl:175-817) as many
+ * independent Markov chains on the device: one workgroup per chain, every
+ * chain with its own temperature and fugacities, all chains advanced by the
+ * same launches (isotherm points and replicas run side by side).  Moves,
+ * acceptance rules, proposal probabilities, the adaptive translation step and
+ * the block statistics are the reference's (src/gcmc.jl:404-727,
+ * src/mc_helpers.jl:17-190); energies are its guest-host and guest-guest
+ * Lennard-Jones sums with the nearest image and overlap -> +Inf
+ * (src/vdw_energetics.jl:25-47,85-120, src/gcmc.jl:111-124), in FP64.
+ * Electrostatics are not part of the reference's GCMC (src/gcmc.jl:378-382).
+ * Random numbers: move t of chain c draws 10 uniforms from Philox4x32-10 keyed
+ * by the seed (the reference draws from Julia's global RNG).
+ */
+#define PM_MAX_GCMC_SPECIES 4
+#define PM_GCMC_BLOCKS 5   /* N_BLOCKS, src/mc_helpers.jl:11 */
+#define PM_GCMC_MOVES 6    /* insertion, deletion, translation, rotation, reinsertion, identity change */
+
+typedef struct pm_gcmc pm_gcmc;
+
+typedef struct pm_gcmc_settings {
+    int64_t n_burn_cycles;       /* src/gcmc.jl:202-205 */
+    int64_t n_sample_cycles;     /* >= PM_GCMC_BLOCKS */
+    int32_t sample_frequency;    /* sample every this many moves (production) */
+    int32_t snapshot_frequency;  /* density-grid update every this many cycles (production) */
+    int32_t capacity;            /* molecules per species per chain */
+    int32_t density_species;     /* adsorbate species tracked by the density grid, -1: no grid */
+    uint32_t density_site_mask;  /* sites of that species counted (bit k: site k) */
+    int32_t density_n_pts[3];
+    int32_t density_sim_box;     /* 1: grid over the simulation box; 0: over the unit cell */
+    double delta;                /* initial half translation step, A (2.0 in the reference) */
+    uint64_t seed;
+} pm_gcmc_settings;
+
+typedef struct pm_gcmc_state {
+    int64_t cycles_done;         /* outer cycles completed */
+    int64_t markov_time;         /* moves proposed */
+    int64_t num_snapshots;
+    int32_t n[PM_MAX_GCMC_SPECIES];
+    int32_t block;               /* current statistics block, 0-based */
+    int32_t error;               /* 1: a species reached the capacity */
+    double delta;
+    double energy_gh, energy_gg; /* running system energy (K) */
+    int64_t n_proposed[PM_GCMC_MOVES], n_accepted[PM_GCMC_MOVES];
+    int64_t block_samples[PM_GCMC_BLOCKS];
+    int64_t block_n[PM_GCMC_BLOCKS][PM_MAX_GCMC_SPECIES];
+    int64_t block_n2[PM_GCMC_BLOCKS][PM_MAX_GCMC_SPECIES];
+    double block_gh[PM_GCMC_BLOCKS], block_gg[PM_GCMC_BLOCKS];
+    double block_gh2[PM_GCMC_BLOCKS], block_gg2[PM_GCMC_BLOCKS];
+    double block_un[PM_GCMC_BLOCKS];
+} pm_gcmc_state;
+
+/*
+ * n_species templates (probes: their LJ spheres are the molecule; charges are
+ * ignored), n_chains chains with temperatures[c] (K) and fugacities[c *
+ * n_species + s] (Pa).  The framework handle is the replicated simulation box
+ * (`replicate(xtal, repfactors)`, src/gcmc.jl:262-264).  Chains start empty.
+ */
+int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_species,
+                   const pm_probe* const* templates, int32_t n_chains, const double* temperatures,
+                   const double* fugacities, const pm_gcmc_settings* settings, pm_gcmc** out);
+int pm_gcmc_destroy(pm_gcmc* gc);
+/* Starting configuration of one species of one chain (before the first
+ * pm_gcmc_run): n molecules, site_xf n x n_sites x 3 (the template's LJ
+ * spheres in order) and com_xf n x 3 (centres of mass), fractional in the
+ * simulation box; the system energy is recomputed before the next run
+ * (src/gcmc.jl:386-402). */
+int pm_gcmc_set_molecules(pm_gcmc* gc, int32_t chain, int32_t species, int32_t n, const double* site_xf,
+                          const double* com_xf);
+/* Advance every chain by n_cycles outer cycles (each of N + 1 moves,
+ * src/gcmc.jl:460-463). */
+int pm_gcmc_run(pm_gcmc* gc, int64_t n_cycles);
+int pm_gcmc_get_state(const pm_gcmc* gc, int32_t chain, pm_gcmc_state* out);
+/* Current molecules of one species: *n of them (at most max_n copied), as
+ * for pm_gcmc_set_molecules. */
+int pm_gcmc_get_molecules(const pm_gcmc* gc, int32_t chain, int32_t species, int32_t max_n, int32_t* n,
+                          double* site_xf, double* com_xf);
+/* System energy recomputed from scratch (`total_vdw_energy`, src/gcmc.jl:735-738). */
+int pm_gcmc_total_energy(pm_gcmc* gc, int32_t chain, double* out_gh, double* out_gg);
+/* Density-grid counts (`update_density!`, src/grid.jl:78-106), i fastest. */
+int pm_gcmc_get_density(const pm_gcmc* gc, int32_t chain, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,11 +19,13 @@ from .forcefield import LJForceField, forcefield_coverage, lennard_jones, replic
 from .energy_min import find_energy_minima, find_energy_minimum, find_energy_minimum_gridsearch
 from .grid import (Grid, box_isapprox, energy_grid, energy_grids, id_to_xf, read_cube, required_n_pts,
                    write_cube, xf_to_id)
+from .gcmc import adsorption_isotherm, muVT_sim, stepwise_adsorption_isotherm, μVT_sim
 from .henry import (K_TO_KJ_PER_MOL, N_BLOCKS, UNIV_GAS_CONST, block_statistics, henry_coefficient,
                     henry_coefficient_gradient, henry_coefficients,
                     henry_result_savename)
 from .molecule import (Cart, Frac, Molecule, arvo_rotation_matrix, center_of_mass, ion, lj_sphere,
                        needs_rotations, point_charge, site_offsets, translate_to)
-from . import device, synthetic
+from .eos import PengRobinsonFluid, VdWFluid, calculate_properties
+from . import device, eos, synthetic
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -47,6 +47,32 @@ class WidomRun(ctypes.Structure):
                 ("reserved", ctypes.c_int32), ("access", P)]
 
 
+PM_MAX_GCMC_SPECIES, PM_GCMC_BLOCKS, PM_GCMC_MOVES = 4, 5, 6
+
+
+class GcmcSettings(ctypes.Structure):
+    """pm_gcmc_settings."""
+    _fields_ = [("n_burn_cycles", ctypes.c_int64), ("n_sample_cycles", ctypes.c_int64),
+                ("sample_frequency", ctypes.c_int32), ("snapshot_frequency", ctypes.c_int32),
+                ("capacity", ctypes.c_int32), ("density_species", ctypes.c_int32),
+                ("density_site_mask", ctypes.c_uint32), ("density_n_pts", ctypes.c_int32 * 3),
+                ("density_sim_box", ctypes.c_int32), ("delta", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class GcmcState(ctypes.Structure):
+    """pm_gcmc_state."""
+    _fields_ = [("cycles_done", ctypes.c_int64), ("markov_time", ctypes.c_int64), ("num_snapshots", ctypes.c_int64),
+                ("n", ctypes.c_int32 * PM_MAX_GCMC_SPECIES), ("block", ctypes.c_int32), ("error", ctypes.c_int32),
+                ("delta", ctypes.c_double), ("energy_gh", ctypes.c_double), ("energy_gg", ctypes.c_double),
+                ("n_proposed", ctypes.c_int64 * PM_GCMC_MOVES), ("n_accepted", ctypes.c_int64 * PM_GCMC_MOVES),
+                ("block_samples", ctypes.c_int64 * PM_GCMC_BLOCKS),
+                ("block_n", (ctypes.c_int64 * PM_MAX_GCMC_SPECIES) * PM_GCMC_BLOCKS),
+                ("block_n2", (ctypes.c_int64 * PM_MAX_GCMC_SPECIES) * PM_GCMC_BLOCKS),
+                ("block_gh", ctypes.c_double * PM_GCMC_BLOCKS), ("block_gg", ctypes.c_double * PM_GCMC_BLOCKS),
+                ("block_gh2", ctypes.c_double * PM_GCMC_BLOCKS), ("block_gg2", ctypes.c_double * PM_GCMC_BLOCKS),
+                ("block_un", ctypes.c_double * PM_GCMC_BLOCKS)]
+
+
 class GridRun(ctypes.Structure):
     """pm_grid_run: one grid of pm_energy_grid_batch."""
     _fields_ = [("framework", P), ("forcefield", P), ("ewald", P), ("probe", P),
@@ -86,6 +112,15 @@ _SIGS = {
     "pm_autotune_widom": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int64,
                                          ctypes.c_int32, D, D, D]),
     "pm_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, D]),
+    "pm_gcmc_create": (ctypes.c_int, [P, P, ctypes.c_int32, ctypes.POINTER(P), ctypes.c_int32, D, D,
+                                      ctypes.POINTER(GcmcSettings), PP]),
+    "pm_gcmc_destroy": (ctypes.c_int, [P]),
+    "pm_gcmc_set_molecules": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, D, D]),
+    "pm_gcmc_run": (ctypes.c_int, [P, ctypes.c_int64]),
+    "pm_gcmc_get_state": (ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(GcmcState)]),
+    "pm_gcmc_get_molecules": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, I32, D, D]),
+    "pm_gcmc_total_energy": (ctypes.c_int, [P, ctypes.c_int32, D, D]),
+    "pm_gcmc_get_density": (ctypes.c_int, [P, ctypes.c_int32, D]),
     "pm_last_kernel_stats": (ctypes.c_int, [D, D]),
     "pm_last_pair_kernel_stats": (ctypes.c_int, [D, D]),
 }

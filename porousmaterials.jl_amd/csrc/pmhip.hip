@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "pm_gcmc.h"
 #include "pm_pockets.h"
 #include "pm_points.h"
 #include "pmhip.h"
@@ -200,6 +201,22 @@ struct pm_probe {
 struct pm_access {
     int32_t n[3] = {0, 0, 0};
     unsigned char* d = nullptr;  // 0 / 1 per unit-cell grid point, i fastest
+};
+
+struct pm_gcmc {
+    int n_chains = 0, n_species = 0, capacity = 0;
+    int64_t n_dens = 0;           // density-grid points per chain (0: no grid)
+    bool energy_stale = false;    // a starting configuration changed the system energy
+    GcmcParams hp{};              // host copy of the shared parameters
+    GcmcParams* d_params = nullptr;
+    GcmcChain* d_chains = nullptr;
+    double4* d_atoms = nullptr;
+    double2* d_fg = nullptr;
+    double2* d_gg = nullptr;
+    double4* d_sites = nullptr;
+    double4* d_com = nullptr;
+    unsigned* d_dens = nullptr;
+    double* d_tot = nullptr;
 };
 
 namespace {
@@ -1791,6 +1808,383 @@ int pm_last_kernel_stats(double* kernel_ms, double* pair_evals) {
 int pm_last_pair_kernel_stats(double* pair_kernel_ms, double* candidate_pairs) {
     if (pair_kernel_ms) *pair_kernel_ms = g_ctx.last_pair_ms;
     if (candidate_pairs) *candidate_pairs = g_ctx.last_cand;
+    return PM_OK;
+}
+
+
+// ------------------------------------------------------------------ GCMC --
+int pm_gcmc_destroy(pm_gcmc* gc) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!gc) return PM_OK;
+    (void)hipStreamSynchronize(g_ctx.stream);
+    (void)hipFree(gc->d_params);
+    (void)hipFree(gc->d_chains);
+    (void)hipFree(gc->d_atoms);
+    (void)hipFree(gc->d_fg);
+    (void)hipFree(gc->d_gg);
+    (void)hipFree(gc->d_sites);
+    (void)hipFree(gc->d_com);
+    (void)hipFree(gc->d_dens);
+    (void)hipFree(gc->d_tot);
+    delete gc;
+    return PM_OK;
+}
+
+int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_species,
+                   const pm_probe* const* templates, int32_t n_chains, const double* temperatures,
+                   const double* fugacities, const pm_gcmc_settings* st, pm_gcmc** out) {
+    std::unique_lock<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (!fw || !ff || !templates || !temperatures || !fugacities || !st || !out)
+        return fail(PM_ERR_ARG, "pm_gcmc_create: null handle or pointer");
+    if (n_species < 1 || n_species > PM_MAX_GCMC_SPECIES)
+        return fail(PM_ERR_ARG, "pm_gcmc_create: n_species outside [1, PM_MAX_GCMC_SPECIES]");
+    if (n_chains < 1 || n_chains > 65536) return fail(PM_ERR_ARG, "pm_gcmc_create: n_chains outside [1, 65536]");
+    if (st->n_burn_cycles < 0 || st->n_sample_cycles < PM_GCMC_BLOCKS)
+        return fail(PM_ERR_ARG, "pm_gcmc_create: need n_burn_cycles >= 0 and n_sample_cycles >= 5");
+    if (st->sample_frequency < 1 || st->snapshot_frequency < 1)
+        return fail(PM_ERR_ARG, "pm_gcmc_create: sample and snapshot frequencies must be >= 1");
+    if (st->capacity < 1 || st->capacity > (1 << 20))
+        return fail(PM_ERR_ARG, "pm_gcmc_create: capacity outside [1, 2^20]");
+    if (!(st->delta > 0.0) || !std::isfinite(st->delta))
+        return fail(PM_ERR_ARG, "pm_gcmc_create: translation step must be positive");
+    if (int rc = check_species(fw, ff)) return rc;
+    for (int32_t c = 0; c < n_chains; ++c) {
+        if (!(temperatures[c] > 0.0) || !std::isfinite(temperatures[c]))
+            return fail(PM_ERR_ARG, "pm_gcmc_create: temperatures must be positive");
+        for (int32_t s = 0; s < n_species; ++s) {
+            const double f = fugacities[(size_t)c * n_species + s];
+            if (!(f > 0.0) || !std::isfinite(f)) return fail(PM_ERR_ARG, "pm_gcmc_create: fugacities must be positive");
+        }
+    }
+    int64_t n_dens = 0;
+    if (st->density_species >= 0) {
+        if (st->density_species >= n_species) return fail(PM_ERR_ARG, "pm_gcmc_create: density species out of range");
+        n_dens = 1;
+        for (int d = 0; d < 3; ++d) {
+            if (st->density_n_pts[d] < 1 || st->density_n_pts[d] > 4096)
+                return fail(PM_ERR_ARG, "pm_gcmc_create: density grid points outside [1, 4096]");
+            n_dens *= st->density_n_pts[d];
+        }
+        if (n_dens * n_chains > ((int64_t)1 << 31))
+            return fail(PM_ERR_ARG, "pm_gcmc_create: density grids too large");
+    }
+    GcmcParams hp{};
+    std::vector<int32_t> ads_ff;  // adsorbate site types -> force-field species
+    bool any_rot = false;
+    for (int32_t s = 0; s < n_species; ++s) {
+        const pm_probe* pr = templates[s];
+        if (!pr) return fail(PM_ERR_ARG, "pm_gcmc_create: null template");
+        int k = 0;
+        for (int i = 0; i < pr->ns; ++i) {
+            if (!pr->site[i].lj) continue;  // charges take no part in the reference's GCMC
+            const int32_t sp = pr->lj_species[i];
+            if (sp >= ff->n) return fail(PM_ERR_ARG, "pm_gcmc_create: template species outside the force field");
+            int t = -1;
+            for (size_t q = 0; q < ads_ff.size(); ++q)
+                if (ads_ff[q] == sp) t = (int)q;
+            if (t < 0) {
+                t = (int)ads_ff.size();
+                ads_ff.push_back(sp);
+            }
+            hp.site_type[s][k] = t;
+            hp.off[s][k][0] = pr->site[i].ox;
+            hp.off[s][k][1] = pr->site[i].oy;
+            hp.off[s][k][2] = pr->site[i].oz;
+            ++k;
+        }
+        if (k == 0) return fail(PM_ERR_ARG, "pm_gcmc_create: a template has no LJ sphere");
+        hp.n_sites[s] = k;
+        hp.rotatable[s] = pr->rotate;  // needs_rotations, src/mc_helpers.jl:8
+        any_rot |= pr->rotate != 0;
+    }
+    if (st->density_species >= 0 && (st->density_site_mask >> hp.n_sites[st->density_species]) != 0u)
+        return fail(PM_ERR_ARG, "pm_gcmc_create: density site mask names sites the template lacks");
+    const int n_ads = (int)ads_ff.size();
+    const int n_fw = (int)fw->species_global.size();
+    std::vector<double2> fg((size_t)std::max(1, n_fw) * n_ads), gg((size_t)n_ads * n_ads);
+    for (int i = 0; i < n_fw; ++i)
+        for (int j = 0; j < n_ads; ++j) {
+            const size_t q = (size_t)fw->species_global[i] * ff->n + ads_ff[j];
+            fg[(size_t)i * n_ads + j] = make_double2(ff->sig2[q], ff->eps[q]);
+        }
+    for (int i = 0; i < n_ads; ++i)
+        for (int j = 0; j < n_ads; ++j) {
+            const size_t q = (size_t)ads_ff[i] * ff->n + ads_ff[j];
+            gg[(size_t)i * n_ads + j] = make_double2(ff->sig2[q], ff->eps[q]);
+        }
+    // proposal probabilities (src/gcmc.jl:404-419), cumulative in move order
+    double prob[PM_GCMC_MOVES] = {0.35, 0.35, 0.25, any_rot ? 0.25 : 0.0, 0.05, n_species > 1 ? 0.1 : 0.0};
+    double tot = 0.0;
+    for (double v : prob) tot += v;
+    int last = 0;
+    double cum = 0.0;
+    for (int k = 0; k < PM_GCMC_MOVES; ++k) {
+        prob[k] /= tot;
+        cum += prob[k];
+        hp.cum[k] = cum;
+        if (prob[k] > 0.0) last = k;
+    }
+    for (int k = last; k < PM_GCMC_MOVES; ++k) hp.cum[k] = 2.0;
+    for (int i = 0; i < 9; ++i) {
+        hp.f[i] = fw->F[i];
+        hp.c2f[i] = fw->c2f[i];
+    }
+    hp.r2_cut = ff->r2_cutoff;
+    hp.omega = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
+    for (int d = 0; d < 3; ++d) hp.rep[d] = (double)fw->rep[d];
+    hp.n_atoms = (int)fw->n_rep;
+    hp.n_species = n_species;
+    hp.n_ads_types = n_ads;
+    hp.capacity = st->capacity;
+    hp.n_burn = st->n_burn_cycles;
+    hp.n_sample = st->n_sample_cycles;
+    hp.adjust_every = st->n_burn_cycles / 12;  // floor(n_burn / 12), src/gcmc.jl:635
+    hp.cycles_per_block = st->n_sample_cycles / PM_GCMC_BLOCKS;
+    hp.sample_frequency = st->sample_frequency;
+    hp.snapshot_frequency = st->snapshot_frequency;
+    hp.dens_species = st->density_species;
+    hp.dens_sim_box = st->density_sim_box;
+    hp.dens_mask = st->density_site_mask;
+    for (int d = 0; d < 3; ++d) hp.dens_n[d] = st->density_species >= 0 ? st->density_n_pts[d] : 1;
+    hp.seed = st->seed;
+    // replicated framework atoms in Xtals order with their compact type
+    std::vector<double4> atoms((size_t)std::max<int64_t>(1, fw->n_rep));
+    {
+        int64_t k = 0;
+        for (int ra = 0; ra < fw->rep[0]; ++ra)
+            for (int rb = 0; rb < fw->rep[1]; ++rb)
+                for (int rc = 0; rc < fw->rep[2]; ++rc)
+                    for (int64_t i = 0; i < fw->n_unit; ++i, ++k)
+                        atoms[(size_t)k] = make_double4((fw->unit_xf[3 * i] + 1.0 * ra) / fw->rep[0],
+                                                        (fw->unit_xf[3 * i + 1] + 1.0 * rb) / fw->rep[1],
+                                                        (fw->unit_xf[3 * i + 2] + 1.0 * rc) / fw->rep[2],
+                                                        (double)fw->unit_type[i]);
+    }
+    auto* gc = new pm_gcmc();
+    gc->n_chains = n_chains;
+    gc->n_species = n_species;
+    gc->capacity = st->capacity;
+    gc->n_dens = n_dens;
+    const size_t per_chain_sites = (size_t)n_species * st->capacity * PM_MAX_SITES;
+    const size_t per_chain_com = (size_t)n_species * st->capacity;
+    lk.unlock();  // pm_gcmc_destroy takes the lock on failure
+    auto bail = [&](int code, const char* msg) {
+        pm_gcmc_destroy(gc);
+        return fail(code, msg);
+    };
+    lk.lock();
+    if (hipMalloc(&gc->d_params, sizeof(GcmcParams)) != hipSuccess ||
+        hipMalloc(&gc->d_chains, sizeof(GcmcChain) * n_chains) != hipSuccess ||
+        hipMalloc(&gc->d_atoms, sizeof(double4) * atoms.size()) != hipSuccess ||
+        hipMalloc(&gc->d_fg, sizeof(double2) * fg.size()) != hipSuccess ||
+        hipMalloc(&gc->d_gg, sizeof(double2) * gg.size()) != hipSuccess ||
+        hipMalloc(&gc->d_sites, sizeof(double4) * per_chain_sites * n_chains) != hipSuccess ||
+        hipMalloc(&gc->d_com, sizeof(double4) * per_chain_com * n_chains) != hipSuccess ||
+        hipMalloc(&gc->d_tot, sizeof(double) * 2 * n_chains) != hipSuccess ||
+        (n_dens > 0 && hipMalloc(&gc->d_dens, sizeof(unsigned) * (size_t)n_dens * n_chains) != hipSuccess)) {
+        lk.unlock();
+        return bail(PM_ERR_OOM, "pm_gcmc_create: device allocation failed");
+    }
+    hp.atoms = gc->d_atoms;
+    hp.lj_fg = gc->d_fg;
+    hp.lj_gg = gc->d_gg;
+    gc->hp = hp;
+    std::vector<GcmcChain> hc((size_t)n_chains);
+    for (int32_t c = 0; c < n_chains; ++c) {
+        GcmcChain& h = hc[(size_t)c];
+        std::memset(&h, 0, sizeof(GcmcChain));
+        h.temperature = temperatures[c];
+        for (int32_t s = 0; s < n_species; ++s) h.fug[s] = fugacities[(size_t)c * n_species + s];
+        h.delta = st->delta;
+        h.sites = gc->d_sites + per_chain_sites * c;
+        h.com = gc->d_com + per_chain_com * c;
+        h.density = n_dens > 0 ? gc->d_dens + (size_t)n_dens * c : nullptr;
+    }
+    if (hipMemcpy(gc->d_params, &hp, sizeof(GcmcParams), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(gc->d_chains, hc.data(), sizeof(GcmcChain) * n_chains, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(gc->d_atoms, atoms.data(), sizeof(double4) * atoms.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(gc->d_fg, fg.data(), sizeof(double2) * fg.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(gc->d_gg, gg.data(), sizeof(double2) * gg.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (n_dens > 0 && hipMemset(gc->d_dens, 0, sizeof(unsigned) * (size_t)n_dens * n_chains) != hipSuccess)) {
+        lk.unlock();
+        return bail(PM_ERR_DEVICE, "pm_gcmc_create: host-to-device copy failed");
+    }
+    *out = gc;
+    return PM_OK;
+}
+
+int pm_gcmc_set_molecules(pm_gcmc* gc, int32_t chain, int32_t species, int32_t n, const double* site_xf,
+                          const double* com_xf) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!gc) return fail(PM_ERR_ARG, "pm_gcmc_set_molecules: null handle");
+    if (chain < 0 || chain >= gc->n_chains || species < 0 || species >= gc->n_species)
+        return fail(PM_ERR_ARG, "pm_gcmc_set_molecules: chain or species out of range");
+    if (n < 0 || n > gc->capacity) return fail(PM_ERR_ARG, "pm_gcmc_set_molecules: n outside [0, capacity]");
+    if (n > 0 && (!site_xf || !com_xf)) return fail(PM_ERR_ARG, "pm_gcmc_set_molecules: null coordinates");
+    GcmcChain h;
+    PM_HIP(hipMemcpy(&h, gc->d_chains + chain, sizeof(GcmcChain), hipMemcpyDeviceToHost));
+    if (h.outer != 0 || h.t != 0) return fail(PM_ERR_ARG, "pm_gcmc_set_molecules: the chain has already run");
+    const int ns = gc->hp.n_sites[species];
+    std::vector<double4> sites((size_t)std::max(1, n) * PM_MAX_SITES, make_double4(0.0, 0.0, 0.0, 0.0));
+    std::vector<double4> com((size_t)std::max(1, n));
+    for (int32_t m = 0; m < n; ++m) {
+        for (int k = 0; k < ns; ++k) {
+            const double* x = site_xf + ((size_t)m * ns + k) * 3;
+            if (!std::isfinite(x[0]) || !std::isfinite(x[1]) || !std::isfinite(x[2]))
+                return fail(PM_ERR_ARG, "pm_gcmc_set_molecules: non-finite coordinate");
+            sites[(size_t)m * PM_MAX_SITES + k] = make_double4(x[0], x[1], x[2], 0.0);
+        }
+        const double* c = com_xf + (size_t)m * 3;
+        if (!(c[0] >= 0.0 && c[0] <= 1.0 && c[1] >= 0.0 && c[1] <= 1.0 && c[2] >= 0.0 && c[2] <= 1.0))
+            return fail(PM_ERR_ARG, "pm_gcmc_set_molecules: a centre of mass lies outside the simulation box");
+        com[(size_t)m] = make_double4(c[0], c[1], c[2], 0.0);
+    }
+    if (n > 0) {
+        PM_HIP(hipMemcpy(h.sites + (size_t)species * gc->capacity * PM_MAX_SITES, sites.data(),
+                         sizeof(double4) * (size_t)n * PM_MAX_SITES, hipMemcpyHostToDevice));
+        PM_HIP(hipMemcpy(h.com + (size_t)species * gc->capacity, com.data(), sizeof(double4) * (size_t)n,
+                         hipMemcpyHostToDevice));
+    }
+    PM_HIP(hipMemcpy((char*)(gc->d_chains + chain) + offsetof(GcmcChain, n) + sizeof(int) * species, &n, sizeof(int),
+                     hipMemcpyHostToDevice));
+    gc->energy_stale = true;
+    return PM_OK;
+}
+
+namespace {
+
+// System energies of every chain from scratch into out[2 c + {0, 1}].
+int gcmc_total_energies(pm_gcmc* gc) {
+    hipLaunchKernelGGL(gcmc_total_energy_kernel, dim3(gc->n_chains), dim3(GC_WG), 0, g_ctx.stream, gc->d_params,
+                       gc->d_chains, gc->d_tot);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipStreamSynchronize(g_ctx.stream));
+    return PM_OK;
+}
+
+}  // namespace
+
+int pm_gcmc_run(pm_gcmc* gc, int64_t n_cycles) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (!gc) return fail(PM_ERR_ARG, "pm_gcmc_run: null handle");
+    if (n_cycles < 0) return fail(PM_ERR_ARG, "pm_gcmc_run: negative cycle count");
+    if (gc->energy_stale) {
+        // starting configurations: system energy of each chain (src/gcmc.jl:398-401)
+        if (int rc = gcmc_total_energies(gc)) return rc;
+        std::vector<double> tot((size_t)2 * gc->n_chains);
+        PM_HIP(hipMemcpy(tot.data(), gc->d_tot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost));
+        for (int c = 0; c < gc->n_chains; ++c) {
+            const double e[2] = {tot[2 * c], tot[2 * c + 1]};
+            PM_HIP(hipMemcpy((char*)(gc->d_chains + c) + offsetof(GcmcChain, e_gh), e, sizeof(e),
+                             hipMemcpyHostToDevice));
+        }
+        gc->energy_stale = false;
+    }
+    if (n_cycles == 0) return PM_OK;
+    (void)hipEventRecord(g_ctx.ev0, g_ctx.stream);
+    hipLaunchKernelGGL(gcmc_kernel, dim3(gc->n_chains), dim3(GC_WG), 0, g_ctx.stream, gc->d_params, gc->d_chains,
+                       (long long)n_cycles);
+    PM_HIP(hipGetLastError());
+    (void)hipEventRecord(g_ctx.ev1, g_ctx.stream);
+    PM_HIP(hipStreamSynchronize(g_ctx.stream));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, g_ctx.ev0, g_ctx.ev1);
+    g_ctx.last_ms = ms;
+    return PM_OK;
+}
+
+int pm_gcmc_get_state(const pm_gcmc* gc, int32_t chain, pm_gcmc_state* out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!gc || !out) return fail(PM_ERR_ARG, "pm_gcmc_get_state: null pointer");
+    if (chain < 0 || chain >= gc->n_chains) return fail(PM_ERR_ARG, "pm_gcmc_get_state: chain out of range");
+    GcmcChain h;
+    PM_HIP(hipMemcpy(&h, gc->d_chains + chain, sizeof(GcmcChain), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(pm_gcmc_state));
+    out->cycles_done = h.outer;
+    out->markov_time = h.t;
+    out->num_snapshots = h.num_snapshots;
+    for (int s = 0; s < PM_MAX_GCMC_SPECIES; ++s) out->n[s] = h.n[s];
+    out->block = h.block;
+    out->error = h.error;
+    out->delta = h.delta;
+    out->energy_gh = h.e_gh;
+    out->energy_gg = h.e_gg;
+    for (int k = 0; k < PM_GCMC_MOVES; ++k) {
+        out->n_proposed[k] = h.n_prop[k];
+        out->n_accepted[k] = h.n_acc[k];
+    }
+    for (int b = 0; b < PM_GCMC_BLOCKS; ++b) {
+        out->block_samples[b] = h.blk_samples[b];
+        for (int s = 0; s < PM_MAX_GCMC_SPECIES; ++s) {
+            out->block_n[b][s] = h.blk_n[b][s];
+            out->block_n2[b][s] = h.blk_n2[b][s];
+        }
+        out->block_gh[b] = h.blk_gh[b];
+        out->block_gg[b] = h.blk_gg[b];
+        out->block_gh2[b] = h.blk_gh2[b];
+        out->block_gg2[b] = h.blk_gg2[b];
+        out->block_un[b] = h.blk_un[b];
+    }
+    return PM_OK;
+}
+
+int pm_gcmc_get_molecules(const pm_gcmc* gc, int32_t chain, int32_t species, int32_t max_n, int32_t* n,
+                          double* site_xf, double* com_xf) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!gc || !n) return fail(PM_ERR_ARG, "pm_gcmc_get_molecules: null pointer");
+    if (chain < 0 || chain >= gc->n_chains || species < 0 || species >= gc->n_species)
+        return fail(PM_ERR_ARG, "pm_gcmc_get_molecules: chain or species out of range");
+    GcmcChain h;
+    PM_HIP(hipMemcpy(&h, gc->d_chains + chain, sizeof(GcmcChain), hipMemcpyDeviceToHost));
+    *n = h.n[species];
+    const int32_t m_copy = std::min(std::max(max_n, 0), h.n[species]);
+    if (m_copy == 0) return PM_OK;
+    if (!site_xf || !com_xf) return fail(PM_ERR_ARG, "pm_gcmc_get_molecules: null coordinates");
+    const int ns = gc->hp.n_sites[species];
+    std::vector<double4> sites((size_t)m_copy * PM_MAX_SITES), com((size_t)m_copy);
+    PM_HIP(hipMemcpy(sites.data(), h.sites + (size_t)species * gc->capacity * PM_MAX_SITES,
+                     sizeof(double4) * sites.size(), hipMemcpyDeviceToHost));
+    PM_HIP(hipMemcpy(com.data(), h.com + (size_t)species * gc->capacity, sizeof(double4) * com.size(),
+                     hipMemcpyDeviceToHost));
+    for (int32_t m = 0; m < m_copy; ++m) {
+        for (int k = 0; k < ns; ++k) {
+            const double4 v = sites[(size_t)m * PM_MAX_SITES + k];
+            double* x = site_xf + ((size_t)m * ns + k) * 3;
+            x[0] = v.x;
+            x[1] = v.y;
+            x[2] = v.z;
+        }
+        com_xf[3 * m] = com[(size_t)m].x;
+        com_xf[3 * m + 1] = com[(size_t)m].y;
+        com_xf[3 * m + 2] = com[(size_t)m].z;
+    }
+    return PM_OK;
+}
+
+int pm_gcmc_total_energy(pm_gcmc* gc, int32_t chain, double* out_gh, double* out_gg) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (!gc || !out_gh || !out_gg) return fail(PM_ERR_ARG, "pm_gcmc_total_energy: null pointer");
+    if (chain < 0 || chain >= gc->n_chains) return fail(PM_ERR_ARG, "pm_gcmc_total_energy: chain out of range");
+    if (int rc = gcmc_total_energies(gc)) return rc;
+    double e[2];
+    PM_HIP(hipMemcpy(e, gc->d_tot + 2 * chain, sizeof(e), hipMemcpyDeviceToHost));
+    *out_gh = e[0];
+    *out_gg = e[1];
+    return PM_OK;
+}
+
+int pm_gcmc_get_density(const pm_gcmc* gc, int32_t chain, double* out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!gc || !out) return fail(PM_ERR_ARG, "pm_gcmc_get_density: null pointer");
+    if (chain < 0 || chain >= gc->n_chains) return fail(PM_ERR_ARG, "pm_gcmc_get_density: chain out of range");
+    if (gc->n_dens == 0) return fail(PM_ERR_ARG, "pm_gcmc_get_density: the handle tracks no density grid");
+    std::vector<unsigned> h((size_t)gc->n_dens);
+    PM_HIP(hipMemcpy(h.data(), gc->d_dens + (size_t)gc->n_dens * chain, sizeof(unsigned) * h.size(),
+                     hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < gc->n_dens; ++i) out[i] = (double)h[(size_t)i];
     return PM_OK;
 }
 

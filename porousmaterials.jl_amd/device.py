@@ -419,3 +419,90 @@ def autotune_widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, pre
                                 int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_blocks), int(n_per_block),
                                 precision_code(precision), ctypes.byref(cp), ctypes.byref(cr), ctypes.byref(ms)))
     return {"cell_points": cp.value, "chunk_r_max": cr.value, "ms": ms.value}
+
+
+class GcmcChains:
+    """Grand-canonical Markov chains on the device (a pm_gcmc handle): one
+    replicated framework, ``templates`` (Molecule objects; their LJ spheres
+    are the simulated molecule), one (temperature, fugacities in Pa) per
+    chain.  ``density``: None or dict(species, site_mask, n_pts, sim_box)."""
+
+    def __init__(self, fw, ljff, templates, temperatures, fugacities, n_burn_cycles, n_sample_cycles,
+                 sample_frequency=1, snapshot_frequency=1, capacity=1024, density=None, delta=2.0, seed=0):
+        init()
+        lib = _lib.load()
+        ffh = _ff_handle(ljff)
+        self.templates = list(templates)
+        self.n_species = len(self.templates)
+        self.n_sites = [m.n_atoms for m in self.templates]
+        self._probes = [probe(m, ljff) for m in self.templates]
+        temps = np.ascontiguousarray(np.asarray(temperatures, dtype=np.float64).reshape(-1))
+        self.n_chains = temps.size
+        fug = np.ascontiguousarray(np.asarray(fugacities, dtype=np.float64).reshape(self.n_chains, self.n_species))
+        st = _lib.GcmcSettings()
+        st.n_burn_cycles = int(n_burn_cycles)
+        st.n_sample_cycles = int(n_sample_cycles)
+        st.sample_frequency = int(sample_frequency)
+        st.snapshot_frequency = int(snapshot_frequency)
+        st.capacity = int(capacity)
+        st.density_species = -1
+        if density is not None:
+            st.density_species = int(density["species"])
+            st.density_site_mask = int(density["site_mask"])
+            for d in range(3):
+                st.density_n_pts[d] = int(density["n_pts"][d])
+            st.density_sim_box = 1 if density.get("sim_box", True) else 0
+        st.delta = float(delta)
+        st.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self.capacity = int(capacity)
+        self.density_n_pts = tuple(int(x) for x in density["n_pts"]) if density is not None else None
+        arr = (ctypes.c_void_p * self.n_species)(*[p.ptr.value for p in self._probes])
+        out = ctypes.c_void_p()
+        check(lib.pm_gcmc_create(fw.ptr, ffh.ptr, self.n_species, arr, self.n_chains, dptr(temps), dptr(fug),
+                                 ctypes.byref(st), ctypes.byref(out)))
+        self._fw, self._ff = fw, ffh  # keep the handles alive
+        self._h = _Handle(out, "pm_gcmc_destroy")
+
+    def set_molecules(self, chain, species, site_xf, com_xf):
+        site_xf = np.ascontiguousarray(np.asarray(site_xf, dtype=np.float64).reshape(-1, self.n_sites[species], 3))
+        com_xf = np.ascontiguousarray(np.asarray(com_xf, dtype=np.float64).reshape(-1, 3))
+        n = site_xf.shape[0]
+        if com_xf.shape[0] != n:
+            raise ValueError("one centre of mass per molecule")
+        check(_lib.load().pm_gcmc_set_molecules(self._h.ptr, int(chain), int(species), n, dptr(site_xf),
+                                                dptr(com_xf)))
+
+    def run(self, n_cycles):
+        check(_lib.load().pm_gcmc_run(self._h.ptr, int(n_cycles)))
+
+    def state(self, chain):
+        st = _lib.GcmcState()
+        check(_lib.load().pm_gcmc_get_state(self._h.ptr, int(chain), ctypes.byref(st)))
+        out = {}
+        for name, _ in st._fields_:
+            v = getattr(st, name)
+            out[name] = np.ctypeslib.as_array(v).copy() if not isinstance(v, (int, float)) else v
+        return out
+
+    def molecules(self, chain, species):
+        lib = _lib.load()
+        n = ctypes.c_int32()
+        check(lib.pm_gcmc_get_molecules(self._h.ptr, int(chain), int(species), 0, ctypes.byref(n), None, None))
+        ns = self.n_sites[species]
+        sites = np.zeros((max(n.value, 1), ns, 3))
+        com = np.zeros((max(n.value, 1), 3))
+        check(lib.pm_gcmc_get_molecules(self._h.ptr, int(chain), int(species), n.value, ctypes.byref(n),
+                                        dptr(sites), dptr(com)))
+        return sites[:n.value], com[:n.value]
+
+    def total_energy(self, chain):
+        gh, gg = ctypes.c_double(), ctypes.c_double()
+        check(_lib.load().pm_gcmc_total_energy(self._h.ptr, int(chain), ctypes.byref(gh), ctypes.byref(gg)))
+        return gh.value, gg.value
+
+    def density(self, chain):
+        if self.density_n_pts is None:
+            raise ValueError("these chains track no density grid")
+        out = np.zeros(int(np.prod(self.density_n_pts)))
+        check(_lib.load().pm_gcmc_get_density(self._h.ptr, int(chain), dptr(out)))
+        return out.reshape(self.density_n_pts, order="F")

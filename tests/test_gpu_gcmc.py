@@ -1,0 +1,163 @@
+"""Grand-canonical Monte Carlo chains on the device (pm_gcmc_*) against the
+oracle's chain (oracle/gcmc.c) and the reference's known answers.
+
+Parity: device chains and oracle chains draw the same random stream, so they
+make the same proposals; with FP64 energies on both sides (different
+summation order) they take the same decisions.  After a run the counts,
+proposal/acceptance counters, block statistics, step size and density
+counts must be equal, the energies equal to 1e-9 (relative) and the
+coordinates to 1e-9.  Cases: Xe in SBMOF-1 (one site), CO2 in a synthetic
+framework (three sites, rotations), Kr/Xe in SBMOF-1 (identity changes) with
+a density grid, and a chain started from a given configuration.
+"""
+import math
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_chains(ogc, pm, x, ff, templates, temps, fugs, n_burn, n_sample, seed, capacity, density=None,
+                   rep=None):
+    tmpl = []
+    for m in templates:
+        off = (m.atoms_x - m.com[:, None]).T
+        tmpl.append((off, [ff.index(s) for s in m.atom_species], pm.needs_rotations(m)))
+    out = []
+    for c, (T, f) in enumerate(zip(temps, fugs)):
+        out.append(ogc.OracleChain((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
+                                   [ff.index(s) for s in x.species], ff.epsilon_table, ff.sigma2_table,
+                                   math.sqrt(ff.r2_cutoff), tmpl, T, f, n_burn, n_sample, seed, chain=c,
+                                   capacity=capacity, density=density, rep=rep))
+    return out
+
+
+def _compare(dev, ochains, n_species, label):
+    for c, oc in enumerate(ochains):
+        d = dev.state(c)
+        o = oc.state
+        assert d["error"] == o.error == 0, label
+        assert d["cycles_done"] == o.outer and d["markov_time"] == o.t, (label, c)
+        assert list(d["n"][:n_species]) == list(o.n[:n_species]), (label, c, d["n"], list(o.n))
+        assert list(d["n_proposed"]) == list(o.n_prop), (label, c)
+        assert list(d["n_accepted"]) == list(o.n_acc), (label, c, list(d["n_accepted"]), list(o.n_acc))
+        assert d["block"] == o.block and d["num_snapshots"] == o.num_snapshots
+        assert list(d["block_samples"]) == list(o.blk_samples)
+        assert np.array_equal(d["block_n"], np.array([list(r) for r in o.blk_n]))
+        assert np.array_equal(d["block_n2"], np.array([list(r) for r in o.blk_n2]))
+        assert d["delta"] == o.delta
+        for a, b in ((d["energy_gh"], o.e_gh), (d["energy_gg"], o.e_gg)):
+            assert abs(a - b) <= 1e-9 * (abs(b) + 1.0), (label, c, a, b)
+        for key, ok in (("block_gh", o.blk_gh), ("block_gg", o.blk_gg), ("block_un", o.blk_un)):
+            want = np.array(list(ok))
+            assert np.allclose(d[key], want, rtol=1e-9, atol=1e-6), (label, key)
+        for s in range(n_species):
+            ds, dc = dev.molecules(c, s)
+            os_, oc_ = oc.molecules(s)
+            assert ds.shape == os_.shape
+            assert np.max(np.abs(ds - os_), initial=0.0) <= 1e-9 and np.max(np.abs(dc - oc_), initial=0.0) <= 1e-9
+
+
+def test_gcmc_xe_sbmof1_matches_oracle(pm, orc):
+    from oracle import gcmc as ogc
+    say("case: GCMC Xe/SBMOF-1 Dreiding, 3 chains (20, 200, 2000 Pa), 300 burn + 100 sample cycles vs oracle")
+    x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
+    ff = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
+    xe = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    rep = pm.replication_factors(x.box, ff)
+    fw = pm.device.framework(pm.replicate(x, rep), ff)
+    fugs = [[20.0], [200.0], [2000.0]]
+    dev = pm.device.GcmcChains(fw, ff, [xe], [298.0] * 3, fugs, 300, 100, capacity=256, seed=17)
+    t = time.time()
+    dev.run(170)
+    dev.run(230)
+    dt = time.time() - t
+    och = _oracle_chains(ogc, pm, x, ff, [xe], [298.0] * 3, fugs, 300, 100, 17, 256)
+    for oc in och:
+        oc.run(400)
+    moves = sum(dev.state(c)["markov_time"] for c in range(3))
+    say(f"   {moves} moves over 3 chains in {dt:.2f} s; N = {[dev.state(c)['n'][0] for c in range(3)]}")
+    _compare(dev, och, 1, "Xe")
+    for c in range(3):
+        gh, gg = dev.total_energy(c)
+        st = dev.state(c)
+        assert abs(gh - st["energy_gh"]) <= 1e-6 * (abs(gh) + 1) and abs(gg - st["energy_gg"]) <= 1e-6 * (abs(gg) + 1)
+
+
+def test_gcmc_co2_rotations_match_oracle(pm, orc):
+    from oracle import gcmc as ogc
+    say("case: GCMC CO2 (3 sites) in the 424-atom synthetic framework, 2 chains, 100 burn + 50 sample cycles")
+    case = pm.synthetic.lj_grid_case()
+    x, ff = case.crystal, case.ljff
+    co2 = pm.synthetic.co2_molecule()
+    rep = pm.replication_factors(x.box, ff)
+    fw = pm.device.framework(pm.replicate(x, rep), ff)
+    fugs = [[2e6], [2e7]]
+    dev = pm.device.GcmcChains(fw, ff, [co2], [260.0, 300.0], fugs, 100, 50, capacity=512, seed=3)
+    dev.run(150)
+    och = _oracle_chains(ogc, pm, x, ff, [co2], [260.0, 300.0], fugs, 100, 50, 3, 512)
+    for oc in och:
+        oc.run(150)
+    st = [dev.state(c) for c in range(2)]
+    say(f"   N = {[s['n'][0] for s in st]}, rotations accepted {[int(s['n_accepted'][3]) for s in st]}")
+    assert all(s["n_proposed"][3] > 0 for s in st)
+    _compare(dev, och, 1, "CO2")
+
+
+def test_gcmc_mixture_identity_density_match_oracle(pm, orc):
+    from oracle import gcmc as ogc
+    say("case: GCMC Kr/Xe (0.9/0.1 of 0.5 bar) in SBMOF-1, UFF, identity changes and a Xe density grid")
+    x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
+    ff = pm.LJForceField.from_csv(golden("forcefields", "UFF.csv"))
+    kr = pm.Molecule.from_dir(golden("molecules", "Kr"))
+    xe = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    rep = pm.replication_factors(x.box, ff)
+    xr = pm.replicate(x, rep)
+    fw = pm.device.framework(xr, ff)
+    fugs = [[0.45e5, 0.05e5], [0.09e5, 0.01e5]]
+    dens = {"species": 1, "site_mask": 1, "n_pts": (9, 11, 13), "sim_box": True}
+    dev = pm.device.GcmcChains(fw, ff, [kr, xe], [298.0] * 2, fugs, 200, 50, capacity=256, seed=11, density=dens)
+    dev.run(60)
+    dev.run(190)
+    och = _oracle_chains(ogc, pm, x, ff, [kr, xe], [298.0] * 2, fugs, 200, 50, 11, 256, density=dens)
+    for oc in och:
+        oc.run(250)
+    st = [dev.state(c) for c in range(2)]
+    say(f"   N(Kr, Xe) = {[list(s['n'][:2]) for s in st]}, identity changes accepted "
+        f"{[int(s['n_accepted'][5]) for s in st]}")
+    assert all(s["n_proposed"][5] > 0 for s in st)
+    _compare(dev, och, 2, "Kr/Xe")
+    for c, oc in enumerate(och):
+        assert np.array_equal(dev.density(c), oc.density())
+    assert dev.density(0).sum() > 0
+
+
+def test_gcmc_start_configuration_matches_oracle(pm, orc):
+    from oracle import gcmc as ogc
+    say("case: GCMC Xe/SBMOF-1 started from a given configuration (energy recomputed), 80 cycles")
+    x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
+    ff = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
+    xe = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    rep = pm.replication_factors(x.box, ff)
+    fw = pm.device.framework(pm.replicate(x, rep), ff)
+    dev0 = pm.device.GcmcChains(fw, ff, [xe], [298.0], [[2000.0]], 400, 5, capacity=256, seed=17)
+    dev0.run(400)
+    sites, com = dev0.molecules(0, 0)
+    assert sites.shape[0] > 5
+    dev = pm.device.GcmcChains(fw, ff, [xe], [298.0], [[500.0]], 40, 40, capacity=256, seed=2)
+    dev.set_molecules(0, 0, sites, com)
+    dev.run(80)
+    oc = _oracle_chains(ogc, pm, x, ff, [xe], [298.0], [[500.0]], 40, 40, 2, 256)[0]
+    oc.set_molecules(0, sites, com)
+    oc.run(80)
+    _compare(dev, [oc], 1, "restart")

@@ -213,15 +213,16 @@ def test_probe_merge_and_limits(pm):
 
 
 def test_batch_structs_match_the_c_layout(tmp_path):
-    """ctypes mirrors of pm_widom_run / pm_grid_run have the C compiler's
-    sizes and field offsets."""
+    """ctypes mirrors of pm_widom_run / pm_grid_run / pm_gcmc_settings /
+    pm_gcmc_state have the C compiler's sizes and field offsets."""
     import shutil
     import subprocess
     from porousmaterials_jl_amd import _lib
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
-    fields = {"pm_widom_run": [f for f, _ in _lib.WidomRun._fields_],
-              "pm_grid_run": [f for f, _ in _lib.GridRun._fields_]}
+    classes = {"pm_widom_run": _lib.WidomRun, "pm_grid_run": _lib.GridRun,
+               "pm_gcmc_settings": _lib.GcmcSettings, "pm_gcmc_state": _lib.GcmcState}
+    fields = {k: [f for f, _ in v._fields_] for k, v in classes.items()}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "pmhip.h"', "int main(void) {"]
     for st, fs in fields.items():
         src.append(f'printf("%zu\\n", sizeof({st}));')
@@ -233,7 +234,7 @@ def test_batch_structs_match_the_c_layout(tmp_path):
     subprocess.run(["gcc", "-I", os.path.dirname(_lib.HEADER), str(c), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = []
-    for cls in (_lib.WidomRun, _lib.GridRun):
+    for cls in classes.values():
         want.append(ctypes.sizeof(cls))
         want += [getattr(cls, f).offset for f, _ in cls._fields_]
     assert got == want
