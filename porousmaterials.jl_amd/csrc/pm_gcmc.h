@@ -80,8 +80,29 @@ __device__ inline void gcmc_uniforms(unsigned long long seed, unsigned chain, un
     }
 }
 
+// The same uniforms, one Philox block per lane 0..4 of each wave and
+// broadcast: identical values at a fifth of the integer work.
+__device__ inline void gcmc_uniforms_wave(unsigned long long seed, unsigned chain, unsigned long long t,
+                                          double u[10]) {
+    const uint32_t w = (uint32_t)min((int)(threadIdx.x & 63), 4);
+    uint32_t c[4] = {(uint32_t)t, (uint32_t)(t >> 32), chain, 0x47434D00u | w};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const uint32_t a = (uint32_t)__shfl((int)c[0], k, 64), b = (uint32_t)__shfl((int)c[1], k, 64);
+        const uint32_t e = (uint32_t)__shfl((int)c[2], k, 64), f = (uint32_t)__shfl((int)c[3], k, 64);
+        u[2 * k] = u53(a, b);
+        u[2 * k + 1] = u53(e, f);
+    }
+}
+
+// sigma^2 / r^2 through the hardware reciprocal and two Newton steps (within
+// an ulp of the quotient), then 4 eps ratio (ratio - 1), ratio = x^3.
 __device__ inline double lj_pair(double r2, double2 se) {
-    const double x = se.x / r2;
+    double inv = __builtin_amdgcn_rcp(r2);
+    inv = fma(fma(-r2, inv, 1.0), inv, inv);
+    inv = fma(fma(-r2, inv, 1.0), inv, inv);
+    const double x = se.x * inv;
     const double ratio = x * x * x;
     return 4.0 * se.y * ratio * (ratio - 1.0);
 }
@@ -105,8 +126,10 @@ __device__ inline void gc_pair(const GcmcParams& p, double4 x, double4 y, double
 // Guest-host and guest-guest energy of a molecule of species ts with sites
 // trial[0..n_sites), against the framework and every stored molecule except
 // (ex_s, ex_m).  Fixed-order workgroup reduction; +Inf on overlap.
-__device__ inline void gc_energy(const GcmcParams& p, const GcmcChain& ch, const int* n, const double4* trial, int ts,
-                                 int ex_s, int ex_m, double* red, double& gh, double& gg) {
+__device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams& p, const GcmcChain& ch,
+                                                                const double4* atoms, const int* n,
+                                                                const double4* trial, int ts, int ex_s, int ex_m,
+                                                                double* red, double& gh, double& gg) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ns = p.n_sites[ts];
     double agh = 0.0, agg = 0.0;
@@ -115,7 +138,7 @@ __device__ inline void gc_energy(const GcmcParams& p, const GcmcChain& ch, const
         const double4 x = trial[k];
         const int tk = p.site_type[ts][k];
         for (int a = tid; a < p.n_atoms; a += GC_WG) {
-            const double4 at = p.atoms[a];
+            const double4 at = atoms[a];
             gc_pair(p, x, at, p.lj_fg[(int)at.w * p.n_ads_types + tk], agh, ogh);
         }
     }
@@ -257,9 +280,12 @@ __device__ inline void gc_density(const GcmcParams& p, GcmcChain& ch, const int*
     __syncthreads();
 }
 
-// Advances every chain (one workgroup each) by n_cycles outer cycles.
+// Advances every chain (one workgroup each) by n_cycles outer cycles.  With
+// LDS_ATOMS the replicated framework is staged in LDS for the launch.
+template <bool LDS_ATOMS>
 __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restrict__ pp, GcmcChain* chains,
                                                      long long n_cycles) {
+    extern __shared__ double4 s_atoms[];
     __shared__ double4 s_trial[PM_MAX_SITES];
     __shared__ double4 s_old[PM_MAX_SITES];
     __shared__ double s_red[4 * (GC_WG / 64)];
@@ -267,6 +293,11 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
     GcmcChain& ch = chains[blockIdx.x];
     const int tid = threadIdx.x;
     if (ch.error) return;
+    const double4* atoms = p.atoms;
+    if constexpr (LDS_ATOMS) {
+        for (int a = tid; a < p.n_atoms; a += GC_WG) s_atoms[a] = p.atoms[a];
+        atoms = s_atoms;
+    }
     const unsigned chain = blockIdx.x;
     const double T = ch.temperature;
     int n[GC_NS];
@@ -312,7 +343,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
         for (long long it = 0; it < inner; ++it) {
             ++t;
             double u[10];
-            gcmc_uniforms(p.seed, chain, (unsigned long long)t, u);
+            gcmc_uniforms_wave(p.seed, chain, (unsigned long long)t, u);
             const int s = S > 1 ? min(S - 1, (int)(u[0] * (double)S)) : 0;
             const int n_i = n[s];
             int mv = 0;
@@ -332,7 +363,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_trial[tid] = gc_new_site(p, s, tid, com, R, rot);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, ch, n, s_trial, s, -1, -1, s_red, gh, gg);
+                gc_energy(p, ch, atoms, n, s_trial, s, -1, -1, s_red, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
                 const double acc = ch.fug[s] * p.omega / ((double)(n_i + 1) * GC_KB * T) * exp(-dE / T);
                 if (u[9] < acc) {
@@ -346,7 +377,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_trial[tid] = *gc_site(p, ch, s, m, tid);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, ch, n, s_trial, s, s, m, s_red, gh, gg);
+                gc_energy(p, ch, atoms, n, s_trial, s, s, m, s_red, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
                 const double acc = (double)n_i * GC_KB * T / (ch.fug[s] * p.omega) * exp(dE / T);
                 if (u[9] < acc) {
@@ -362,7 +393,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_old[tid] = *gc_site(p, ch, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, ch, n, s_old, s, s, m, s_red, gh0, gg0);
+                gc_energy(p, ch, atoms, n, s_old, s, s, m, s_red, gh0, gg0);
                 double4 com1 = com0;
                 {
 #pragma clang fp contract(off)
@@ -407,7 +438,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                     }
                 }
                 __syncthreads();
-                gc_energy(p, ch, n, s_trial, s, s, m, s_red, gh1, gg1);
+                gc_energy(p, ch, atoms, n, s_trial, s, s, m, s_red, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
                 if (u[9] < exp(-(e_new - e_old) / T)) {
                     gc_store(p, ch, s, m, s_trial, com1);
@@ -422,7 +453,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_old[tid] = *gc_site(p, ch, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, ch, n, s_old, s, s, m, s_red, gh0, gg0);
+                gc_energy(p, ch, atoms, n, s_old, s, s, m, s_red, gh0, gg0);
                 gc_remove(p, ch, s, m, n_i);
                 n[s] = n_i - 1;
                 int pick = min(S - 2, (int)(u[3] * (double)(S - 1)));
@@ -437,7 +468,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (rot2) arvo_rotation(&u[6], R);
                 if (tid < p.n_sites[s2]) s_trial[tid] = gc_new_site(p, s2, tid, com0, R, rot2);
                 __syncthreads();
-                gc_energy(p, ch, n, s_trial, s2, -1, -1, s_red, gh1, gg1);
+                gc_energy(p, ch, atoms, n, s_trial, s2, -1, -1, s_red, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
                 const double acc = ((double)n_i * ch.fug[s2] / ((double)(n_j + 1) * ch.fug[s])) * exp(-(e_new - e_old) / T);
                 if (u[9] < acc) {
@@ -514,6 +545,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_total_energy_kernel(const GcmcPara
     __shared__ double s_red[4 * (GC_WG / 64)];
     const GcmcParams& p = *pp;
     GcmcChain& ch = chains[blockIdx.x];
+    const double4* atoms = p.atoms;
     int n[GC_NS];
 #pragma unroll
     for (int s = 0; s < GC_NS; ++s) n[s] = ch.n[s];
@@ -523,7 +555,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_total_energy_kernel(const GcmcPara
             if (threadIdx.x < p.n_sites[s]) s_trial[threadIdx.x] = *gc_site(p, ch, s, m, threadIdx.x);
             __syncthreads();
             double gh, gg;
-            gc_energy(p, ch, n, s_trial, s, s, m, s_red, gh, gg);
+            gc_energy(p, ch, atoms, n, s_trial, s, s, m, s_red, gh, gg);
             tgh += gh;
             tgg += gg;
         }

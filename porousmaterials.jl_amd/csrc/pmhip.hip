@@ -43,6 +43,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
     double last_ms = 0.0, last_pair_ms = 0.0, last_pairs = 0.0, last_cand = 0.0;
+    size_t max_lds = 0;  // LDS bytes one workgroup may allocate
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     // host staging for the per-call LJ table: outlives the asynchronous copy
@@ -1064,6 +1065,9 @@ int pm_init(int32_t device) {
     PM_HIP(hipEventCreate(&g_ctx.ev1));
     PM_HIP(hipEventCreate(&g_ctx.evp0));
     PM_HIP(hipEventCreate(&g_ctx.evp1));
+    int lds = 0;
+    PM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    g_ctx.max_lds = (size_t)std::max(lds, 0);
     g_ctx.device = device;
     return PM_OK;
 }
@@ -2083,9 +2087,16 @@ int pm_gcmc_run(pm_gcmc* gc, int64_t n_cycles) {
         gc->energy_stale = false;
     }
     if (n_cycles == 0) return PM_OK;
+    // the framework goes to LDS when it fits beside the kernel's own LDS
+    const size_t lds_atoms = sizeof(double4) * (size_t)gc->hp.n_atoms;
+    const bool use_lds = gc->hp.n_atoms > 0 && lds_atoms + 4096 <= g_ctx.max_lds;
     (void)hipEventRecord(g_ctx.ev0, g_ctx.stream);
-    hipLaunchKernelGGL(gcmc_kernel, dim3(gc->n_chains), dim3(GC_WG), 0, g_ctx.stream, gc->d_params, gc->d_chains,
-                       (long long)n_cycles);
+    if (use_lds)
+        hipLaunchKernelGGL(gcmc_kernel<true>, dim3(gc->n_chains), dim3(GC_WG), lds_atoms, g_ctx.stream,
+                           gc->d_params, gc->d_chains, (long long)n_cycles);
+    else
+        hipLaunchKernelGGL(gcmc_kernel<false>, dim3(gc->n_chains), dim3(GC_WG), 0, g_ctx.stream, gc->d_params,
+                           gc->d_chains, (long long)n_cycles);
     PM_HIP(hipGetLastError());
     (void)hipEventRecord(g_ctx.ev1, g_ctx.stream);
     PM_HIP(hipStreamSynchronize(g_ctx.stream));
