@@ -94,7 +94,7 @@ def _check_inputs(xtal, templates, ljff):
 
 
 def _molecular_weight(xtal):
-    return sum(ATOMIC_MASSES[s] for s in xtal.species)
+    return float(sum(ATOMIC_MASSES[s] for s in xtal.species))
 
 
 def _results(chain_state, xtal, xtal_rep, templates, temperature, pressures, fugacities, ljff, repfactors, n_burn,
@@ -146,9 +146,11 @@ def _results(chain_state, xtal, xtal_rep, templates, temperature, pressures, fug
     r["err ⟨U⟩ (K)"] = err_gg + 0.0 + err_gh + 0.0
     r["⟨N⟩ (molecules/unit cell)"] = avg_n / nr
     r["err ⟨N⟩ (molecules/unit cell)"] = err_n / nr
-    to_mmol_g = 1000 / (6.022140857e23 * _molecular_weight(xtal_rep) * 1.66054e-24) * nr
-    r["⟨N⟩ (mmol/g)"] = r["⟨N⟩ (molecules/unit cell)"] * to_mmol_g
-    r["err ⟨N⟩ (mmol/g)"] = r["err ⟨N⟩ (molecules/unit cell)"] * to_mmol_g
+    # an empty framework has no mass: Inf (NaN at zero loading), as Julia's float division gives
+    with np.errstate(invalid="ignore", divide="ignore"):
+        to_mmol_g = np.float64(1000.0) / (6.022140857e23 * _molecular_weight(xtal_rep) * 1.66054e-24) * nr
+        r["⟨N⟩ (mmol/g)"] = r["⟨N⟩ (molecules/unit cell)"] * to_mmol_g
+        r["err ⟨N⟩ (mmol/g)"] = r["err ⟨N⟩ (molecules/unit cell)"] * to_mmol_g
     for k, desc in PROPOSAL_ENCODINGS.items():
         prop, acc = int(st["n_proposed"][k - 1]), int(st["n_accepted"][k - 1])
         r["Total # %s proposals" % desc] = prop
