@@ -86,6 +86,66 @@ def cpu_baseline(pm, case, xr, seed, target_s):
     }
 
 
+GCMC_WORKLOAD = ("xe_sbmof1: Xe in SBMOF-1 (tests/golden), UFF, r_c 14 A, replicated to 4320 atoms, 298 K, "
+                 "0.5 bar (about 71 molecules per chain), after 1500 untimed cycles from an empty box")
+FILL_CYCLES = 1500
+
+
+def _gcmc_setup(pm):
+    g = os.path.join(ROOT, "tests", "golden")
+    x = pm.read_cif(os.path.join(g, "crystals", "SBMOF-1.cif"))
+    ff = pm.LJForceField.from_csv(os.path.join(g, "forcefields", "UFF.csv"))
+    xe = pm.Molecule.from_dir(os.path.join(g, "molecules", "Xe"))
+    return x, ff, xe
+
+
+def gcmc_leg(pm, cycles=100):
+    """Grand-canonical chains through pm_gcmc_run: Markov steps per second
+    with one chain per CU (256 chains) and with a single chain."""
+    x, ff, xe = _gcmc_setup(pm)
+    xr = pm.replicate(x, pm.replication_factors(x.box, ff))
+    fw = pm.device.framework(xr, ff)
+    out = {"workload": GCMC_WORKLOAD, "unit": "MC steps/s"}
+    for key, n in (("chains_256", 256), ("chain_1", 1)):
+        ch = pm.device.GcmcChains(fw, ff, [xe], [298.0] * n, [[0.5e5]] * n, 10 ** 6, 10 ** 6, capacity=512,
+                                  seed=1)
+        ch.run(FILL_CYCLES)
+        t0 = np.array([ch.state(c)["markov_time"] for c in range(n)])
+        t = time.perf_counter()
+        ch.run(cycles)
+        dt = time.perf_counter() - t
+        per_chain = np.array([ch.state(c)["markov_time"] for c in range(n)]) - t0
+        moves = int(per_chain.sum())
+        mean_n = float(np.mean([ch.state(c)["n"][0] for c in range(n)]))
+        # a launch lasts as long as its longest chain
+        out[key] = {"chains": n, "cycles": cycles, "steps": moves, "seconds": dt, "value": moves / dt,
+                    "us_per_step_longest_chain": dt / int(per_chain.max()) * 1e6, "mean_molecules": mean_n}
+        print(f"gcmc {key}: {moves / dt:.3e} steps/s, {dt / int(per_chain.max()) * 1e6:.2f} us/step on the "
+              f"longest chain, <N> {mean_n:.1f}", file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_gcmc_baseline(pm, target_s):
+    """The oracle's chain (oracle/gcmc.c, one core) on the GCMC leg's
+    workload: Markov steps per second."""
+    from oracle import gcmc as ogc
+    x, ff, xe = _gcmc_setup(pm)
+    oc = ogc.OracleChain((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
+                         [ff.index(s) for s in x.species], ff.epsilon_table, ff.sigma2_table,
+                         math.sqrt(ff.r2_cutoff), [((xe.atoms_x - xe.com[:, None]).T,
+                                                    [ff.index(s) for s in xe.atom_species], False)],
+                         298.0, [0.5e5], 10 ** 6, 10 ** 6, 1, capacity=512)
+    oc.run(FILL_CYCLES)
+    t0, steps, t = oc.state.t, 0, time.perf_counter()
+    while time.perf_counter() - t < target_s:
+        oc.run(10)
+    dt = time.perf_counter() - t
+    steps = oc.state.t - t0
+    return {"value": steps / dt, "unit": "MC steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} Markov steps of one chain of the GCMC leg's workload after {FILL_CYCLES} cycles, "
+                      f"<N> {oc.state.n[0]}"}
+
+
 def screening_leg(pm, steps, n_per_block=20000):
     """256 synthetic frameworks x (CH4, CO2) x 10^5 insertions in one
     pm_widom_batch call per step (mixed precision)."""
@@ -280,6 +340,7 @@ def main():
     ap.add_argument("--no-screening", action="store_true", help="skip the batched screening leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (gradient) leg")
     ap.add_argument("--no-grids", action="store_true", help="skip the energy-grid leg")
+    ap.add_argument("--no-gcmc", action="store_true", help="skip the grand-canonical Monte Carlo leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (pm_copy_probe) three times: the "
                          "calibration launch for FETCH_SIZE / WRITE_SIZE counter runs")
@@ -390,10 +451,15 @@ def main():
         out["screening"] = screening_leg(pm, max(1, min(args.steps, 3)))
     if not args.no_grids:
         out["grids"] = grids_leg(pm, max(1, min(args.steps, 3)))
+    if not args.no_gcmc:
+        out["gcmc"] = gcmc_leg(pm)
     if not args.no_cpu:
         cb = cpu_baseline(pm, case, xr, seed, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+        if not args.no_gcmc:
+            cb["gcmc"] = cpu_gcmc_baseline(pm, min(5.0, args.cpu_seconds))
+            out["gcmc"]["gpu_over_cpu"] = out["gcmc"]["chains_256"]["value"] / cb["gcmc"]["value"]
     if args.copy_probe:
         ms = ctypes.c_double()
         _lib = pm._lib

@@ -42,6 +42,7 @@ struct GcmcParams {
     double off[GC_NS][PM_MAX_SITES][3];  // Cartesian offsets from the centre of mass
     const double2* lj_fg;    // [framework type][adsorbate type] (sigma^2, eps)
     const double2* lj_gg;    // [adsorbate type][adsorbate type]
+    int n_fw_types;          // rows of lj_fg
     double cum[GC_NM];       // cumulative proposal probabilities (last used entry 2.0)
     long long n_burn, n_sample, adjust_every, cycles_per_block;
     int sample_frequency, snapshot_frequency;
@@ -80,20 +81,58 @@ __device__ inline void gcmc_uniforms(unsigned long long seed, unsigned chain, un
     }
 }
 
-// The same uniforms, one Philox block per lane 0..4 of each wave and
-// broadcast: identical values at a fifth of the integer work.
-__device__ inline void gcmc_uniforms_wave(unsigned long long seed, unsigned chain, unsigned long long t,
-                                          double u[10]) {
-    const uint32_t w = (uint32_t)min((int)(threadIdx.x & 63), 4);
-    uint32_t c[4] = {(uint32_t)t, (uint32_t)(t >> 32), chain, 0x47434D00u | w};
-    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+// The same uniforms for GC_BATCH consecutive moves at once: lane l < 60 of
+// every wave holds the Philox block (t0 + l / 5, word l % 5), so one pass of
+// integer work serves 12 moves; a move reads its five blocks with readlane.
+constexpr int GC_BATCH = 12;
+struct GcRng {
+    unsigned long long t0;
+    uint32_t c[4];
+};
+
+__device__ inline void gc_rng_fill(GcRng& r, unsigned long long seed, unsigned chain, unsigned long long t0) {
+    const uint32_t l = (uint32_t)min((int)(threadIdx.x & 63), 5 * GC_BATCH - 1);
+    const unsigned long long t = t0 + l / 5;
+    r.c[0] = (uint32_t)t;
+    r.c[1] = (uint32_t)(t >> 32);
+    r.c[2] = chain;
+    r.c[3] = 0x47434D00u | (l % 5);
+    philox4x32_10(r.c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    r.t0 = t0;
+}
+
+__device__ inline void gc_rng_get(const GcRng& r, unsigned long long t, double u[10]) {
+    const int l0 = 5 * (int)(t - r.t0);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        const uint32_t a = (uint32_t)__shfl((int)c[0], k, 64), b = (uint32_t)__shfl((int)c[1], k, 64);
-        const uint32_t e = (uint32_t)__shfl((int)c[2], k, 64), f = (uint32_t)__shfl((int)c[3], k, 64);
+        const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)r.c[0], l0 + k);
+        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)r.c[1], l0 + k);
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)r.c[2], l0 + k);
+        const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)r.c[3], l0 + k);
         u[2 * k] = u53(a, b);
         u[2 * k + 1] = u53(e, f);
     }
+}
+
+// Sum over the 64 lanes of a wave by DPP row shifts and row broadcasts
+// (the total lands in lane 63), returned to every lane.
+template <int CTRL, int ROW_MASK>
+__device__ inline double gc_dpp(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline double gc_wave_total(double v) {
+    v += gc_dpp<0x111, 0xf>(v);  // row_shr:1
+    v += gc_dpp<0x112, 0xf>(v);  // row_shr:2
+    v += gc_dpp<0x114, 0xf>(v);  // row_shr:4
+    v += gc_dpp<0x118, 0xf>(v);  // row_shr:8  -> lane 15 of each row holds the row's sum
+    v += gc_dpp<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += gc_dpp<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3 -> lane 63 holds the total
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 // sigma^2 / r^2 through the hardware reciprocal and two Newton steps (within
@@ -123,94 +162,127 @@ __device__ inline void gc_pair(const GcmcParams& p, double4 x, double4 y, double
     }
 }
 
+// Guest site jj of the flattened (species, molecule, site) list of stored
+// molecules, unless it lies past the end or belongs to the excluded molecule.
+__device__ inline bool gc_guest(const GcmcParams& p, const double4* sites, const int* n, int jj, int total,
+                                int ex_s, int ex_m, double4& y, int& ty) {
+    if (jj >= total) return false;
+    int s2 = 0;
+    while (jj >= n[s2] * p.n_sites[s2]) {
+        jj -= n[s2] * p.n_sites[s2];
+        ++s2;
+    }
+    const int ns2 = p.n_sites[s2];
+    const int m2 = jj / ns2, k2 = jj - m2 * ns2;
+    if (s2 == ex_s && m2 == ex_m) return false;
+    y = sites[((long long)s2 * p.capacity + m2) * PM_MAX_SITES + k2];
+    ty = p.site_type[s2][k2];
+    return true;
+}
+
 // Guest-host and guest-guest energy of a molecule of species ts with sites
 // trial[0..n_sites), against the framework and every stored molecule except
-// (ex_s, ex_m).  Fixed-order workgroup reduction; +Inf on overlap.
-__device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams& p, const GcmcChain& ch,
-                                                                const double4* atoms, const int* n,
+// (ex_s, ex_m).  Each work-item sums its atoms in index order, then its guest
+// sites; waves reduce by DPP and the four wave sums are added in wave order.
+// +Inf on overlap.  `red` alternates between two halves (`par`), so the next
+// evaluation needs no barrier before writing its partial sums.
+__device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams& p, const double4* sites,
+                                                                const double4* atoms, const double2* fg,
+                                                                const double2* ggt, const int* n,
                                                                 const double4* trial, int ts, int ex_s, int ex_m,
-                                                                double* red, double& gh, double& gg) {
+                                                                double* red, int& par, double& gh, double& gg) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ns = p.n_sites[ts];
+    const int na = p.n_ads_types;
+    int total = 0;
+#pragma unroll
+    for (int q = 0; q < GC_NS; ++q) total += q < p.n_species ? n[q] * p.n_sites[q] : 0;
+    // the first guest site is loaded ahead of the framework loop
+    int jj = tid;
+    double4 y = make_double4(0.0, 0.0, 0.0, 0.0);
+    int ty = 0;
+    bool have = gc_guest(p, sites, n, jj, total, ex_s, ex_m, y, ty);
     double agh = 0.0, agg = 0.0;
     bool ogh = false, ogg = false;
     for (int k = 0; k < ns; ++k) {
         const double4 x = trial[k];
-        const int tk = p.site_type[ts][k];
-        for (int a = tid; a < p.n_atoms; a += GC_WG) {
+        const double2* col = fg + p.site_type[ts][k];
+        int a = tid;
+        for (; a + 3 * GC_WG < p.n_atoms; a += 4 * GC_WG) {
+            const double4 a0 = atoms[a], a1 = atoms[a + GC_WG], a2 = atoms[a + 2 * GC_WG], a3 = atoms[a + 3 * GC_WG];
+            const double2 e0 = col[(int)a0.w * na], e1 = col[(int)a1.w * na];
+            const double2 e2 = col[(int)a2.w * na], e3 = col[(int)a3.w * na];
+            gc_pair(p, x, a0, e0, agh, ogh);
+            gc_pair(p, x, a1, e1, agh, ogh);
+            gc_pair(p, x, a2, e2, agh, ogh);
+            gc_pair(p, x, a3, e3, agh, ogh);
+        }
+        for (; a < p.n_atoms; a += GC_WG) {
             const double4 at = atoms[a];
-            gc_pair(p, x, at, p.lj_fg[(int)at.w * p.n_ads_types + tk], agh, ogh);
+            gc_pair(p, x, at, col[(int)at.w * na], agh, ogh);
         }
     }
-    for (int s2 = 0; s2 < p.n_species; ++s2) {
-        const int ns2 = p.n_sites[s2];
-        const int cnt = n[s2] * ns2;
-        for (int j = tid; j < cnt; j += GC_WG) {
-            const int m2 = j / ns2, k2 = j - m2 * ns2;
-            if (s2 == ex_s && m2 == ex_m) continue;
-            const double4 y = ch.sites[((long long)s2 * p.capacity + m2) * PM_MAX_SITES + k2];
-            const int t2 = p.site_type[s2][k2];
-            for (int k = 0; k < ns; ++k)
-                gc_pair(p, trial[k], y, p.lj_gg[p.site_type[ts][k] * p.n_ads_types + t2], agg, ogg);
-        }
+    while (jj < total) {
+        if (have)
+            for (int k = 0; k < ns; ++k) gc_pair(p, trial[k], y, ggt[p.site_type[ts][k] * na + ty], agg, ogg);
+        jj += GC_WG;
+        have = gc_guest(p, sites, n, jj, total, ex_s, ex_m, y, ty);
     }
-    agh = wave_sum(agh);
-    agg = wave_sum(agg);
+    agh = gc_wave_total(agh);
+    agg = gc_wave_total(agg);
     const bool bh = __ballot(ogh) != 0ull, bg = __ballot(ogg) != 0ull;
+    double* r = red + par * (4 * (GC_WG / 64));
+    par ^= 1;
     if (lane == 0) {
-        red[wave] = agh;
-        red[GC_WG / 64 + wave] = agg;
-        red[2 * (GC_WG / 64) + wave] = bh ? 1.0 : 0.0;
-        red[3 * (GC_WG / 64) + wave] = bg ? 1.0 : 0.0;
+        r[wave] = agh;
+        r[GC_WG / 64 + wave] = agg;
+        r[2 * (GC_WG / 64) + wave] = bh ? 1.0 : 0.0;
+        r[3 * (GC_WG / 64) + wave] = bg ? 1.0 : 0.0;
     }
     __syncthreads();
     double sh = 0.0, sg = 0.0;
     bool oh = false, og = false;
 #pragma unroll
     for (int w = 0; w < GC_WG / 64; ++w) {
-        sh += red[w];
-        sg += red[GC_WG / 64 + w];
-        oh |= red[2 * (GC_WG / 64) + w] != 0.0;
-        og |= red[3 * (GC_WG / 64) + w] != 0.0;
+        sh += r[w];
+        sg += r[GC_WG / 64 + w];
+        oh |= r[2 * (GC_WG / 64) + w] != 0.0;
+        og |= r[3 * (GC_WG / 64) + w] != 0.0;
     }
-    __syncthreads();  // red is reused by the next evaluation
     gh = oh ? INFINITY : sh;
     gg = og ? INFINITY : sg;
 }
 
-__device__ inline double4* gc_site(const GcmcParams& p, const GcmcChain& ch, int s, int m, int k) {
-    return ch.sites + ((long long)s * p.capacity + m) * PM_MAX_SITES + k;
+__device__ inline double4* gc_site(const GcmcParams& p, double4* sites, int s, int m, int k) {
+    return sites + ((long long)s * p.capacity + m) * PM_MAX_SITES + k;
 }
 
 // Removes molecule m of species s, keeping the order of the others (splice!,
 // src/mc_helpers.jl:58-60): chunks of work-items read, then write one
 // molecule lower.
-__device__ inline void gc_remove(const GcmcParams& p, GcmcChain& ch, int s, int m, int n_s) {
+__device__ inline void gc_remove(const GcmcParams& p, double4* sites, double4* coms, int s, int m, int n_s) {
     const long long base = (long long)s * p.capacity;
     const long long first = (base + m + 1) * PM_MAX_SITES, last = (base + n_s) * PM_MAX_SITES;
-    for (long long i0 = first; i0 < last; i0 += GC_WG) {
-        const long long i = i0 + threadIdx.x;
-        double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (i < last) v = ch.sites[i];
+    const long long cfirst = base + m + 1, clast = base + n_s;
+    // sites and centres move down together, one chunk of work-items each round
+    for (long long i0 = first, j0 = cfirst; i0 < last; i0 += GC_WG, j0 += GC_WG) {
+        const long long i = i0 + threadIdx.x, j = j0 + threadIdx.x;
+        double4 v = make_double4(0.0, 0.0, 0.0, 0.0), c = v;
+        if (i < last) v = sites[i];
+        if (j < clast) c = coms[j];
         __syncthreads();
-        if (i < last) ch.sites[i - PM_MAX_SITES] = v;
-        __syncthreads();
-    }
-    for (long long j0 = base + m + 1; j0 < base + n_s; j0 += GC_WG) {
-        const long long j = j0 + threadIdx.x;
-        double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (j < base + n_s) v = ch.com[j];
-        __syncthreads();
-        if (j < base + n_s) ch.com[j - 1] = v;
+        if (i < last) sites[i - PM_MAX_SITES] = v;
+        if (j < clast) coms[j - 1] = c;
         __syncthreads();
     }
 }
 
 // Writes the trial molecule (LDS) as molecule m of species s.
-__device__ inline void gc_store(const GcmcParams& p, GcmcChain& ch, int s, int m, const double4* trial, double4 com) {
+__device__ inline void gc_store(const GcmcParams& p, double4* sites, double4* coms, int s, int m,
+                                const double4* trial, double4 com) {
     const int tid = threadIdx.x;
-    if (tid < p.n_sites[s]) *gc_site(p, ch, s, m, tid) = trial[tid];
-    if (tid == 0) ch.com[(long long)s * p.capacity + m] = com;
+    if (tid < p.n_sites[s]) *gc_site(p, sites, s, m, tid) = trial[tid];
+    if (tid == 0) coms[(long long)s * p.capacity + m] = com;
     __syncthreads();
 }
 
@@ -260,22 +332,23 @@ __device__ inline long long gc_voxel(double x, int n) {
 // Density-grid update of one snapshot (update_density!, src/grid.jl:78-106):
 // every counted site of every molecule of the tracked species; over the unit
 // cell only molecules whose centre of mass lies in it.
-__device__ inline void gc_density(const GcmcParams& p, GcmcChain& ch, const int* n) {
+__device__ inline void gc_density(const GcmcParams& p, const double4* sites, const double4* coms, unsigned* density,
+                                  const int* n) {
     const int s = p.dens_species;
     const int ns = p.n_sites[s];
     const int n0 = p.dens_n[0], n1 = p.dens_n[1], n2 = p.dens_n[2];
     for (int j = threadIdx.x; j < n[s] * ns; j += GC_WG) {
         const int m = j / ns, k = j - m * ns;
         if (!((p.dens_mask >> k) & 1u)) continue;
-        double4 x = *gc_site(p, ch, s, m, k);
+        double4 x = sites[((long long)s * p.capacity + m) * PM_MAX_SITES + k];
         if (!p.dens_sim_box) {
-            const double4 c = ch.com[(long long)s * p.capacity + m];
+            const double4 c = coms[(long long)s * p.capacity + m];
             const double c0 = c.x * p.rep[0], c1 = c.y * p.rep[1], c2 = c.z * p.rep[2];
             if (c0 > 1.0 || c1 > 1.0 || c2 > 1.0 || c0 < 0.0 || c1 < 0.0 || c2 < 0.0) continue;
             x = make_double4(x.x * p.rep[0], x.y * p.rep[1], x.z * p.rep[2], 0.0);
         }
         const long long v = gc_voxel(x.x, n0) + (long long)n0 * (gc_voxel(x.y, n1) + (long long)n1 * gc_voxel(x.z, n2));
-        atomicAdd(&ch.density[v], 1u);
+        atomicAdd(&density[v], 1u);
     }
     __syncthreads();
 }
@@ -285,19 +358,33 @@ __device__ inline void gc_density(const GcmcParams& p, GcmcChain& ch, const int*
 template <bool LDS_ATOMS>
 __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restrict__ pp, GcmcChain* chains,
                                                      long long n_cycles) {
-    extern __shared__ double4 s_atoms[];
+    // dynamic LDS: LJ tables (framework x adsorbate, adsorbate x adsorbate),
+    // then with LDS_ATOMS the replicated framework
+    extern __shared__ double4 s_dyn[];
     __shared__ double4 s_trial[PM_MAX_SITES];
     __shared__ double4 s_old[PM_MAX_SITES];
-    __shared__ double s_red[4 * (GC_WG / 64)];
+    __shared__ double s_red[2 * 4 * (GC_WG / 64)];
     const GcmcParams& p = *pp;
     GcmcChain& ch = chains[blockIdx.x];
     const int tid = threadIdx.x;
     if (ch.error) return;
+    const int n_fg = p.n_fw_types * p.n_ads_types, n_gg = p.n_ads_types * p.n_ads_types;
+    double2* s_fg = reinterpret_cast<double2*>(s_dyn);
+    double2* s_gg = s_fg + n_fg;
+    for (int i = tid; i < n_fg; i += GC_WG) s_fg[i] = p.lj_fg[i];
+    for (int i = tid; i < n_gg; i += GC_WG) s_gg[i] = p.lj_gg[i];
     const double4* atoms = p.atoms;
     if constexpr (LDS_ATOMS) {
+        double4* s_atoms = s_dyn + (n_fg + n_gg + 1) / 2;
         for (int a = tid; a < p.n_atoms; a += GC_WG) s_atoms[a] = p.atoms[a];
         atoms = s_atoms;
     }
+    double4* const sites = ch.sites;
+    double4* const coms = ch.com;
+    double fug[GC_NS];
+#pragma unroll
+    for (int q = 0; q < GC_NS; ++q) fug[q] = ch.fug[q];
+    int par = 0;
     const unsigned chain = blockIdx.x;
     const double T = ch.temperature;
     int n[GC_NS];
@@ -310,6 +397,14 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
     for (int k = 0; k < GC_NM; ++k) { n_prop[k] = ch.n_prop[k]; n_acc[k] = ch.n_acc[k]; }
     int block = ch.block, error = 0;
     const int S = p.n_species;
+    // countdowns to the next multiple of each period (no 64-bit division per move)
+    long long to_sample = p.sample_frequency - t % p.sample_frequency;
+    long long to_snap = p.snapshot_frequency - outer % p.snapshot_frequency;
+    long long to_adjust = p.adjust_every > 0 ? p.adjust_every - outer % p.adjust_every : 0;
+    long long to_block = 0;  // cycles to the next block boundary once sampling
+    if (outer >= p.n_burn) to_block = p.cycles_per_block - (outer - p.n_burn) % p.cycles_per_block;
+    GcRng rng;
+    gc_rng_fill(rng, p.seed, chain, (unsigned long long)t + 1);
     // sums of the current statistics block, added to the chain's block at a
     // block change and at the end of the launch
     long long acc_samples = 0, acc_n[GC_NS], acc_n2[GC_NS];
@@ -342,8 +437,9 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
         const long long inner = (long long)n_tot + 1;
         for (long long it = 0; it < inner; ++it) {
             ++t;
+            if ((unsigned long long)t >= rng.t0 + GC_BATCH) gc_rng_fill(rng, p.seed, chain, (unsigned long long)t);
             double u[10];
-            gcmc_uniforms_wave(p.seed, chain, (unsigned long long)t, u);
+            gc_rng_get(rng, (unsigned long long)t, u);
             const int s = S > 1 ? min(S - 1, (int)(u[0] * (double)S)) : 0;
             const int n_i = n[s];
             int mv = 0;
@@ -363,25 +459,25 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_trial[tid] = gc_new_site(p, s, tid, com, R, rot);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, ch, atoms, n, s_trial, s, -1, -1, s_red, gh, gg);
+                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s, -1, -1, s_red, par, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
-                const double acc = ch.fug[s] * p.omega / ((double)(n_i + 1) * GC_KB * T) * exp(-dE / T);
+                const double acc = fug[s] * p.omega / ((double)(n_i + 1) * GC_KB * T) * exp(-dE / T);
                 if (u[9] < acc) {
-                    gc_store(p, ch, s, n_i, s_trial, com);
+                    gc_store(p, sites, coms, s, n_i, s_trial, com);
                     n[s] = n_i + 1;
                     n_acc[mv] += 1;
                     e_gh += gh;
                     e_gg += gg;
                 }
             } else if (mv == MV_DELETION && n_i != 0) {
-                if (tid < ns) s_trial[tid] = *gc_site(p, ch, s, m, tid);
+                if (tid < ns) s_trial[tid] = *gc_site(p, sites, s, m, tid);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, ch, atoms, n, s_trial, s, s, m, s_red, gh, gg);
+                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
-                const double acc = (double)n_i * GC_KB * T / (ch.fug[s] * p.omega) * exp(dE / T);
+                const double acc = (double)n_i * GC_KB * T / (fug[s] * p.omega) * exp(dE / T);
                 if (u[9] < acc) {
-                    gc_remove(p, ch, s, m, n_i);
+                    gc_remove(p, sites, coms, s, m, n_i);
                     n[s] = n_i - 1;
                     n_acc[mv] += 1;
                     e_gh -= gh;
@@ -389,11 +485,11 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 }
             } else if ((mv == MV_TRANSLATION || (mv == MV_ROTATION && rot) || mv == MV_REINSERTION) && n_i != 0) {
                 if (mv == MV_TRANSLATION) ntp += 1;
-                const double4 com0 = ch.com[(long long)s * p.capacity + m];
-                if (tid < ns) s_old[tid] = *gc_site(p, ch, s, m, tid);
+                const double4 com0 = coms[(long long)s * p.capacity + m];
+                if (tid < ns) s_old[tid] = *gc_site(p, sites, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, ch, atoms, n, s_old, s, s, m, s_red, gh0, gg0);
+                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
                 double4 com1 = com0;
                 {
 #pragma clang fp contract(off)
@@ -438,10 +534,10 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                     }
                 }
                 __syncthreads();
-                gc_energy(p, ch, atoms, n, s_trial, s, s, m, s_red, gh1, gg1);
+                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
                 if (u[9] < exp(-(e_new - e_old) / T)) {
-                    gc_store(p, ch, s, m, s_trial, com1);
+                    gc_store(p, sites, coms, s, m, s_trial, com1);
                     n_acc[mv] += 1;
                     if (mv == MV_TRANSLATION) nta += 1;
                     e_gh += gh1 - gh0;
@@ -449,12 +545,12 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 }
             } else if (mv == MV_IDENTITY && n_i != 0) {
                 // src/gcmc.jl:566-613
-                const double4 com0 = ch.com[(long long)s * p.capacity + m];
-                if (tid < ns) s_old[tid] = *gc_site(p, ch, s, m, tid);
+                const double4 com0 = coms[(long long)s * p.capacity + m];
+                if (tid < ns) s_old[tid] = *gc_site(p, sites, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, ch, atoms, n, s_old, s, s, m, s_red, gh0, gg0);
-                gc_remove(p, ch, s, m, n_i);
+                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
+                gc_remove(p, sites, coms, s, m, n_i);
                 n[s] = n_i - 1;
                 int pick = min(S - 2, (int)(u[3] * (double)(S - 1)));
                 const int s2 = pick < s ? pick : pick + 1;
@@ -468,22 +564,24 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (rot2) arvo_rotation(&u[6], R);
                 if (tid < p.n_sites[s2]) s_trial[tid] = gc_new_site(p, s2, tid, com0, R, rot2);
                 __syncthreads();
-                gc_energy(p, ch, atoms, n, s_trial, s2, -1, -1, s_red, gh1, gg1);
+                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s2, -1, -1, s_red, par, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
-                const double acc = ((double)n_i * ch.fug[s2] / ((double)(n_j + 1) * ch.fug[s])) * exp(-(e_new - e_old) / T);
+                const double acc = ((double)n_i * fug[s2] / ((double)(n_j + 1) * fug[s])) * exp(-(e_new - e_old) / T);
                 if (u[9] < acc) {
-                    gc_store(p, ch, s2, n_j, s_trial, com0);
+                    gc_store(p, sites, coms, s2, n_j, s_trial, com0);
                     n[s2] = n_j + 1;
                     n_acc[mv] += 1;
                     e_gh += gh1 - gh0;
                     e_gg += gg1 - gg0;
                 } else {
                     // the original molecule goes back at the end of its species' list
-                    gc_store(p, ch, s, n[s], s_old, com0);
+                    gc_store(p, sites, coms, s, n[s], s_old, com0);
                     n[s] += 1;
                 }
             }
-            if (outer > p.n_burn && t % p.sample_frequency == 0) {
+            const bool sample_now = --to_sample == 0;
+            if (sample_now) to_sample = p.sample_frequency;
+            if (outer > p.n_burn && sample_now) {
                 long long ntot = 0;
 #pragma unroll
                 for (int q = 0; q < GC_NS; ++q) {
@@ -501,7 +599,12 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
         }
         if (error) break;
         // adaptive step during burn-in (src/gcmc.jl:634-637, src/mc_helpers.jl:107-129)
-        if (outer <= p.n_burn && p.adjust_every > 0 && outer % p.adjust_every == 0) {
+        bool adjust_now = false;
+        if (p.adjust_every > 0 && --to_adjust == 0) {
+            adjust_now = true;
+            to_adjust = p.adjust_every;
+        }
+        if (outer <= p.n_burn && adjust_now) {
             const double rate = (double)nta / (double)ntp;
             if (rate > 0.5) delta = 1.2 * delta;
             else if (rate < 0.4) delta = delta / 1.2;
@@ -509,13 +612,20 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
             nta = 0;
         }
         // statistics blocks (src/gcmc.jl:639-653)
-        if (outer > p.n_burn && block != GC_NB - 1 && (outer - p.n_burn) % p.cycles_per_block == 0) {
+        bool block_now = false;
+        if (outer > p.n_burn) {
+            if (to_block == 0) to_block = p.cycles_per_block;  // first sampling cycle
+            if (--to_block == 0) block_now = true;
+        }
+        if (outer > p.n_burn && block != GC_NB - 1 && block_now) {
             flush();
             ++block;
         }
         // snapshots (src/gcmc.jl:663-678)
-        if (outer > p.n_burn && outer % p.snapshot_frequency == 0) {
-            if (p.dens_species >= 0) gc_density(p, ch, n);
+        const bool snap_now = --to_snap == 0;
+        if (snap_now) to_snap = p.snapshot_frequency;
+        if (outer > p.n_burn && snap_now) {
+            if (p.dens_species >= 0) gc_density(p, sites, coms, ch.density, n);
             ++nsnap;
         }
     }
@@ -542,20 +652,21 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
 __global__ __launch_bounds__(GC_WG) void gcmc_total_energy_kernel(const GcmcParams* __restrict__ pp,
                                                                   GcmcChain* chains, double* out) {
     __shared__ double4 s_trial[PM_MAX_SITES];
-    __shared__ double s_red[4 * (GC_WG / 64)];
+    __shared__ double s_red[2 * 4 * (GC_WG / 64)];
     const GcmcParams& p = *pp;
     GcmcChain& ch = chains[blockIdx.x];
-    const double4* atoms = p.atoms;
+    double4* const sites = ch.sites;
     int n[GC_NS];
 #pragma unroll
     for (int s = 0; s < GC_NS; ++s) n[s] = ch.n[s];
+    int par = 0;
     double tgh = 0.0, tgg = 0.0;
     for (int s = 0; s < p.n_species; ++s) {
         for (int m = 0; m < n[s]; ++m) {
-            if (threadIdx.x < p.n_sites[s]) s_trial[threadIdx.x] = *gc_site(p, ch, s, m, threadIdx.x);
+            if (threadIdx.x < p.n_sites[s]) s_trial[threadIdx.x] = *gc_site(p, sites, s, m, threadIdx.x);
             __syncthreads();
             double gh, gg;
-            gc_energy(p, ch, atoms, n, s_trial, s, s, m, s_red, gh, gg);
+            gc_energy(p, sites, p.atoms, p.lj_fg, p.lj_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
             tgh += gh;
             tgg += gg;
         }

@@ -1940,6 +1940,7 @@ int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sp
     hp.n_atoms = (int)fw->n_rep;
     hp.n_species = n_species;
     hp.n_ads_types = n_ads;
+    hp.n_fw_types = std::max(1, n_fw);
     hp.capacity = st->capacity;
     hp.n_burn = st->n_burn_cycles;
     hp.n_sample = st->n_sample_cycles;
@@ -2087,16 +2088,19 @@ int pm_gcmc_run(pm_gcmc* gc, int64_t n_cycles) {
         gc->energy_stale = false;
     }
     if (n_cycles == 0) return PM_OK;
-    // the framework goes to LDS when it fits beside the kernel's own LDS
+    // dynamic LDS: the LJ tables (at most 48 KiB), then the framework when it
+    // fits beside them and the kernel's own LDS
+    const size_t n_tab = (size_t)gc->hp.n_fw_types * gc->hp.n_ads_types + (size_t)gc->hp.n_ads_types * gc->hp.n_ads_types;
+    const size_t lds_tables = sizeof(double4) * ((n_tab + 1) / 2);
     const size_t lds_atoms = sizeof(double4) * (size_t)gc->hp.n_atoms;
-    const bool use_lds = gc->hp.n_atoms > 0 && lds_atoms + 4096 <= g_ctx.max_lds;
+    const bool use_lds = gc->hp.n_atoms > 0 && lds_tables + lds_atoms + 4096 <= g_ctx.max_lds;
     (void)hipEventRecord(g_ctx.ev0, g_ctx.stream);
     if (use_lds)
-        hipLaunchKernelGGL(gcmc_kernel<true>, dim3(gc->n_chains), dim3(GC_WG), lds_atoms, g_ctx.stream,
-                           gc->d_params, gc->d_chains, (long long)n_cycles);
+        hipLaunchKernelGGL(gcmc_kernel<true>, dim3(gc->n_chains), dim3(GC_WG), lds_tables + lds_atoms,
+                           g_ctx.stream, gc->d_params, gc->d_chains, (long long)n_cycles);
     else
-        hipLaunchKernelGGL(gcmc_kernel<false>, dim3(gc->n_chains), dim3(GC_WG), 0, g_ctx.stream, gc->d_params,
-                           gc->d_chains, (long long)n_cycles);
+        hipLaunchKernelGGL(gcmc_kernel<false>, dim3(gc->n_chains), dim3(GC_WG), lds_tables, g_ctx.stream,
+                           gc->d_params, gc->d_chains, (long long)n_cycles);
     PM_HIP(hipGetLastError());
     (void)hipEventRecord(g_ctx.ev1, g_ctx.stream);
     PM_HIP(hipStreamSynchronize(g_ctx.stream));
