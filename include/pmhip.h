@@ -404,6 +404,13 @@ int pm_gcmc_get_molecules(const pm_gcmc* gc, int32_t chain, int32_t species, int
 int pm_gcmc_total_energy(pm_gcmc* gc, int32_t chain, double* out_gh, double* out_gg);
 /* Density-grid counts (`update_density!`, src/grid.jl:78-106), i fastest. */
 int pm_gcmc_get_density(const pm_gcmc* gc, int32_t chain, double* out);
+/* No counterpart in the reference (a schedule setting, like pm_set_tuning):
+ * the most molecules per species a launch keeps in LDS.  -1 (the default):
+ * as many as fit beside the framework; 0: molecules stay in device memory.
+ * A chain that could outgrow the LDS store within its next cycle finishes
+ * the call with its molecules in device memory.  Results never depend on
+ * this setting. */
+int pm_gcmc_set_lds_molecules(pm_gcmc* gc, int32_t max_per_species);
 
 #ifdef __cplusplus
 }

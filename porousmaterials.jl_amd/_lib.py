@@ -121,6 +121,7 @@ _SIGS = {
     "pm_gcmc_get_molecules": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, I32, D, D]),
     "pm_gcmc_total_energy": (ctypes.c_int, [P, ctypes.c_int32, D, D]),
     "pm_gcmc_get_density": (ctypes.c_int, [P, ctypes.c_int32, D]),
+    "pm_gcmc_set_lds_molecules": (ctypes.c_int, [P, ctypes.c_int32]),
     "pm_last_kernel_stats": (ctypes.c_int, [D, D]),
     "pm_last_pair_kernel_stats": (ctypes.c_int, [D, D]),
 }

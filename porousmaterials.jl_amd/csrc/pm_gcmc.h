@@ -16,6 +16,8 @@
 // (src/gcmc.jl:111-124, src/vdw_energetics.jl:85-120), all FP64.
 #pragma once
 
+#include <type_traits>
+
 #include "pm_device.h"
 
 namespace pmk {
@@ -37,6 +39,8 @@ struct GcmcParams {
     const double4* atoms;    // fractional xyz + compact framework type in w
     int n_species, n_ads_types, capacity;
     int n_sites[GC_NS];
+    int site_prefix[GC_NS];  // sites of the species before each one
+    int total_sites;         // sites of one molecule of every species
     int rotatable[GC_NS];
     int site_type[GC_NS][PM_MAX_SITES];
     double off[GC_NS][PM_MAX_SITES][3];  // Cartesian offsets from the centre of mass
@@ -67,7 +71,51 @@ struct GcmcChain {
     double4* sites;          // [species][capacity][PM_MAX_SITES] fractional xyz
     double4* com;            // [species][capacity]
     unsigned* density;       // counts per grid point, or null
+    long long outer_target;  // outer cycle the current pm_gcmc_run call ends at
 };
+
+// Where a chain's molecules live during a launch.  GcMolsMem: device memory
+// in the handle's layout ([species][capacity][PM_MAX_SITES] sites,
+// [species][capacity] centres).  GcMolsLds: LDS, compact per species
+// ([cap][n_sites] sites of species s after those of the species before it),
+// the centres ([species][cap]) after all sites.
+struct GcMolsMem {
+    double4* sites;
+    double4* coms;
+    int cap;
+    __device__ double4* site_base(const GcmcParams& p, int s) const { return sites + s * cap * PM_MAX_SITES; }
+    __device__ int stride(const GcmcParams& p, int s) const { return PM_MAX_SITES; }
+    __device__ double4* com_base(const GcmcParams& p, int s) const { return coms + s * cap; }
+    __device__ double4& site(const GcmcParams& p, int s, int m, int k) const {
+        return site_base(p, s)[m * PM_MAX_SITES + k];
+    }
+    __device__ double4& com(const GcmcParams& p, int s, int m) const { return com_base(p, s)[m]; }
+};
+
+struct GcMolsLds {
+    double4* base;
+    int cap;
+    __device__ double4* site_base(const GcmcParams& p, int s) const { return base + p.site_prefix[s] * cap; }
+    __device__ int stride(const GcmcParams& p, int s) const { return p.n_sites[s]; }
+    __device__ double4* com_base(const GcmcParams& p, int s) const { return base + (p.total_sites + s) * cap; }
+    __device__ double4& site(const GcmcParams& p, int s, int m, int k) const {
+        return site_base(p, s)[m * p.n_sites[s] + k];
+    }
+    __device__ double4& com(const GcmcParams& p, int s, int m) const { return com_base(p, s)[m]; }
+};
+
+// Copies the stored molecules of every species from one store to another.
+template <typename From, typename To>
+__device__ inline void gc_mols_copy(const GcmcParams& p, const From& from, const To& to, const int* n) {
+    for (int s = 0; s < p.n_species; ++s) {
+        const int ns = p.n_sites[s];
+        for (int j = threadIdx.x; j < n[s] * ns; j += GC_WG) {
+            const int m = j / ns, k = j - m * ns;
+            to.site(p, s, m, k) = from.site(p, s, m, k);
+        }
+        for (int m = threadIdx.x; m < n[s]; m += GC_WG) to.com(p, s, m) = from.com(p, s, m);
+    }
+}
 
 // The 10 uniforms of move t of a chain: species, move, molecule, position or
 // step (3), rotation (3), acceptance.
@@ -164,7 +212,8 @@ __device__ inline void gc_pair(const GcmcParams& p, double4 x, double4 y, double
 
 // Guest site jj of the flattened (species, molecule, site) list of stored
 // molecules, unless it lies past the end or belongs to the excluded molecule.
-__device__ inline bool gc_guest(const GcmcParams& p, const double4* sites, const int* n, int jj, int total,
+template <typename Mols>
+__device__ inline bool gc_guest(const GcmcParams& p, const Mols& mols, const int* n, int jj, int total,
                                 int ex_s, int ex_m, double4& y, int& ty) {
     if (jj >= total) return false;
     int s2 = 0;
@@ -175,7 +224,7 @@ __device__ inline bool gc_guest(const GcmcParams& p, const double4* sites, const
     const int ns2 = p.n_sites[s2];
     const int m2 = jj / ns2, k2 = jj - m2 * ns2;
     if (s2 == ex_s && m2 == ex_m) return false;
-    y = sites[((long long)s2 * p.capacity + m2) * PM_MAX_SITES + k2];
+    y = mols.site(p, s2, m2, k2);
     ty = p.site_type[s2][k2];
     return true;
 }
@@ -186,7 +235,8 @@ __device__ inline bool gc_guest(const GcmcParams& p, const double4* sites, const
 // sites; waves reduce by DPP and the four wave sums are added in wave order.
 // +Inf on overlap.  `red` alternates between two halves (`par`), so the next
 // evaluation needs no barrier before writing its partial sums.
-__device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams& p, const double4* sites,
+template <typename Mols>
+__device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams& p, const Mols& mols,
                                                                 const double4* atoms, const double2* fg,
                                                                 const double2* ggt, const int* n,
                                                                 const double4* trial, int ts, int ex_s, int ex_m,
@@ -201,7 +251,7 @@ __device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams
     int jj = tid;
     double4 y = make_double4(0.0, 0.0, 0.0, 0.0);
     int ty = 0;
-    bool have = gc_guest(p, sites, n, jj, total, ex_s, ex_m, y, ty);
+    bool have = gc_guest(p, mols, n, jj, total, ex_s, ex_m, y, ty);
     double agh = 0.0, agg = 0.0;
     bool ogh = false, ogg = false;
     for (int k = 0; k < ns; ++k) {
@@ -226,7 +276,7 @@ __device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams
         if (have)
             for (int k = 0; k < ns; ++k) gc_pair(p, trial[k], y, ggt[p.site_type[ts][k] * na + ty], agg, ogg);
         jj += GC_WG;
-        have = gc_guest(p, sites, n, jj, total, ex_s, ex_m, y, ty);
+        have = gc_guest(p, mols, n, jj, total, ex_s, ex_m, y, ty);
     }
     agh = gc_wave_total(agh);
     agg = gc_wave_total(agg);
@@ -253,36 +303,35 @@ __device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams
     gg = og ? INFINITY : sg;
 }
 
-__device__ inline double4* gc_site(const GcmcParams& p, double4* sites, int s, int m, int k) {
-    return sites + ((long long)s * p.capacity + m) * PM_MAX_SITES + k;
-}
-
 // Removes molecule m of species s, keeping the order of the others (splice!,
 // src/mc_helpers.jl:58-60): chunks of work-items read, then write one
 // molecule lower.
-__device__ inline void gc_remove(const GcmcParams& p, double4* sites, double4* coms, int s, int m, int n_s) {
-    const long long base = (long long)s * p.capacity;
-    const long long first = (base + m + 1) * PM_MAX_SITES, last = (base + n_s) * PM_MAX_SITES;
-    const long long cfirst = base + m + 1, clast = base + n_s;
+template <typename Mols>
+__device__ inline void gc_remove(const GcmcParams& p, const Mols& mols, int s, int m, int n_s) {
+    double4* const sb = mols.site_base(p, s);
+    double4* const cb = mols.com_base(p, s);
+    const int st = mols.stride(p, s);
+    const int first = (m + 1) * st, last = n_s * st;
     // sites and centres move down together, one chunk of work-items each round
-    for (long long i0 = first, j0 = cfirst; i0 < last; i0 += GC_WG, j0 += GC_WG) {
-        const long long i = i0 + threadIdx.x, j = j0 + threadIdx.x;
+    for (int i0 = first, j0 = m + 1; i0 < last; i0 += GC_WG, j0 += GC_WG) {
+        const int i = i0 + threadIdx.x, j = j0 + threadIdx.x;
         double4 v = make_double4(0.0, 0.0, 0.0, 0.0), c = v;
-        if (i < last) v = sites[i];
-        if (j < clast) c = coms[j];
+        if (i < last) v = sb[i];
+        if (j < n_s) c = cb[j];
         __syncthreads();
-        if (i < last) sites[i - PM_MAX_SITES] = v;
-        if (j < clast) coms[j - 1] = c;
+        if (i < last) sb[i - st] = v;
+        if (j < n_s) cb[j - 1] = c;
         __syncthreads();
     }
 }
 
 // Writes the trial molecule (LDS) as molecule m of species s.
-__device__ inline void gc_store(const GcmcParams& p, double4* sites, double4* coms, int s, int m,
-                                const double4* trial, double4 com) {
+template <typename Mols>
+__device__ inline void gc_store(const GcmcParams& p, const Mols& mols, int s, int m, const double4* trial,
+                                double4 com) {
     const int tid = threadIdx.x;
-    if (tid < p.n_sites[s]) *gc_site(p, sites, s, m, tid) = trial[tid];
-    if (tid == 0) coms[(long long)s * p.capacity + m] = com;
+    if (tid < p.n_sites[s]) mols.site(p, s, m, tid) = trial[tid];
+    if (tid == 0) mols.com(p, s, m) = com;
     __syncthreads();
 }
 
@@ -332,17 +381,17 @@ __device__ inline long long gc_voxel(double x, int n) {
 // Density-grid update of one snapshot (update_density!, src/grid.jl:78-106):
 // every counted site of every molecule of the tracked species; over the unit
 // cell only molecules whose centre of mass lies in it.
-__device__ inline void gc_density(const GcmcParams& p, const double4* sites, const double4* coms, unsigned* density,
-                                  const int* n) {
+template <typename Mols>
+__device__ inline void gc_density(const GcmcParams& p, const Mols& mols, unsigned* density, const int* n) {
     const int s = p.dens_species;
     const int ns = p.n_sites[s];
     const int n0 = p.dens_n[0], n1 = p.dens_n[1], n2 = p.dens_n[2];
     for (int j = threadIdx.x; j < n[s] * ns; j += GC_WG) {
         const int m = j / ns, k = j - m * ns;
         if (!((p.dens_mask >> k) & 1u)) continue;
-        double4 x = sites[((long long)s * p.capacity + m) * PM_MAX_SITES + k];
+        double4 x = mols.site(p, s, m, k);
         if (!p.dens_sim_box) {
-            const double4 c = coms[(long long)s * p.capacity + m];
+            const double4 c = mols.com(p, s, m);
             const double c0 = c.x * p.rep[0], c1 = c.y * p.rep[1], c2 = c.z * p.rep[2];
             if (c0 > 1.0 || c1 > 1.0 || c2 > 1.0 || c0 < 0.0 || c1 < 0.0 || c2 < 0.0) continue;
             x = make_double4(x.x * p.rep[0], x.y * p.rep[1], x.z * p.rep[2], 0.0);
@@ -353,13 +402,20 @@ __device__ inline void gc_density(const GcmcParams& p, const double4* sites, con
     __syncthreads();
 }
 
-// Advances every chain (one workgroup each) by n_cycles outer cycles.  With
-// LDS_ATOMS the replicated framework is staged in LDS for the launch.
-template <bool LDS_ATOMS>
+// Advances every chain (one workgroup each) to its target outer cycle: the
+// chain's current cycle plus n_cycles, or with `resume` the target a
+// previous launch of this call recorded.  With LDS_ATOMS the replicated
+// framework is staged in LDS for the launch; with LDS_MOLS the chain's
+// molecules are too (up to lds_cap per species), and a chain whose next
+// cycle could outgrow them stops at that cycle's start, counts itself in
+// *incomplete and leaves the rest of the call to a device-memory launch.
+template <bool LDS_ATOMS, bool LDS_MOLS>
 __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restrict__ pp, GcmcChain* chains,
-                                                     long long n_cycles) {
+                                                     long long n_cycles, int resume, int lds_cap,
+                                                     int* incomplete) {
     // dynamic LDS: LJ tables (framework x adsorbate, adsorbate x adsorbate),
-    // then with LDS_ATOMS the replicated framework
+    // then with LDS_ATOMS the replicated framework, then with LDS_MOLS the
+    // molecules
     extern __shared__ double4 s_dyn[];
     __shared__ double4 s_trial[PM_MAX_SITES];
     __shared__ double4 s_old[PM_MAX_SITES];
@@ -374,13 +430,18 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
     for (int i = tid; i < n_fg; i += GC_WG) s_fg[i] = p.lj_fg[i];
     for (int i = tid; i < n_gg; i += GC_WG) s_gg[i] = p.lj_gg[i];
     const double4* atoms = p.atoms;
+    double4* lds_next = s_dyn + (n_fg + n_gg + 1) / 2;
     if constexpr (LDS_ATOMS) {
-        double4* s_atoms = s_dyn + (n_fg + n_gg + 1) / 2;
+        double4* s_atoms = lds_next;
         for (int a = tid; a < p.n_atoms; a += GC_WG) s_atoms[a] = p.atoms[a];
         atoms = s_atoms;
+        lds_next += p.n_atoms;
     }
-    double4* const sites = ch.sites;
-    double4* const coms = ch.com;
+    const GcMolsMem gmols{ch.sites, ch.com, p.capacity};
+    using Mols = typename std::conditional<LDS_MOLS, GcMolsLds, GcMolsMem>::type;
+    Mols mols;
+    if constexpr (LDS_MOLS) mols = GcMolsLds{lds_next, lds_cap};
+    else mols = gmols;
     double fug[GC_NS];
 #pragma unroll
     for (int q = 0; q < GC_NS; ++q) fug[q] = ch.fug[q];
@@ -390,8 +451,17 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
     int n[GC_NS];
 #pragma unroll
     for (int s = 0; s < GC_NS; ++s) n[s] = ch.n[s];
+    if constexpr (LDS_MOLS) {
+        // a chain already over the LDS capacity leaves the call to device memory
+        bool fits = true;
+        for (int s = 0; s < GC_NS; ++s) fits &= n[s] <= lds_cap;
+        if (fits) gc_mols_copy(p, gmols, mols, n);
+    }
     double delta = ch.delta, e_gh = ch.e_gh, e_gg = ch.e_gg;
     long long t = ch.t, outer = ch.outer, ntp = ch.n_trans_prop, nta = ch.n_trans_acc, nsnap = ch.num_snapshots;
+    const long long target = resume ? ch.outer_target : outer + n_cycles;
+    const long long outer_start = outer;
+    bool stopped = false;
     long long n_prop[GC_NM], n_acc[GC_NM];
 #pragma unroll
     for (int k = 0; k < GC_NM; ++k) { n_prop[k] = ch.n_prop[k]; n_acc[k] = ch.n_acc[k]; }
@@ -430,10 +500,19 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
     };
     __syncthreads();
 
-    for (long long cyc = 0; cyc < n_cycles && !error; ++cyc) {
-        ++outer;
+    while (outer < target && !error) {
         int n_tot = 0;
         for (int s = 0; s < S; ++s) n_tot += n[s];
+        if constexpr (LDS_MOLS) {
+            // a cycle of n_tot + 1 moves adds at most n_tot + 1 molecules to a species
+            bool fits = true;
+            for (int s = 0; s < S; ++s) fits &= n[s] + n_tot + 1 <= lds_cap;
+            if (!fits) {
+                stopped = true;
+                break;
+            }
+        }
+        ++outer;
         const long long inner = (long long)n_tot + 1;
         for (long long it = 0; it < inner; ++it) {
             ++t;
@@ -459,25 +538,25 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_trial[tid] = gc_new_site(p, s, tid, com, R, rot);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s, -1, -1, s_red, par, gh, gg);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, -1, -1, s_red, par, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
                 const double acc = fug[s] * p.omega / ((double)(n_i + 1) * GC_KB * T) * exp(-dE / T);
                 if (u[9] < acc) {
-                    gc_store(p, sites, coms, s, n_i, s_trial, com);
+                    gc_store(p, mols, s, n_i, s_trial, com);
                     n[s] = n_i + 1;
                     n_acc[mv] += 1;
                     e_gh += gh;
                     e_gg += gg;
                 }
             } else if (mv == MV_DELETION && n_i != 0) {
-                if (tid < ns) s_trial[tid] = *gc_site(p, sites, s, m, tid);
+                if (tid < ns) s_trial[tid] = mols.site(p, s, m, tid);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
                 const double acc = (double)n_i * GC_KB * T / (fug[s] * p.omega) * exp(dE / T);
                 if (u[9] < acc) {
-                    gc_remove(p, sites, coms, s, m, n_i);
+                    gc_remove(p, mols, s, m, n_i);
                     n[s] = n_i - 1;
                     n_acc[mv] += 1;
                     e_gh -= gh;
@@ -485,11 +564,11 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 }
             } else if ((mv == MV_TRANSLATION || (mv == MV_ROTATION && rot) || mv == MV_REINSERTION) && n_i != 0) {
                 if (mv == MV_TRANSLATION) ntp += 1;
-                const double4 com0 = coms[(long long)s * p.capacity + m];
-                if (tid < ns) s_old[tid] = *gc_site(p, sites, s, m, tid);
+                const double4 com0 = mols.com(p, s, m);
+                if (tid < ns) s_old[tid] = mols.site(p, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
                 double4 com1 = com0;
                 {
 #pragma clang fp contract(off)
@@ -534,10 +613,10 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                     }
                 }
                 __syncthreads();
-                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh1, gg1);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
                 if (u[9] < exp(-(e_new - e_old) / T)) {
-                    gc_store(p, sites, coms, s, m, s_trial, com1);
+                    gc_store(p, mols, s, m, s_trial, com1);
                     n_acc[mv] += 1;
                     if (mv == MV_TRANSLATION) nta += 1;
                     e_gh += gh1 - gh0;
@@ -545,12 +624,12 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 }
             } else if (mv == MV_IDENTITY && n_i != 0) {
                 // src/gcmc.jl:566-613
-                const double4 com0 = coms[(long long)s * p.capacity + m];
-                if (tid < ns) s_old[tid] = *gc_site(p, sites, s, m, tid);
+                const double4 com0 = mols.com(p, s, m);
+                if (tid < ns) s_old[tid] = mols.site(p, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
-                gc_remove(p, sites, coms, s, m, n_i);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
+                gc_remove(p, mols, s, m, n_i);
                 n[s] = n_i - 1;
                 int pick = min(S - 2, (int)(u[3] * (double)(S - 1)));
                 const int s2 = pick < s ? pick : pick + 1;
@@ -564,18 +643,18 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (rot2) arvo_rotation(&u[6], R);
                 if (tid < p.n_sites[s2]) s_trial[tid] = gc_new_site(p, s2, tid, com0, R, rot2);
                 __syncthreads();
-                gc_energy(p, sites, atoms, s_fg, s_gg, n, s_trial, s2, -1, -1, s_red, par, gh1, gg1);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s2, -1, -1, s_red, par, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
                 const double acc = ((double)n_i * fug[s2] / ((double)(n_j + 1) * fug[s])) * exp(-(e_new - e_old) / T);
                 if (u[9] < acc) {
-                    gc_store(p, sites, coms, s2, n_j, s_trial, com0);
+                    gc_store(p, mols, s2, n_j, s_trial, com0);
                     n[s2] = n_j + 1;
                     n_acc[mv] += 1;
                     e_gh += gh1 - gh0;
                     e_gg += gg1 - gg0;
                 } else {
                     // the original molecule goes back at the end of its species' list
-                    gc_store(p, sites, coms, s, n[s], s_old, com0);
+                    gc_store(p, mols, s, n[s], s_old, com0);
                     n[s] += 1;
                 }
             }
@@ -625,13 +704,20 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
         const bool snap_now = --to_snap == 0;
         if (snap_now) to_snap = p.snapshot_frequency;
         if (outer > p.n_burn && snap_now) {
-            if (p.dens_species >= 0) gc_density(p, sites, coms, ch.density, n);
+            if (p.dens_species >= 0) gc_density(p, mols, ch.density, n);
             ++nsnap;
         }
     }
     flush();
     __syncthreads();
+    if constexpr (LDS_MOLS) {
+        bool fits = true;
+        for (int s = 0; s < GC_NS; ++s) fits &= n[s] <= lds_cap;
+        if (fits && outer != outer_start) gc_mols_copy(p, mols, gmols, n);
+        if (stopped && tid == 0) atomicAdd(incomplete, 1);
+    }
     if (tid == 0) {
+        ch.outer_target = target;
         for (int s = 0; s < GC_NS; ++s) ch.n[s] = n[s];
         ch.delta = delta;
         ch.e_gh = e_gh;
@@ -655,7 +741,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_total_energy_kernel(const GcmcPara
     __shared__ double s_red[2 * 4 * (GC_WG / 64)];
     const GcmcParams& p = *pp;
     GcmcChain& ch = chains[blockIdx.x];
-    double4* const sites = ch.sites;
+    const GcMolsMem mols{ch.sites, ch.com, p.capacity};
     int n[GC_NS];
 #pragma unroll
     for (int s = 0; s < GC_NS; ++s) n[s] = ch.n[s];
@@ -663,10 +749,10 @@ __global__ __launch_bounds__(GC_WG) void gcmc_total_energy_kernel(const GcmcPara
     double tgh = 0.0, tgg = 0.0;
     for (int s = 0; s < p.n_species; ++s) {
         for (int m = 0; m < n[s]; ++m) {
-            if (threadIdx.x < p.n_sites[s]) s_trial[threadIdx.x] = *gc_site(p, sites, s, m, threadIdx.x);
+            if (threadIdx.x < p.n_sites[s]) s_trial[threadIdx.x] = mols.site(p, s, m, threadIdx.x);
             __syncthreads();
             double gh, gg;
-            gc_energy(p, sites, p.atoms, p.lj_fg, p.lj_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
+            gc_energy(p, mols, p.atoms, p.lj_fg, p.lj_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
             tgh += gh;
             tgg += gg;
         }

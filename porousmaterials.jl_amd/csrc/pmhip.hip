@@ -218,6 +218,9 @@ struct pm_gcmc {
     double4* d_com = nullptr;
     unsigned* d_dens = nullptr;
     double* d_tot = nullptr;
+    int* d_incomplete = nullptr;  // chains a launch with molecules in LDS left unfinished
+    bool mols_in_memory = false;  // a chain outgrew the LDS molecule store: launches keep them in memory
+    int lds_mols_limit = -1;      // pm_gcmc_set_lds_molecules
 };
 
 namespace {
@@ -1830,6 +1833,7 @@ int pm_gcmc_destroy(pm_gcmc* gc) {
     (void)hipFree(gc->d_com);
     (void)hipFree(gc->d_dens);
     (void)hipFree(gc->d_tot);
+    (void)hipFree(gc->d_incomplete);
     delete gc;
     return PM_OK;
 }
@@ -1941,6 +1945,11 @@ int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sp
     hp.n_species = n_species;
     hp.n_ads_types = n_ads;
     hp.n_fw_types = std::max(1, n_fw);
+    hp.total_sites = 0;
+    for (int s = 0; s < PM_MAX_GCMC_SPECIES; ++s) {
+        hp.site_prefix[s] = hp.total_sites;
+        hp.total_sites += s < n_species ? hp.n_sites[s] : 0;
+    }
     hp.capacity = st->capacity;
     hp.n_burn = st->n_burn_cycles;
     hp.n_sample = st->n_sample_cycles;
@@ -1987,6 +1996,7 @@ int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sp
         hipMalloc(&gc->d_sites, sizeof(double4) * per_chain_sites * n_chains) != hipSuccess ||
         hipMalloc(&gc->d_com, sizeof(double4) * per_chain_com * n_chains) != hipSuccess ||
         hipMalloc(&gc->d_tot, sizeof(double) * 2 * n_chains) != hipSuccess ||
+        hipMalloc(&gc->d_incomplete, sizeof(int)) != hipSuccess ||
         (n_dens > 0 && hipMalloc(&gc->d_dens, sizeof(unsigned) * (size_t)n_dens * n_chains) != hipSuccess)) {
         lk.unlock();
         return bail(PM_ERR_OOM, "pm_gcmc_create: device allocation failed");
@@ -2089,18 +2099,56 @@ int pm_gcmc_run(pm_gcmc* gc, int64_t n_cycles) {
     }
     if (n_cycles == 0) return PM_OK;
     // dynamic LDS: the LJ tables (at most 48 KiB), then the framework when it
-    // fits beside them and the kernel's own LDS
-    const size_t n_tab = (size_t)gc->hp.n_fw_types * gc->hp.n_ads_types + (size_t)gc->hp.n_ads_types * gc->hp.n_ads_types;
+    // fits, then the chain's molecules when at least GC_MIN_LDS_MOLS per
+    // species fit; static LDS and slack take the last 4 KiB
+    constexpr int GC_MIN_LDS_MOLS = 16;
+    const GcmcParams& hp = gc->hp;
+    const size_t n_tab = (size_t)hp.n_fw_types * hp.n_ads_types + (size_t)hp.n_ads_types * hp.n_ads_types;
     const size_t lds_tables = sizeof(double4) * ((n_tab + 1) / 2);
-    const size_t lds_atoms = sizeof(double4) * (size_t)gc->hp.n_atoms;
-    const bool use_lds = gc->hp.n_atoms > 0 && lds_tables + lds_atoms + 4096 <= g_ctx.max_lds;
+    const size_t lds_atoms = sizeof(double4) * (size_t)hp.n_atoms;
+    const size_t avail = g_ctx.max_lds > lds_tables + 4096 ? g_ctx.max_lds - lds_tables - 4096 : 0;
+    const bool atoms_lds = hp.n_atoms > 0 && lds_atoms <= avail;
+    size_t per_mol = 0;  // LDS bytes per unit of the molecule capacity
+    for (int s = 0; s < hp.n_species; ++s) per_mol += sizeof(double4) * (size_t)(hp.n_sites[s] + 1);
+    const size_t mol_room = avail - (atoms_lds ? lds_atoms : 0);
+    int lds_cap = (int)std::min<size_t>((size_t)gc->capacity, mol_room / per_mol);
+    if (gc->lds_mols_limit >= 0) lds_cap = std::min(lds_cap, gc->lds_mols_limit);
+    // a smaller store only when the caller asked for one
+    const int min_cap = gc->lds_mols_limit > 0 ? std::min(GC_MIN_LDS_MOLS, gc->lds_mols_limit) : GC_MIN_LDS_MOLS;
+    const bool mols_lds = !gc->mols_in_memory && lds_cap >= min_cap;
+    const size_t dyn = lds_tables + (atoms_lds ? lds_atoms : 0) + (mols_lds ? per_mol * (size_t)lds_cap : 0);
+    auto launch = [&](bool a_lds, bool m_lds, int resume, size_t bytes) {
+        const dim3 grid(gc->n_chains), block(GC_WG);
+        const long long nc = (long long)n_cycles;
+        if (a_lds && m_lds)
+            hipLaunchKernelGGL((gcmc_kernel<true, true>), grid, block, bytes, g_ctx.stream, gc->d_params,
+                               gc->d_chains, nc, resume, lds_cap, gc->d_incomplete);
+        else if (a_lds)
+            hipLaunchKernelGGL((gcmc_kernel<true, false>), grid, block, bytes, g_ctx.stream, gc->d_params,
+                               gc->d_chains, nc, resume, 0, gc->d_incomplete);
+        else if (m_lds)
+            hipLaunchKernelGGL((gcmc_kernel<false, true>), grid, block, bytes, g_ctx.stream, gc->d_params,
+                               gc->d_chains, nc, resume, lds_cap, gc->d_incomplete);
+        else
+            hipLaunchKernelGGL((gcmc_kernel<false, false>), grid, block, bytes, g_ctx.stream, gc->d_params,
+                               gc->d_chains, nc, resume, 0, gc->d_incomplete);
+    };
     (void)hipEventRecord(g_ctx.ev0, g_ctx.stream);
-    if (use_lds)
-        hipLaunchKernelGGL(gcmc_kernel<true>, dim3(gc->n_chains), dim3(GC_WG), lds_tables + lds_atoms,
-                           g_ctx.stream, gc->d_params, gc->d_chains, (long long)n_cycles);
-    else
-        hipLaunchKernelGGL(gcmc_kernel<false>, dim3(gc->n_chains), dim3(GC_WG), lds_tables, g_ctx.stream,
-                           gc->d_params, gc->d_chains, (long long)n_cycles);
+    if (mols_lds) {
+        PM_HIP(hipMemsetAsync(gc->d_incomplete, 0, sizeof(int), g_ctx.stream));
+        launch(atoms_lds, true, 0, dyn);
+        PM_HIP(hipGetLastError());
+        int incomplete = 0;
+        PM_HIP(hipMemcpyAsync(&incomplete, gc->d_incomplete, sizeof(int), hipMemcpyDeviceToHost, g_ctx.stream));
+        PM_HIP(hipStreamSynchronize(g_ctx.stream));
+        if (incomplete > 0) {
+            // the rest of the call, and later calls, with the molecules in device memory
+            gc->mols_in_memory = true;
+            launch(atoms_lds, false, 1, lds_tables + (atoms_lds ? lds_atoms : 0));
+        }
+    } else {
+        launch(atoms_lds, false, 0, lds_tables + (atoms_lds ? lds_atoms : 0));
+    }
     PM_HIP(hipGetLastError());
     (void)hipEventRecord(g_ctx.ev1, g_ctx.stream);
     PM_HIP(hipStreamSynchronize(g_ctx.stream));
@@ -2188,6 +2236,15 @@ int pm_gcmc_total_energy(pm_gcmc* gc, int32_t chain, double* out_gh, double* out
     PM_HIP(hipMemcpy(e, gc->d_tot + 2 * chain, sizeof(e), hipMemcpyDeviceToHost));
     *out_gh = e[0];
     *out_gg = e[1];
+    return PM_OK;
+}
+
+int pm_gcmc_set_lds_molecules(pm_gcmc* gc, int32_t max_per_species) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!gc) return fail(PM_ERR_ARG, "pm_gcmc_set_lds_molecules: null handle");
+    if (max_per_species < -1) return fail(PM_ERR_ARG, "pm_gcmc_set_lds_molecules: limit below -1");
+    gc->lds_mols_limit = max_per_species;
+    gc->mols_in_memory = false;
     return PM_OK;
 }
 

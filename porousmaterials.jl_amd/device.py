@@ -475,6 +475,11 @@ class GcmcChains:
     def run(self, n_cycles):
         check(_lib.load().pm_gcmc_run(self._h.ptr, int(n_cycles)))
 
+    def set_lds_molecules(self, max_per_species):
+        """Most molecules per species a launch keeps in LDS (-1: as many as
+        fit, 0: none); a schedule setting that never changes a result."""
+        check(_lib.load().pm_gcmc_set_lds_molecules(self._h.ptr, int(max_per_species)))
+
     def state(self, chain):
         st = _lib.GcmcState()
         check(_lib.load().pm_gcmc_get_state(self._h.ptr, int(chain), ctypes.byref(st)))

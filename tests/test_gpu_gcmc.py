@@ -161,3 +161,60 @@ def test_gcmc_start_configuration_matches_oracle(pm, orc):
     oc.set_molecules(0, sites, com)
     oc.run(80)
     _compare(dev, [oc], 1, "restart")
+
+
+def _states(dev, n_chains, n_species):
+    out = []
+    for c in range(n_chains):
+        st = dev.state(c)
+        mols = [dev.molecules(c, s) for s in range(n_species)]
+        out.append((st, mols))
+    return out
+
+
+def test_gcmc_molecule_store_placement_is_invisible(pm, orc):
+    """The chains keep their molecules in LDS when they fit and move them to
+    device memory when a chain could outgrow the LDS store; a 4-molecule
+    store makes every chain overflow in mid-call.  Every placement gives the
+    same chains bit for bit, and the oracle's."""
+    from oracle import gcmc as ogc
+    say("case: GCMC Xe/SBMOF-1 Dreiding 3 chains and Kr/Xe UFF 2 chains with the molecule store in LDS "
+        "(default, 4 per species) and in device memory")
+    x = pm.read_cif(golden("crystals", "SBMOF-1.cif"))
+    cases = []
+    ff = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
+    xe = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    cases.append(("Xe", ff, [xe], [298.0] * 3, [[20.0], [200.0], [2000.0]], 300, 100, 17, None))
+    uff = pm.LJForceField.from_csv(golden("forcefields", "UFF.csv"))
+    kr = pm.Molecule.from_dir(golden("molecules", "Kr"))
+    xe_u = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    dens = {"species": 1, "site_mask": 1, "n_pts": (9, 11, 13), "sim_box": True}
+    cases.append(("Kr/Xe", uff, [kr, xe_u], [298.0] * 2, [[0.45e5, 0.05e5], [0.09e5, 0.01e5]], 200, 50, 11, dens))
+    for label, f, tmpl, temps, fugs, nb, ns, seed, density in cases:
+        rep = pm.replication_factors(x.box, f)
+        fw = pm.device.framework(pm.replicate(x, rep), f)
+        runs = {}
+        for limit in (-1, 4, 0):
+            dev = pm.device.GcmcChains(fw, f, tmpl, temps, fugs, nb, ns, capacity=256, seed=seed, density=density)
+            dev.set_lds_molecules(limit)
+            dev.run(150)
+            dev.run(nb + ns - 150)
+            runs[limit] = (dev, _states(dev, len(temps), len(tmpl)))
+        base = runs[-1][1]
+        for limit in (4, 0):
+            other = runs[limit][1]
+            for (sa, ma), (sb, mb) in zip(base, other):
+                for key in sa:
+                    assert np.array_equal(sa[key], sb[key]), (label, limit, key)
+                for (a_s, a_c), (b_s, b_c) in zip(ma, mb):
+                    assert np.array_equal(a_s, b_s) and np.array_equal(a_c, b_c), (label, limit)
+            if density is not None:
+                for c in range(len(temps)):
+                    assert np.array_equal(runs[-1][0].density(c), runs[limit][0].density(c))
+        # the 4-molecule store overflowed: some chain ended with n_s + N + 1 > 4
+        assert max(int(st["n"].max()) + int(st["n"].sum()) + 1 for st, _ in base) > 4
+        och = _oracle_chains(ogc, pm, x, f, tmpl, temps, fugs, nb, ns, seed, 256, density=density)
+        for oc in och:
+            oc.run(nb + ns)
+        _compare(runs[4][0], och, len(tmpl), label + " (4-molecule LDS store)")
+        say(f"   {label}: N = {[list(s['n'][:len(tmpl)]) for s, _ in base]}, placements agree bit for bit")
