@@ -263,13 +263,7 @@ def _run(xtal, templates, temperature, pressure_sets, ljff, molecule_sets, n_cyc
         mols = []
         for s in range(S):
             sites, coms = chains.molecules(c, s)
-            tmpl = templates[s]
-            lst = []
-            for i in range(sites.shape[0]):
-                m = Molecule(tmpl.species, tmpl.atom_species, box.f_to_c @ sites[i].T, [], np.zeros((3, 0)),
-                             box.f_to_c @ coms[i])
-                lst.append(m)
-            mols.append(lst)
+            mols.append([_device_molecule(templates[s], sites[i], coms[i], box) for i in range(sites.shape[0])])
         if verbose:
             print_results(res)
         if autosave:
@@ -283,6 +277,19 @@ def _run(xtal, templates, temperature, pressure_sets, ljff, molecule_sets, n_cyc
                 print("\tresults dictionary saved in ", name)
         out.append((res, mols))
     return out
+
+
+def _device_molecule(tmpl, sites_xf, com_xf, box):
+    """A device molecule as the reference returns it (Cartesian in the
+    simulation box); the charges follow the rigid motion that takes the
+    template's LJ spheres onto the device's (Kabsch)."""
+    x = box.f_to_c @ np.asarray(sites_xf).T
+    c = box.f_to_c @ np.asarray(com_xf)
+    P = tmpl.atoms_x - tmpl.com[:, None]
+    U, _, Vt = np.linalg.svd((x - c[:, None]) @ P.T)
+    R = U @ np.diag([1.0, 1.0, np.sign(np.linalg.det(U @ Vt))]) @ Vt
+    qx = c[:, None] + R @ (tmpl.charges_x - tmpl.com[:, None])
+    return Molecule(tmpl.species, list(tmpl.atom_species), x, np.array(tmpl.charges_q, dtype=np.float64), qx, c)
 
 
 def _to_frac(mol, box):

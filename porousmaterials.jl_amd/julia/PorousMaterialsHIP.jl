@@ -64,6 +64,8 @@ mutable struct Handle
                     ccall((:pm_ewald_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
                 elseif x.destroy === :pm_access_destroy
                     ccall((:pm_access_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
+                elseif x.destroy === :pm_gcmc_destroy
+                    ccall((:pm_gcmc_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
                 else
                     ccall((:pm_probe_destroy, libpmhip), Cint, (Ptr{Cvoid},), x.ptr)
                 end
@@ -745,11 +747,329 @@ function find_energy_minimum(xtal::Crystal, molecule::Molecule, ljff::LJForceFie
     return Frac(Cart(m, xtal_rep.box), xtal.box), Optim.minimum(res)
 end
 
+# ------------------------------------------------------------- GCMC ----
+const PM_MAX_GCMC_SPECIES = 4
+const PM_GCMC_BLOCKS = 5
+const KB_PA_A3 = 1.38064852e7  # src/PorousMaterials.jl:14
+
+"""Mirror of `pm_gcmc_settings` (include/pmhip.h), same C layout."""
+struct GcmcSettings
+    n_burn_cycles::Int64
+    n_sample_cycles::Int64
+    sample_frequency::Int32
+    snapshot_frequency::Int32
+    capacity::Int32
+    density_species::Int32
+    density_site_mask::UInt32
+    density_n_pts::NTuple{3, Int32}
+    density_sim_box::Int32
+    delta::Float64
+    seed::UInt64
+end
+
+"""Mirror of `pm_gcmc_state` (include/pmhip.h); `block_n[b * 4 + s + 1]` is
+block b (0-based), species s."""
+struct GcmcState
+    cycles_done::Int64
+    markov_time::Int64
+    num_snapshots::Int64
+    n::NTuple{4, Int32}
+    block::Int32
+    error::Int32
+    delta::Float64
+    energy_gh::Float64
+    energy_gg::Float64
+    n_proposed::NTuple{6, Int64}
+    n_accepted::NTuple{6, Int64}
+    block_samples::NTuple{5, Int64}
+    block_n::NTuple{20, Int64}
+    block_n2::NTuple{20, Int64}
+    block_gh::NTuple{5, Float64}
+    block_gg::NTuple{5, Float64}
+    block_gh2::NTuple{5, Float64}
+    block_gg2::NTuple{5, Float64}
+    block_un::NTuple{5, Float64}
+end
+
+function gcmc_state(gc::Handle, chain::Integer)
+    st = Ref{GcmcState}()
+    check(ccall((:pm_gcmc_get_state, libpmhip), Cint, (Ptr{Cvoid}, Int32, Ref{GcmcState}), gc.ptr, Int32(chain), st))
+    return st[]
+end
+
+# the reference's per-block statistics (src/gcmc.jl:30-62) from the device's block sums
+function gcmc_blocks(st::GcmcState, nb_species::Int)
+    E(gh, gg) = PorousMaterials.SystemPotentialEnergy(PorousMaterials.PotentialEnergy(gh, 0.0),
+                                                      PorousMaterials.PotentialEnergy(gg, 0.0))
+    return [PorousMaterials.GCMCstats(Int(st.block_samples[b]),
+                                      [Int(st.block_n[(b - 1) * PM_MAX_GCMC_SPECIES + s]) for s in 1:nb_species],
+                                      [Int(st.block_n2[(b - 1) * PM_MAX_GCMC_SPECIES + s]) for s in 1:nb_species],
+                                      E(st.block_gh[b], st.block_gg[b]), E(st.block_gh2[b], st.block_gg2[b]),
+                                      st.block_un[b]) for b in 1:PM_GCMC_BLOCKS]
+end
+
+# a device molecule as the reference returns it: Cartesian in the simulation
+# box, charges carried by the rigid motion that takes the template's LJ
+# spheres onto the device's (Kabsch)
+function device_molecule(template::Molecule{Cart}, sites_xf::Matrix{Float64}, com_xf::Vector{Float64}, box::Box)
+    x = box.f_to_c * sites_xf
+    c = box.f_to_c * com_xf
+    P = template.atoms.coords.x .- template.com.x
+    F = svd((x .- c) * P')
+    R = F.U * Diagonal([1.0, 1.0, sign(det(F.U * F.Vt))]) * F.Vt
+    qx = c .+ R * (template.charges.coords.x .- template.com.x)
+    return Molecule(template.species, Atoms(template.atoms.species, Cart(x)),
+                    Charges(template.charges.q, Cart(qx)), Cart(c))
+end
+
+# chains for every pressure set in one pm_gcmc handle (src/gcmc.jl:175-817 per chain)
+function gcmc_sims(xtal::Crystal, molecule_templates::Array{Molecule{Cart}, 1}, temperature::Float64,
+                   pressure_sets::Vector{Vector{Float64}}, ljff::LJForceField, molecule_sets::Vector;
+                   n_cycles_per_volume::Int=200, fraction_burn_cycles::Float64=0.5, sample_frequency::Int=1,
+                   verbose::Bool=true, ewald_precision::Float64=1e-6, eos::Symbol=:ideal, autosave::Bool=true,
+                   show_progress_bar::Bool=false, write_adsorbate_snapshots::Bool=false,
+                   snapshot_frequency::Int=1, calculate_density_grid::Bool=false, density_grid_dx::Float64=1.0,
+                   density_grid_molecular_species::Union{Nothing, Symbol}=nothing,
+                   density_grid_sim_box::Bool=true, results_filename_comment::String="",
+                   n_burn_cycles::Union{Nothing, Int}=nothing, n_sample_cycles::Union{Nothing, Int}=nothing,
+                   seed::Integer=0, capacity::Union{Nothing, Int}=nothing)
+    assert_P1_symmetry(xtal)
+    write_adsorbate_snapshots && error("xyz snapshots of the adsorbates are not written by the device chains")
+    start_time = time()
+    nb_species = length(molecule_templates)
+    molecular_species = [mt.species for mt in molecule_templates]
+    all(p -> length(p) == nb_species, pressure_sets) || error("# molecules in simulation: $nb_species; " *
+                                                              "each pressure set needs as many partial pressures")
+    if has_charges(xtal) && any(has_charges, molecule_templates)
+        error("sorry, electrostatics not supported")  # src/gcmc.jl:378-382
+    end
+    nb_cycles = max(N_BLOCKS, ceil(Int, n_cycles_per_volume * xtal.box.Ω))
+    @assert (0.0 < fraction_burn_cycles) && (fraction_burn_cycles < 1.0)
+    n_burn = isnothing(n_burn_cycles) ? ceil(Int, nb_cycles * fraction_burn_cycles) : n_burn_cycles
+    n_sample = isnothing(n_sample_cycles) ? ceil(Int, nb_cycles * (1 - fraction_burn_cycles)) : n_sample_cycles
+    # fugacities (Pa) from an equation of state (src/gcmc.jl:228-243)
+    fugacity_sets = map(pressure_sets) do pressures
+        if eos == :ideal
+            pressures * 100000.0
+        elseif eos == :PengRobinson
+            nb_species == 1 || error("Peng Robinson EOS not implemented for >1 species")
+            props = calculate_properties(PengRobinsonFluid(molecule_templates[1].species), temperature,
+                                         pressures[1], verbose=false)
+            [props["fugacity (bar)"] * 100000.0]
+        else
+            error("eos=:ideal and eos=:PengRobinson are the only valid options for an equation of state.")
+        end
+    end
+    repfactors = replication_factors(xtal.box, ljff)
+    xtal_rep = replicate(xtal, repfactors)
+    box = xtal_rep.box
+    # density grid (src/gcmc.jl:286-325)
+    dens_species, dens_mask, n_pts = Int32(-1), UInt32(0), (Int32(1), Int32(1), Int32(1))
+    if calculate_density_grid
+        if isnothing(density_grid_molecular_species)
+            nb_species == 1 || error("Pass `density_grid_molecular_species` to specify which adsorbate to make a grid for.")
+            density_grid_molecular_species = molecular_species[1]
+        end
+        sid = findfirst(==(density_grid_molecular_species), molecular_species)
+        dens_species = Int32(sid - 1)
+        dens_mask = UInt32(sum(UInt32(1) << (k - 1) for (k, a) in enumerate(molecule_templates[sid].atoms.species)
+                               if a == density_grid_molecular_species; init=UInt32(0)))
+        n_pts = Int32.(required_n_pts(density_grid_sim_box ? box : xtal.box, density_grid_dx))
+    end
+    cap = isnothing(capacity) ? Int(min(1 << 18, max(256, ceil(Int, 0.1 * box.Ω)))) : capacity
+    settings = GcmcSettings(n_burn, n_sample, Int32(sample_frequency), Int32(snapshot_frequency), Int32(cap),
+                            dens_species, dens_mask, n_pts, Int32(density_grid_sim_box), 2.0, UInt64(seed))
+    fw = framework_handle(xtal, ljff, repfactors)
+    ff = forcefield_handle(ljff)
+    probes = [probe_handle(m, ljff) for m in molecule_templates]
+    n_chains = length(pressure_sets)
+    fug = Float64[f for fs in fugacity_sets for f in fs]   # [chain * n_species + s]
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pm_gcmc_create, libpmhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Int32, Ptr{Float64}, Ptr{Float64},
+                 Ref{GcmcSettings}, Ref{Ptr{Cvoid}}),
+                fw.ptr, ff.ptr, Int32(nb_species), [p.ptr for p in probes], Int32(n_chains),
+                fill(temperature, n_chains), fug, settings, out))
+    gc = Handle(out[], :pm_gcmc_destroy)
+    # starting configurations (src/gcmc.jl:386-402)
+    for (c, mols) in enumerate(molecule_sets)
+        isnothing(mols) && continue
+        for (s, ms) in enumerate(mols)
+            isempty(ms) && continue
+            all(m -> m.species == molecular_species[s], ms) || error("initializing with wrong molecule species")
+            sites = reduce(vcat, [vec(box.c_to_f * m.atoms.coords.x) for m in ms])
+            coms = reduce(vcat, [box.c_to_f * m.com.x for m in ms])
+            check(ccall((:pm_gcmc_set_molecules, libpmhip), Cint,
+                        (Ptr{Cvoid}, Int32, Int32, Int32, Ptr{Float64}, Ptr{Float64}),
+                        gc.ptr, Int32(c - 1), Int32(s - 1), Int32(length(ms)), sites, coms))
+        end
+    end
+    verbose && @printf("μVT simulation of %s in %s at %f K (%d chains), %d burn + %d sample cycles\n",
+                       join(molecular_species, ", "), xtal.name, temperature, n_chains, n_burn, n_sample)
+    # launches grow towards ~1 s of device time; the chains stay on the device between them
+    done, step = 0, 1
+    while done < n_burn + n_sample
+        k = min(step, n_burn + n_sample - done)
+        t = @elapsed check(ccall((:pm_gcmc_run, libpmhip), Cint, (Ptr{Cvoid}, Int64), gc.ptr, Int64(k)))
+        done += k
+        step = max(1, Int(floor(k * clamp(1.0 / max(t, 1e-3), 0.5, 4.0))))
+        show_progress_bar && @printf("\tcycle %d / %d\n", done, n_burn + n_sample)
+    end
+    elapsed_time = time() - start_time
+    nr = repfactors[1] * repfactors[2] * repfactors[3]
+    outputs = Tuple{Dict{String, Any}, Vector{Vector{Molecule{Cart}}}}[]
+    for c in 1:n_chains
+        st = gcmc_state(gc, c - 1)
+        st.error == 0 || error("chain $c reached the molecule capacity $cap; pass a larger capacity")
+        gh, gg = Ref(0.0), Ref(0.0)
+        check(ccall((:pm_gcmc_total_energy, libpmhip), Cint, (Ptr{Cvoid}, Int32, Ref{Float64}, Ref{Float64}),
+                    gc.ptr, Int32(c - 1), gh, gg))
+        if !(isapprox(gh[], st.energy_gh, atol=0.01) && isapprox(gg[], st.energy_gg, atol=0.01))
+            error("energy incremented improperly during simulation...")  # src/gcmc.jl:729-741
+        end
+        stats = gcmc_blocks(st, nb_species)
+        avg_n, err_n, avg_U, err_U = PorousMaterials.mean_stderr_n_U(stats)
+        tot = sum(stats)
+        results = Dict{String, Any}()
+        results["xtal"] = xtal.name
+        results["adsorbate"] = molecular_species
+        results["forcefield"] = ljff.name
+        results["pressure (bar)"] = pressure_sets[c]
+        results["fugacity (bar)"] = fugacity_sets[c] / 100000.0
+        results["temperature (K)"] = temperature
+        results["repfactors"] = repfactors
+        results["# sample cycles"] = n_sample
+        results["# burn cycles"] = n_burn
+        results["# samples"] = tot.n_samples
+        results["elapsed time (min)"] = elapsed_time / 60
+        results["⟨N⟩ (molecules)"] = avg_n
+        results["⟨U_gh, vdw⟩ (K)"] = avg_U.gh.vdw
+        results["⟨U_gh, electro⟩ (K)"] = avg_U.gh.es
+        results["⟨U_gg, vdw⟩ (K)"] = avg_U.gg.vdw
+        results["⟨U_gg, electro⟩ (K)"] = avg_U.gg.es
+        results["⟨U⟩ (K)"] = sum(avg_U)
+        results["var(N)"] = (tot.n² / tot.n_samples) - (results["⟨N⟩ (molecules)"] .^ 2)
+        results["Q_st (K)"] = temperature .- (tot.Un / tot.n_samples .- results["⟨U⟩ (K)"] * results["⟨N⟩ (molecules)"]) ./
+                              results["var(N)"]
+        results["err ⟨N⟩ (molecules)"] = err_n
+        results["err ⟨U_gh, vdw⟩ (K)"] = err_U.gh.vdw
+        results["err ⟨U_gh, electro⟩ (K)"] = err_U.gh.es
+        results["err ⟨U_gg, vdw⟩ (K)"] = err_U.gg.vdw
+        results["err ⟨U_gg, electro⟩ (K)"] = err_U.gg.es
+        results["err ⟨U⟩ (K)"] = sum(err_U)
+        results["⟨N⟩ (molecules/unit cell)"] = avg_n / nr
+        results["err ⟨N⟩ (molecules/unit cell)"] = err_n / nr
+        to_mmol_g = 1000 / (6.022140857e23 * molecular_weight(xtal_rep) * 1.66054e-24) * nr
+        results["⟨N⟩ (mmol/g)"] = results["⟨N⟩ (molecules/unit cell)"] * to_mmol_g
+        results["err ⟨N⟩ (mmol/g)"] = results["err ⟨N⟩ (molecules/unit cell)"] * to_mmol_g
+        for (id, desc) in PorousMaterials.PROPOSAL_ENCODINGS
+            results[@sprintf("Total # %s proposals", desc)] = Int(st.n_proposed[id])
+            results[@sprintf("Fraction of %s proposals accepted", desc)] = st.n_accepted[id] / st.n_proposed[id]
+        end
+        dbox = density_grid_sim_box ? box : xtal.box
+        density = Grid(dbox, Tuple(Int.(n_pts)), zeros(Int.(n_pts)...), :inverse_A3, [0.0, 0.0, 0.0])
+        if calculate_density_grid
+            counts = zeros(prod(Int.(n_pts)))
+            check(ccall((:pm_gcmc_get_density, libpmhip), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                        gc.ptr, Int32(c - 1), counts))
+            density.data .= reshape(counts, Int.(n_pts)...) ./ max(st.num_snapshots, 1)
+        end
+        results["density grid"] = density
+        results["num snapshots"] = Int(st.num_snapshots)
+        verbose && PorousMaterials.print_results(results, print_title=false)
+        molecules = Vector{Vector{Molecule{Cart}}}()
+        for s in 1:nb_species
+            n = Ref{Int32}(0)
+            check(ccall((:pm_gcmc_get_molecules, libpmhip), Cint,
+                        (Ptr{Cvoid}, Int32, Int32, Int32, Ref{Int32}, Ptr{Float64}, Ptr{Float64}),
+                        gc.ptr, Int32(c - 1), Int32(s - 1), Int32(0), n, C_NULL, C_NULL))
+            ns = molecule_templates[s].atoms.n
+            sites = zeros(3, ns, max(n[], 1))
+            coms = zeros(3, max(n[], 1))
+            check(ccall((:pm_gcmc_get_molecules, libpmhip), Cint,
+                        (Ptr{Cvoid}, Int32, Int32, Int32, Ref{Int32}, Ptr{Float64}, Ptr{Float64}),
+                        gc.ptr, Int32(c - 1), Int32(s - 1), n[], n, sites, coms))
+            push!(molecules, [device_molecule(molecule_templates[s], sites[:, :, i], coms[:, i], box)
+                              for i in 1:n[]])
+        end
+        if autosave
+            path = rc[:paths][:simulations]
+            isdir(path) || mkdir(path)
+            savename = joinpath(path, μVT_output_filename(xtal_rep, molecule_templates, temperature,
+                                                          pressure_sets[c], ljff, n_burn, n_sample,
+                                                          comment=results_filename_comment))
+            PorousMaterials.JLD2.@save savename results
+            verbose && println("\tresults dictionary saved in ", savename)
+        end
+        push!(outputs, (results, molecules))
+    end
+    return outputs
+end
+
+"""
+    results, molecules = μVT_sim(xtal, molecule_templates, temperature, pressures, ljff; molecules=nothing,
+                                 <the reference's keywords>, seed=0, capacity=nothing)
+
+`src/gcmc.jl:175-817` with the Markov chain on the device (`pm_gcmc_*`): the
+same cycle counts, fugacities, replication, moves, acceptance rules, block
+statistics (the reference's own `GCMCstats` and `mean_stderr_n_U`), energy
+check and results `Dict`, saved as JLD2.  The chain draws from the Philox
+stream keyed by `seed`.
+"""
+function μVT_sim(xtal::Crystal, molecule_templates::Array{Molecule{Cart}, 1}, temperature::Float64,
+                 pressures::Array{Float64, 1}, ljff::LJForceField;
+                 molecules::Union{Nothing, Array{Array{Molecule{Cart}, 1}, 1}}=nothing, kwargs...)
+    return gcmc_sims(xtal, molecule_templates, temperature, [pressures], ljff, [molecules]; kwargs...)[1]
+end
+
+μVT_sim(xtal::Crystal, molecule_template::Molecule{Cart}, temperature::Float64, pressure::Float64,
+        ljff::LJForceField; kwargs...) = μVT_sim(xtal, [molecule_template], temperature, [pressure], ljff; kwargs...)
+
+"""
+    results = adsorption_isotherm(xtal, molecule_templates, temperature, pressures, ljff; kwargs...)
+
+`src/gcmc.jl:996-1030`: every pressure set is a chain of one device handle
+(the reference's `pmap` over pressures), all advanced by the same launches.
+"""
+function adsorption_isotherm(xtal::Crystal, molecule_templates::Array{Molecule{Cart}, 1}, temperature::Float64,
+                             pressures::Array{Array{Float64, 1}, 1}, ljff::LJForceField; kwargs...)
+    return [r for (r, _) in gcmc_sims(xtal, molecule_templates, temperature, pressures, ljff,
+                                      fill(nothing, length(pressures)); kwargs...)]
+end
+
+adsorption_isotherm(xtal::Crystal, molecule_template::Molecule{Cart}, temperature::Float64,
+                    pressures::Array{Float64, 1}, ljff::LJForceField; kwargs...) =
+    adsorption_isotherm(xtal, [molecule_template], temperature, [[p] for p in pressures], ljff; kwargs...)
+
+"""
+    results = stepwise_adsorption_isotherm(xtal, molecule_templates, temperature, pressures, ljff; kwargs...)
+
+`src/gcmc.jl:940-975`: one device chain per pressure in turn, each starting
+from the previous one's molecules; pressure i draws from stream `seed + i - 1`.
+"""
+function stepwise_adsorption_isotherm(xtal::Crystal, molecule_templates::Array{Molecule{Cart}, 1},
+                                      temperature::Float64, pressures::Array{Array{Float64, 1}, 1},
+                                      ljff::LJForceField; seed::Integer=0, kwargs...)
+    length(molecule_templates) == 1 || error("We only calculate pure gas isotherms")
+    results = Dict{String, Any}[]
+    molecules = nothing
+    for (i, pressure) in enumerate(pressures)
+        result, molecules = μVT_sim(xtal, molecule_templates, temperature, pressure, ljff; molecules=molecules,
+                                    seed=seed + i - 1, kwargs...)
+        push!(results, result)
+    end
+    return results
+end
+
+stepwise_adsorption_isotherm(xtal::Crystal, molecule_template::Molecule{Cart}, temperature::Float64,
+                             pressures::Array{Float64, 1}, ljff::LJForceField; kwargs...) =
+    stepwise_adsorption_isotherm(xtal, [molecule_template], temperature, [[p] for p in pressures], ljff; kwargs...)
+
 """
     enable!()
 
-Route PorousMaterials' own `henry_coefficient` and `energy_grid` through the
-device (type-specific methods added to the reference's functions).
+Route PorousMaterials' own `henry_coefficient`, `energy_grid` and `μVT_sim`
+through the device (methods added to, or replacing, the reference's).
 """
 function enable!()
     @eval PorousMaterials.henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Float64,
@@ -758,6 +1078,11 @@ function enable!()
     @eval PorousMaterials.energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljff::LJForceField;
                                       kwargs...) =
         PorousMaterialsHIP.energy_grid(crystal, molecule, ljff; kwargs...)
+    # the reference's isotherm drivers call μVT_sim, so they reach the device too
+    @eval PorousMaterials.μVT_sim(xtal::Crystal, molecule_templates::Array{Molecule{Cart}, 1},
+                                  temperature::Float64, pressures::Array{Float64, 1}, ljff::LJForceField;
+                                  kwargs...) =
+        PorousMaterialsHIP.μVT_sim(xtal, molecule_templates, temperature, pressures, ljff; kwargs...)
     return nothing
 end
 

@@ -294,3 +294,23 @@ def test_batched_lbfgs_on_quadratics_and_a_ridge(pm):
     x, f, g, it, conv = _lbfgs(Obj(), np.zeros((P, 3)), gtol=1e-7, max_iter=500)
     assert conv.all(), (it, np.linalg.norm(g, axis=1))
     np.testing.assert_allclose(x[:-2], c[:-2], atol=1e-8)
+
+
+def test_gcmc_molecules_carry_charges(pm):
+    """μVT_sim returns molecules as the reference does, charges included: the
+    rigid motion taking the template's LJ spheres onto the device's sites
+    (Kabsch) carries the off-site charges along; a reflection is never used."""
+    import math
+    tmpl = pm.Molecule("W", ["O", "H", "H"], np.array([[0.0, 0.757, -0.757], [0.0, 0.586, 0.586], [0.0, 0.0, 0.0]]),
+                       np.array([-0.8, 0.4, 0.4]), np.array([[0.0, 0.5, -0.5], [0.15, 0.3, 0.3], [0.0, 0.1, -0.1]]),
+                       np.array([0.0, 0.07, 0.0]))
+    box = pm.Box(20.0, 22.0, 25.0, math.pi / 2, 1.9, math.pi / 2)
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        R = pm.arvo_rotation_matrix(*rng.random(3))
+        c = rng.random(3) * 10.0 + 2.0
+        sites = c[:, None] + R @ (tmpl.atoms_x - tmpl.com[:, None])
+        qx = c[:, None] + R @ (tmpl.charges_x - tmpl.com[:, None])
+        m = pm.gcmc._device_molecule(tmpl, (box.c_to_f @ sites).T, box.c_to_f @ c, box)
+        assert np.allclose(m.atoms_x, sites, atol=1e-12) and np.allclose(m.charges_x, qx, atol=1e-12)
+        assert np.allclose(m.com, c, atol=1e-12) and list(m.charges_q) == [-0.8, 0.4, 0.4]
