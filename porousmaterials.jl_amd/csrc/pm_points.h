@@ -31,6 +31,9 @@
 
 namespace pmk {
 
+// Images per spatial tile of an image list (one wave loads a tile).
+constexpr int TILE_IMAGES = 64;
+
 constexpr int ERFC_SEG = 512;      // cubic segments of erfc(alpha r) on [0, r_sr] (FP32 path)
 constexpr int SCAN_WG = 1024;      // work-items of the scan kernels
 constexpr int SCAN_PER = 4;        // cells per work-item in the scan kernels
@@ -51,10 +54,12 @@ struct RunArgs {
     double inv4pe;
     int esm;                 // 0 no electrostatics, 1 LJ and Ewald share r_c, 2 separate cutoffs
     int fast;                // the image list covers r_cmax + chunk_r around every chunk
-    int nsite, n_types, n_img, n_atoms;
+    int nsite, n_types, n_img, n_atoms, n_tiles;
     unsigned site_lj_mask, site_q_mask;
-    const double4* img;      // images near the box: Cartesian xyz (A) + charge
+    const double4* img;      // images near the box: Cartesian xyz (A) + charge, in spatial tiles
     const int* img_type;
+    const double4* tile_sph; // per tile: bounding-sphere centre (Cartesian) and radius
+    const int2* tile_rng;    // per tile: first image, image count (<= 64)
     const double4* atoms;    // replicated atoms (fallback path)
     const int* species;
     const void* ljtab;       // [n_types][nsite] LJP<Real>
@@ -348,8 +353,11 @@ __global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_
 
 // ---------------------------------------------------------------- pairs ----
 // One candidate pair: LJ into plj, real-space Ewald into psr.
+// FP32 pairs: the masked terms accumulate by one FMA each (eps4 * w and
+// q / r * erfc), the table offset is v_fract of the scaled distance, and the
+// candidate's charge arrives in FP32 (qf).
 template <int ESM, typename Real>
-__device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool is_lj, double r2lj,
+__device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>& p, bool is_lj, double r2lj,
                                  double r2sr, double alpha, float inv_h, const float4* tab, bool& ovl,
                                  Real& plj, Real& psr) {
     const bool in = r2 < r2lj;
@@ -360,14 +368,18 @@ __device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool i
         const float rinv = __builtin_amdgcn_rsqf(rf2);
         const float s = p.sig2 * (rinv * rinv);
         const float s3 = s * s * s;
-        plj += in ? p.eps4 * fmaf(s3, s3, -s3) : 0.0f;
-        const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (q != 0.0);
+        plj = fmaf(p.eps4, in ? fmaf(s3, s3, -s3) : 0.0f, plj);
+        const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (qf != 0.0f);
         const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
         const int i = (int)x;
-        const float t = x - (float)i;
+        const float t = __builtin_amdgcn_fractf(x);
         const float4 c = tab[i];
         const float ec = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-        psr += ins ? (float)q * ec * rinv : 0.0f;
+        psr = fmaf(ins ? qf * rinv : 0.0f, ec, psr);
+    } else if constexpr (sizeof(Real) == 4) {
+        const float s = p.sig2 * __builtin_amdgcn_rcpf((float)r2);
+        const float s3 = s * s * s;
+        plj = fmaf(p.eps4, in ? fmaf(s3, s3, -s3) : 0.0f, plj);
     } else {
         const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
         plj += in ? e : (Real)0;
@@ -383,7 +395,8 @@ __device__ inline void pair_term(double r2, double q, const LJP<Real>& p, bool i
 // cQ += erfc(alpha r) / r, all for in-cutoff pairs.  The energy terms are
 // computed exactly as in pair_term.
 template <int ESM, typename Real>
-__device__ inline void pair_term_grad(double r2, double q, const LJP<Real>& p, Real csig, bool is_lj, double r2lj,
+__device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<Real>& p, Real csig, bool is_lj,
+                                      double r2lj,
                                       double r2sr, double alpha, float inv_h, const float4* tab, bool& ovl,
                                       Real& plj, Real& psr, double& cE, double& cS, double& cQ) {
     const bool in = r2 < r2lj;
@@ -395,7 +408,7 @@ __device__ inline void pair_term_grad(double r2, double q, const LJP<Real>& p, R
                                              : __builtin_amdgcn_rcpf((float)r2));
         const float u = s * s * s;
         const float w = fmaf(u, u, -u);
-        plj += in ? p.eps4 * w : 0.0f;
+        plj = fmaf(p.eps4, in ? w : 0.0f, plj);
         cE += lj_in ? (double)w : 0.0;
         cS += lj_in ? (double)(csig * u * fmaf(12.0f, u, -6.0f)) : 0.0;
     } else {
@@ -412,10 +425,10 @@ __device__ inline void pair_term_grad(double r2, double q, const LJP<Real>& p, R
             const float rinv = __builtin_amdgcn_rsqf(rf2);
             const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
             const int i = (int)x;
-            const float t = x - (float)i;
+            const float t = __builtin_amdgcn_fractf(x);
             const float4 c = tab[i];
             const float e = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-            psr += (ins & (q != 0.0)) ? (float)q * e * rinv : 0.0f;
+            psr = fmaf((ins & (qf != 0.0f)) ? qf * rinv : 0.0f, e, psr);
             cQ += ins ? (double)(e * rinv) : 0.0;
         } else {
             const double r = sqrt(r2);
@@ -429,6 +442,8 @@ __device__ inline void pair_term_grad(double r2, double q, const LJP<Real>& p, R
 struct PairLds {
     double4* c;     // candidate records (Cartesian relative to the chunk centre, charge)
     int* t;         // candidate type * nsite
+    float* qf;      // candidate charge in FP32
+    int* sel;       // selected tiles of the current batch
     const float4* erfc;
     int* wcnt;      // per-wave candidate counts
 };
@@ -500,54 +515,84 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
         const double ccz = f33 * cf2;
         const double Rc = a.r_cmax + a.chunk_r + 1e-3;
         const double R2 = Rc * Rc;
-        const int n_img = a.n_img;
-        for (int base = 0; base < n_img; base += TILE) {
-            const int t = base + tid;
-            bool keep = false;
-            double4 rec = make_double4(0.0, 0.0, 0.0, 0.0);
-            int ty = 0;
-            if (t < n_img) {
-                rec = a.img[t];
-                const double cx = rec.x - ccx, cy = rec.y - ccy, cz = rec.z - ccz;
-                keep = cx * cx + cy * cy + cz * cz < R2;
-                ty = a.img_type[t] * nsite;
+        const int n_tiles = a.n_tiles;
+        for (int tb = 0; tb < n_tiles; tb += WG) {
+            // the tiles of this batch whose bounding sphere meets the chunk's
+            bool hit = false;
+            const int ti = tb + tid;
+            if (ti < n_tiles) {
+                const double4 sp = a.tile_sph[ti];
+                const double sx = sp.x - ccx, sy = sp.y - ccy, sz = sp.z - ccz;
+                const double rr = Rc + sp.w;
+                hit = sx * sx + sy * sy + sz * sz < rr * rr;
             }
-            const unsigned long long m = __ballot(keep);
-            const int before = __popcll(m & ((1ull << lane) - 1ull));
-            __syncthreads();  // the previous tile's candidates are consumed
-            if (lane == 0) L.wcnt[wave] = __popcll(m);
+            const unsigned long long hm = __ballot(hit);
+            const int hb = __popcll(hm & ((1ull << lane) - 1ull));
+            __syncthreads();  // the previous batch's tile list and candidates are consumed
+            if (lane == 0) L.wcnt[wave] = __popcll(hm);
             __syncthreads();
-            int woff[WG / 64 + 1];
-            woff[0] = 0;
+            int hoff = 0, n_hit = 0;
 #pragma unroll
-            for (int w = 0; w < WG / 64; ++w) woff[w + 1] = woff[w] + L.wcnt[w];
-            if (keep) {
-                L.c[woff[wave] + before] = rec;
-                L.t[woff[wave] + before] = ty;
+            for (int w = 0; w < WG / 64; ++w) {
+                const int c = L.wcnt[w];
+                hoff += w < wave ? c : 0;
+                n_hit += c;
             }
+            if (hit) L.sel[hoff + hb] = ti;
             __syncthreads();
-            n_cand += (unsigned long long)woff[WG / 64];
-            if (wave_live) {
-                // candidates of each 64-image group accumulate in Real, flushed to FP64
-#pragma unroll
-                for (int gi = 0; gi < WG / 64; ++gi) {
-                    Real plj = (Real)0, psr = (Real)0;
-                    for (int j = woff[gi]; j < woff[gi + 1]; ++j) {
-                        const double4 c = L.c[j];
-                        const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
-                        const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        if constexpr (GRAD) {
-                            const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
-                            ts.type(tt, nsite);
-                            pair_term_grad<ESM, Real>(r2, c.w, s_lj[tt + site], s_gc[tt + site], is_lj, r2lj, r2sr,
-                                                      alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
-                        } else {
-                            pair_term<ESM, Real>(r2, c.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
-                                                 L.erfc, ovl, plj, psr);
-                        }
+            // rounds of WG / 64 selected tiles, one per wave, in tile order
+            for (int k = 0; k < n_hit; k += WG / 64) {
+                bool keep = false;
+                double4 rec = make_double4(0.0, 0.0, 0.0, 0.0);
+                int ty = 0;
+                if (k + wave < n_hit) {
+                    const int2 rg = a.tile_rng[L.sel[k + wave]];
+                    if (lane < rg.y) {
+                        rec = a.img[rg.x + lane];
+                        const double cx = rec.x - ccx, cy = rec.y - ccy, cz = rec.z - ccz;
+                        keep = cx * cx + cy * cy + cz * cz < R2;
+                        ty = a.img_type[rg.x + lane] * nsite;
                     }
-                    acc_lj += (double)plj;
-                    acc_sr += (double)psr;
+                }
+                const unsigned long long m = __ballot(keep);
+                const int before = __popcll(m & ((1ull << lane) - 1ull));
+                __syncthreads();  // the previous round's candidates (and the tile counts) are consumed
+                if (lane == 0) L.wcnt[wave] = __popcll(m);
+                __syncthreads();
+                int woff[WG / 64 + 1];
+                woff[0] = 0;
+#pragma unroll
+                for (int w = 0; w < WG / 64; ++w) woff[w + 1] = woff[w] + L.wcnt[w];
+                if (keep) {
+                    L.c[woff[wave] + before] = rec;
+                    L.t[woff[wave] + before] = ty;
+                    L.qf[woff[wave] + before] = (float)rec.w;
+                }
+                __syncthreads();
+                n_cand += (unsigned long long)woff[WG / 64];
+                if (wave_live) {
+                    // the candidates of each tile accumulate in Real, flushed to FP64
+#pragma unroll
+                    for (int gi = 0; gi < WG / 64; ++gi) {
+                        Real plj = (Real)0, psr = (Real)0;
+                        for (int j = woff[gi]; j < woff[gi + 1]; ++j) {
+                            const double4 c = L.c[j];
+                            const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
+                            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            if constexpr (GRAD) {
+                                const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
+                                ts.type(tt, nsite);
+                                pair_term_grad<ESM, Real>(r2, c.w, L.qf[j], s_lj[tt + site], s_gc[tt + site], is_lj,
+                                                          r2lj, r2sr, alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE,
+                                                          ts.cS, ts.cQ);
+                            } else {
+                                pair_term<ESM, Real>(r2, c.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr,
+                                                     alpha, inv_h, L.erfc, ovl, plj, psr);
+                            }
+                        }
+                        acc_lj += (double)plj;
+                        acc_sr += (double)psr;
+                    }
                 }
             }
         }
@@ -559,6 +604,7 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
             if (tid < nt) {
                 L.c[tid] = a.atoms[base + tid];
                 L.t[tid] = a.species[base + tid] * nsite;
+                L.qf[tid] = (float)a.atoms[base + tid].w;
             }
             __syncthreads();
             n_cand += (unsigned long long)nt;
@@ -579,11 +625,12 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                         if constexpr (GRAD) {
                             const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
                             ts.type(tt, nsite);
-                            pair_term_grad<ESM, Real>(r2, at.w, s_lj[tt + site], s_gc[tt + site], is_lj, r2lj, r2sr,
-                                                      alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
+                            pair_term_grad<ESM, Real>(r2, at.w, L.qf[j], s_lj[tt + site], s_gc[tt + site], is_lj,
+                                                      r2lj, r2sr, alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE, ts.cS,
+                                                      ts.cQ);
                         } else {
-                            pair_term<ESM, Real>(r2, at.w, s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha, inv_h,
-                                                 L.erfc, ovl, plj, psr);
+                            pair_term<ESM, Real>(r2, at.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha,
+                                                 inv_h, L.erfc, ovl, plj, psr);
                         }
                     }
                     acc_lj += (double)plj;
@@ -595,11 +642,13 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
     if constexpr (GRAD) ts.flush(nsite);
 }
 
-// LDS: TILE candidate records and types, the erfc table (FP32 kernels), then
-// the LJ table of the run (and for a gradient run its d/d sigma factors).
+// LDS: TILE candidate records, types and FP32 charges, the selected-tile list,
+// the erfc table (FP32 kernels), then the LJ table of the run (and for a gradient run its d/d
+// sigma factors).
 template <typename Real>
 constexpr int pair_lds_fixed() {
-    return TILE * (int)(sizeof(double4) + sizeof(int)) + (sizeof(Real) == 4 ? ERFC_SEG * (int)sizeof(float4) : 0);
+    return TILE * (int)(sizeof(double4) + sizeof(int) + sizeof(float) + sizeof(int)) +
+           (sizeof(Real) == 4 ? ERFC_SEG * (int)sizeof(float4) : 0);
 }
 
 template <typename Real, bool GRAD>
@@ -609,7 +658,9 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     PairLds L;
     L.c = reinterpret_cast<double4*>(smem);
     L.t = reinterpret_cast<int*>(smem + TILE * sizeof(double4));
-    float4* s_erfc = reinterpret_cast<float4*>(smem + TILE * (sizeof(double4) + sizeof(int)));
+    L.qf = reinterpret_cast<float*>(smem + TILE * (sizeof(double4) + sizeof(int)));
+    L.sel = reinterpret_cast<int*>(smem + TILE * (sizeof(double4) + sizeof(int) + sizeof(float)));
+    float4* s_erfc = reinterpret_cast<float4*>(smem + TILE * (sizeof(double4) + 2 * sizeof(int) + sizeof(float)));
     L.erfc = s_erfc;
     L.wcnt = s_wcnt;
     LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + pair_lds_fixed<Real>());

@@ -168,8 +168,11 @@ struct pm_framework {
     struct Images {
         double r = 0.0;
         int64_t n = 0;
+        int64_t n_tiles = 0;
         double4* d_img = nullptr;
         int32_t* d_type = nullptr;
+        double4* d_tile_sph = nullptr;  // tile bounding spheres
+        int2* d_tile_rng = nullptr;     // tile (first image, count)
     };
     std::vector<Images> images;
 };
@@ -406,8 +409,9 @@ int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
     img.reserve((size_t)est);
     ty.reserve((size_t)est);
     const double* F = fw->F;
-    // sorted by species (then unit-cell atom): a gradient run's per-type sums
-    // change type about once per tile
+    // by species (then unit-cell atom) before the spatial ordering below, so
+    // the species of a tile come in order and a gradient run's per-type sums
+    // change type at most once per species in a tile
     std::vector<int64_t> by_type((size_t)fw->n_unit);
     for (int64_t u = 0; u < fw->n_unit; ++u) by_type[(size_t)u] = u;
     std::stable_sort(by_type.begin(), by_type.end(),
@@ -429,21 +433,103 @@ int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
                     ty.push_back(fw->unit_type[u]);
                 }
     }
+    // spatial tiles: bins of about TILE_IMAGES images over the list's bounding
+    // box; the list is ordered by (bin, species, previous order) and every bin
+    // is cut into tiles of at most TILE_IMAGES images, each with its bounding
+    // sphere.  A chunk visits only the tiles whose sphere meets its own; the
+    // FP32 partial sums of a point run per tile, so they depend on this list
+    // alone.
+    std::vector<double4> sph;
+    std::vector<int2> rng;
+    if (!img.empty()) {
+        double lo[3] = {img[0].x, img[0].y, img[0].z}, hi[3] = {lo[0], lo[1], lo[2]};
+        for (const auto& v : img) {
+            const double c[3] = {v.x, v.y, v.z};
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], c[d]);
+                hi[d] = std::max(hi[d], c[d]);
+            }
+        }
+        double ext[3], bvol = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            ext[d] = std::max(hi[d] - lo[d], 1e-6);
+            bvol *= ext[d];
+        }
+        const double b = std::cbrt(bvol * TILE_IMAGES / (double)img.size());
+        int nb[3];
+        for (int d = 0; d < 3; ++d) nb[d] = (int)std::max(1.0, std::min(4096.0, std::ceil(ext[d] / b)));
+        std::vector<int64_t> key(img.size());
+        for (size_t i = 0; i < img.size(); ++i) {
+            const double c[3] = {img[i].x, img[i].y, img[i].z};
+            int64_t k = 0;
+            for (int d = 0; d < 3; ++d) {
+                const int q = std::min(nb[d] - 1, std::max(0, (int)((c[d] - lo[d]) / ext[d] * nb[d])));
+                k = k * nb[d] + q;
+            }
+            key[i] = k;
+        }
+        std::vector<size_t> perm(img.size());
+        for (size_t i = 0; i < perm.size(); ++i) perm[i] = i;
+        std::stable_sort(perm.begin(), perm.end(), [&](size_t x, size_t y) {
+            return key[x] != key[y] ? key[x] < key[y] : ty[x] < ty[y];
+        });
+        std::vector<double4> img2(img.size());
+        std::vector<int32_t> ty2(ty.size());
+        for (size_t i = 0; i < perm.size(); ++i) {
+            img2[i] = img[perm[i]];
+            ty2[i] = ty[perm[i]];
+        }
+        img.swap(img2);
+        ty.swap(ty2);
+        size_t i0 = 0;
+        while (i0 < img.size()) {
+            size_t i1 = i0 + 1;
+            while (i1 < img.size() && i1 - i0 < (size_t)TILE_IMAGES && key[perm[i1]] == key[perm[i0]]) ++i1;
+            double bl[3] = {img[i0].x, img[i0].y, img[i0].z}, bh[3] = {bl[0], bl[1], bl[2]};
+            for (size_t i = i0; i < i1; ++i) {
+                const double c[3] = {img[i].x, img[i].y, img[i].z};
+                for (int d = 0; d < 3; ++d) {
+                    bl[d] = std::min(bl[d], c[d]);
+                    bh[d] = std::max(bh[d], c[d]);
+                }
+            }
+            const double cx = 0.5 * (bl[0] + bh[0]), cy = 0.5 * (bl[1] + bh[1]), cz = 0.5 * (bl[2] + bh[2]);
+            double r2 = 0.0;
+            for (size_t i = i0; i < i1; ++i) {
+                const double dx = img[i].x - cx, dy = img[i].y - cy, dz = img[i].z - cz;
+                r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+            }
+            // slack for the kernel's rounding of the sphere test
+            sph.push_back(make_double4(cx, cy, cz, std::sqrt(r2) * (1.0 + 1e-12) + 1e-9));
+            rng.push_back(make_int2((int)i0, (int)(i1 - i0)));
+            i0 = i1;
+        }
+    }
     pm_framework::Images im;
     im.r = r;
     im.n = (int64_t)img.size();
+    im.n_tiles = (int64_t)sph.size();
     if (im.n > (int64_t)INT32_MAX) return fail(PM_ERR_ARG, "image list too large");
     const size_t n = std::max<size_t>(img.size(), 1);
+    const size_t nt = std::max<size_t>(sph.size(), 1);
     PM_HIP(hipMalloc(&im.d_img, n * sizeof(double4)));
     hipError_t e = hipMalloc(&im.d_type, n * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&im.d_tile_sph, nt * sizeof(double4));
+    if (e == hipSuccess) e = hipMalloc(&im.d_tile_rng, nt * sizeof(int2));
     if (e == hipSuccess && !img.empty()) {
         e = hipMemcpy(im.d_img, img.data(), img.size() * sizeof(double4), hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(im.d_type, ty.data(), ty.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(im.d_tile_sph, sph.data(), sph.size() * sizeof(double4), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(im.d_tile_rng, rng.data(), rng.size() * sizeof(int2), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         (void)hipFree(im.d_img);
         (void)hipFree(im.d_type);
+        (void)hipFree(im.d_tile_sph);
+        (void)hipFree(im.d_tile_rng);
         return fail(PM_ERR_DEVICE, std::string("image list upload: ") + hipGetErrorString(e));
     }
     fw->images.push_back(im);
@@ -671,6 +757,9 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         a.n_img = 0;
         a.img = nullptr;
         a.img_type = nullptr;
+        a.tile_sph = nullptr;
+        a.tile_rng = nullptr;
+        a.n_tiles = 0;
         if (a.chunk_r <= g_ctx.chunk_r_max) {
             const pm_framework::Images* im = nullptr;
             const int rc = get_images(r.fw, a.r_cmax + 0.01, &im);
@@ -679,6 +768,9 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
                 a.img = im->d_img;
                 a.img_type = im->d_type;
                 a.n_img = (int)im->n;
+                a.tile_sph = im->d_tile_sph;
+                a.tile_rng = im->d_tile_rng;
+                a.n_tiles = (int)im->n_tiles;
             } else if (rc != PM_ERR_ARG) {
                 return rc;
             }
@@ -1187,6 +1279,8 @@ int pm_framework_destroy(pm_framework* fw) {
     for (auto& im : fw->images) {
         (void)hipFree(im.d_img);
         (void)hipFree(im.d_type);
+        (void)hipFree(im.d_tile_sph);
+        (void)hipFree(im.d_tile_rng);
     }
     delete fw;
     return PM_OK;
@@ -1737,7 +1831,7 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
         std::vector<PointRun> probe_run(1);
         if (int rc = widom_prepare(in, probe_run[0])) return rc;  // validates the arguments
     }
-    const double cps[] = {192.0, 256.0, 384.0, 512.0, 640.0, 896.0, 1280.0};
+    const double cps[] = {64.0, 96.0, 128.0, 192.0, 256.0, 384.0, 512.0, 640.0, 896.0, 1280.0};
     const double crs[] = {3.0, 4.5, 6.0};
     const double keep_cp = g_ctx.cell_points, keep_cr = g_ctx.chunk_r_max;
     double best_ms = INFINITY, best_cp = keep_cp, best_cr = keep_cr;
