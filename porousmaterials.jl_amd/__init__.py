@@ -19,7 +19,8 @@ from .forcefield import LJForceField, forcefield_coverage, lennard_jones, replic
 from .energy_min import find_energy_minima, find_energy_minimum, find_energy_minimum_gridsearch
 from .grid import (Grid, box_isapprox, energy_grid, energy_grids, id_to_xf, read_cube, required_n_pts,
                    write_cube, xf_to_id)
-from .gcmc import adsorption_isotherm, muVT_sim, stepwise_adsorption_isotherm, μVT_sim
+from .gcmc import (adsorption_isotherm, isotherm_sim_results_to_dataframe, muVT_sim, stepwise_adsorption_isotherm,
+                   μVT_output_filename, μVT_sim)
 from .henry import (K_TO_KJ_PER_MOL, N_BLOCKS, UNIV_GAS_CONST, block_statistics, henry_coefficient,
                     henry_coefficient_gradient, henry_coefficients,
                     henry_result_savename)

@@ -31,6 +31,7 @@ import warnings
 import numpy as np
 
 from . import device
+from .box import Box
 from .crystal import ATOMIC_MASSES, assert_P1_symmetry, has_charges as xtal_has_charges, replicate
 from .eos import PengRobinsonFluid, calculate_properties
 from .forcefield import replication_factors
@@ -164,10 +165,58 @@ def _jsonable(v):
     if isinstance(v, np.ndarray):
         return v.tolist()
     if isinstance(v, Grid):
-        return {"n_pts": list(v.n_pts), "units": v.units, "data": np.asarray(v.data).ravel(order="F").tolist()}
+        return {"grid": True, "f_to_c": np.asarray(v.box.f_to_c).tolist(), "n_pts": list(v.n_pts), "units": v.units,
+                "data": np.asarray(v.data).ravel(order="F").tolist(), "origin": np.asarray(v.origin).tolist()}
     if isinstance(v, (np.floating, np.integer)):
         return v.item()
     return v
+
+
+def _from_json(key, v):
+    """Inverse of ``_jsonable`` for a saved results dict."""
+    if isinstance(v, dict) and v.get("grid"):
+        return Grid(Box(np.array(v["f_to_c"])), tuple(v["n_pts"]),
+                    np.array(v["data"], dtype=np.float64).reshape(tuple(v["n_pts"]), order="F"), v["units"],
+                    np.array(v["origin"], dtype=np.float64))
+    if key == "repfactors":
+        return tuple(v)
+    if isinstance(v, list) and all(isinstance(x, (int, float)) for x in v) and key not in ("pressure (bar)",
+                                                                                           "fugacity (bar)"):
+        return np.array(v)
+    return v
+
+
+def save_results(results, path):
+    """Write a μVT_sim results dict as JSON (the reference writes JLD2)."""
+    with open(path, "w") as f:
+        json.dump({k: _jsonable(v) for k, v in results.items()}, f, indent=1, ensure_ascii=False)
+
+
+def load_results(path):
+    """Read a results dict written by ``save_results``."""
+    with open(path) as f:
+        return {k: _from_json(k, v) for k, v in json.load(f).items()}
+
+
+def isotherm_sim_results_to_dataframe(desired_props, xtal, molecule_templates, temperature, pressures, ljff,
+                                      n_burn_cycles, n_sample_cycles, comment="", where_are_jld_files=None):
+    """``src/gcmc.jl:1081-1128``: a pandas DataFrame with one row per pressure
+    set and the ``desired_props`` of each saved μVT_sim results dict (found by
+    ``μVT_output_filename`` under ``PATHS["simulations"]`` or
+    ``where_are_jld_files``)."""
+    import pandas as pd
+
+    if isinstance(molecule_templates, Molecule):
+        molecule_templates = [molecule_templates]
+        pressures = [[float(p)] for p in pressures]
+    where = PATHS["simulations"] if where_are_jld_files is None else where_are_jld_files
+    rows = []
+    for p in pressures:
+        name = μVT_output_filename(xtal, molecule_templates, temperature, list(p), ljff, n_burn_cycles,
+                                   n_sample_cycles, comment=comment)
+        res = load_results(os.path.join(where, name))
+        rows.append({prop: res[prop] for prop in desired_props})
+    return pd.DataFrame(rows, columns=list(desired_props))
 
 
 def _run(xtal, templates, temperature, pressure_sets, ljff, molecule_sets, n_cycles_per_volume,
@@ -271,8 +320,7 @@ def _run(xtal, templates, temperature, pressure_sets, ljff, molecule_sets, n_cyc
             name = os.path.join(PATHS["simulations"],
                                 μVT_output_filename(xtal, templates, temperature, pressures, ljff, n_burn, n_sample,
                                                     comment=results_filename_comment))
-            with open(name, "w") as f:
-                json.dump({k: _jsonable(v) for k, v in res.items()}, f, indent=1, ensure_ascii=False)
+            save_results(res, name)
             if verbose:
                 print("\tresults dictionary saved in ", name)
         out.append((res, mols))

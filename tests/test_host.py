@@ -314,3 +314,32 @@ def test_gcmc_molecules_carry_charges(pm):
         m = pm.gcmc._device_molecule(tmpl, (box.c_to_f @ sites).T, box.c_to_f @ c, box)
         assert np.allclose(m.atoms_x, sites, atol=1e-12) and np.allclose(m.charges_x, qx, atol=1e-12)
         assert np.allclose(m.com, c, atol=1e-12) and list(m.charges_q) == [-0.8, 0.4, 0.4]
+
+
+def test_gcmc_results_round_trip_and_dataframe(pm, tmp_path):
+    """μVT_sim's saved results read back with their types (the density grid a
+    Grid, repfactors a tuple) and `isotherm_sim_results_to_dataframe` picks
+    the reference's columns per pressure (test/gcmc_quick.jl)."""
+    import math
+    box = pm.Box(10.0, 11.0, 12.0, math.pi / 2, math.pi / 2, math.pi / 2)
+    xtal = pm.Crystal("SBMOF-1.cif", box, ["Ca"], np.zeros((3, 1)))
+    xe = pm.Molecule.from_dir(golden("molecules", "Xe"))
+    ff = pm.LJForceField.from_csv(golden("forcefields", "UFF.csv"))
+    pressures = [0.01, 0.1, 1.0]
+    for i, p in enumerate(pressures):
+        grid = pm.Grid(box, (2, 3, 4), np.arange(24.0).reshape((2, 3, 4), order="F") * p, "inverse_A3", np.zeros(3))
+        res = {"xtal": xtal.name, "adsorbate": ["Xe"], "pressure (bar)": [p], "repfactors": (3, 6, 2),
+               "⟨N⟩ (mmol/g)": np.array([1.0 + i]), "density grid": grid}
+        pm.gcmc.save_results(res, str(tmp_path / pm.μVT_output_filename(xtal, [xe], 298.0, [p], ff, 50, 60)))
+    back = pm.gcmc.load_results(str(tmp_path / pm.μVT_output_filename(xtal, [xe], 298.0, [1.0], ff, 50, 60)))
+    assert back["repfactors"] == (3, 6, 2) and back["pressure (bar)"] == [1.0]
+    assert isinstance(back["density grid"], pm.Grid) and back["density grid"].n_pts == (2, 3, 4)
+    assert np.array_equal(back["density grid"].data, np.arange(24.0).reshape((2, 3, 4), order="F"))
+    props = ["xtal", "adsorbate", "pressure (bar)", "repfactors", "⟨N⟩ (mmol/g)", "density grid"]
+    df = pm.isotherm_sim_results_to_dataframe(props, xtal, xe, 298.0, pressures, ff, 50, 60,
+                                              where_are_jld_files=str(tmp_path))
+    assert list(df.columns) == props and len(df) == 3
+    assert all(df["pressure (bar)"][i][0] == pytest.approx(pressures[i]) for i in range(3))
+    assert [float(v[0]) for v in df["⟨N⟩ (mmol/g)"]] == [1.0, 2.0, 3.0]
+    assert all(df["xtal"] == "SBMOF-1.cif") and all(isinstance(g, pm.Grid) for g in df["density grid"])
+    assert all(r == (3, 6, 2) for r in df["repfactors"])
