@@ -802,7 +802,9 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
     const double2 Eb = make_double2(c, -s);
     sincospi(2.0 * t2, &s, &c);
     const double zr = c, zi = -s;  // z_c
-    const int Ma = a.Ma, Mb = a.Mb, Mc = a.Mc;
+    // wave-uniform: scalar loop control (the loads are not provably uniform)
+    const int Ma = __builtin_amdgcn_readfirstlane(a.Ma), Mb = __builtin_amdgcn_readfirstlane(a.Mb),
+              Mc = __builtin_amdgcn_readfirstlane(a.Mc);
     double2 Eb_neg = make_double2(1.0, 0.0);
     const double2 Eb_c = make_double2(Eb.x, -Eb.y);
     for (int q = 0; q < Mb; ++q) Eb_neg = cmul(Eb_neg, Eb_c);
@@ -824,10 +826,12 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
                 Pa = cmul(Pa, Ea);
                 Pb = cmul(Pa, Eb_neg);
             }
-            const int2 lh = a.krows[r];
+            const int2 lh0 = a.krows[r];
+            const int2 lh = make_int2(__builtin_amdgcn_readfirstlane(lh0.x), __builtin_amdgcn_readfirstlane(lh0.y));
             if (lh.x <= lh.y) {
                 const double2* row = s_a + (r - r0) * rowlen + Mc;  // row[mc], mc in [-Mc, Mc]
                 double qr = 0.0, qi = 0.0;                           // sum_{mc >= 0} A_mc z^mc
+#pragma unroll 4
                 for (int mc = lh.y; mc >= 0; --mc) {
                     const double2 A = row[mc];
                     const double nr = fma(qr, zr, fma(-qi, zi, A.x));
@@ -835,6 +839,7 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
                     qr = nr;
                 }
                 double nr_ = 0.0, ni_ = 0.0;                         // sum_{j >= 1} A_{-j} (z*)^{j-1}
+#pragma unroll 4
                 for (int j = -lh.x; j >= 1; --j) {
                     const double2 A = row[-j];
                     const double tr = fma(nr_, zr, fma(ni_, zi, A.x));
@@ -1009,7 +1014,9 @@ __global__ __launch_bounds__(WG) void point_fourier_grad_kernel(GroupArgs g) {
     const double2 Eb = make_double2(c, -s);
     sincospi(2.0 * t2, &s, &c);
     const double2 Ec = make_double2(c, -s);
-    const int Ma = a.Ma, Mb = a.Mb, Mc = a.Mc;
+    // wave-uniform: scalar loop control (the loads are not provably uniform)
+    const int Ma = __builtin_amdgcn_readfirstlane(a.Ma), Mb = __builtin_amdgcn_readfirstlane(a.Mb),
+              Mc = __builtin_amdgcn_readfirstlane(a.Mc);
     double2 Eb_neg = make_double2(1.0, 0.0), Ec_neg = make_double2(1.0, 0.0);
     const double2 Eb_c = make_double2(Eb.x, -Eb.y), Ec_c = make_double2(Ec.x, -Ec.y);
     for (int q = 0; q < Mb; ++q) Eb_neg = cmul(Eb_neg, Eb_c);
