@@ -110,7 +110,7 @@ struct GroupArgs {
     const SegArgs* segs;
     int n_seg, mode;
     double4* pos;            // Widom: per slot, fractional xyz in [0,1)^3, w = 1 (valid) or 0
-    unsigned* cell;          // Widom: per slot, cell in the group or 0xffffffff
+    uint2* cell;             // Widom: per slot, (cell in the group or 0xffffffff, rank within the cell)
     unsigned* count;         // per cell
     const unsigned* point_off;
     const uint4* items;      // (cell, slice, segment, 0)
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     if (!emit) {
         for (int s = 0; s < ns; ++s) {
             g.pos[p0 + s] = make_double4(0.0, 0.0, 0.0, 0.0);
-            g.cell[p0 + s] = 0xffffffffu;
+            g.cell[p0 + s] = make_uint2(0xffffffffu, 0u);
         }
         return;
     }
@@ -229,8 +229,9 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
         const int ic = min((int)(z * n2), n2 - 1);
         const unsigned cid = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
         g.pos[p0 + s] = make_double4(x, y, z, 1.0);
-        g.cell[p0 + s] = cid;
-        atomicAdd(&g.count[cid], 1u);
+        // the count before this point is its rank in the cell (the order within
+        // a cell changes no point's energy)
+        g.cell[p0 + s] = make_uint2(cid, atomicAdd(&g.count[cid], 1u));
     }
 }
 
@@ -332,14 +333,13 @@ __global__ __launch_bounds__(SCAN_WG) void scan_down_kernel(const unsigned* coun
     }
 }
 
-__global__ void scatter_points_kernel(const unsigned* cell, long long n_pts, const unsigned* point_off,
-                                      unsigned* cursor, unsigned* order) {
+__global__ void scatter_points_kernel(const uint2* cell, long long n_pts, const unsigned* point_off,
+                                      unsigned* order) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pts) return;
-    unsigned c = cell[p];
-    if (c == 0xffffffffu) return;
-    unsigned slot = point_off[c] + atomicAdd(&cursor[c], 1u);
-    order[slot] = (unsigned)p;
+    const uint2 c = cell[p];
+    if (c.x == 0xffffffffu) return;
+    order[point_off[c.x] + c.y] = (unsigned)p;
 }
 
 __global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_off, const SegArgs* segs,

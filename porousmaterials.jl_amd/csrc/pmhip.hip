@@ -887,14 +887,13 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const size_t o_sums = ar.take(widom ? (size_t)blocks_total * 3 * sizeof(double) : 0);
     const size_t o_grid = ar.take(widom ? 0 : (size_t)grid_total);
     const size_t o_pos = ar.take(widom ? (size_t)max_slots * sizeof(double4) : 0);
-    const size_t o_cell = ar.take(widom ? (size_t)max_slots * sizeof(unsigned) : 0);
+    const size_t o_cell = ar.take(widom ? (size_t)max_slots * sizeof(uint2) : 0);
     const size_t o_order = ar.take(widom ? (size_t)max_slots * sizeof(unsigned) : 0);
     const size_t o_lj = ar.take((size_t)max_slots * sizeof(double));
     const size_t o_sr = ar.take((size_t)max_slots * sizeof(double));
     const size_t o_lr = ar.take((size_t)max_slots * sizeof(double));
     const long long max_spans = max_cells / SCAN_SPAN + 1;
     const size_t o_count = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
-    const size_t o_cursor = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
     const size_t o_poff = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
     const size_t o_soff = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
     const size_t o_span = ar.take(widom ? (size_t)max_spans * 2 * sizeof(unsigned) : 0);
@@ -942,7 +941,6 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const RunArgs* d_runs = reinterpret_cast<const RunArgs*>(base + o_runs);
     const SegArgs* d_segs = reinterpret_cast<const SegArgs*>(base + o_segs);
     unsigned* d_count = reinterpret_cast<unsigned*>(base + o_count);
-    unsigned* d_cursor = reinterpret_cast<unsigned*>(base + o_cursor);
     unsigned* d_poff = reinterpret_cast<unsigned*>(base + o_poff);
     unsigned* d_soff = reinterpret_cast<unsigned*>(base + o_soff);
     unsigned* d_span_p = reinterpret_cast<unsigned*>(base + o_span);
@@ -955,7 +953,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     g.runs = d_runs;
     g.mode = mode;
     g.pos = reinterpret_cast<double4*>(base + o_pos);
-    g.cell = reinterpret_cast<unsigned*>(base + o_cell);
+    g.cell = reinterpret_cast<uint2*>(base + o_cell);
     g.count = d_count;
     g.point_off = d_poff;
     g.items = reinterpret_cast<const uint4*>(base + o_items);
@@ -981,8 +979,6 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             const long long n_spans = (n_cells + SCAN_SPAN - 1) / SCAN_SPAN;
             hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_cells + 1, 256)), dim3(256), 0, g_ctx.stream, d_count,
                                n_cells + 1, 0u);
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_cells + 1, 256)), dim3(256), 0, g_ctx.stream, d_cursor,
-                               n_cells + 1, 0u);
             hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
             hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)n_spans), dim3(SCAN_WG), 0, g_ctx.stream, d_count,
                                (int)n_cells, d_span_p, d_span_s);
@@ -991,7 +987,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)n_spans), dim3(SCAN_WG), 0, g_ctx.stream, d_count,
                                (int)n_cells, d_span_p, d_span_s, d_poff, d_soff);
             hipLaunchKernelGGL(scatter_points_kernel, dim3(grid1d(gr.n_slots, 256)), dim3(256), 0, g_ctx.stream,
-                               g.cell, gr.n_slots, d_poff, d_cursor, const_cast<unsigned*>(g.order));
+                               g.cell, gr.n_slots, d_poff, const_cast<unsigned*>(g.order));
             hipLaunchKernelGGL(slice_items_kernel, dim3(grid1d(n_cells, 256)), dim3(256), 0, g_ctx.stream, d_count,
                                d_soff, g.segs, gr.n_seg, const_cast<uint4*>(g.items), (int)n_cells);
             pair_wg = n_cells + (gr.n_slots + WG - 1) / WG;  // >= the slices the scan counts
