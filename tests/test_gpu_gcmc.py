@@ -218,3 +218,80 @@ def test_gcmc_molecule_store_placement_is_invisible(pm, orc):
             oc.run(nb + ns)
         _compare(runs[4][0], och, len(tmpl), label + " (4-molecule LDS store)")
         say(f"   {label}: N = {[list(s['n'][:len(tmpl)]) for s, _ in base]}, placements agree bit for bit")
+
+
+def _gg_box(pm, length, ff, templates):
+    """An empty cubic box and a GCMC handle whose from-scratch energy
+    (pm_gcmc_total_energy) is the guest-guest energy of the molecules set."""
+    box = pm.Box(length, length, length, math.pi / 2, math.pi / 2, math.pi / 2)
+    x = pm.Crystal("empty", box, [], np.zeros((3, 0)))
+    dev = pm.device.GcmcChains(pm.device.framework(x, ff), ff, templates, [298.0], [[1.0] * len(templates)], 0, 5,
+                               capacity=8)
+    return box, dev
+
+
+def _placed(box, tmpl, cart_coms):
+    """translate_to! of unrotated templates: sites (n, n_sites, 3) and centres
+    (n, 3), fractional."""
+    off = tmpl.atoms_x - tmpl.com[:, None]
+    sites = np.array([(box.c_to_f @ (np.asarray(c, dtype=float)[:, None] + off)).T for c in cart_coms])
+    coms = np.array([box.c_to_f @ np.asarray(c, dtype=float) for c in cart_coms])
+    return sites, coms
+
+
+def test_gcmc_guest_guest_energy_known_answers(pm):
+    """The device's guest-guest energy against test/vdw_energetics.jl:97-227:
+    Xe-He at 6 A directly and through the periodic boundary, a molecule on top
+    of another (+Inf), a three-axis periodic pair, the cutoff, two CO2 6 A
+    apart (directly and through the boundary), CO2 pairs with only some sites
+    inside the cutoff, and a Xe/Xe/Kr mixture.  The total counts every guest
+    pair once."""
+    say("case: guest-guest energies of Xe/He, CO2/CO2 and Xe/Xe/Kr in empty boxes (the reference's closed forms)")
+    dre = pm.LJForceField.from_csv(golden("forcefields", "Dreiding.csv"), r_cutoff=12.5)
+    xe, he = pm.lj_sphere("Xe"), pm.lj_sphere("He")
+    co2 = pm.Molecule.from_dir(golden("molecules", "CO2"))
+
+    def lj(ff, r2, a, b):
+        i, j = ff.index(a), ff.index(b)
+        return pm.lennard_jones(r2, ff.sigma2_table[i, j], ff.epsilon_table[i, j])
+
+    def gg(dev):
+        gh, g = dev.total_energy(0)
+        assert gh == 0.0
+        return g
+
+    box, dev = _gg_box(pm, 25.0, dre, [xe, he])
+    for a, b, r2 in (([5.0, 12.0, 12.0], [11.0, 12.0, 12.0], 36.0), ([5.0, 12.0, 12.0], [24.0, 12.0, 12.0], 36.0),
+                     ([24.0, 23.0, 11.0], [22.0, 2.0, 12.0], 21.0), ([11.0, 1.0, 12.0], [11.0, 4.0, 12.0], 9.0),
+                     ([11.0, 1.0, 12.0], [11.0, 23.0, 12.0], 9.0)):
+        dev.set_molecules(0, 0, *_placed(box, xe, [a]))
+        dev.set_molecules(0, 1, *_placed(box, he, [b]))
+        assert gg(dev) == pytest.approx(lj(dre, r2, "Xe", "He"), rel=1e-12)
+    dev.set_molecules(0, 0, *_placed(box, xe, [[0.0, 0.0, 0.0]]))
+    dev.set_molecules(0, 1, *_placed(box, he, [[12.0, 12.0, 12.0]]))
+    assert gg(dev) == 0.0  # beyond the cutoff
+    dev.set_molecules(0, 0, *_placed(box, xe, [[5.0, 12.0, 12.0], [5.0, 12.0, 12.0]]))
+    dev.set_molecules(0, 1, *_placed(box, he, [[24.0, 12.0, 12.0]]))
+    assert gg(dev) == math.inf  # a molecule on top of another
+
+    box, dev = _gg_box(pm, 25.0, dre, [co2])
+    r2_co, r2_oo = 1.16 ** 2, (2.0 * 1.16) ** 2
+    for a, b in (([12.0, 9.0, 12.0], [12.0, 15.0, 12.0]), ([12.0, 2.0, 12.0], [12.0, 21.0, 12.0])):
+        dev.set_molecules(0, 0, *_placed(box, co2, [a, b]))
+        want = (2.0 * lj(dre, 36.0, "O_CO2", "O_CO2") + 4.0 * lj(dre, 36.0 + r2_co, "O_CO2", "C_CO2")
+                + 2.0 * lj(dre, 36.0 + r2_oo, "O_CO2", "O_CO2") + lj(dre, 36.0, "C_CO2", "C_CO2"))
+        assert gg(dev) == pytest.approx(want, rel=1e-12)
+    box, dev = _gg_box(pm, 50.0, dre, [co2])
+    dev.set_molecules(0, 0, *_placed(box, co2, [[0.0, 0.0, 0.0], [13.0, 0.0, 0.0]]))
+    want = lj(dre, (13.0 - 2.0 * 1.16) ** 2, "O_CO2", "O_CO2") + 2.0 * lj(dre, (13.0 - 1.16) ** 2, "O_CO2", "C_CO2")
+    assert gg(dev) == pytest.approx(want, rel=1e-12)
+
+    uff = pm.LJForceField.from_csv(golden("forcefields", "UFF.csv"), r_cutoff=6.0)
+    box, dev = _gg_box(pm, 10.0, uff, [xe, pm.lj_sphere("Kr")])
+    dev.set_molecules(0, 0, *_placed(box, xe, [[1.0, 1.0, 1.0], [1.0, 5.0, 1.0]]))
+    dev.set_molecules(0, 1, *_placed(box, pm.lj_sphere("Kr"), [[6.0, 1.0, 1.0]]))
+    # Xe1-Xe2 at 4 A, Xe1-Kr at 5 A, Xe2-Kr at sqrt(41) A > r_c
+    assert gg(dev) == pytest.approx(lj(uff, 16.0, "Xe", "Xe") + lj(uff, 25.0, "Xe", "Kr"), rel=1e-12)
+    dev.set_molecules(0, 0, *_placed(box, xe, [[1.0, 1.0, 1.0]]))
+    dev.set_molecules(0, 1, np.zeros((0, 1, 3)), np.zeros((0, 3)))
+    assert gg(dev) == 0.0
