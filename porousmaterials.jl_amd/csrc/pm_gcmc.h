@@ -37,6 +37,11 @@ struct GcmcParams {
     double rep[3];           // replication of the unit cell (density grid over the unit cell)
     int n_atoms;             // replicated framework atoms
     const double4* atoms;    // fractional xyz + compact framework type in w
+    // cell lists: the atoms within r_c + the cell's half diagonal of each cell
+    // of an n_cell[0] x n_cell[1] x n_cell[2] grid over the box (null: none)
+    int n_cell[3];
+    const int* cell_start;   // [cells + 1]
+    const int* cell_atoms;   // atom indices, cell after cell
     int n_species, n_ads_types, capacity;
     int n_sites[GC_NS];
     int site_prefix[GC_NS];  // sites of the species before each one
@@ -240,7 +245,8 @@ __device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams
                                                                 const double4* atoms, const double2* fg,
                                                                 const double2* ggt, const int* n,
                                                                 const double4* trial, int ts, int ex_s, int ex_m,
-                                                                double* red, int& par, double& gh, double& gg) {
+                                                                bool cells, double* red, int& par, double& gh,
+                                                                double& gg) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ns = p.n_sites[ts];
     const int na = p.n_ads_types;
@@ -257,6 +263,30 @@ __device__ __attribute__((always_inline)) inline void gc_energy(const GcmcParams
     for (int k = 0; k < ns; ++k) {
         const double4 x = trial[k];
         const double2* col = fg + p.site_type[ts][k];
+        if (cells) {
+            // the atoms of the site's cell list; the others are beyond r_c
+            const double w0 = x.x - floor(x.x), w1 = x.y - floor(x.y), w2 = x.z - floor(x.z);
+            const int c0 = min(p.n_cell[0] - 1, (int)(w0 * p.n_cell[0]));
+            const int c1 = min(p.n_cell[1] - 1, (int)(w1 * p.n_cell[1]));
+            const int c2 = min(p.n_cell[2] - 1, (int)(w2 * p.n_cell[2]));
+            const int cell = __builtin_amdgcn_readfirstlane(c0 + p.n_cell[0] * (c1 + p.n_cell[1] * c2));
+            const int e = p.cell_start[cell + 1];
+            int j = p.cell_start[cell] + tid;
+            for (; j + 3 * GC_WG < e; j += 4 * GC_WG) {
+                const int i0 = p.cell_atoms[j], i1 = p.cell_atoms[j + GC_WG];
+                const int i2 = p.cell_atoms[j + 2 * GC_WG], i3 = p.cell_atoms[j + 3 * GC_WG];
+                const double4 a0 = atoms[i0], a1 = atoms[i1], a2 = atoms[i2], a3 = atoms[i3];
+                gc_pair(p, x, a0, col[(int)a0.w * na], agh, ogh);
+                gc_pair(p, x, a1, col[(int)a1.w * na], agh, ogh);
+                gc_pair(p, x, a2, col[(int)a2.w * na], agh, ogh);
+                gc_pair(p, x, a3, col[(int)a3.w * na], agh, ogh);
+            }
+            for (; j < e; j += GC_WG) {
+                const double4 at = atoms[p.cell_atoms[j]];
+                gc_pair(p, x, at, col[(int)at.w * na], agh, ogh);
+            }
+            continue;
+        }
         int a = tid;
         for (; a + 3 * GC_WG < p.n_atoms; a += 4 * GC_WG) {
             const double4 a0 = atoms[a], a1 = atoms[a + GC_WG], a2 = atoms[a + 2 * GC_WG], a3 = atoms[a + 3 * GC_WG];
@@ -446,6 +476,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
 #pragma unroll
     for (int q = 0; q < GC_NS; ++q) fug[q] = ch.fug[q];
     int par = 0;
+    const bool cells_on = p.cell_start != nullptr;
     const unsigned chain = blockIdx.x;
     const double T = ch.temperature;
     int n[GC_NS];
@@ -538,7 +569,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_trial[tid] = gc_new_site(p, s, tid, com, R, rot);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, -1, -1, s_red, par, gh, gg);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, -1, -1, cells_on, s_red, par, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
                 const double acc = fug[s] * p.omega / ((double)(n_i + 1) * GC_KB * T) * exp(-dE / T);
                 if (u[9] < acc) {
@@ -552,7 +583,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_trial[tid] = mols.site(p, s, m, tid);
                 __syncthreads();
                 double gh, gg;
-                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, s, m, cells_on, s_red, par, gh, gg);
                 const double dE = (gg + 0.0) + gh + 0.0;
                 const double acc = (double)n_i * GC_KB * T / (fug[s] * p.omega) * exp(dE / T);
                 if (u[9] < acc) {
@@ -568,7 +599,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_old[tid] = mols.site(p, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_old, s, s, m, cells_on, s_red, par, gh0, gg0);
                 double4 com1 = com0;
                 {
 #pragma clang fp contract(off)
@@ -613,7 +644,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                     }
                 }
                 __syncthreads();
-                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, s, m, s_red, par, gh1, gg1);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s, s, m, cells_on, s_red, par, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
                 if (u[9] < exp(-(e_new - e_old) / T)) {
                     gc_store(p, mols, s, m, s_trial, com1);
@@ -628,7 +659,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (tid < ns) s_old[tid] = mols.site(p, s, m, tid);
                 __syncthreads();
                 double gh0, gg0, gh1, gg1;
-                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_old, s, s, m, s_red, par, gh0, gg0);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_old, s, s, m, cells_on, s_red, par, gh0, gg0);
                 gc_remove(p, mols, s, m, n_i);
                 n[s] = n_i - 1;
                 int pick = min(S - 2, (int)(u[3] * (double)(S - 1)));
@@ -643,7 +674,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_kernel(const GcmcParams* __restric
                 if (rot2) arvo_rotation(&u[6], R);
                 if (tid < p.n_sites[s2]) s_trial[tid] = gc_new_site(p, s2, tid, com0, R, rot2);
                 __syncthreads();
-                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s2, -1, -1, s_red, par, gh1, gg1);
+                gc_energy(p, mols, atoms, s_fg, s_gg, n, s_trial, s2, -1, -1, cells_on, s_red, par, gh1, gg1);
                 const double e_new = (gg1 + 0.0) + gh1 + 0.0, e_old = (gg0 + 0.0) + gh0 + 0.0;
                 const double acc = ((double)n_i * fug[s2] / ((double)(n_j + 1) * fug[s])) * exp(-(e_new - e_old) / T);
                 if (u[9] < acc) {
@@ -752,7 +783,7 @@ __global__ __launch_bounds__(GC_WG) void gcmc_total_energy_kernel(const GcmcPara
             if (threadIdx.x < p.n_sites[s]) s_trial[threadIdx.x] = mols.site(p, s, m, threadIdx.x);
             __syncthreads();
             double gh, gg;
-            gc_energy(p, mols, p.atoms, p.lj_fg, p.lj_gg, n, s_trial, s, s, m, s_red, par, gh, gg);
+            gc_energy(p, mols, p.atoms, p.lj_fg, p.lj_gg, n, s_trial, s, s, m, false, s_red, par, gh, gg);
             tgh += gh;
             tgg += gg;
         }

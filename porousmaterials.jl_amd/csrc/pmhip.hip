@@ -222,6 +222,8 @@ struct pm_gcmc {
     unsigned* d_dens = nullptr;
     double* d_tot = nullptr;
     int* d_incomplete = nullptr;  // chains a launch with molecules in LDS left unfinished
+    int* d_cell_start = nullptr;  // framework cell lists (GcmcParams::cell_start, cell_atoms)
+    int* d_cell_atoms = nullptr;
     bool mols_in_memory = false;  // a chain outgrew the LDS molecule store: launches keep them in memory
     int lds_mols_limit = -1;      // pm_gcmc_set_lds_molecules
 };
@@ -1924,6 +1926,8 @@ int pm_gcmc_destroy(pm_gcmc* gc) {
     (void)hipFree(gc->d_dens);
     (void)hipFree(gc->d_tot);
     (void)hipFree(gc->d_incomplete);
+    (void)hipFree(gc->d_cell_start);
+    (void)hipFree(gc->d_cell_atoms);
     delete gc;
     return PM_OK;
 }
@@ -2065,6 +2069,51 @@ int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sp
                                                         (fw->unit_xf[3 * i + 2] + 1.0 * rc) / fw->rep[2],
                                                         (double)fw->unit_type[i]);
     }
+    // cell lists for the guest-host sums: cells of about GC_CELL_EDGE over the
+    // box, each listing the atoms whose nearest image lies within r_c + the
+    // cell's half diagonal of its centre (so within r_c of any point of the
+    // cell); kept when they are clearly shorter than the framework
+    constexpr double GC_CELL_EDGE = 4.0;
+    std::vector<int> cell_start, cell_atoms;
+    int ncell[3] = {1, 1, 1};
+    if (fw->n_rep > 0) {
+        for (int d = 0; d < 3; ++d)
+            ncell[d] = (int)std::max(1.0, std::min(64.0, std::floor(fw->width[d] / GC_CELL_EDGE)));
+        const int nc = ncell[0] * ncell[1] * ncell[2];
+        if ((double)nc * (double)fw->n_rep <= 4e8) {
+            double ea[3], eb[3], ec[3];
+            for (int i = 0; i < 3; ++i) {
+                ea[i] = fw->F[i] / ncell[0];
+                eb[i] = fw->F[3 + i] / ncell[1];
+                ec[i] = fw->F[6 + i] / ncell[2];
+            }
+            const double R = std::sqrt(ff->r2_cutoff) + half_diagonal(ea, eb, ec) + 1e-6;
+            const double R2 = R * R;
+            const double* F = fw->F;
+            cell_start.assign((size_t)nc + 1, 0);
+            for (int c2 = 0; c2 < ncell[2]; ++c2)
+                for (int c1 = 0; c1 < ncell[1]; ++c1)
+                    for (int c0 = 0; c0 < ncell[0]; ++c0) {
+                        const double cf[3] = {(c0 + 0.5) / ncell[0], (c1 + 0.5) / ncell[1], (c2 + 0.5) / ncell[2]};
+                        for (size_t a = 0; a < atoms.size() && (int64_t)a < fw->n_rep; ++a) {
+                            double d0 = atoms[a].x - cf[0], d1 = atoms[a].y - cf[1], d2 = atoms[a].z - cf[2];
+                            d0 -= std::nearbyint(d0);
+                            d1 -= std::nearbyint(d1);
+                            d2 -= std::nearbyint(d2);
+                            const double x = F[0] * d0 + F[3] * d1 + F[6] * d2;
+                            const double y = F[4] * d1 + F[7] * d2;
+                            const double z = F[8] * d2;
+                            if (x * x + y * y + z * z <= R2) cell_atoms.push_back((int)a);
+                        }
+                        cell_start[(size_t)(c0 + ncell[0] * (c1 + ncell[1] * c2)) + 1] = (int)cell_atoms.size();
+                    }
+            if ((double)cell_atoms.size() > 0.6 * (double)nc * (double)fw->n_rep) {
+                cell_start.clear();
+                cell_atoms.clear();
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) hp.n_cell[d] = ncell[d];
     auto* gc = new pm_gcmc();
     gc->n_chains = n_chains;
     gc->n_species = n_species;
@@ -2087,11 +2136,15 @@ int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sp
         hipMalloc(&gc->d_com, sizeof(double4) * per_chain_com * n_chains) != hipSuccess ||
         hipMalloc(&gc->d_tot, sizeof(double) * 2 * n_chains) != hipSuccess ||
         hipMalloc(&gc->d_incomplete, sizeof(int)) != hipSuccess ||
+        (!cell_start.empty() && hipMalloc(&gc->d_cell_start, sizeof(int) * cell_start.size()) != hipSuccess) ||
+        (!cell_atoms.empty() && hipMalloc(&gc->d_cell_atoms, sizeof(int) * cell_atoms.size()) != hipSuccess) ||
         (n_dens > 0 && hipMalloc(&gc->d_dens, sizeof(unsigned) * (size_t)n_dens * n_chains) != hipSuccess)) {
         lk.unlock();
         return bail(PM_ERR_OOM, "pm_gcmc_create: device allocation failed");
     }
     hp.atoms = gc->d_atoms;
+    hp.cell_start = cell_atoms.empty() ? nullptr : gc->d_cell_start;
+    hp.cell_atoms = cell_atoms.empty() ? nullptr : gc->d_cell_atoms;
     hp.lj_fg = gc->d_fg;
     hp.lj_gg = gc->d_gg;
     gc->hp = hp;
@@ -2111,6 +2164,11 @@ int pm_gcmc_create(const pm_framework* fw, const pm_forcefield* ff, int32_t n_sp
         hipMemcpy(gc->d_atoms, atoms.data(), sizeof(double4) * atoms.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(gc->d_fg, fg.data(), sizeof(double2) * fg.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(gc->d_gg, gg.data(), sizeof(double2) * gg.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (!cell_atoms.empty() &&
+         (hipMemcpy(gc->d_cell_start, cell_start.data(), sizeof(int) * cell_start.size(), hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          hipMemcpy(gc->d_cell_atoms, cell_atoms.data(), sizeof(int) * cell_atoms.size(), hipMemcpyHostToDevice) !=
+              hipSuccess)) ||
         (n_dens > 0 && hipMemset(gc->d_dens, 0, sizeof(unsigned) * (size_t)n_dens * n_chains) != hipSuccess)) {
         lk.unlock();
         return bail(PM_ERR_DEVICE, "pm_gcmc_create: host-to-device copy failed");
