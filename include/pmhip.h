@@ -286,9 +286,9 @@ int pm_energy_grid_batch(int32_t n_runs, const pm_grid_run* runs, int32_t precis
  * Point-engine tuning (no counterpart in the reference, whose CPU loops have
  * no tiling).  The chunk shapes only schedule work: every energy is a
  * function of the point and its run alone, so no setting changes a result.
- *   PM_TUNE_CELL_POINTS  target points per Widom cell (default 640)
+ *   PM_TUNE_CELL_POINTS  target points per Widom cell (default 192)
  *   PM_TUNE_CHUNK_R_MAX  largest chunk radius (A) served by the image-list
- *                        path; wider chunks take the all-atoms path (default 3)
+ *                        path; wider chunks take the all-atoms path (default 6)
  * Settings are process-wide.
  */
 enum { PM_TUNE_CELL_POINTS = 0, PM_TUNE_CHUNK_R_MAX = 1 };
@@ -298,7 +298,8 @@ int pm_get_tuning(int32_t key, double* value);
 /*
  * Autotuner: runs the given Widom run (pm_widom's arguments) on the device for
  * a grid of (cell points, chunk radius) settings, keeps the fastest as the
- * process-wide setting and reports it and its device time (ms).
+ * process-wide setting (a narrower chunk-radius limit only when it is more
+ * than 1% faster) and reports it and its device time (ms).
  */
 int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, const pm_probe* pr,
                       double temperature, uint64_t seed, int32_t n_blocks, int64_t n_per_block, int32_t precision,

@@ -27,8 +27,8 @@ namespace {
 
 // Point-engine tunables.
 // Point-engine tunables (defaults; pm_set_tuning / pm_autotune_widom change them).
-constexpr double DEFAULT_CHUNK_R_MAX = 3.0;   // A: largest chunk radius of the image-list path
-constexpr double DEFAULT_CELL_POINTS = 640.0; // target points per Widom cell
+constexpr double DEFAULT_CHUNK_R_MAX = 6.0;   // A: largest chunk radius of the image-list path
+constexpr double DEFAULT_CELL_POINTS = 192.0; // target points per Widom cell
 constexpr long long MAX_POINT_SLOTS = 1LL << 24;  // point slots per launch group
 constexpr long long MAX_IMAGES = 1LL << 26;       // images per list
 constexpr size_t MAX_IMAGE_LISTS = 8;             // image lists (radii) per framework
@@ -1830,7 +1830,10 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
         if (int rc = widom_prepare(in, probe_run[0])) return rc;  // validates the arguments
     }
     const double cps[] = {64.0, 96.0, 128.0, 192.0, 256.0, 384.0, 512.0, 640.0, 896.0, 1280.0};
-    const double crs[] = {3.0, 4.5, 6.0};
+    // the widest radius limit first: a narrower one must be more than 1% faster
+    // to be kept, since a wider limit keeps sparser runs (other frameworks,
+    // fewer insertions) on the image-list path
+    const double crs[] = {6.0, 4.5, 3.0};
     const double keep_cp = g_ctx.cell_points, keep_cr = g_ctx.chunk_r_max;
     double best_ms = INFINITY, best_cp = keep_cp, best_cr = keep_cr;
     std::vector<double> sums((size_t)n_blocks * 3);
@@ -1850,7 +1853,7 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
                 }
                 ms = std::min(ms, g_ctx.last_ms);
             }
-            if (ms < best_ms) {
+            if (ms < (cr == best_cr ? 1.0 : 0.99) * best_ms) {
                 best_ms = ms;
                 best_cp = cp;
                 best_cr = cr;
