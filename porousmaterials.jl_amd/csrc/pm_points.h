@@ -575,6 +575,9 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
 #pragma unroll
                     for (int gi = 0; gi < WG / 64; ++gi) {
                         Real plj = (Real)0, psr = (Real)0;
+                        // two candidates per iteration share the LDS address registers (FP32 energies)
+                        constexpr int PAIR_UNROLL = (sizeof(Real) == 4 && !GRAD) ? 2 : 1;
+#pragma unroll PAIR_UNROLL
                         for (int j = woff[gi]; j < woff[gi + 1]; ++j) {
                             const double4 c = L.c[j];
                             const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
