@@ -759,6 +759,21 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
 }
 
 // -------------------------------------------------------------- Fourier ----
+// A pointer into read-only device memory (the constant address space: the
+// kernel never writes what it points to) that the compiler may treat as
+// uniform across the wave (it is: every lane holds the same value), so that
+// loads through uniform indices become scalar loads.
+template <typename T>
+using gconst_ptr = const T __attribute__((address_space(4)))*;
+
+template <typename T>
+__device__ inline gconst_ptr<T> uniform_ptr(const T* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<gconst_ptr<T>>(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ inline double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -767,11 +782,12 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 // fractional), FP64.  With z_a = e^{-i 2 pi u_a}: for every (ma, mb) row the
 // sum over the row's kept mc range is evaluated as two polynomials by Horner,
 // sum_{mc >= 0} A_mc z_c^mc and z_c^* sum_{j >= 1} A_{-j} (z_c^*)^{j-1}, and
-// multiplied by z_a^ma z_b^mb, carried along the rows by recurrence.  Point
-// slots of a segment start at a multiple of WG, so a workgroup lies in one
-// segment.
+// multiplied by z_a^ma z_b^mb, carried along the rows by recurrence.  The
+// coefficients are the same for every lane: they are read by scalar loads
+// (through the scalar cache, into SGPR operands of the FMAs) rather than
+// broadcast from LDS, which would move 64 copies of each one.  Point slots of
+// a segment start at a multiple of WG, so a workgroup lies in one segment.
 __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
-    __shared__ double2 s_a[FT];
     const int tid = threadIdx.x;
     const long long p0 = (long long)blockIdx.x * WG;
     const SegArgs sg = g.segs[find_segment<SEG_BY_SLOT>(g.segs, g.n_seg, p0)];
@@ -805,6 +821,9 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
     const double2 Eb = make_double2(c, -s);
     sincospi(2.0 * t2, &s, &c);
     const double zr = c, zi = -s;  // z_c
+    double nzr = -zr, nzi = -zi;   // opaque to the optimiser, which would fold -q * -z back to q * z
+    asm volatile("" : "+v"(nzr));
+    asm volatile("" : "+v"(nzi));
     // wave-uniform: scalar loop control (the loads are not provably uniform)
     const int Ma = __builtin_amdgcn_readfirstlane(a.Ma), Mb = __builtin_amdgcn_readfirstlane(a.Mb),
               Mc = __builtin_amdgcn_readfirstlane(a.Mc);
@@ -814,48 +833,47 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
 
     const int rowlen = 2 * Mc + 1, nb = 2 * Mb + 1;
     const int n_rows = (Ma + 1) * nb;
-    const int rows_per_tile = max(1, FT / rowlen);
+    // (plain scalars through these pointers: the vector types have no copy
+    // from that address space)
+    const gconst_ptr<double> Abox = uniform_ptr(reinterpret_cast<const double*>(a.A));
+    const gconst_ptr<int> krows = uniform_ptr(reinterpret_cast<const int*>(a.krows));
     double2 Pa = make_double2(1.0, 0.0);
     double2 Pb = Eb_neg;  // z_a^ma z_b^mb of the current row
     double acc = 0.0;
-    for (int r0 = 0; r0 < n_rows; r0 += rows_per_tile) {
-        const int r1 = min(n_rows, r0 + rows_per_tile);
-        const int n_ent = (r1 - r0) * rowlen;
-        __syncthreads();
-        for (int e = tid; e < n_ent; e += WG) s_a[e] = a.A[(long long)r0 * rowlen + e];
-        __syncthreads();
-        for (int r = r0; r < r1; ++r) {
-            if (r > 0 && (r % nb) == 0) {  // next m_a
-                Pa = cmul(Pa, Ea);
-                Pb = cmul(Pa, Eb_neg);
-            }
-            const int2 lh0 = a.krows[r];
-            const int2 lh = make_int2(__builtin_amdgcn_readfirstlane(lh0.x), __builtin_amdgcn_readfirstlane(lh0.y));
-            if (lh.x <= lh.y) {
-                const double2* row = s_a + (r - r0) * rowlen + Mc;  // row[mc], mc in [-Mc, Mc]
-                double qr = 0.0, qi = 0.0;                           // sum_{mc >= 0} A_mc z^mc
-#pragma unroll 4
-                for (int mc = lh.y; mc >= 0; --mc) {
-                    const double2 A = row[mc];
-                    const double nr = fma(qr, zr, fma(-qi, zi, A.x));
-                    qi = fma(qr, zi, fma(qi, zr, A.y));
-                    qr = nr;
-                }
-                double nr_ = 0.0, ni_ = 0.0;                         // sum_{j >= 1} A_{-j} (z*)^{j-1}
-#pragma unroll 4
-                for (int j = -lh.x; j >= 1; --j) {
-                    const double2 A = row[-j];
-                    const double tr = fma(nr_, zr, fma(ni_, zi, A.x));
-                    ni_ = fma(ni_, zr, fma(-nr_, zi, A.y));
-                    nr_ = tr;
-                }
-                // R = q + z* n;  acc += Re(Pb R)
-                const double Rr = qr + (zr * nr_ + zi * ni_);
-                const double Ri = qi + (zr * ni_ - zi * nr_);
-                acc += Pb.x * Rr - Pb.y * Ri;
-            }
-            Pb = cmul(Pb, Eb);
+    for (int r = 0; r < n_rows; ++r) {
+        if (r > 0 && (r % nb) == 0) {  // next m_a
+            Pa = cmul(Pa, Ea);
+            Pb = cmul(Pa, Eb_neg);
         }
+        const int2 lh = make_int2(krows[2 * r], krows[2 * r + 1]);
+        if (lh.x <= lh.y) {
+            const gconst_ptr<double> row = Abox + 2 * ((long long)r * rowlen + Mc);  // mc in [-Mc, Mc]
+            // Every inner FMA carries a negated operand (-q * -z where the sign
+            // is not wanted), which keeps it in the three-operand form that
+            // takes the scalar coefficient directly; the products are exact
+            // negations of each other, so the sums are unchanged.
+            double qr = 0.0, qi = 0.0;                           // sum_{mc >= 0} A_mc z^mc
+#pragma unroll 4
+            for (int mc = lh.y; mc >= 0; --mc) {
+                const double Ar = row[2 * mc], Ai = row[2 * mc + 1];
+                const double nr = fma(qr, zr, fma(-qi, zi, Ar));
+                qi = fma(qr, zi, fma(-qi, nzr, Ai));
+                qr = nr;
+            }
+            double nr_ = 0.0, ni_ = 0.0;                         // sum_{j >= 1} A_{-j} (z*)^{j-1}
+#pragma unroll 4
+            for (int j = -lh.x; j >= 1; --j) {
+                const double Ar = row[-2 * j], Ai = row[-2 * j + 1];
+                const double tr = fma(nr_, zr, fma(-ni_, nzi, Ar));
+                ni_ = fma(ni_, zr, fma(-nr_, zi, Ai));
+                nr_ = tr;
+            }
+            // R = q + z* n;  acc += Re(Pb R)
+            const double Rr = qr + (zr * nr_ + zi * ni_);
+            const double Ri = qi + (zr * ni_ - zi * nr_);
+            acc += Pb.x * Rr - Pb.y * Ri;
+        }
+        Pb = cmul(Pb, Eb);
     }
     if (valid) g.out_lr[pid] = charged ? acc : 0.0;
 }
