@@ -35,6 +35,12 @@ namespace pmk {
 constexpr int TILE_IMAGES = 64;
 
 constexpr int ERFC_SEG = 512;      // cubic segments of erfc(alpha r) on [0, r_sr] (FP32 path)
+// The largest float below ERFC_SEG: the scaled distance is clamped to it, so
+// the last segment keeps its own t in [0, 1) up to r_sr, a distance beyond r_sr
+// (a masked term) reads a finite value, and r = 0 (0 * inf = NaN) reads
+// segment ERFC_SEG - 1 rather than a NaN.
+constexpr float ERFC_XMAX = (float)ERFC_SEG - (float)ERFC_SEG / 16777216.0f;
+constexpr int ERFC64_SEG = 512;    // quintic segments of erfc(alpha r) on [0, r_sr] (FP64 path)
 constexpr int SCAN_WG = 1024;      // work-items of the scan kernels
 constexpr int SCAN_PER = 4;        // cells per work-item in the scan kernels
 constexpr int SCAN_SPAN = SCAN_WG * SCAN_PER;
@@ -64,6 +70,8 @@ struct RunArgs {
     const int* species;
     const void* ljtab;       // [n_types][nsite] LJP<Real>
     const float4* erfc_tab;  // ERFC_SEG cubics of erfc(alpha r) in t = r/h - i
+    const double2* erfc64_tab;  // ERFC64_SEG quintics (c0, c1), (c2, c3), (c4, c5) in t = r/h - i
+    double erfc64_inv_h;
     // Fourier box: ma in [0, Ma], mb in [-Mb, Mb], mc in [-Mc, Mc]
     int Ma, Mb, Mc;
     const double2* A;        // wt * N_R * S(m), zeros off the kept set
@@ -352,14 +360,35 @@ __global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_
 }
 
 // ---------------------------------------------------------------- pairs ----
+// The erfc(alpha r) tables of a run in LDS: FP32 cubics (mixed path) or FP64
+// quintics (FP64 path), each with 1/h of its segments.
+struct ErfcTabs {
+    float inv_h;
+    const float4* tab;
+    double inv_h64;
+    const double2* tab64;
+};
+
+// erfc(alpha r) from the FP64 segment table, Horner in t = r/h - i (beyond
+// r_sr the last segment extrapolates: finite, and the term is masked there).
+__device__ inline double erfc_tab64(double r, const ErfcTabs& et) {
+    const double x = r * et.inv_h64;
+    const int i = min((int)x, ERFC64_SEG - 1);
+    const double t = x - (double)i;
+    const double2 c01 = et.tab64[3 * i], c23 = et.tab64[3 * i + 1], c45 = et.tab64[3 * i + 2];
+    return fma(fma(fma(fma(fma(c45.y, t, c45.x), t, c23.y), t, c23.x), t, c01.y), t, c01.x);
+}
+
 // One candidate pair: LJ into plj, real-space Ewald into psr.
 // FP32 pairs: the masked terms accumulate by one FMA each (eps4 * w and
 // q / r * erfc), the table offset is v_fract of the scaled distance, and the
-// candidate's charge arrives in FP32 (qf).
+// candidate's charge arrives in FP32 (qf).  FP64 pairs: IEEE division and
+// square root, erfc(alpha r) from the FP64 table.
 template <int ESM, typename Real>
 __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>& p, bool is_lj, double r2lj,
-                                 double r2sr, double alpha, float inv_h, const float4* tab, bool& ovl,
-                                 Real& plj, Real& psr) {
+                                 double r2sr, const ErfcTabs& et, bool& ovl, Real& plj, Real& psr) {
+    const float inv_h = et.inv_h;
+    const float4* tab = et.tab;
     const bool in = r2 < r2lj;
     ovl |= in & (r2 < R2_OVERLAP) & is_lj;
     if constexpr (ESM != 0 && sizeof(Real) == 4) {
@@ -370,7 +399,7 @@ __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>&
         const float s3 = s * s * s;
         plj = fmaf(p.eps4, in ? fmaf(s3, s3, -s3) : 0.0f, plj);
         const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (qf != 0.0f);
-        const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
+        const float x = fminf(rf2 * rinv * inv_h, ERFC_XMAX);
         const int i = (int)x;
         const float t = __builtin_amdgcn_fractf(x);
         const float4 c = tab[i];
@@ -385,7 +414,8 @@ __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>&
         plj += in ? e : (Real)0;
         if constexpr (ESM != 0) {
             const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (q != 0.0);
-            psr += ins ? PairMath<Real>::coul(r2, q, alpha) : (Real)0;
+            const double r = sqrt(r2);
+            psr += ins ? q / r * erfc_tab64(r, et) : (Real)0;
         }
     }
 }
@@ -396,9 +426,10 @@ __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>&
 // computed exactly as in pair_term.
 template <int ESM, typename Real>
 __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<Real>& p, Real csig, bool is_lj,
-                                      double r2lj,
-                                      double r2sr, double alpha, float inv_h, const float4* tab, bool& ovl,
-                                      Real& plj, Real& psr, double& cE, double& cS, double& cQ) {
+                                      double r2lj, double r2sr, const ErfcTabs& et, bool& ovl, Real& plj,
+                                      Real& psr, double& cE, double& cS, double& cQ) {
+    const float inv_h = et.inv_h;
+    const float4* tab = et.tab;
     const bool in = r2 < r2lj;
     ovl |= in & (r2 < R2_OVERLAP) & is_lj;
     const bool lj_in = in & is_lj;
@@ -423,7 +454,7 @@ __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<R
         if constexpr (sizeof(Real) == 4) {
             const float rf2 = (float)r2;
             const float rinv = __builtin_amdgcn_rsqf(rf2);
-            const float x = fminf(rf2 * rinv * inv_h, (float)(ERFC_SEG - 1));
+            const float x = fminf(rf2 * rinv * inv_h, ERFC_XMAX);
             const int i = (int)x;
             const float t = __builtin_amdgcn_fractf(x);
             const float4 c = tab[i];
@@ -432,7 +463,7 @@ __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<R
             cQ += ins ? (double)(e * rinv) : 0.0;
         } else {
             const double r = sqrt(r2);
-            const double ec = erfc(r * alpha);
+            const double ec = erfc_tab64(r, et);
             psr += (ins & (q != 0.0)) ? q / r * ec : 0.0;
             cQ += ins ? ec / r : 0.0;
         }
@@ -444,7 +475,8 @@ struct PairLds {
     int* t;         // candidate type * nsite
     float* qf;      // candidate charge in FP32
     int* sel;       // selected tiles of the current batch
-    const float4* erfc;
+    const float4* erfc;    // FP32 kernels
+    const double2* erfc64;  // FP64 kernels
     int* wcnt;      // per-wave candidate counts
 };
 
@@ -496,8 +528,8 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                                    TypeSums& ts) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
-    const double r2lj = a.r2_lj, r2sr = a.r2_sr, alpha = a.alpha;
-    const float inv_h = (float)a.erfc_inv_h;
+    const double r2lj = a.r2_lj, r2sr = a.r2_sr;
+    const ErfcTabs et{(float)a.erfc_inv_h, L.erfc, a.erfc64_inv_h, L.erfc64};
     const bool is_lj = (a.site_lj_mask >> site) & 1u;
     const int nsite = a.nsite;
     // a wave with no valid point skips the pair loops; it still takes part in
@@ -586,11 +618,10 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                                 const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
                                 ts.type(tt, nsite);
                                 pair_term_grad<ESM, Real>(r2, c.w, L.qf[j], s_lj[tt + site], s_gc[tt + site], is_lj,
-                                                          r2lj, r2sr, alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE,
-                                                          ts.cS, ts.cQ);
+                                                          r2lj, r2sr, et, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
                             } else {
-                                pair_term<ESM, Real>(r2, c.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr,
-                                                     alpha, inv_h, L.erfc, ovl, plj, psr);
+                                pair_term<ESM, Real>(r2, c.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr, et,
+                                                     ovl, plj, psr);
                             }
                         }
                         acc_lj += (double)plj;
@@ -629,11 +660,10 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                             const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
                             ts.type(tt, nsite);
                             pair_term_grad<ESM, Real>(r2, at.w, L.qf[j], s_lj[tt + site], s_gc[tt + site], is_lj,
-                                                      r2lj, r2sr, alpha, inv_h, L.erfc, ovl, plj, psr, ts.cE, ts.cS,
-                                                      ts.cQ);
+                                                      r2lj, r2sr, et, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
                         } else {
-                            pair_term<ESM, Real>(r2, at.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr, alpha,
-                                                 inv_h, L.erfc, ovl, plj, psr);
+                            pair_term<ESM, Real>(r2, at.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr, et, ovl,
+                                                 plj, psr);
                         }
                     }
                     acc_lj += (double)plj;
@@ -646,12 +676,12 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
 }
 
 // LDS: TILE candidate records, types and FP32 charges, the selected-tile list,
-// the erfc table (FP32 kernels), then the LJ table of the run (and for a gradient run its d/d
-// sigma factors).
+// the erfc table (FP32 cubics or FP64 quintics), then the LJ table of the run
+// (and for a gradient run its d/d sigma factors).
 template <typename Real>
 constexpr int pair_lds_fixed() {
     return TILE * (int)(sizeof(double4) + sizeof(int) + sizeof(float) + sizeof(int)) +
-           (sizeof(Real) == 4 ? ERFC_SEG * (int)sizeof(float4) : 0);
+           (sizeof(Real) == 4 ? ERFC_SEG * (int)sizeof(float4) : 3 * ERFC64_SEG * (int)sizeof(double2));
 }
 
 template <typename Real, bool GRAD>
@@ -665,6 +695,8 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     L.sel = reinterpret_cast<int*>(smem + TILE * (sizeof(double4) + sizeof(int) + sizeof(float)));
     float4* s_erfc = reinterpret_cast<float4*>(smem + TILE * (sizeof(double4) + 2 * sizeof(int) + sizeof(float)));
     L.erfc = s_erfc;
+    double2* s_erfc64 = reinterpret_cast<double2*>(s_erfc);
+    L.erfc64 = s_erfc64;
     L.wcnt = s_wcnt;
     LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + pair_lds_fixed<Real>());
 
@@ -728,6 +760,9 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     if constexpr (sizeof(Real) == 4)
         if (a.esm != 0)
             for (int t = tid; t < ERFC_SEG; t += WG) s_erfc[t] = a.erfc_tab[t];
+    if constexpr (sizeof(Real) == 8)
+        if (a.esm != 0)
+            for (int t = tid; t < 3 * ERFC64_SEG; t += WG) s_erfc64[t] = a.erfc64_tab[t];
 
     double acc_lj = 0.0, acc_sr = 0.0;
     bool ovl = false;

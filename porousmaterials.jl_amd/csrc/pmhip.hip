@@ -191,6 +191,7 @@ struct pm_ewald {
     double2* d_box = nullptr;
     int2* d_rows = nullptr;    // per (ma, mb) row of the box: kept mc range (lo > hi: empty)
     float4* d_erfc = nullptr;  // ERFC_SEG cubics of erfc(alpha r) on [0, sr_cutoff]
+    double2* d_erfc64 = nullptr;  // ERFC64_SEG quintics of erfc(alpha r) on [0, sr_cutoff] (FP64)
     double2* d_box_t = nullptr;  // per-type boxes (gradient runs), built on first use
 };
 
@@ -552,6 +553,8 @@ void fill_run_pair(RunArgs& a, const pm_framework* fw, const pm_forcefield* ff, 
     a.r_cmax = std::sqrt(std::max(a.r2_lj, a.r2_sr));
     a.erfc_tab = es ? ew->d_erfc : nullptr;
     a.erfc_inv_h = es ? ERFC_SEG / ew->sr_cutoff : 1.0;
+    a.erfc64_tab = es ? ew->d_erfc64 : nullptr;
+    a.erfc64_inv_h = es ? ERFC64_SEG / ew->sr_cutoff : 1.0;
     a.inv4pe = INV_4PI_EPS0;
     a.esm = !es ? 0 : (a.r2_lj == a.r2_sr ? 1 : 2);
     a.nsite = nsite;
@@ -1358,6 +1361,7 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
     double* d_w = nullptr;
     auto cleanup = [&]() {
         (void)hipFree(ew->d_k); (void)hipFree(ew->d_box); (void)hipFree(ew->d_erfc); (void)hipFree(ew->d_rows);
+        (void)hipFree(ew->d_erfc64);
         (void)hipFree(d_m); (void)hipFree(d_w);
         delete ew;
     };
@@ -1377,6 +1381,47 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
                                   (float)(2.0 * (f0 - f1) + d0 + d1));
         }
     }
+    // FP64 path: degree-5 segments of f(r) = erfc(alpha r) on [0, sr_cutoff],
+    // h = sr_cutoff / ERFC64_SEG, in t = r/h - i, from the Taylor expansion about
+    // the segment's midpoint m (in u = t - 1/2: sum_k f^(k)(m) h^k u^k / k!),
+    // re-expanded in powers of t.  f^(k)(r) = alpha^k erfc^(k)(alpha r) with
+    // erfc^(k)(x) = -(2/sqrt(pi)) (-1)^(k-1) H_(k-1)(x) e^(-x^2) (Hermite
+    // polynomials); truncation error <= (h/2)^6 max|f^(6)| / 720, below 1e-15
+    // for alpha h <= 0.01 (alpha r_c is about 3 to 4 for Ewald precisions
+    // 1e-6 to 1e-8).  Evaluated in long double, stored as (c0, c1), (c2, c3),
+    // (c4, c5).
+    std::vector<double2> etab64(3 * ERFC64_SEG);
+    {
+        const long double h = (long double)sr_cutoff / ERFC64_SEG, al = alpha;
+        const long double two_rpi = 2.0L / std::sqrt((long double)M_PI);
+        for (int i = 0; i < ERFC64_SEG; ++i) {
+            const long double x = al * h * (i + 0.5L);
+            long double H[5];  // H_n(x), n = 0..4: H_(n+1) = 2x H_n - 2n H_(n-1)
+            H[0] = 1.0L;
+            H[1] = 2.0L * x;
+            for (int n = 1; n < 4; ++n) H[n + 1] = 2.0L * x * H[n] - 2.0L * n * H[n - 1];
+            const long double g = std::exp(-x * x);
+            long double d[6];  // f^(k)(m) h^k / k!
+            d[0] = std::erfc(x);
+            long double scale = 1.0L;  // (alpha h)^k / k!
+            for (int k = 1; k <= 5; ++k) {
+                scale *= al * h / k;
+                d[k] = -two_rpi * ((k - 1) % 2 ? -1.0L : 1.0L) * H[k - 1] * g * scale;
+            }
+            long double c[6] = {0.0L, 0.0L, 0.0L, 0.0L, 0.0L, 0.0L};
+            for (int k = 0; k <= 5; ++k) {
+                long double binom = 1.0L, pw = 1.0L;  // C(k, j) (-1/2)^(k-j), from j = k down
+                for (int j = k; j >= 0; --j) {
+                    c[j] += d[k] * binom * pw;
+                    binom = binom * j / (k - j + 1);
+                    pw *= -0.5L;
+                }
+            }
+            etab64[3 * i] = make_double2((double)c[0], (double)c[1]);
+            etab64[3 * i + 1] = make_double2((double)c[2], (double)c[3]);
+            etab64[3 * i + 2] = make_double2((double)c[4], (double)c[5]);
+        }
+    }
     // kept mc range of every (ma, mb) row of the box
     const size_t n_rows = (size_t)(Ma + 1) * (2 * Mb + 1);
     std::vector<int2> rows(n_rows, make_int2(1, 0));
@@ -1388,6 +1433,7 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
     }
     if (hipMalloc(&ew->d_rows, n_rows * sizeof(int2)) != hipSuccess ||
         hipMalloc(&ew->d_erfc, ERFC_SEG * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&ew->d_erfc64, 3 * ERFC64_SEG * sizeof(double2)) != hipSuccess ||
         hipMalloc(&ew->d_box, box_n * sizeof(double2)) != hipSuccess ||
         hipMalloc(&ew->d_k, std::max<size_t>(ew->nk, 1) * sizeof(KVec)) != hipSuccess ||
         hipMalloc(&d_m, std::max<size_t>(m.size(), 1) * sizeof(int)) != hipSuccess ||
@@ -1398,6 +1444,9 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
     hipError_t e = hipMemsetAsync(ew->d_box, 0, box_n * sizeof(double2), g_ctx.stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ew->d_erfc, etab.data(), ERFC_SEG * sizeof(float4), hipMemcpyHostToDevice,
+                           g_ctx.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ew->d_erfc64, etab64.data(), 3 * ERFC64_SEG * sizeof(double2), hipMemcpyHostToDevice,
                            g_ctx.stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ew->d_rows, rows.data(), n_rows * sizeof(int2), hipMemcpyHostToDevice, g_ctx.stream);
@@ -1437,6 +1486,7 @@ int pm_ewald_destroy(pm_ewald* ew) {
     (void)hipFree(ew->d_k);
     (void)hipFree(ew->d_box);
     (void)hipFree(ew->d_erfc);
+    (void)hipFree(ew->d_erfc64);
     (void)hipFree(ew->d_box_t);
     (void)hipFree(ew->d_rows);
     delete ew;
