@@ -316,6 +316,32 @@ def test_point_charge_grid(pm, orc, precision):
     _assert_energy(g.data.ravel(order="F"), want, scale, TOL[precision], "charge grid")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_charge_grid_last_erfc_segment(pm, orc, precision):
+    """Ion-probe grid points whose distance to a charge lies in the last
+    segment of the erfc tables, [r_c - h, r_c) with h = r_c / 512, against the
+    oracle.  A +1/-1 pair in a 40 A cubic box (no replication) and 3131 points
+    along a, spaced 40/3130 A, put points at 12.78 and 12.79 A from the +1
+    charge.  Reading the last segment at its start instead of at the point's
+    own offset is an error of about 1e-7 of the condition scale there."""
+    say(f"case: 3131 x 2 x 2 ion grid through the last erfc segment ({precision})")
+    ff = pm.LJForceField("e", {"A": 3.0, "B": 3.0}, {"A": 50.0, "B": 50.0}, 12.8)
+    box = pm.Box(40.0, 40.0, 40.0, math.pi / 2, math.pi / 2, math.pi / 2)
+    x = pm.Crystal("pair", box, ["A", "B"], np.array([[0.0, 0.5], [0.0, 0.5], [0.0, 0.5]]), atom_q=[1.0, -1.0])
+    probe = pm.point_charge(1.0)
+    n = (3131, 2, 2)
+    g = pm.energy_grid(x, probe, ff, resolution=n, units="K", verbose=False, precision=precision)
+    o = _oracle(orc, pm, x, ff, probe)
+    assert tuple(o.rep) == (1, 1, 1)
+    r = np.arange(n[0]) * 40.0 / (n[0] - 1)
+    last = (r >= 12.8 * 511 / 512) & (r < 12.8)
+    assert last.sum() == 2
+    want = o.energy_grid(n)
+    scale = o.energy_grid_scales(n)
+    worst = _assert_energy(g.data.ravel(order="F"), want, scale, TOL[precision], "last-segment grid")
+    say(f"   max scaled error {worst:.2e}")
+
+
 # ------------------------------------------------------- edge cases ----
 def test_edge_cases(pm):
     say("case: edge cases (empty framework, uncharged framework, zero poses, bad species)")
