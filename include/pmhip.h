@@ -299,7 +299,8 @@ int pm_get_tuning(int32_t key, double* value);
  * Autotuner: runs the given Widom run (pm_widom's arguments) on the device for
  * a grid of (cell points, chunk radius) settings, keeps the fastest as the
  * process-wide setting (a narrower chunk-radius limit only when it is more
- * than 1% faster) and reports it and its device time (ms).
+ * than 1% faster, and a setting is timed only when it changes the run's cell
+ * layout or path) and reports it and its device time (ms).
  */
 int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew, const pm_probe* pr,
                       double temperature, uint64_t seed, int32_t n_blocks, int64_t n_per_block, int32_t precision,

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1887,10 +1888,27 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
     const double keep_cp = g_ctx.cell_points, keep_cr = g_ctx.chunk_r_max;
     double best_ms = INFINITY, best_cp = keep_cp, best_cr = keep_cr;
     std::vector<double> sums((size_t)n_blocks * 3);
+    // cell layouts already timed: (cells per axis, image-list path).  A
+    // candidate giving the same layout runs the same launches, so it could only
+    // win by timing noise and is not timed again (the wider radius limit, timed
+    // first, stands).
+    std::vector<std::array<int, 4>> seen;
     for (double cr : crs)
         for (double cp : cps) {
             g_ctx.cell_points = cp;
             g_ctx.chunk_r_max = cr;
+            {
+                std::vector<PointRun> runs(1);
+                if (int rc = widom_prepare(in, runs[0])) {
+                    g_ctx.cell_points = keep_cp;
+                    g_ctx.chunk_r_max = keep_cr;
+                    return rc;
+                }
+                const RunArgs& ra = runs[0].ra;
+                const std::array<int, 4> layout{ra.ncell[0], ra.ncell[1], ra.ncell[2], ra.chunk_r <= cr ? 1 : 0};
+                if (std::find(seen.begin(), seen.end(), layout) != seen.end()) continue;
+                seen.push_back(layout);
+            }
             double ms = INFINITY;
             for (int rep = 0; rep < 2; ++rep) {
                 std::vector<PointRun> runs(1);
