@@ -369,21 +369,38 @@ struct ErfcTabs {
     const double2* tab64;
 };
 
-// erfc(alpha r) from the FP64 segment table, Horner in t = r/h - i (beyond
-// r_sr the last segment extrapolates: finite, and the term is masked there).
+// The largest double below ERFC64_SEG (the FP64 twin of ERFC_XMAX).
+constexpr double ERFC64_XMAX = (double)ERFC64_SEG - (double)ERFC64_SEG / 9007199254740992.0;
+
+// erfc(alpha r) from the FP64 segment table, Horner in t = r/h - i.  The
+// scaled distance is clamped below ERFC64_SEG: beyond r_sr (a masked term)
+// the last segment gives a finite value, and so does r = NaN (r2 = 0, where
+// the caller's 1/r is +inf).
 __device__ inline double erfc_tab64(double r, const ErfcTabs& et) {
-    const double x = r * et.inv_h64;
-    const int i = min((int)x, ERFC64_SEG - 1);
+    const double x = fmin(r * et.inv_h64, ERFC64_XMAX);
+    const int i = (int)x;
     const double t = x - (double)i;
     const double2 c01 = et.tab64[3 * i], c23 = et.tab64[3 * i + 1], c45 = et.tab64[3 * i + 2];
     return fma(fma(fma(fma(fma(c45.y, t, c45.x), t, c23.y), t, c23.x), t, c01.y), t, c01.x);
 }
 
+// 1/sqrt(r2) in FP64: v_rsq_f64 and two Newton steps y <- y (3/2 - r2 y^2 / 2)
+// (the estimate's ~2^-22 relative error squares at each step, so the result
+// is within a few ulp); r2 below the smallest normal gives +inf, as
+// 1 / sqrt(0) does.
+__device__ inline double rsq64(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    const double h = 0.5 * r2;
+    y = fma(y, fma(-(h * y), y, 0.5), y);
+    y = fma(y, fma(-(h * y), y, 0.5), y);
+    return r2 >= 2.2250738585072014e-308 ? y : INFINITY;
+}
+
 // One candidate pair: LJ into plj, real-space Ewald into psr.
 // FP32 pairs: the masked terms accumulate by one FMA each (eps4 * w and
 // q / r * erfc), the table offset is v_fract of the scaled distance, and the
-// candidate's charge arrives in FP32 (qf).  FP64 pairs: IEEE division and
-// square root, erfc(alpha r) from the FP64 table.
+// candidate's charge arrives in FP32 (qf).  FP64 pairs: 1/r from rsq64,
+// 1/r^2 as its square, erfc(alpha r) from the FP64 table.
 template <int ESM, typename Real>
 __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>& p, bool is_lj, double r2lj,
                                  double r2sr, const ErfcTabs& et, bool& ovl, Real& plj, Real& psr) {
@@ -410,12 +427,13 @@ __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>&
         const float s3 = s * s * s;
         plj = fmaf(p.eps4, in ? fmaf(s3, s3, -s3) : 0.0f, plj);
     } else {
-        const Real e = PairMath<Real>::lj(r2, p.sig2, p.eps4);
-        plj += in ? e : (Real)0;
+        const double y = rsq64(r2);
+        const double s = p.sig2 * (y * y);
+        const double s3 = s * s * s;
+        plj += in ? p.eps4 * s3 * (s3 - 1.0) : 0.0;
         if constexpr (ESM != 0) {
             const bool ins = ((ESM == 1) ? in : (r2 < r2sr)) & (q != 0.0);
-            const double r = sqrt(r2);
-            psr += ins ? q / r * erfc_tab64(r, et) : (Real)0;
+            psr += ins ? q * y * erfc_tab64(r2 * y, et) : 0.0;
         }
     }
 }
@@ -443,7 +461,8 @@ __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<R
         cE += lj_in ? (double)w : 0.0;
         cS += lj_in ? (double)(csig * u * fmaf(12.0f, u, -6.0f)) : 0.0;
     } else {
-        const double s = p.sig2 / r2;
+        const double y = rsq64(r2);  // 1/r^2 exactly as pair_term computes it
+        const double s = p.sig2 * (y * y);
         const double u = s * s * s;
         plj += in ? p.eps4 * u * (u - 1.0) : 0.0;
         cE += lj_in ? u * (u - 1.0) : 0.0;
@@ -462,10 +481,10 @@ __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<R
             psr = fmaf((ins & (qf != 0.0f)) ? qf * rinv : 0.0f, e, psr);
             cQ += ins ? (double)(e * rinv) : 0.0;
         } else {
-            const double r = sqrt(r2);
-            const double ec = erfc_tab64(r, et);
-            psr += (ins & (q != 0.0)) ? q / r * ec : 0.0;
-            cQ += ins ? ec / r : 0.0;
+            const double y = rsq64(r2);
+            const double ec = erfc_tab64(r2 * y, et);
+            psr += (ins & (q != 0.0)) ? q * y * ec : 0.0;
+            cQ += ins ? ec * y : 0.0;
         }
     }
 }
