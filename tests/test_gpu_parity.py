@@ -216,6 +216,30 @@ def test_widom_insertions_match_oracle(pm, co2case, precision):
         np.testing.assert_allclose(sums[:, 2] / sums[:, 0], want[:, 2] / want[:, 0], rtol=tol)
 
 
+@pytest.mark.parametrize("eps", [1e-8, 1e-10])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_widom_energies_tight_ewald(pm, orc, precision, eps):
+    """Per-insertion energies at tighter Ewald precisions (alpha r_c 3.7 and
+    4.3 against 3.1 at 1e-6, so each segment of the erfc tables spans more of
+    erfc): 5 x 120 CO2 insertions vs the oracle at that precision."""
+    say(f"case: Widom CO2 5x120 at Ewald precision {eps:g} vs oracle ({precision})")
+    case = pm.synthetic.co2_widom_case()
+    x, ff, co2 = case.crystal, case.ljff, case.probes["CO2"]
+    xr = pm.replicate(x, pm.replication_factors(x.box, ff))
+    ep = pm.setup_Ewald_sum(xr.box, math.sqrt(ff.r2_cutoff), eps=eps)
+    o = _oracle(orc, pm, x, ff, co2, eps=eps)
+    assert math.isclose(o.alpha, ep.alpha, rel_tol=1e-12)
+    fw = pm.device.framework(xr, ff)
+    ew = pm.device.ewald(xr, fw, ep)
+    pr = pm.device.probe(co2, ff)
+    nb, seed = 120, 99
+    _, en = pm.device.widom(fw, ff, ew, pr, 298.0, seed, 5, nb, precision, return_energies=True)
+    v, e = o.widom_energies(seed, 0, 5 * nb)
+    scale = o.widom_scales(seed, 0, 5 * nb)
+    worst = _assert_energy(en[:, 0] + en[:, 1], v + e, scale, TOL[precision], "Widom E")
+    say(f"   alpha r_c {ep.alpha * math.sqrt(ff.r2_cutoff):.2f}, max scaled error {worst:.2e}")
+
+
 def test_widom_deterministic(pm, co2case):
     say("case: Widom determinism (two identical mixed runs)")
     case, xr, ep, o = co2case
