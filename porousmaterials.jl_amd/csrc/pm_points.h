@@ -489,8 +489,17 @@ __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<R
     }
 }
 
+// FP32 kernels pack a candidate's type offset (high word) and FP32 charge
+// (low word) into the fourth double of its record, so one 32-byte LDS record
+// carries the candidate (raw bits: no arithmetic touches the packed value).
+__device__ inline double pack_tq(int t, float q) {
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)t << 32) | (unsigned)__float_as_int(q)));
+}
+__device__ inline int unpack_t(double w) { return (int)(__double_as_longlong(w) >> 32); }
+__device__ inline float unpack_q(double w) { return __int_as_float((int)__double_as_longlong(w)); }
+
 struct PairLds {
-    double4* c;     // candidate records (Cartesian relative to the chunk centre, charge)
+    double4* c;     // candidate records: Cartesian xyz; charge (FP64 kernels) or packed type and charge
     int* t;         // candidate type * nsite
     float* qf;      // candidate charge in FP32
     int* sel;       // selected tiles of the current batch
@@ -615,9 +624,12 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
 #pragma unroll
                 for (int w = 0; w < WG / 64; ++w) woff[w + 1] = woff[w] + L.wcnt[w];
                 if (keep) {
-                    L.c[woff[wave] + before] = rec;
-                    L.t[woff[wave] + before] = ty;
-                    L.qf[woff[wave] + before] = (float)rec.w;
+                    if constexpr (sizeof(Real) == 4) {
+                        L.c[woff[wave] + before] = make_double4(rec.x, rec.y, rec.z, pack_tq(ty, (float)rec.w));
+                    } else {
+                        L.c[woff[wave] + before] = rec;
+                        L.t[woff[wave] + before] = ty;
+                    }
                 }
                 __syncthreads();
                 n_cand += (unsigned long long)woff[WG / 64];
@@ -633,14 +645,17 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                             const double4 c = L.c[j];
                             const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
                             const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            const bool f32 = sizeof(Real) == 4;
+                            const int tj = f32 ? unpack_t(c.w) : L.t[j];
+                            const float qj = f32 ? unpack_q(c.w) : 0.0f;  // the FP64 terms take c.w
                             if constexpr (GRAD) {
-                                const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
+                                const int tt = __builtin_amdgcn_readfirstlane(tj);
                                 ts.type(tt, nsite);
-                                pair_term_grad<ESM, Real>(r2, c.w, L.qf[j], s_lj[tt + site], s_gc[tt + site], is_lj,
+                                pair_term_grad<ESM, Real>(r2, c.w, qj, s_lj[tt + site], s_gc[tt + site], is_lj,
                                                           r2lj, r2sr, et, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
                             } else {
-                                pair_term<ESM, Real>(r2, c.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr, et,
-                                                     ovl, plj, psr);
+                                pair_term<ESM, Real>(r2, c.w, qj, s_lj[tj + site], is_lj, r2lj, r2sr, et, ovl, plj,
+                                                     psr);
                             }
                         }
                         acc_lj += (double)plj;
