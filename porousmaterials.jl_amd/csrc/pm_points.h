@@ -131,6 +131,21 @@ struct GroupArgs {
     double* out_g;           // gradient runs: per slot GQ x GT per-type sums
 };
 
+// A pointer into read-only device memory (the constant address space: the
+// kernel never writes what it points to) that the compiler may treat as
+// uniform across the wave (it is: every lane holds the same value), so that
+// loads through uniform indices become scalar loads.
+template <typename T>
+using gconst_ptr = const T __attribute__((address_space(4)))*;
+
+template <typename T>
+__device__ inline gconst_ptr<T> uniform_ptr(const T* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<gconst_ptr<T>>(((unsigned long long)hi << 32) | lo);
+}
+
 enum { SEG_BY_WG = 0, SEG_BY_CELL = 1, SEG_BY_SLOT = 2 };
 
 // Last segment whose first workgroup / cell / slot is <= v.
@@ -216,30 +231,47 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
         return;
     }
     double R[9];
-    if (a.rotate) arvo_rotation(&u[3], R);
+    const bool rotate = a.rotate != 0;
+    if (rotate) arvo_rotation(&u[3], R);
     const int n0 = a.ncell[0], n1 = a.ncell[1], n2 = a.ncell[2];
-    for (int s = 0; s < ns; ++s) {
-        double y0 = a.off[s][0], y1 = a.off[s][1], y2 = a.off[s][2];
-        if (a.rotate) {
+    // The site offsets and the basis change come in by scalar loads before the
+    // first atomic, so that the returning atomics of an insertion's sites are
+    // in flight together (a vector load after an atomic would wait for it).
+    const gconst_ptr<double> offp = uniform_ptr(&a.off[0][0]);
+    const gconst_ptr<double> c2fp = uniform_ptr(&a.c2f[0]);
+    double c2f[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c2f[k] = c2fp[k];
+    unsigned cid[PM_MAX_SITES], rank[PM_MAX_SITES];
+#pragma unroll
+    for (int s = 0; s < PM_MAX_SITES; ++s) {
+        if (s >= ns) break;
+        double y0 = offp[3 * s], y1 = offp[3 * s + 1], y2 = offp[3 * s + 2];
+        if (rotate) {
             const double t0 = R[0] * y0 + R[3] * y1 + R[6] * y2;
             const double t1 = R[1] * y0 + R[4] * y1 + R[7] * y2;
             const double t2 = R[2] * y0 + R[5] * y1 + R[8] * y2;
             y0 = t0; y1 = t1; y2 = t2;
         }
-        double x = u[0] + (a.c2f[0] * y0 + a.c2f[3] * y1 + a.c2f[6] * y2);
-        double y = u[1] + (a.c2f[1] * y0 + a.c2f[4] * y1 + a.c2f[7] * y2);
-        double z = u[2] + (a.c2f[2] * y0 + a.c2f[5] * y1 + a.c2f[8] * y2);
+        double x = u[0] + (c2f[0] * y0 + c2f[3] * y1 + c2f[6] * y2);
+        double y = u[1] + (c2f[1] * y0 + c2f[4] * y1 + c2f[7] * y2);
+        double z = u[2] + (c2f[2] * y0 + c2f[5] * y1 + c2f[8] * y2);
         x -= floor(x);
         y -= floor(y);
         z -= floor(z);
         const int ia = min((int)(x * n0), n0 - 1);
         const int ib = min((int)(y * n1), n1 - 1);
         const int ic = min((int)(z * n2), n2 - 1);
-        const unsigned cid = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
+        cid[s] = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
         g.pos[p0 + s] = make_double4(x, y, z, 1.0);
         // the count before this point is its rank in the cell (the order within
         // a cell changes no point's energy)
-        g.cell[p0 + s] = make_uint2(cid, atomicAdd(&g.count[cid], 1u));
+        rank[s] = atomicAdd(&g.count[cid[s]], 1u);
+    }
+#pragma unroll
+    for (int s = 0; s < PM_MAX_SITES; ++s) {
+        if (s >= ns) break;
+        g.cell[p0 + s] = make_uint2(cid[s], rank[s]);
     }
 }
 
@@ -828,21 +860,6 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
 }
 
 // -------------------------------------------------------------- Fourier ----
-// A pointer into read-only device memory (the constant address space: the
-// kernel never writes what it points to) that the compiler may treat as
-// uniform across the wave (it is: every lane holds the same value), so that
-// loads through uniform indices become scalar loads.
-template <typename T>
-using gconst_ptr = const T __attribute__((address_space(4)))*;
-
-template <typename T>
-__device__ inline gconst_ptr<T> uniform_ptr(const T* p) {
-    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return reinterpret_cast<gconst_ptr<T>>(((unsigned long long)hi << 32) | lo);
-}
-
 __device__ inline double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
