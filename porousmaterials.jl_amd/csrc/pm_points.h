@@ -864,22 +864,24 @@ __device__ inline double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Fourier part per point: sum_m Re(A_m e^{-i 2 pi m.u}), u = R o x (unit-cell
-// fractional), FP64.  With z_a = e^{-i 2 pi u_a}: for every (ma, mb) row the
-// sum over the row's kept mc range is evaluated as two polynomials by Horner,
-// sum_{mc >= 0} A_mc z_c^mc and z_c^* sum_{j >= 1} A_{-j} (z_c^*)^{j-1}, and
-// multiplied by z_a^ma z_b^mb, carried along the rows by recurrence.  The
-// coefficients are the same for every lane: they are read by scalar loads
-// (through the scalar cache, into SGPR operands of the FMAs) rather than
-// broadcast from LDS, which would move 64 copies of each one.  Point slots of
-// a segment start at a multiple of WG, so a workgroup lies in one segment.
-__global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
-    const int tid = threadIdx.x;
-    const long long p0 = (long long)blockIdx.x * WG;
-    const SegArgs sg = g.segs[find_segment<SEG_BY_SLOT>(g.segs, g.n_seg, p0)];
-    const RunArgs& a = g.runs[sg.run];
-    if (a.esm == 0) return;  // uniform: the whole workgroup leaves
-    const long long pid = p0 + tid, lp = pid - sg.slot_base;
+// Points per work-item of the Fourier kernel: the points of a work-item share
+// every coefficient load and the loop control, and their Horner chains
+// interleave.
+constexpr int FOURIER_PTS = 2;
+constexpr int FOURIER_WG = WG / FOURIER_PTS;
+
+struct FourierPoint {
+    long long pid;
+    bool valid, charged;
+    double2 Ea, Eb;  // z_a, z_b
+    double zr, zi;   // z_c
+};
+
+__device__ inline FourierPoint fourier_point(const GroupArgs& g, const SegArgs& sg, const RunArgs& a,
+                                             long long pid) {
+    FourierPoint P;
+    P.pid = pid;
+    const long long lp = pid - sg.slot_base;
     bool valid = lp < sg.n_slots;
     double x = 0.0, y = 0.0, z = 0.0;
     int site = 0;
@@ -895,27 +897,63 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
         y = grid_coord((double)j, n2, a.repd[1]);
         z = grid_coord((double)k, n3, a.repd[2]);
     }
-    const bool charged = valid && ((a.site_q_mask >> site) & 1u);
+    P.valid = valid;
+    P.charged = valid && ((a.site_q_mask >> site) & 1u);
     double t0 = x * a.repd[0], t1 = y * a.repd[1], t2 = z * a.repd[2];
     t0 -= rint(t0);
     t1 -= rint(t1);
     t2 -= rint(t2);
     double s, c;
     sincospi(2.0 * t0, &s, &c);
-    const double2 Ea = make_double2(c, -s);
+    P.Ea = make_double2(c, -s);
     sincospi(2.0 * t1, &s, &c);
-    const double2 Eb = make_double2(c, -s);
+    P.Eb = make_double2(c, -s);
     sincospi(2.0 * t2, &s, &c);
-    const double zr = c, zi = -s;  // z_c
-    double nzr = -zr, nzi = -zi;   // opaque to the optimiser, which would fold -q * -z back to q * z
-    asm volatile("" : "+v"(nzr));
-    asm volatile("" : "+v"(nzi));
+    P.zr = c;
+    P.zi = -s;
+    return P;
+}
+
+// Fourier part per point: sum_m Re(A_m e^{-i 2 pi m.u}), u = R o x (unit-cell
+// fractional), FP64.  With z_a = e^{-i 2 pi u_a}: for every (ma, mb) row the
+// sum over the row's kept mc range is evaluated as two polynomials by Horner,
+// sum_{mc >= 0} A_mc z_c^mc and z_c^* sum_{j >= 1} A_{-j} (z_c^*)^{j-1}, and
+// multiplied by z_a^ma z_b^mb, carried along the rows by recurrence.  The
+// coefficients are the same for every lane: they are read by scalar loads
+// (through the scalar cache, into SGPR operands of the FMAs) rather than
+// broadcast from LDS, which would move 64 copies of each one.  Point slots of
+// a segment start at a multiple of WG, so a workgroup lies in one segment.
+__global__ __launch_bounds__(FOURIER_WG) void point_fourier_kernel(GroupArgs g) {
+    const int tid = threadIdx.x;
+    const long long p0 = (long long)blockIdx.x * WG;  // WG slots per workgroup, FOURIER_PTS per work-item
+    const SegArgs sg = g.segs[find_segment<SEG_BY_SLOT>(g.segs, g.n_seg, p0)];
+    const RunArgs& a = g.runs[sg.run];
+    if (a.esm == 0) return;  // uniform: the whole workgroup leaves
+    FourierPoint P[FOURIER_PTS];
+    double nzr[FOURIER_PTS], nzi[FOURIER_PTS];
+#pragma unroll
+    for (int k = 0; k < FOURIER_PTS; ++k) {
+        P[k] = fourier_point(g, sg, a, p0 + tid + k * FOURIER_WG);
+        // opaque to the optimiser, which would fold -q * -z back to q * z
+        nzr[k] = -P[k].zr;
+        nzi[k] = -P[k].zi;
+        asm volatile("" : "+v"(nzr[k]));
+        asm volatile("" : "+v"(nzi[k]));
+    }
     // wave-uniform: scalar loop control (the loads are not provably uniform)
     const int Ma = __builtin_amdgcn_readfirstlane(a.Ma), Mb = __builtin_amdgcn_readfirstlane(a.Mb),
               Mc = __builtin_amdgcn_readfirstlane(a.Mc);
-    double2 Eb_neg = make_double2(1.0, 0.0);
-    const double2 Eb_c = make_double2(Eb.x, -Eb.y);
-    for (int q = 0; q < Mb; ++q) Eb_neg = cmul(Eb_neg, Eb_c);
+    double2 Eb_neg[FOURIER_PTS], Pa[FOURIER_PTS], Pb[FOURIER_PTS];  // Pb: z_a^ma z_b^mb of the current row
+    double acc[FOURIER_PTS];
+#pragma unroll
+    for (int k = 0; k < FOURIER_PTS; ++k) {
+        Eb_neg[k] = make_double2(1.0, 0.0);
+        const double2 Eb_c = make_double2(P[k].Eb.x, -P[k].Eb.y);
+        for (int q = 0; q < Mb; ++q) Eb_neg[k] = cmul(Eb_neg[k], Eb_c);
+        Pa[k] = make_double2(1.0, 0.0);
+        Pb[k] = Eb_neg[k];
+        acc[k] = 0.0;
+    }
 
     const int rowlen = 2 * Mc + 1, nb = 2 * Mb + 1;
     const int n_rows = (Ma + 1) * nb;
@@ -923,13 +961,13 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
     // from that address space)
     const gconst_ptr<double> Abox = uniform_ptr(reinterpret_cast<const double*>(a.A));
     const gconst_ptr<int> krows = uniform_ptr(reinterpret_cast<const int*>(a.krows));
-    double2 Pa = make_double2(1.0, 0.0);
-    double2 Pb = Eb_neg;  // z_a^ma z_b^mb of the current row
-    double acc = 0.0;
     for (int r = 0; r < n_rows; ++r) {
         if (r > 0 && (r % nb) == 0) {  // next m_a
-            Pa = cmul(Pa, Ea);
-            Pb = cmul(Pa, Eb_neg);
+#pragma unroll
+            for (int k = 0; k < FOURIER_PTS; ++k) {
+                Pa[k] = cmul(Pa[k], P[k].Ea);
+                Pb[k] = cmul(Pa[k], Eb_neg[k]);
+            }
         }
         const int2 lh = make_int2(krows[2 * r], krows[2 * r + 1]);
         if (lh.x <= lh.y) {
@@ -938,30 +976,47 @@ __global__ __launch_bounds__(WG) void point_fourier_kernel(GroupArgs g) {
             // is not wanted), which keeps it in the three-operand form that
             // takes the scalar coefficient directly; the products are exact
             // negations of each other, so the sums are unchanged.
-            double qr = 0.0, qi = 0.0;                           // sum_{mc >= 0} A_mc z^mc
+            double qr[FOURIER_PTS], qi[FOURIER_PTS];  // sum_{mc >= 0} A_mc z^mc
+#pragma unroll
+            for (int k = 0; k < FOURIER_PTS; ++k) qr[k] = qi[k] = 0.0;
 #pragma unroll 4
             for (int mc = lh.y; mc >= 0; --mc) {
                 const double Ar = row[2 * mc], Ai = row[2 * mc + 1];
-                const double nr = fma(qr, zr, fma(-qi, zi, Ar));
-                qi = fma(qr, zi, fma(-qi, nzr, Ai));
-                qr = nr;
+#pragma unroll
+                for (int k = 0; k < FOURIER_PTS; ++k) {
+                    const double nr = fma(qr[k], P[k].zr, fma(-qi[k], P[k].zi, Ar));
+                    qi[k] = fma(qr[k], P[k].zi, fma(-qi[k], nzr[k], Ai));
+                    qr[k] = nr;
+                }
             }
-            double nr_ = 0.0, ni_ = 0.0;                         // sum_{j >= 1} A_{-j} (z*)^{j-1}
+            double nr_[FOURIER_PTS], ni_[FOURIER_PTS];  // sum_{j >= 1} A_{-j} (z*)^{j-1}
+#pragma unroll
+            for (int k = 0; k < FOURIER_PTS; ++k) nr_[k] = ni_[k] = 0.0;
 #pragma unroll 4
             for (int j = -lh.x; j >= 1; --j) {
                 const double Ar = row[-2 * j], Ai = row[-2 * j + 1];
-                const double tr = fma(nr_, zr, fma(-ni_, nzi, Ar));
-                ni_ = fma(ni_, zr, fma(-nr_, zi, Ai));
-                nr_ = tr;
+#pragma unroll
+                for (int k = 0; k < FOURIER_PTS; ++k) {
+                    const double tr = fma(nr_[k], P[k].zr, fma(-ni_[k], nzi[k], Ar));
+                    ni_[k] = fma(ni_[k], P[k].zr, fma(-nr_[k], P[k].zi, Ai));
+                    nr_[k] = tr;
+                }
             }
-            // R = q + z* n;  acc += Re(Pb R)
-            const double Rr = qr + (zr * nr_ + zi * ni_);
-            const double Ri = qi + (zr * ni_ - zi * nr_);
-            acc += Pb.x * Rr - Pb.y * Ri;
+#pragma unroll
+            for (int k = 0; k < FOURIER_PTS; ++k) {
+                // R = q + z* n;  acc += Re(Pb R)
+                const double zr = P[k].zr, zi = P[k].zi;
+                const double Rr = qr[k] + (zr * nr_[k] + zi * ni_[k]);
+                const double Ri = qi[k] + (zr * ni_[k] - zi * nr_[k]);
+                acc[k] += Pb[k].x * Rr - Pb[k].y * Ri;
+            }
         }
-        Pb = cmul(Pb, Eb);
+#pragma unroll
+        for (int k = 0; k < FOURIER_PTS; ++k) Pb[k] = cmul(Pb[k], P[k].Eb);
     }
-    if (valid) g.out_lr[pid] = charged ? acc : 0.0;
+#pragma unroll
+    for (int k = 0; k < FOURIER_PTS; ++k)
+        if (P[k].valid) g.out_lr[P[k].pid] = P[k].charged ? acc[k] : 0.0;
 }
 
 // -------------------------------------------------------------- combine ----

@@ -1008,7 +1008,8 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         }
         PM_HIP(hipEventRecord(g_ctx.evp1, g_ctx.stream));
         const long long slot_wg = (gr.n_slots + WG - 1) / WG;
-        if (gr.es) hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
+        if (gr.es)
+            hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)slot_wg), dim3(FOURIER_WG), 0, g_ctx.stream, g);
         if (grad && gr.es)
             hipLaunchKernelGGL(point_fourier_grad_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
         if (widom && grad)
