@@ -1351,7 +1351,7 @@ int pm_ewald_create(const pm_framework* fw, double alpha, double sr_cutoff, int6
         Mb = std::max(Mb, std::abs(mb));
         Mc = std::max(Mc, std::abs(mc));
     }
-    if ((int64_t)(Ma + 1) * (2 * Mb + 1) * (2 * Mc + 1) > (1LL << 26) || 2 * Mc + 1 > FT)
+    if ((int64_t)(Ma + 1) * (2 * Mb + 1) * (2 * Mc + 1) > (1LL << 26))
         return fail(PM_ERR_ARG, "pm_ewald_create: k-vector box too large");
     auto* ew = new pm_ewald();
     ew->alpha = alpha;
