@@ -472,12 +472,13 @@ __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>&
 
 // pair_term with the per-type derivative sums of a gradient run: cE += u^2 - u
 // (d LJ / d eps_ts = 4 (u^2 - u), u = (sigma_ts^2 / r^2)^3), cS += d LJ / d sigma_ts,
-// cQ += erfc(alpha r) / r, all for in-cutoff pairs.  The energy terms are
-// computed exactly as in pair_term.
+// cQ += erfc(alpha r) / r, all for in-cutoff pairs, in Real (the caller folds
+// them into FP64 per tile).  The energy terms are computed exactly as in
+// pair_term.
 template <int ESM, typename Real>
 __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<Real>& p, Real csig, bool is_lj,
                                       double r2lj, double r2sr, const ErfcTabs& et, bool& ovl, Real& plj,
-                                      Real& psr, double& cE, double& cS, double& cQ) {
+                                      Real& psr, Real& cE, Real& cS, Real& cQ) {
     const float inv_h = et.inv_h;
     const float4* tab = et.tab;
     const bool in = r2 < r2lj;
@@ -490,8 +491,8 @@ __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<R
         const float u = s * s * s;
         const float w = fmaf(u, u, -u);
         plj = fmaf(p.eps4, in ? w : 0.0f, plj);
-        cE += lj_in ? (double)w : 0.0;
-        cS += lj_in ? (double)(csig * u * fmaf(12.0f, u, -6.0f)) : 0.0;
+        cE += lj_in ? w : 0.0f;
+        cS += lj_in ? csig * u * fmaf(12.0f, u, -6.0f) : 0.0f;
     } else {
         const double y = rsq64(r2);  // 1/r^2 exactly as pair_term computes it
         const double s = p.sig2 * (y * y);
@@ -511,7 +512,7 @@ __device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<R
             const float4 c = tab[i];
             const float e = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
             psr = fmaf((ins & (qf != 0.0f)) ? qf * rinv : 0.0f, e, psr);
-            cQ += ins ? (double)(e * rinv) : 0.0;
+            cQ += ins ? e * rinv : 0.0f;
         } else {
             const double y = rsq64(r2);
             const double ec = erfc_tab64(r2 * y, et);
@@ -540,21 +541,32 @@ struct PairLds {
     int* wcnt;      // per-wave candidate counts
 };
 
-// Per-type derivative sums of a gradient run's point: the running sums of the
-// current framework type (cE, cS, cQ) are added to g[] when the type changes.
-// The type is the same for every lane of the wave (all lanes read the same
-// candidate), so the flush is a uniform branch; image lists are sorted by type,
-// so the types of a point's candidate stream never decrease and it runs at
-// most once per type.
+// Per-type derivative sums of a gradient run's point.  The pair terms of the
+// current framework type accumulate in Real (fE, fS, fQ), folded into FP64
+// (cE, cS, cQ) at the end of every tile and when the type changes, as the
+// energies are; the FP64 sums of a type are added to g[] when the type
+// changes.  The type is the same for every lane of the wave (all lanes read
+// the same candidate), so the flush is a uniform branch; the tiles of an image
+// list are sorted by species inside each spatial bin, so it runs about once
+// per species per bin.
+template <typename Real>
 struct TypeSums {
     int cur = -1;  // current type * nsite
     double cE = 0.0, cS = 0.0, cQ = 0.0;
+    Real fE = (Real)0, fS = (Real)0, fQ = (Real)0;
     double g[3 * GT];
     __device__ TypeSums() {
 #pragma unroll
         for (int k = 0; k < 3 * GT; ++k) g[k] = 0.0;
     }
+    __device__ __attribute__((always_inline)) inline void fold() {
+        cE += (double)fE;
+        cS += (double)fS;
+        cQ += (double)fQ;
+        fE = fS = fQ = (Real)0;
+    }
     __device__ __attribute__((always_inline)) inline void flush(int nsite) {
+        fold();
         if (cur >= 0) {
             const int t = cur / nsite;
             // every entry is updated (a conditional update of one entry would be
@@ -585,7 +597,7 @@ template <int ESM, typename Real, bool GRAD>
 __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs& a, bool valid, double px, double py, double pz, double cf0,
                                    double cf1, double cf2, int site, const LJP<Real>* s_lj, const Real* s_gc,
                                    PairLds L, double& acc_lj, double& acc_sr, bool& ovl, unsigned long long& n_cand,
-                                   TypeSums& ts) {
+                                   TypeSums<Real>& ts) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
     const double r2lj = a.r2_lj, r2sr = a.r2_sr;
@@ -684,7 +696,7 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                                 const int tt = __builtin_amdgcn_readfirstlane(tj);
                                 ts.type(tt, nsite);
                                 pair_term_grad<ESM, Real>(r2, c.w, qj, s_lj[tt + site], s_gc[tt + site], is_lj,
-                                                          r2lj, r2sr, et, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
+                                                          r2lj, r2sr, et, ovl, plj, psr, ts.fE, ts.fS, ts.fQ);
                             } else {
                                 pair_term<ESM, Real>(r2, c.w, qj, s_lj[tj + site], is_lj, r2lj, r2sr, et, ovl, plj,
                                                      psr);
@@ -692,6 +704,7 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                         }
                         acc_lj += (double)plj;
                         acc_sr += (double)psr;
+                        if constexpr (GRAD) ts.fold();
                     }
                 }
             }
@@ -726,7 +739,7 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                             const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
                             ts.type(tt, nsite);
                             pair_term_grad<ESM, Real>(r2, at.w, L.qf[j], s_lj[tt + site], s_gc[tt + site], is_lj,
-                                                      r2lj, r2sr, et, ovl, plj, psr, ts.cE, ts.cS, ts.cQ);
+                                                      r2lj, r2sr, et, ovl, plj, psr, ts.fE, ts.fS, ts.fQ);
                         } else {
                             pair_term<ESM, Real>(r2, at.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr, et, ovl,
                                                  plj, psr);
@@ -734,6 +747,7 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                     }
                     acc_lj += (double)plj;
                     acc_sr += (double)psr;
+                    if constexpr (GRAD) ts.fold();
                 }
             }
         }
@@ -833,7 +847,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     double acc_lj = 0.0, acc_sr = 0.0;
     bool ovl = false;
     unsigned long long n_cand = 0;
-    TypeSums ts;
+    TypeSums<Real> ts;
     if (a.esm == 0)
         chunk_pairs<0, Real, GRAD>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
                                    n_cand, ts);
