@@ -907,6 +907,17 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const size_t o_nitems = ar.take(sizeof(unsigned) * 4);
     const size_t o_cand = ar.take(sizeof(unsigned long long));
     const size_t o_g = ar.take(grad ? (size_t)max_slots * GQ * GT * sizeof(double) : 0);
+    // separable Fourier part of charged grids: phases, B and C of the largest run
+    size_t gf_elems = 0;
+    if (!widom)
+        for (const PointRun& r : runs)
+            if (r.es) {
+                const RunArgs& a = r.ra;
+                const size_t n3 = (size_t)a.grid_n[2], n2 = (size_t)a.grid_n[1];
+                gf_elems = std::max(gf_elems, (size_t)a.grid_n[0] + n2 + n3 +
+                                                  (size_t)(a.Ma + 1) * (2 * a.Mb + 1) * n3 + (size_t)(a.Ma + 1) * n2 * n3);
+            }
+    const size_t o_gf = ar.take(gf_elems * sizeof(double2));
     const size_t o_gsums = ar.take(grad ? (size_t)blocks_total * 3 * GT * sizeof(double) : 0);
     if (int rc = ensure_scratch(ar.used)) return rc;
     unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
@@ -1008,8 +1019,34 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         }
         PM_HIP(hipEventRecord(g_ctx.evp1, g_ctx.stream));
         const long long slot_wg = (gr.n_slots + WG - 1) / WG;
-        if (gr.es)
+        if (gr.es && widom) {
             hipLaunchKernelGGL(point_fourier_kernel, dim3((unsigned)slot_wg), dim3(FOURIER_WG), 0, g_ctx.stream, g);
+        } else if (gr.es) {
+            // grids: the separable form, run by run
+            for (int si = 0; si < gr.n_seg; ++si) {
+                const SegArgs& sg = segs[gr.seg0 + si];
+                const PointRun& r = runs[sg.run];
+                if (!r.es) continue;
+                const RunArgs& a = r.ra;
+                GridFourierArgs f{};
+                f.A = a.A;
+                f.Ma = a.Ma; f.Mb = a.Mb; f.Mc = a.Mc;
+                for (int d = 0; d < 3; ++d) { f.n[d] = a.grid_n[d]; f.repd[d] = a.repd[d]; }
+                f.slot_base = sg.slot_base;
+                f.out_lr = g.out_lr;
+                f.z = reinterpret_cast<double2*>(base + o_gf);
+                f.B = f.z + (f.n[0] + f.n[1] + f.n[2]);
+                f.C = f.B + (long long)(a.Ma + 1) * (2 * a.Mb + 1) * f.n[2];
+                const long long n_z = (long long)f.n[0] + f.n[1] + f.n[2];
+                const long long n_b = (long long)(a.Ma + 1) * (2 * a.Mb + 1) * f.n[2];
+                const long long n_c = (long long)(a.Ma + 1) * f.n[1] * f.n[2];
+                const long long n_p = (long long)f.n[0] * f.n[1] * f.n[2];
+                hipLaunchKernelGGL(grid_phase_kernel, dim3(grid1d(n_z, 256)), dim3(256), 0, g_ctx.stream, f);
+                hipLaunchKernelGGL(grid_fourier_c_kernel, dim3(grid1d(n_b, 256)), dim3(256), 0, g_ctx.stream, f);
+                hipLaunchKernelGGL(grid_fourier_b_kernel, dim3(grid1d(n_c, 256)), dim3(256), 0, g_ctx.stream, f);
+                hipLaunchKernelGGL(grid_fourier_a_kernel, dim3(grid1d(n_p, 256)), dim3(256), 0, g_ctx.stream, f);
+            }
+        }
         if (grad && gr.es)
             hipLaunchKernelGGL(point_fourier_grad_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
         if (widom && grad)
