@@ -44,6 +44,8 @@
 #define M_PI 3.14159265358979323846
 #endif
 
+#include "oracle_cfg.h"
+
 /* src/vdw_energetics.jl:2 */
 #define R2_OVERLAP 0.1
 
@@ -581,21 +583,7 @@ void orc_insertion_pose(uint64_t seed, uint64_t g, const double c_to_f[9],
 /* Energies of insertions [g0, g0+count) go to out_vdw/out_es; block sums */
 /* (S, W_vdw, W_es) are formed sequentially in insertion order.           */
 /* ------------------------------------------------------------------ */
-typedef struct {
-    int64_t n_rep;
-    const double* xf; const int32_t* species; const double* q;
-    int64_t nq; const double* xq; const double* qq; /* nonzero charges only */
-    double F[9], c_to_f[9];
-    int32_t rep[3];
-    int32_t n_species; const double* eps; const double* sig2; double r2_cutoff;
-    int32_t n_lj; const double* lj_off; const int32_t* lj_species;
-    int32_t n_q; const double* q_off; const double* q_site;
-    int32_t charged; double alpha; double sr_cutoff;
-    int32_t es_mode;
-    int64_t nk_sk; const int32_t* m_sk; const double* a_re; const double* a_im;
-    int32_t kreps[3]; int64_t nk; const int32_t* kidx; const double* wt;
-    uint64_t seed;
-} orc_widom_cfg;
+/* orc_widom_cfg: oracle_cfg.h */
 
 static void widom_energy(const orc_widom_cfg* c, uint64_t g, double* vdw, double* es) {
     int32_t rotate = (c->n_lj + c->n_q) > 1;

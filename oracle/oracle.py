@@ -16,7 +16,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# PM_ORACLE_LIB names another build of the oracle (a sanitizer build, say)
+LIB_PATH = os.environ.get("PM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 D = ctypes.POINTER(ctypes.c_double)
 I32 = ctypes.POINTER(ctypes.c_int32)
@@ -73,6 +74,9 @@ def lib():
             "orc_phi_sr_abs": (ctypes.c_double, [ctypes.c_int64, D, D, D, D, ctypes.c_double,
                                                  ctypes.c_double]),
             "orc_widom_scales": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, D]),
+            "orc_widom_param_grad": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                                      ctypes.c_int64, D, D, D]),
+            "orc_pose_grad": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, I32, D, ctypes.c_int64, D, D, D]),
             "orc_energy_grid_scales": (ctypes.c_int32, [ctypes.c_void_p, I32, D]),
         }
         for name, (res, args) in sig.items():
@@ -244,6 +248,11 @@ class _Cfg(ctypes.Structure):
     ]
 
 
+class _ParamDual(ctypes.Structure):
+    _fields_ = [("n_ft", ctypes.c_int32), ("ft", I32), ("pure_eps", D), ("pure_sig", D),
+                ("n_unit", ctypes.c_int64), ("unit_xf", D), ("unit_q", D), ("unit_species", I32)]
+
+
 class OracleSystem:
     """A framework + force field + probe, replicated and Ewald-set-up by the
     oracle itself from raw inputs.
@@ -266,10 +275,13 @@ class OracleSystem:
         self.c_to_f = np.linalg.inv(self.F)
         unit_xf = _c(unit_xf).reshape(-1, 3)
         n_unit = unit_xf.shape[0]
+        self.unit_xf = unit_xf
+        self.unit_species = _c(np.asarray(species, dtype=np.int32), np.int32)
         nr = int(np.prod(R))
         self.xf = replicate(unit_xf, R)
         self.species = _c(np.tile(np.asarray(species, dtype=np.int32), nr), np.int32)
         q_unit = np.zeros(n_unit) if q is None else _c(q)
+        self.unit_q = q_unit
         self.q = _c(np.tile(q_unit, nr))
         nz = self.q != 0.0
         self.xq = _c(self.xf[nz])
@@ -407,3 +419,37 @@ class OracleSystem:
         out = np.zeros(int(np.prod(n_pts)))
         lib().orc_energy_grid(ctypes.byref(self.cfg), _i(n), _d(out))
         return out
+
+    # -- dual-number derivatives (oracle/dual.c) -----------------------------
+    def widom_param_grad(self, seed, g0, count, pure_eps, pure_sig, ft):
+        """Insertions [g0, g0 + count) of the stream: (E_vdw, E_es, dE) with
+        dE (count, len(ft), 3) = dE/d(eps_t, sigma_t, q_t) for the force-field
+        species ft (framework side; Lorentz-Berthelot mixing of the pure
+        parameters pure_eps / pure_sig, indexed by force-field species; q_t a
+        uniform shift of every framework atom of species t)."""
+        ft = np.asarray(ft, dtype=np.int32)
+        pe, ps = _c(pure_eps), _c(pure_sig)
+        p = _ParamDual(len(ft), _i(ft), _d(pe), _d(ps), self.unit_xf.shape[0], _d(self.unit_xf),
+                       _d(self.unit_q), _i(self.unit_species))
+        self.cfg.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        v, e = np.zeros(count), np.zeros(count)
+        d = np.zeros((count, len(ft), 3))
+        rc = lib().orc_widom_param_grad(ctypes.byref(self.cfg), ctypes.byref(p), int(g0), int(count), _d(v), _d(e),
+                                        _d(d))
+        if rc:
+            raise ValueError("orc_widom_param_grad: too many species or sites")
+        return v, e, d
+
+    def pose_grad(self, site_species, site_q, site_xf):
+        """Energies (n_poses,) and dE/d(site Cartesian xyz) (n_poses, n_sites,
+        3) of explicit poses, site_xf (n_poses, n_sites, 3) fractional in the
+        replicated box; site_species -1 for a site without an LJ sphere."""
+        x = _c(site_xf)
+        n_poses, n_sites = x.shape[0], x.shape[1]
+        sp = _c(np.asarray(site_species, dtype=np.int32), np.int32)
+        q = _c(site_q)
+        e = np.zeros(n_poses)
+        g = np.zeros((n_poses, n_sites, 3))
+        if lib().orc_pose_grad(ctypes.byref(self.cfg), n_sites, _i(sp), _d(q), n_poses, _d(x), _d(e), _d(g)):
+            raise ValueError("orc_pose_grad: at most 8 sites")
+        return e, g
