@@ -12,7 +12,9 @@ Differences, by design:
 * the reference runs Nelder-Mead (Optim) on ``vdw_energy``; here an L-BFGS
   iteration uses the analytic forces of ``pm_pose_gradient``, and many start
   poses advance together (``find_energy_minima``: one device call per
-  iteration for all of them);
+  iteration for all of them), followed by a batched Nelder-Mead polish of
+  every end point, which finds the lower energies that lie across the
+  cutoff's energy steps;
 * when both the crystal and the molecule carry charges the Ewald energy and
   its forces are included (the reference raises an error there).
 """
@@ -152,7 +154,9 @@ def _minimise(obj, x0, gtol, max_iter, probe=0.05, rounds=4):
     """L-BFGS to a stationary point, then a second-order check: the energy
     at 14 points probe A away (axes and diagonals) must not be lower; a pose
     that sits on a saddle or maximum (a zero gradient by symmetry, say)
-    resumes from its lowest probe."""
+    resumes from its lowest probe.  Last, a shell search and a Nelder-Mead
+    polish (the reference's own derivative-free method, src/energy_min.jl:52)
+    from each end point replace it where lower (``_step_search``)."""
     x, f, g, it, conv = _lbfgs(obj, x0, max_iter=max_iter, gtol=gtol)
     total = it
     for _ in range(rounds):
@@ -173,7 +177,137 @@ def _minimise(obj, x0, gtol, max_iter, probe=0.05, rounds=4):
         xs, fs, gs, its, cs = _lbfgs(lambda xx, ii: obj(xx, sub[ii]), x1[sub], max_iter=max_iter, gtol=gtol)
         x[sub], f[sub], g[sub], conv[sub] = xs, fs, gs, cs
         total += its
+    # derivative-free polish across the cutoff's energy steps
+    f_before = f.copy()
+    x, f = _step_search(obj, x, f)
+    moved = f < f_before
+    if moved.any():
+        mv = np.nonzero(moved)[0]
+        fb, gb = obj(x[mv], mv)
+        f[mv], g[mv] = fb, gb
     return x, f, g, total, conv
+
+
+def _nelder_mead(obj, x0, f0, step=0.1, max_iter=1000, ftol=1e-10, xtol=1e-7):
+    """Batched Nelder-Mead over P independent 3-dimensional problems, with
+    the dimension-adaptive coefficients of Gao and Han (2012, Comput. Optim.
+    Appl. 51:259) that Optim's NelderMead uses by default (reflection 1,
+    expansion 1 + 2/n, contraction 3/4 - 1/(2n), shrink 1 - 1/n).  The
+    initial simplex is x0 plus ``step`` A along each Cartesian axis.  A pose
+    stops when the spread (standard deviation) of its vertex energies is
+    below ``ftol`` K or every vertex lies within ``xtol`` A of its best.  At
+    most three device calls per iteration serve all poses.  Returns the best
+    vertex and its energy per pose."""
+    n = 3
+    a_r, a_e, a_c, a_s = 1.0, 1.0 + 2.0 / n, 0.75 - 0.5 / n, 1.0 - 1.0 / n
+    x0 = np.asarray(x0, dtype=np.float64)
+    P = x0.shape[0]
+    S = np.repeat(x0[:, None, :], n + 1, axis=1)
+    for i in range(n):
+        S[:, i + 1, i] += step
+    F = np.empty((P, n + 1))
+    F[:, 0] = f0
+    fv, _ = obj(S[:, 1:].reshape(-1, 3), np.repeat(np.arange(P), n))
+    F[:, 1:] = fv.reshape(P, n)
+    F = np.where(np.isfinite(F), F, np.inf)
+    active = np.isfinite(F[:, 0])
+    rows = np.arange(P)
+    for _ in range(max_iter):
+        o = np.argsort(F, axis=1, kind="stable")
+        S = S[rows[:, None], o]
+        F = F[rows[:, None], o]
+        spread = np.where(np.isfinite(F).all(axis=1), np.std(F, axis=1), np.inf)
+        size = np.max(np.linalg.norm(S[:, 1:] - S[:, :1], axis=2), axis=1)
+        active &= ~((spread < ftol) | (size < xtol))
+        if not active.any():
+            break
+        idx = np.nonzero(active)[0]
+        c = S[idx, :n].mean(axis=1)
+        worst = S[idx, n]
+        xr = c + a_r * (c - worst)
+        fr, _ = obj(xr, idx)
+        fr = np.where(np.isfinite(fr), fr, np.inf)
+        f0_, fn1, fn = F[idx, 0], F[idx, n - 1], F[idx, n]
+        take_r = (fr >= f0_) & (fr < fn1)
+        expand = fr < f0_
+        outside = (fr >= fn1) & (fr < fn)
+        inside = fr >= fn
+        x2 = np.where(expand[:, None], c + a_e * (xr - c),
+                      np.where(outside[:, None], c + a_c * (xr - c), c + a_c * (worst - c)))
+        need = ~take_r
+        f2 = np.full(len(idx), np.inf)
+        if need.any():
+            fv, _ = obj(x2[need], idx[need])
+            f2[need] = np.where(np.isfinite(fv), fv, np.inf)
+        new_x = np.where(take_r[:, None], xr, worst)
+        new_f = np.where(take_r, fr, fn)
+        # expansion: the better of the expanded and the reflected point
+        e_ok = expand & (f2 < fr)
+        new_x = np.where(e_ok[:, None], x2, np.where((expand & ~e_ok)[:, None], xr, new_x))
+        new_f = np.where(e_ok, f2, np.where(expand & ~e_ok, fr, new_f))
+        oc_ok = outside & (f2 <= fr)
+        ic_ok = inside & (f2 < fn)
+        new_x = np.where((oc_ok | ic_ok)[:, None], x2, new_x)
+        new_f = np.where(oc_ok | ic_ok, f2, new_f)
+        shrink = (outside & ~oc_ok) | (inside & ~ic_ok)
+        S[idx, n] = new_x
+        F[idx, n] = new_f
+        if shrink.any():
+            sh = idx[shrink]
+            S[sh, 1:] = S[sh, :1] + a_s * (S[sh, 1:] - S[sh, :1])
+            fv, _ = obj(S[sh, 1:].reshape(-1, 3), np.repeat(sh, n))
+            F[sh, 1:] = np.where(np.isfinite(fv), fv, np.inf).reshape(-1, n)
+    best = np.argmin(F, axis=1)
+    return S[rows, best], F[rows, best]
+
+
+def _shell_points(n_dirs=64, radii=(0.004, 0.008, 0.012, 0.016, 0.02, 0.025, 0.032, 0.04)):
+    """Offsets (A) on spheres of the given radii, n_dirs Fibonacci-lattice
+    directions each (the same set for every pose)."""
+    k = np.arange(n_dirs) + 0.5
+    z = 1.0 - 2.0 * k / n_dirs
+    phi = math.pi * (1.0 + math.sqrt(5.0)) * k
+    r = np.sqrt(1.0 - z * z)
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+    return np.concatenate([rad * d for rad in radii])
+
+
+_SHELL = _shell_points()
+
+
+def _step_search(obj, x, f, rounds=3, step=0.01):
+    """Search the shells around each end point for lower energies and, from
+    the lowest point found, polish by Nelder-Mead; repeated while it helps.
+    The hard cutoff makes the energy jump by 4 eps ((sigma/r_c)^12 -
+    (sigma/r_c)^6) wherever an atom crosses a cutoff sphere; next to a
+    stationary point of the smooth part the lowest energy can lie just across
+    such a step, in a slab a few thousandths of an A thick, where no gradient
+    points."""
+    P = x.shape[0]
+    active = np.isfinite(f)
+    for _ in range(rounds):
+        idx = np.nonzero(active)[0]
+        if len(idx) == 0:
+            break
+        m = len(_SHELL)
+        trial = (x[idx, None, :] + _SHELL[None, :, :]).reshape(-1, 3)
+        ft, _ = obj(trial, np.repeat(idx, m))
+        ft = np.where(np.isfinite(ft), ft, np.inf).reshape(len(idx), m)
+        best = np.argmin(ft, axis=1)
+        fb = ft[np.arange(len(idx)), best]
+        start_x = x[idx].copy()
+        start_f = f[idx].copy()
+        lower = fb < start_f
+        start_x[lower] = trial.reshape(len(idx), m, 3)[lower, best[lower]]
+        start_f[lower] = fb[lower]
+        xn, fn = _nelder_mead(lambda xx, ii: obj(xx, idx[ii]), start_x, start_f, step=step)
+        improved = fn < f[idx]
+        if not improved.any():
+            break
+        x[idx[improved]], f[idx[improved]] = xn[improved], fn[improved]
+        active = np.zeros(P, dtype=bool)
+        active[idx[improved]] = True
+    return x, f
 
 
 def _start_template(molecule, box):

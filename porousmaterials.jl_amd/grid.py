@@ -166,7 +166,10 @@ def required_n_pts(box: Box, dx: float):
     return tuple(int(math.ceil(n / dx)) + 1 for n in norms)
 
 
-def _grid_setup(crystal, molecule, ljff, resolution, units, center, verbose, ewald_precision):
+def _grid_setup(crystal, molecule, ljff, resolution, units, center, verbose, ewald_precision,
+                electrostatics=False):
+    """Device handles of one grid, or ``None`` in place of them for a probe
+    without LJ spheres when electrostatics are off (its grid is all zeros)."""
     assert_P1_symmetry(crystal)
     if units not in ("kJ_mol", "K"):
         raise ValueError("Pass kJ_mol or K for units of kJ/mol or K, respectively.")
@@ -177,12 +180,16 @@ def _grid_setup(crystal, molecule, ljff, resolution, units, center, verbose, ewa
     if needs_rotations(molecule):
         raise NotImplementedError("energy grids of multi-site probes (Boltzmann average over "
                                   "rotations) are not supported")
-    charged = xtal_has_charges(crystal) and mol_has_charges(molecule)
+    charged = electrostatics and xtal_has_charges(crystal) and mol_has_charges(molecule)
     grid_origin = crystal.box.f_to_c @ np.array([-0.5, -0.5, -0.5]) if center else np.zeros(3)
     if verbose:
         print(f"Computing energy grid of {molecule.species} in {crystal.name}")
         print(f"\tRegular grid (in fractional space) of {n_pts[0]} by {n_pts[1]} by {n_pts[2]} "
               f"points superimposed over the unit cell.")
+    if molecule.n_atoms == 0 and not charged:
+        # a single-site probe without an LJ sphere is a bare point charge: the
+        # reference's grid is its vdw_energy, zero everywhere (src/grid.jl:337-338)
+        return (None, None, None, None, n_pts), grid_origin
     repfactors = replication_factors(crystal.box, ljff)
     xtal = replicate(crystal, repfactors)
     fw = device.framework(xtal, ljff)
@@ -203,22 +210,35 @@ def _to_grid(crystal, n_pts, flat, units, origin):
 
 
 def energy_grid(crystal, molecule, ljff, resolution=1.0, n_rotations=1000, temperature=math.nan,
-                units="kJ_mol", center=False, verbose=True, precision="mixed",
-                ewald_precision=1e-6) -> Grid:
+                units="kJ_mol", center=False, verbose=True, precision="fp64",
+                ewald_precision=1e-6, electrostatics=False) -> Grid:
+    """``src/grid.jl:291-362`` for single-site probes, on the device.
+
+    As in the reference, the energy at a point is the probe's ``vdw_energy``
+    alone (src/grid.jl:337-338), so a bare point charge (the only charged
+    single-site probe) gives a grid of zeros.  ``electrostatics=True`` (no
+    counterpart in the reference) adds the probe charge times the framework's
+    Ewald potential, the point-charge energy of
+    ``electrostatic_potential_energy(crystal, ion(q, xf), ...)``."""
     run, origin = _grid_setup(crystal, molecule, ljff, resolution, units, center, verbose,
-                              ewald_precision)
-    flat = device.energy_grid(*run, precision)
+                              ewald_precision, electrostatics)
+    if run[3] is None:
+        flat = np.zeros(int(np.prod(run[4])))
+    else:
+        flat = device.energy_grid(*run, precision)
     return _to_grid(crystal, run[4], flat, units, origin)
 
 
 def energy_grids(crystals, molecule, ljff, resolution=1.0, units="kJ_mol", center=False,
-                 verbose=False, precision="mixed", ewald_precision=1e-6):
+                 verbose=False, precision="fp64", ewald_precision=1e-6, electrostatics=False):
     """``energy_grid`` of ``molecule`` in every crystal of ``crystals``, all
     grids in one device call (pm_energy_grid_batch).  Each grid equals what
     ``energy_grid`` returns for that crystal."""
     crystals = list(crystals)
-    setups = [_grid_setup(x, molecule, ljff, resolution, units, center, verbose, ewald_precision)
+    setups = [_grid_setup(x, molecule, ljff, resolution, units, center, verbose, ewald_precision, electrostatics)
               for x in crystals]
-    flats = device.energy_grid_batch([run for run, _ in setups], precision)
-    return [_to_grid(x, run[4], flat, units, origin)
-            for x, (run, origin), flat in zip(crystals, setups, flats)]
+    live = [run for run, _ in setups if run[3] is not None]
+    flats = iter(device.energy_grid_batch(live, precision) if live else [])
+    return [_to_grid(x, run[4], next(flats) if run[3] is not None else np.zeros(int(np.prod(run[4]))),
+                     units, origin)
+            for x, (run, origin) in zip(crystals, setups)]

@@ -150,7 +150,7 @@ def _finish(results, crystal, molecule, temperature, ljff, insertions_per_volume
 
 def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volume=200,
                       verbose=True, ewald_precision=1e-6, autosave=True, filename_comment="",
-                      accessibility_grid=None, seed=0, precision="mixed", return_block_sums=False):
+                      accessibility_grid=None, seed=0, precision="fp64", return_block_sums=False):
     """K_H and <U> of ``molecule`` (Cartesian template) in ``crystal``."""
     t0 = time.time()
     acc = _access_handle(crystal, accessibility_grid, verbose)
@@ -167,7 +167,7 @@ def henry_coefficient(crystal, molecule, temperature, ljff, insertions_per_volum
 
 def henry_coefficients(pairs, temperature, ljff, insertions_per_volume=200, verbose=False,
                        ewald_precision=1e-6, autosave=False, filename_comment="", seeds=None,
-                       precision="mixed", return_block_sums=False, accessibility_grids=None):
+                       precision="fp64", return_block_sums=False, accessibility_grids=None):
     """Screening: ``henry_coefficient`` for every (crystal, molecule) in
     ``pairs``, all runs in one device call (pm_widom_batch).  Each result is
     what ``henry_coefficient`` returns for that pair with the same seed
@@ -199,7 +199,7 @@ def henry_coefficients(pairs, temperature, ljff, insertions_per_volume=200, verb
 
 
 def henry_coefficient_gradient(crystal, molecule, temperature, ljff, insertions_per_volume=200,
-                               ewald_precision=1e-6, seed=0, precision="mixed", verbose=False,
+                               ewald_precision=1e-6, seed=0, precision="fp64", verbose=False,
                                accessibility_grid=None):
     """K_H (the ``henry_coefficient`` results dict) and its logarithmic
     derivatives with respect to the framework species' pure LJ parameters and

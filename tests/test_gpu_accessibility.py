@@ -25,6 +25,7 @@ import sys
 import numpy as np
 import pytest
 
+from _parity import TOL, check_energies, describe
 from conftest import golden
 
 pytestmark = pytest.mark.gpu
@@ -145,10 +146,9 @@ def test_widom_with_accessibility_matches_oracle(pm, orc, oacc, uff, ch4, lta_bl
     assert np.array_equal(np.isinf(en[open_, 0]), np.isinf(v[open_]))
     fin = open_ & np.isfinite(v)
     scale = np.array([o.vdw_scale(orc.insertion_uniforms(seed, i)[:3].reshape(1, 3)) for i in np.flatnonzero(fin)])
-    err = np.abs(en[fin, 0] - v[fin]) / (scale + 1.0)
-    tol = 1e-10 if precision == "fp64" else 1e-5
-    say(f"   max scaled energy error {err.max():.2e}")
-    assert err.max() <= tol
+    res = check_energies(en[fin, 0], 0.0, v[fin], 0.0, TOL[precision], "open insertions", cond_scale=scale,
+                         groups=np.repeat(np.arange(5), nb)[fin])
+    say(f"   {describe(res)}")
     assert np.all(en[open_, 1] == 0.0)
     if precision == "fp64":
         ve = np.where(blocked, np.inf, v)

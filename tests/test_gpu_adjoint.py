@@ -4,8 +4,12 @@ d ln K_H / d theta for theta = (eps_t, sigma_t, q_t) of the 5 framework
 species is checked against central finite differences (h = 1e-4 relative) of
 the oracle's FP64 ln K_H on the same insertion stream, to 1e-4 relative
 (floor: 1e-6 of the largest component).  q_t is a uniform shift of every
-atom of species t (h = 1e-4 of their mean |q|).  The pass's block sums must
-equal pm_widom's bit for bit."""
+atom of species t (h = 1e-4 of their mean |q|).  The per-block weighted
+derivative sums sum_i w_i dE_i/d theta are checked against the oracle's
+dual-number derivatives (oracle/dual.c, the forward-mode check) on the same
+stream: 1e-8 (FP64) and 1e-4 (mixed) relative, floor 1e-8 / 1e-6 of the
+block's largest component.  The pass's block sums must equal pm_widom's bit
+for bit."""
 import math
 import sys
 
@@ -104,6 +108,44 @@ def test_dlnKH_matches_oracle_finite_differences(pm, orc, adj, precision):
             f"(FD {fd[3 * sp_i + 1]:+.6e}), d/d q {dev[3 * sp_i + 2]:+.6e} (FD {fd[3 * sp_i + 2]:+.6e})")
     say(f"   max relative deviation from FD: {err.max():.2e}")
     assert err.max() <= 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_block_derivative_sums_match_dual_oracle(pm, orc, adj, precision):
+    case, fw, ew, pr = adj
+    ff, x, co2 = case.ljff, case.crystal, case.probes["CO2"]
+    nb, seed = 2000, 31
+    say(f"case: sum_i w_i dE_i/d theta per block, CO2 5 x {nb} insertions, vs the dual-number oracle ({precision})")
+    sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, T, seed, N_BLOCKS, nb, precision, ff)
+    lj_off, q_off = pm.site_offsets(co2)
+    o = orc.OracleSystem((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
+                         [ff.index(s) for s in x.species], x.atom_q, ff.epsilon_table, ff.sigma2_table,
+                         math.sqrt(ff.r2_cutoff), lj_off.T, [ff.index(s) for s in co2.atom_species], q_off.T,
+                         co2.charges_q, ewald_precision=case.ewald_precision)
+    pe = np.array([ff.pure_epsilon[s] for s in ff.species])
+    ps = np.array([ff.pure_sigma[s] for s in ff.species])
+    ft = sorted({ff.index(s) for s in x.species})
+    v, e, d = o.widom_param_grad(seed, 0, N_BLOCKS * nb, pe, ps, ft)
+    E = v + e
+    fin = np.isfinite(E)
+    w = np.where(fin, np.exp(-np.where(fin, E, 0.0) / T), 0.0)
+    want = np.stack([(w[b * nb:(b + 1) * nb, None, None] * d[b * nb:(b + 1) * nb]).sum(axis=0)
+                     for b in range(N_BLOCKS)])                      # (blocks, species, 3)
+    got = grads[:, ft, :]
+    assert np.all(grads[:, [k for k in range(grads.shape[1]) if k not in ft], :] == 0.0)
+    rtol, floor = (1e-8, 1e-8) if precision == "fp64" else (1e-4, 1e-6)
+    worst = 0.0
+    for b in range(N_BLOCKS):
+        scale = np.maximum(np.abs(want[b]), floor * np.abs(want[b]).max())
+        err = np.abs(got[b] - want[b]) / scale
+        worst = max(worst, float(err.max()))
+    S = sums[:, 0].sum()
+    g_dev = -got.sum(axis=0) / (T * S)
+    g_dual = -want.sum(axis=0) / (T * w.sum())
+    dl = np.abs(g_dev - g_dual) / np.maximum(np.abs(g_dual), floor * np.abs(g_dual).max())
+    say(f"   per-block weighted derivative sums: max relative deviation {worst:.2e}; d ln K_H/d theta: "
+        f"{dl.max():.2e}")
+    assert worst <= rtol and dl.max() <= rtol
 
 
 def test_henry_coefficient_gradient_api(pm):

@@ -5,21 +5,16 @@ Cartesian coordinates, FP32 partial sums ordered by the framework's image
 list), so a batch must reproduce every run's stand-alone result bit for bit,
 whatever the batch composition, scheduling order or launch-group split.  The
 screening regime itself (small cells replicated up to 5x5x5, thousands of
-k-vectors) is checked against the oracle per insertion."""
+k-vectors) is checked against the oracle per insertion, to the tolerances of
+tests/_parity.py."""
 import math
-import sys
 
 import numpy as np
 import pytest
 
+from _parity import TOL, check_energies, describe, say
+
 pytestmark = pytest.mark.gpu
-
-TOL = {"fp64": 1e-10, "mixed": 1e-5}
-
-
-def say(msg):
-    print(msg, flush=True)
-    sys.stdout.flush()
 
 
 def _handles(pm, case, probe_name):
@@ -70,8 +65,8 @@ def test_widom_batch_split_across_launch_groups(pm):
 
 def test_screening_regime_vs_oracle(pm, orc, screen):
     """Per-insertion CO2 energies in the smallest and the largest replicated
-    screening frameworks vs the oracle (FP64 to 1e-10, mixed to 1e-5 of the
-    condition scale)."""
+    screening frameworks vs the oracle, to the bounds of tests/_parity.py
+    (FP64 1e-10, mixed 1e-5)."""
     sizes = [(pm.replication_factors(c.crystal.box, c.ljff), c) for c in screen]
     by_nrep = sorted(sizes, key=lambda t: int(np.prod(t[0])) * t[1].crystal.n_atoms)
     for rep, c in (by_nrep[0], by_nrep[-1]):
@@ -91,13 +86,9 @@ def test_screening_regime_vs_oracle(pm, orc, screen):
         scale = o.widom_scales(31, 0, 5 * nb)
         for precision in ("fp64", "mixed"):
             _, en = pm.device.widom(fw, ff, ew, pr, 298.0, 31, 5, nb, precision, return_energies=True)
-            got, want = en[:, 0] + en[:, 1], v + e
-            inf_g, inf_w = np.isinf(got), np.isinf(want)
-            assert np.array_equal(inf_g, inf_w)
-            fin = ~inf_w
-            err = np.abs(got[fin] - want[fin]) / (scale[fin] + 1.0)
-            say(f"   {precision}: max scaled error {err.max():.2e} over {fin.sum()} finite insertions")
-            assert err.max() <= TOL[precision]
+            res = check_energies(en[:, 0], en[:, 1], v, e, TOL[precision], f"screening ({precision})",
+                                 cond_scale=scale, groups=np.repeat(np.arange(5), nb))
+            say(f"   {precision}: {describe(res)}")
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])

@@ -155,6 +155,17 @@ def test_synthetic_cases_deterministic(pm):
     assert g.crystal.n_atoms == 424 and pm.replication_factors(g.crystal.box, g.ljff) == (1, 1, 1)
 
 
+def test_energy_grid_reference_mode_bare_charge(pm):
+    """src/grid.jl:337-338: the reference's grid of a single-site probe is its
+    vdw_energy, so a bare point charge gives zeros in K and kJ/mol (no device
+    work is needed for it)."""
+    case = pm.synthetic.co2_widom_case(n_per_block=1)
+    for units in ("K", "kJ_mol"):
+        g = pm.energy_grid(case.crystal, pm.point_charge(1.0), case.ljff, resolution=(4, 3, 2), units=units,
+                           verbose=False)
+        assert g.data.shape == (4, 3, 2) and not g.data.any() and g.units == units
+
+
 # ------------------------------------------------------------- C ABI ----
 def test_library_exports_every_declared_symbol(pm):
     from porousmaterials_jl_amd import _lib
