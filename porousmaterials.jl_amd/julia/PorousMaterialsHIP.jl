@@ -92,10 +92,50 @@ function ff_tables(ljff::LJForceField)
     return sp, eps, sig2
 end
 
-const FF_CACHE = WeakKeyDict{LJForceField, Handle}()
+# ------------------------------------------------------------ caches ----
+# Device handles of reference objects are cached under a fingerprint of the
+# object's contents.  `LJForceField` (src/forcefield.jl:12) and Xtals'
+# `Crystal` are immutable structs: Julia refuses finalizers on them, so they
+# can key neither a WeakKeyDict nor a finalizer.  A content fingerprint also
+# changes when a caller edits a crystal's arrays in place, so a stale device
+# copy is never used.  Each cache keeps at most CACHE_LIMIT handles (it is
+# emptied when full); `release_handles!()` empties both.  A handle removed from
+# a cache is freed by its finalizer once no call holds it.
+const CACHE_LIMIT = 64
+const FF_CACHE = Dict{UInt64, Handle}()
+const FW_CACHE = Dict{UInt64, Handle}()
+
+fingerprint(ljff::LJForceField) =
+    hash((ljff.name, ljff.pure_σ, ljff.pure_ϵ, ljff.σ², ljff.ϵ, ljff.r²_cutoff))
+
+fingerprint(crystal::Crystal) =
+    hash((crystal.box.f_to_c, crystal.atoms.species, crystal.atoms.coords.xf, crystal.charges.q,
+          crystal.charges.coords.xf))
+
+function cached!(make::Function, cache::Dict{UInt64, Handle}, key::UInt64)
+    h = get(cache, key, nothing)
+    h === nothing || return h
+    length(cache) >= CACHE_LIMIT && empty!(cache)
+    h = make()
+    cache[key] = h
+    return h
+end
+
+"""
+    release_handles!()
+
+Drop every cached device handle (force fields, frameworks, Ewald sums); their
+device memory is freed by the handles' finalizers once no call holds them.
+"""
+function release_handles!()
+    empty!(FF_CACHE)
+    empty!(FW_CACHE)
+    empty!(EW_CACHE)
+    return nothing
+end
 
 function forcefield_handle(ljff::LJForceField)
-    get!(FF_CACHE, ljff) do
+    cached!(FF_CACHE, fingerprint(ljff)) do
         sp, eps, sig2 = ff_tables(ljff)
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:pm_forcefield_create, libpmhip), Cint,
@@ -135,28 +175,21 @@ function framework_arrays(crystal::Crystal, ljff::LJForceField)
     return xf, species, q
 end
 
-const FW_CACHE = WeakKeyDict{Crystal, Dict{Any, Handle}}()
-
 # species-only force field (every epsilon 0) for Ewald-only device frameworks
-const ES_FF_CACHE = WeakKeyDict{Crystal, LJForceField}()
-
 function es_only_forcefield(crystal::Crystal, r_cutoff::Float64)
-    get!(ES_FF_CACHE, crystal) do
-        species = unique(crystal.atoms.species)
-        pure_σ = Dict{Symbol, Float64}(s => 1.0 for s in species)
-        pure_ϵ = Dict{Symbol, Float64}(s => 0.0 for s in species)
-        σ² = Dict{Symbol, Dict{Symbol, Float64}}(a => Dict{Symbol, Float64}(b => 1.0 for b in species)
-                                                 for a in species)
-        ϵ = Dict{Symbol, Dict{Symbol, Float64}}(a => Dict{Symbol, Float64}(b => 0.0 for b in species)
-                                                for a in species)
-        LJForceField("electrostatics-only", pure_σ, pure_ϵ, σ², ϵ, r_cutoff^2)
-    end
+    species = unique(crystal.atoms.species)
+    pure_σ = Dict{Symbol, Float64}(s => 1.0 for s in species)
+    pure_ϵ = Dict{Symbol, Float64}(s => 0.0 for s in species)
+    σ² = Dict{Symbol, Dict{Symbol, Float64}}(a => Dict{Symbol, Float64}(b => 1.0 for b in species)
+                                             for a in species)
+    ϵ = Dict{Symbol, Dict{Symbol, Float64}}(a => Dict{Symbol, Float64}(b => 0.0 for b in species)
+                                            for a in species)
+    return LJForceField("electrostatics-only", pure_σ, pure_ϵ, σ², ϵ, r_cutoff^2)
 end
 
 # `crystal` is the unit cell; `rep` the replication factors the reference uses
 function framework_handle(crystal::Crystal, ljff::LJForceField, rep::Tuple{Int, Int, Int})
-    cache = get!(() -> Dict{Any, Handle}(), FW_CACHE, crystal)
-    get!(cache, (objectid(ljff), rep)) do
+    cached!(FW_CACHE, hash((fingerprint(crystal), fingerprint(ljff), rep))) do
         xf, species, q = framework_arrays(crystal, ljff)
         out = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:pm_framework_create, libpmhip), Cint,
@@ -168,14 +201,30 @@ function framework_handle(crystal::Crystal, ljff::LJForceField, rep::Tuple{Int, 
     end
 end
 
+fingerprint(eparams::EwaldParams) =
+    hash((eparams.α, eparams.sr_cutoff_r, [(k.ka, k.kb, k.kc, k.wt) for k in eparams.kvecs]))
+
+# Ewald handles per (framework handle, Ewald parameters); an entry holds its
+# framework handle, so that handle stays alive (and its address unreused)
+# while the entry exists
+const EW_CACHE = Dict{UInt64, Tuple{Handle, Handle}}()
+
 function ewald_handle(fw::Handle, eparams::EwaldParams)
+    key = hash((objectid(fw), fingerprint(eparams)))
+    hit = get(EW_CACHE, key, nothing)
+    if hit !== nothing && hit[1] === fw
+        return hit[2]
+    end
     kidx = Int32[v for kv in eparams.kvecs for v in (kv.ka, kv.kb, kv.kc)]
     wt = Float64[kv.wt for kv in eparams.kvecs]
     out = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:pm_ewald_create, libpmhip), Cint,
                 (Ptr{Cvoid}, Float64, Float64, Int64, Ptr{Int32}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
                 fw.ptr, eparams.α, eparams.sr_cutoff_r, Int64(length(wt)), kidx, wt, out))
-    return Handle(out[], :pm_ewald_destroy)
+    ew = Handle(out[], :pm_ewald_destroy)
+    length(EW_CACHE) >= CACHE_LIMIT && empty!(EW_CACHE)
+    EW_CACHE[key] = (fw, ew)
+    return ew
 end
 
 function probe_handle(molecule::Molecule{Cart}, ljff::LJForceField)
@@ -289,7 +338,7 @@ end
 """
     henry_coefficient(crystal, molecule, temperature, ljff; insertions_per_volume=200,
                       verbose=true, ewald_precision=1e-6, autosave=true,
-                      filename_comment="", seed=0, precision=:mixed)
+                      filename_comment="", seed=0, precision=:fp64)
 
 `src/henry.jl:34-181` with the insertions on the device: same insertion count,
 blocks, replication, statistics and 16-key results `Dict`.  Insertions come
@@ -300,7 +349,7 @@ function henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Fl
                            verbose::Bool=true, ewald_precision::Float64=1e-6,
                            autosave::Bool=true, filename_comment::AbstractString="",
                            accessibility_grid::Union{Nothing, Grid{Bool}}=nothing, seed::Integer=0,
-                           precision::Symbol=:mixed)
+                           precision::Symbol=:fp64)
     assert_P1_symmetry(crystal)
     acc = access_handle(crystal, accessibility_grid, verbose)
     time_start = time()
@@ -361,7 +410,7 @@ end
 """
     henry_coefficients(pairs, temperature, ljff; insertions_per_volume=200,
                        ewald_precision=1e-6, seeds=zeros(Int, length(pairs)),
-                       precision=:mixed)
+                       precision=:fp64)
 
 Screening: `henry_coefficient` for every `(crystal, molecule)` in `pairs`, all
 runs in one `pm_widom_batch` call.  Each results `Dict` equals the one
@@ -370,7 +419,7 @@ runs in one `pm_widom_batch` call.  Each results `Dict` equals the one
 function henry_coefficients(pairs::Vector{<:Tuple{Crystal, Molecule}}, temperature::Float64,
                             ljff::LJForceField; insertions_per_volume::Union{Int, Float64}=200,
                             ewald_precision::Float64=1e-6, seeds=zeros(Int, length(pairs)),
-                            precision::Symbol=:mixed,
+                            precision::Symbol=:fp64,
                             accessibility_grids=fill(nothing, length(pairs)))
     time_start = time()
     handles = Any[]   # keeps every handle alive until the call returns
@@ -418,26 +467,38 @@ end
 """
     energy_grid(crystal, molecule, ljff; resolution=1.0, n_rotations=1000,
                 temperature=NaN, units=:kJ_mol, center=false, verbose=true,
-                precision=:mixed)
+                precision=:fp64, electrostatics=false)
 
-`src/grid.jl:291-362` for single-site probes on the device; a single
-point-charge probe gives the per-point Ewald energy.
+`src/grid.jl:291-362` for single-site probes on the device.  As in the
+reference, a point's energy is the probe's `vdw_energy` alone
+(src/grid.jl:337-338), so a bare point charge (the only charged single-site
+probe) gives a grid of zeros.  `electrostatics=true` (no counterpart in the
+reference) adds the probe charge times the framework's Ewald potential, the
+point-charge energy of `electrostatic_potential_energy(crystal, ion(q, xf), ...)`.
 """
 function energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljforcefield::LJForceField;
                      resolution::Union{Float64, Tuple{Int, Int, Int}}=1.0, n_rotations::Int=1000,
                      temperature::Float64=NaN, units::Symbol=:kJ_mol, center::Bool=false,
-                     verbose::Bool=true, precision::Symbol=:mixed, ewald_precision::Float64=1e-6)
+                     verbose::Bool=true, precision::Symbol=:fp64, ewald_precision::Float64=1e-6,
+                     electrostatics::Bool=false)
     assert_P1_symmetry(crystal)
     units in (:kJ_mol, :K) || error("Pass :kJ_mol or :K for units of kJ/mol or K, respectively.")
     needs_rotations(molecule) &&
         error("energy grids of multi-site probes are not supported by the device path")
     n_pts = isa(resolution, Tuple{Int, Int, Int}) ? resolution : required_n_pts(crystal.box, resolution)
+    origin = center ? crystal.box.f_to_c * [-0.5, -0.5, -0.5] : [0.0, 0.0, 0.0]
+    charged = electrostatics && has_charges(crystal) && has_charges(molecule)
+    # a single-site probe without an LJ sphere is a bare point charge: the
+    # reference's grid is its vdw_energy, zero everywhere (src/grid.jl:337-338)
+    if molecule.atoms.n == 0 && !charged
+        return Grid(crystal.box, n_pts, zeros(Float64, n_pts...), units, origin)
+    end
     repfactors = replication_factors(crystal.box, ljforcefield)
     fw = framework_handle(crystal, ljforcefield, repfactors)
     ff = forcefield_handle(ljforcefield)
     ew = C_NULL
     ew_h = nothing
-    if has_charges(crystal) && has_charges(molecule)
+    if charged
         xtal_rep = replicate(crystal, repfactors)
         eparams = setup_Ewald_sum(xtal_rep.box, sqrt(ljforcefield.r²_cutoff), ϵ=ewald_precision)
         ew_h = ewald_handle(fw, eparams)
@@ -445,14 +506,15 @@ function energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljforcefield::L
     end
     pr = probe_handle(molecule, ljforcefield)
     data = zeros(Float64, n_pts...)
-    check(ccall((:pm_energy_grid, libpmhip), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Float64}),
-                fw.ptr, ff.ptr, ew, pr.ptr, Int32[n_pts...], precision_code(precision), data))
+    GC.@preserve ew_h begin
+        check(ccall((:pm_energy_grid, libpmhip), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Float64}),
+                    fw.ptr, ff.ptr, ew, pr.ptr, Int32[n_pts...], precision_code(precision), data))
+    end
     if units == :kJ_mol
         data .= data .* 8.314 ./ 1000.0   # the reference's order of operations (src/grid.jl:358)
     end
-    return Grid(crystal.box, n_pts, data, units,
-                center ? crystal.box.f_to_c * [-0.5, -0.5, -0.5] : [0.0, 0.0, 0.0])
+    return Grid(crystal.box, n_pts, data, units, origin)
 end
 
 # ------------------------------------------------------ accessibility ----
@@ -586,7 +648,7 @@ charges of its atoms: `grad[species] = (epsilon=..., sigma=..., charge=...)`.
 function henry_coefficient_gradient(crystal::Crystal, molecule::Molecule, temperature::Float64,
                                     ljff::LJForceField; mixing_rules::AbstractString="Lorentz-Berthelot",
                                     insertions_per_volume::Union{Int, Float64}=200,
-                                    ewald_precision::Float64=1e-6, seed::Integer=0, precision::Symbol=:mixed,
+                                    ewald_precision::Float64=1e-6, seed::Integer=0, precision::Symbol=:fp64,
                                     accessibility_grid::Union{Nothing, Grid{Bool}}=nothing)
     acc = access_handle(crystal, accessibility_grid, false)
     nb_insertions = max(N_BLOCKS, ceil(Int, insertions_per_volume * crystal.box.Ω))
@@ -1068,8 +1130,13 @@ stepwise_adsorption_isotherm(xtal::Crystal, molecule_template::Molecule{Cart}, t
 """
     enable!()
 
-Route PorousMaterials' own `henry_coefficient`, `energy_grid` and `μVT_sim`
-through the device (methods added to, or replacing, the reference's).
+Route PorousMaterials' own `henry_coefficient`, `energy_grid`, `vdw_energy`,
+`electrostatic_potential_energy`, `total_electrostatic_potential_energy` and
+`μVT_sim` through the device (methods replacing the reference's for the same
+argument types).  Every routed call computes in FP64, as the reference does;
+`precision=:mixed` stays an explicit opt-in of this module's own methods.
+The reference's guest-guest `vdw_energy` methods (molecule vs molecules) are
+left as they are.
 """
 function enable!()
     @eval PorousMaterials.henry_coefficient(crystal::Crystal, molecule::Molecule, temperature::Float64,
@@ -1078,6 +1145,15 @@ function enable!()
     @eval PorousMaterials.energy_grid(crystal::Crystal, molecule::Molecule{Cart}, ljff::LJForceField;
                                       kwargs...) =
         PorousMaterialsHIP.energy_grid(crystal, molecule, ljff; kwargs...)
+    @eval PorousMaterials.vdw_energy(crystal::Crystal, molecule::Molecule, ljff::LJForceField) =
+        PorousMaterialsHIP.vdw_energy(crystal, molecule, ljff)
+    @eval PorousMaterials.electrostatic_potential_energy(crystal::Crystal, molecule::Molecule{Frac},
+                                                         eparams::EwaldParams, eikr::Eikr) =
+        PorousMaterialsHIP.electrostatic_potential_energy(crystal, molecule, eparams, eikr)
+    @eval PorousMaterials.total_electrostatic_potential_energy(crystal::Crystal,
+                                                               molecules::Array{Molecule{Frac}, 1},
+                                                               eparams::EwaldParams, eikr::Eikr) =
+        PorousMaterialsHIP.total_electrostatic_potential_energy(crystal, molecules, eparams, eikr)
     # the reference's isotherm drivers call μVT_sim, so they reach the device too
     @eval PorousMaterials.μVT_sim(xtal::Crystal, molecule_templates::Array{Molecule{Cart}, 1},
                                   temperature::Float64, pressures::Array{Float64, 1}, ljff::LJForceField;
