@@ -52,6 +52,18 @@ def flops_all_pairs(n_rep, omega_rep, r_cut, n_lj, n_q, n_k, sr_cut):
             + n_q * in_cutoff(n_rep, omega_rep, sr_cut) * C_SR + n_q * n_k * C_K)
 
 
+def cpu_model():
+    """Model name of this host's CPU (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(pm, case, xr, seed, target_s):
     from oracle import oracle as orc
     x, ff, co2 = case.crystal, case.ljff, case.probes["CO2"]
@@ -78,7 +90,7 @@ def cpu_baseline(pm, case, xr, seed, target_s):
     o.widom_energies(seed, 0, m, es_mode=0)
     sk_rate = m / (time.perf_counter() - t)
     return {
-        "value": ref_rate, "unit": "insertions/s", "cores": threads, "kind": "port",
+        "value": ref_rate, "unit": "insertions/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
         "sample": (f"{n} insertions of this workload (stream offset {n0}), reference form of the "
                    f"Fourier sum (e^(ik.r) tables over {xr.n_atoms} replicated charges x "
                    f"{o.wt.size} replicated-box k-vectors per site), OpenMP over insertions"),
@@ -126,24 +138,53 @@ def gcmc_leg(pm, cycles=100):
 
 
 def cpu_gcmc_baseline(pm, target_s):
-    """The oracle's chain (oracle/gcmc.c, one core) on the GCMC leg's
-    workload: Markov steps per second."""
+    """The oracle's chain (oracle/gcmc.c) on the GCMC leg's workload, one
+    independent chain per host thread (the oracle's OpenMP thread count; each
+    chain's C loop runs with the GIL released): Markov steps per second summed
+    over the chains, as the device figure sums over its chains."""
+    import threading
+
     from oracle import gcmc as ogc
+    from oracle import oracle as orc
     x, ff, xe = _gcmc_setup(pm)
-    oc = ogc.OracleChain((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
-                         [ff.index(s) for s in x.species], ff.epsilon_table, ff.sigma2_table,
-                         math.sqrt(ff.r2_cutoff), [((xe.atoms_x - xe.com[:, None]).T,
-                                                    [ff.index(s) for s in xe.atom_species], False)],
-                         298.0, [0.5e5], 10 ** 6, 10 ** 6, 1, capacity=512)
-    oc.run(FILL_CYCLES)
-    t0, steps, t = oc.state.t, 0, time.perf_counter()
-    while time.perf_counter() - t < target_s:
-        oc.run(10)
+    n_thr = orc.lib().orc_num_threads()
+
+    def chain(seed):
+        return ogc.OracleChain((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
+                               [ff.index(s) for s in x.species], ff.epsilon_table, ff.sigma2_table,
+                               math.sqrt(ff.r2_cutoff), [((xe.atoms_x - xe.com[:, None]).T,
+                                                          [ff.index(s) for s in xe.atom_species], False)],
+                               298.0, [0.5e5], 10 ** 6, 10 ** 6, seed, capacity=512)
+
+    chains = [chain(1 + c) for c in range(n_thr)]
+
+    def fill(oc):
+        oc.run(FILL_CYCLES)
+
+    ths = [threading.Thread(target=fill, args=(oc,)) for oc in chains]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    t0 = [oc.state.t for oc in chains]
+    stop = time.perf_counter() + target_s
+
+    def go(oc):
+        while time.perf_counter() < stop:
+            oc.run(10)
+
+    t = time.perf_counter()
+    ths = [threading.Thread(target=go, args=(oc,)) for oc in chains]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
     dt = time.perf_counter() - t
-    steps = oc.state.t - t0
-    return {"value": steps / dt, "unit": "MC steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} Markov steps of one chain of the GCMC leg's workload after {FILL_CYCLES} cycles, "
-                      f"<N> {oc.state.n[0]}"}
+    steps = sum(oc.state.t - a for oc, a in zip(chains, t0))
+    return {"value": steps / dt, "unit": "MC steps/s", "cores": n_thr, "kind": "port", "cpu_model": cpu_model(),
+            "sample": (f"{steps} Markov steps of {n_thr} independent chains (one per host thread, seeds 1..{n_thr}) "
+                       f"of the GCMC leg's workload after {FILL_CYCLES} cycles each, "
+                       f"<N> {np.mean([oc.state.n[0] for oc in chains]):.1f}")}
 
 
 def screening_leg(pm, steps, n_per_block=20000):
@@ -223,7 +264,7 @@ def grids_leg(pm, steps):
     def measure(case, probe_name, precision):
         probe = case.probes[probe_name]
         run, _ = _grid_setup(case.crystal, probe, case.ljff, case.meta["n_pts"], "K", False, False,
-                             case.ewald_precision)
+                             case.ewald_precision, electrostatics=True)
         pm.device.energy_grid(*run, precision)  # warmup: image list, scratch
         pipe, pair, cand = [], [], []
         t = time.perf_counter()
@@ -251,7 +292,7 @@ def grids_leg(pm, steps):
               f"{out[key]['fp64']['value']:.3e} points/s", file=sys.stderr, flush=True)
     cases = pm.synthetic.small_grid_cases()
     runs = [_grid_setup(c.crystal, c.probes["ion"], c.ljff, c.meta["n_pts"], "K", False, False,
-                        c.ewald_precision)[0] for c in cases]
+                        c.ewald_precision, electrostatics=True)[0] for c in cases]
     pm.device.energy_grid_batch(runs, "mixed")
     pipe = []
     t = time.perf_counter()
@@ -317,6 +358,39 @@ def adjoint_leg(pm, case, fw, ff, ew, pr, xr, n_per_block, seed):
     }
 
 
+def fp64_leg(pm, run, n_ins, nb, case, sums32, value32, steps, flop_of):
+    """The headline workload through the plain FP64 path: insertions/s over
+    ``steps`` calls, the pair kernel's algorithmic TFLOP/s against the FP64
+    peak, and K_H / <U> of the mixed run (``sums32``; one untimed mixed call
+    when None) relative to the FP64 run."""
+    run("fp64")
+    k64s, p64s, c64s = [], [], []
+    t = time.perf_counter()
+    for _ in range(steps):
+        s64 = run("fp64")
+        k64s.append(pm.device.last_kernel_stats()[0])
+        pk, pc = pm.device.last_pair_kernel_stats()
+        p64s.append(pk)
+        c64s.append(pc)
+    dt64 = (time.perf_counter() - t) / steps
+    k64, p64 = float(np.mean(k64s)), float(np.mean(p64s))
+    if sums32 is None:
+        sums32 = run("mixed")
+    r64 = pm.block_statistics(s64, nb, case.temperature, 1.0)
+    r32 = pm.block_statistics(sums32, nb, case.temperature, 1.0)
+    kh = "henry coefficient [mol/(m³-bar)]"
+    tf = flop_of(float(np.mean(c64s))) / (p64 * 1e-3) / 1e12
+    out = {"value": n_ins / dt64, "steps": steps, "device_pipeline_ms": k64, "pair_kernel_ms": p64,
+           "pair_kernel_tflops": tf, "peak_tflops": PEAK_FP64_TFLOPS, "frac": tf / PEAK_FP64_TFLOPS,
+           "kh_rel_diff_mixed_vs_fp64": abs(r32[kh] - r64[kh]) / abs(r64[kh]),
+           "u_rel_diff_mixed_vs_fp64": abs(r32["⟨U⟩ (K)"] - r64["⟨U⟩ (K)"]) / abs(r64["⟨U⟩ (K)"])}
+    if value32 is not None:
+        out["mixed_over_fp64"] = value32 / (n_ins / dt64)
+    print(f"fp64: {steps} steps, {dt64 * 1e3:.3f} ms/step, device pipeline {k64:.3f} ms, pair kernel "
+          f"{p64:.3f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def _traffic_record():
     """HBM bytes per launch of the pair kernel measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE counters (scaled by a plain copy of known size),
@@ -348,6 +422,9 @@ def main():
                     help="chunk tuning: points per Widom cell (default: autotuned on the workload)")
     ap.add_argument("--chunk-r-max", type=float, default=None,
                     help="chunk tuning: largest chunk radius in A (default: autotuned on the workload)")
+    ap.add_argument("--only", choices=("fp64", "adjoint", "screening", "grids", "gcmc"), default=None,
+                    help="run one side leg alone (no headline timing, no autotuning: the library's default "
+                         "chunk shape unless given) and print its JSON object; for per-leg kernel traces")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -372,6 +449,27 @@ def main():
 
     def run(precision):
         return pm.device.widom(fw, ff, ew, pr, case.temperature, seed, 5, nb, precision)
+
+    if args.only is not None:
+        if args.cell_points is not None or args.chunk_r_max is not None:
+            pm.device.set_tuning(args.cell_points, args.chunk_r_max)
+        r_lj, r_sr = math.sqrt(ff.r2_cutoff), ep.sr_cutoff_r
+        n_pts = n_ins * max(co2.n_atoms, co2.n_charges)
+        if args.only == "fp64":
+            res = fp64_leg(pm, run, n_ins, nb, case, None, None, max(1, min(args.steps, 5)),
+                           lambda c: C_GEOM * c + n_ins * (co2.n_atoms * in_cutoff(xr.n_atoms, xr.box.Omega, r_lj)
+                                                           * C_LJ + co2.n_charges
+                                                           * in_cutoff(xr.n_atoms, xr.box.Omega, r_sr) * C_SR))
+        elif args.only == "adjoint":
+            res = adjoint_leg(pm, case, fw, ff, ew, pr, xr, nb, seed)
+        elif args.only == "screening":
+            res = screening_leg(pm, max(1, min(args.steps, 3)))
+        elif args.only == "grids":
+            res = grids_leg(pm, max(1, min(args.steps, 3)))
+        else:
+            res = gcmc_leg(pm)
+        print(json.dumps({"leg": args.only, "points": n_pts, **res}), flush=True)
+        return
 
     # chunk shapes: autotuned on this workload (untimed) unless given; they
     # change the schedule only, never a result
@@ -430,21 +528,11 @@ def main():
         "all_pairs_equivalent_tflops": flop_ap * n_ins / (k_ms * 1e-3) / 1e12,
     }
     if not args.no_fp64:
-        run("fp64")
-        t = time.perf_counter()
-        s64 = run("fp64")
-        dt64 = time.perf_counter() - t
-        k64 = pm.device.last_kernel_stats()[0]
-        p64, c64 = pm.device.last_pair_kernel_stats()
-        r64 = pm.block_statistics(s64, nb, case.temperature, 1.0)
-        r32 = pm.block_statistics(sums, nb, case.temperature, 1.0)
-        kh = "henry coefficient [mol/(m³-bar)]"
-        out["fp64"] = {"value": n_ins / dt64, "device_pipeline_ms": k64, "pair_kernel_ms": p64,
-                       "pair_kernel_tflops": pair_flop / (p64 * 1e-3) / 1e12,
-                       "peak_tflops": PEAK_FP64_TFLOPS}
-        out["mixed_over_fp64"] = value / (n_ins / dt64)
-        out["kh_rel_diff_mixed_vs_fp64"] = abs(r32[kh] - r64[kh]) / abs(r64[kh])
-        out["u_rel_diff_mixed_vs_fp64"] = abs(r32["⟨U⟩ (K)"] - r64["⟨U⟩ (K)"]) / abs(r64["⟨U⟩ (K)"])
+        out["fp64"] = fp64_leg(pm, run, n_ins, nb, case, sums, value, max(1, min(args.steps, 5)),
+                               lambda c: pair_flop)
+        out["mixed_over_fp64"] = out["fp64"].pop("mixed_over_fp64")
+        out["kh_rel_diff_mixed_vs_fp64"] = out["fp64"].pop("kh_rel_diff_mixed_vs_fp64")
+        out["u_rel_diff_mixed_vs_fp64"] = out["fp64"].pop("u_rel_diff_mixed_vs_fp64")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(pm, case, fw, ff, ew, pr, xr, nb, seed)
     if not args.no_screening:
