@@ -10,7 +10,9 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpmhip.so")
+# PMHIP_LIB names another build of the library to load (a sanitizer build of
+# its host code, say); by default the in-tree build
+LIB_PATH = os.environ.get("PMHIP_LIB") or os.path.join(HERE, "libpmhip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "pmhip.h")
 
 PM_OK, PM_ERR_ARG, PM_ERR_DEVICE, PM_ERR_OOM, PM_ERR_STATE = 0, 1, 2, 3, 4
