@@ -236,6 +236,30 @@ int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_
                       int64_t n_per_block, int32_t precision, const double* jac,
                       double* out_block_sums, double* out_block_grads, const pm_access* access);
 
+/*
+ * pm_widom without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as pm_widom (block sums bit for bit), without
+ * per-insertion energies.  Its buffers (point slots, cell lists, per-workgroup
+ * partials, block sums, staged tables) live in a fixed device array of
+ * PM_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the
+ * process ends; a run whose buffers exceed it fails with PM_ERR_ARG before any
+ * device work (about 210 bytes per insertion of a 3-site probe: some 3e5 CO2
+ * insertions per call).  The framework's image list for the force field's
+ * cutoff must already exist (any pm_widom call with the same framework and
+ * force field builds it); otherwise PM_ERR_STATE.  It calls no hipMalloc,
+ * hipFree or allocating routine.  Calls are serialised by the library mutex
+ * and complete before they return, so two calls never use the array at once;
+ * it keeps the last call's contents, and the call clears what must start at
+ * zero.
+ */
+#define PM_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int pm_widom_noalloc(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+                     const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
+                     int64_t n_per_block, int32_t precision, double* out_block_sums,
+                     const pm_access* access);
+
 #define PM_MAX_BATCH_RUNS 65536  /* runs per pm_widom_batch / pm_energy_grid_batch call */
 
 /* One Widom run of a batch: the arguments of pm_widom. */

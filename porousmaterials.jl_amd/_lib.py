@@ -104,6 +104,8 @@ _SIGS = {
     "pm_pose_gradient": (ctypes.c_int, [P, P, P, ctypes.c_int32, I32, D, ctypes.c_int64, D, ctypes.c_int32, D, D]),
     "pm_widom": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
                                 ctypes.c_int64, ctypes.c_int32, D, D, P]),
+    "pm_widom_noalloc": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
+                                        ctypes.c_int64, ctypes.c_int32, D, P]),
     "pm_energy_grid": (ctypes.c_int, [P, P, P, P, I32, ctypes.c_int32, D]),
     "pm_widom_gradient": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32,
                                          ctypes.c_int64, ctypes.c_int32, D, D, D, P]),

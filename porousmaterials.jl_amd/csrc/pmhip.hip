@@ -24,6 +24,13 @@
 
 using namespace pmk;
 
+// Scratch of pm_widom_noalloc, the entry point that allocates nothing: every
+// GPU that loads the library holds this array from module load until the
+// process ends.  Host code never uses the name itself as an address (on the
+// host it is a stand-in, not device memory): pm_init resolves its device
+// address once, with hipGetSymbolAddress, and everything uses that.
+static __device__ __align__(256) unsigned char scratch[PM_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 // Point-engine tunables.
@@ -47,6 +54,7 @@ struct Ctx {
     size_t max_lds = 0;  // LDS bytes one workgroup may allocate
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    void* noalloc_scratch = nullptr;  // device address of ::scratch on this device
     // host staging for the per-call LJ table: outlives the asynchronous copy
     std::vector<unsigned char> host_tab;
     double cell_points = DEFAULT_CELL_POINTS;
@@ -396,12 +404,17 @@ int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* e
 // box [0,1)^3 is present; stored in Cartesian coordinates.  Lists are cached
 // per framework and radius, so a run's list -- and with it the order of its
 // FP32 partial sums -- depends on the framework and the radius alone.
-int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
+const pm_framework::Images* find_images(const pm_framework* fw, double r) {
     for (const auto& im : fw->images)
-        if (im.r == r) {
-            *out = &im;
-            return PM_OK;
-        }
+        if (im.r == r) return &im;
+    return nullptr;
+}
+
+int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
+    if (const pm_framework::Images* im = find_images(fw, r)) {
+        *out = im;
+        return PM_OK;
+    }
     if (fw->images.size() >= MAX_IMAGE_LISTS) return fail(PM_ERR_ARG, "too many image-list radii");
     double m[3];
     for (int i = 0; i < 3; ++i) m[i] = r / fw->width[i];
@@ -747,13 +760,22 @@ void pair_launch_t(const GroupArgs& g, int n_wg, size_t shm) {
 
 unsigned grid1d(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Device memory a call must use instead of the library's growable scratch
+// (pm_widom_noalloc): the call then allocates nothing, and fails before any
+// device work if its buffers do not fit or an image list is not built yet.
+struct FixedScratch {
+    void* base;
+    size_t bytes;
+    const char* caller;
+};
+
 // Runs every run of a call through the point engine.  Widom: out_sums gets
 // n_blocks x (S, W_vdw, W_es) per run in input order; with grad (gradient
 // runs, their gtab/jac filled) out_gsums gets n_blocks x 3 GT sums of
 // w dE/d(eps_t, sigma_t, q_t).  Grids: each run's host_out gets its grid
 // (i fastest).
 int execute_points(std::vector<PointRun>& runs, int mode, int precision, double* out_sums, bool grad = false,
-                   double* out_gsums = nullptr) {
+                   double* out_gsums = nullptr, const FixedScratch* fixed = nullptr) {
     const bool fp64 = precision == PM_FP64;
     const int n_runs = (int)runs.size();
     // image lists and paths
@@ -768,7 +790,17 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         a.n_tiles = 0;
         if (a.chunk_r <= g_ctx.chunk_r_max) {
             const pm_framework::Images* im = nullptr;
-            const int rc = get_images(r.fw, a.r_cmax + 0.01, &im);
+            int rc = PM_OK;
+            if (fixed) {
+                im = find_images(r.fw, a.r_cmax + 0.01);
+                if (!im)
+                    return fail(PM_ERR_STATE, std::string(fixed->caller) +
+                                                  ": the framework's image list for this cutoff is not built yet "
+                                                  "(one pm_widom call with the same framework and force field "
+                                                  "builds it; building it allocates device memory)");
+            } else {
+                rc = get_images(r.fw, a.r_cmax + 0.01, &im);
+            }
             if (rc == PM_OK) {
                 a.fast = 1;
                 a.img = im->d_img;
@@ -919,8 +951,17 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             }
     const size_t o_gf = ar.take(gf_elems * sizeof(double2));
     const size_t o_gsums = ar.take(grad ? (size_t)blocks_total * 3 * GT * sizeof(double) : 0);
-    if (int rc = ensure_scratch(ar.used)) return rc;
-    unsigned char* base = static_cast<unsigned char*>(g_ctx.scratch);
+    unsigned char* base = nullptr;
+    if (fixed) {
+        if (ar.used > fixed->bytes)
+            return fail(PM_ERR_ARG, std::string(fixed->caller) + ": the call needs " + std::to_string(ar.used) +
+                                        " bytes of scratch, above the fixed maximum of " +
+                                        std::to_string(fixed->bytes) + " (PM_NOALLOC_SCRATCH_BYTES)");
+        base = static_cast<unsigned char*>(fixed->base);
+    } else {
+        if (int rc = ensure_scratch(ar.used)) return rc;
+        base = static_cast<unsigned char*>(g_ctx.scratch);
+    }
 
     // device pointers of each run, then one staged upload of runs, segments, tables
     for (int i = 0; i < n_runs; ++i) {
@@ -1203,6 +1244,12 @@ int pm_init(int32_t device) {
     int lds = 0;
     PM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     g_ctx.max_lds = (size_t)std::max(lds, 0);
+    // this device's copy of the no-alloc scratch array (the library binds one
+    // device per process, so one address)
+    void* p = nullptr;
+    PM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(PM_ERR_DEVICE, "pm_init: no device address for the no-alloc scratch array");
+    g_ctx.noalloc_scratch = p;
     g_ctx.device = device;
     return PM_OK;
 }
@@ -1779,6 +1826,32 @@ int pm_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew
         (void)hipFree(d_energies);
     }
     return rc;
+}
+
+int pm_widom_noalloc(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,
+                     const pm_probe* pr, double temperature, uint64_t seed, int32_t n_blocks,
+                     int64_t n_per_block, int32_t precision, double* out_block_sums,
+                     const pm_access* access) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    if (!out_block_sums) return fail(PM_ERR_ARG, "pm_widom_noalloc: null output pointer");
+    if (precision != PM_FP64 && precision != PM_MIXED)
+        return fail(PM_ERR_ARG, "precision must be PM_FP64 or PM_MIXED");
+    if (!g_ctx.noalloc_scratch) return fail(PM_ERR_STATE, "pm_widom_noalloc: no scratch address for this device");
+    pm_widom_run in{};
+    in.framework = fw;
+    in.forcefield = ff;
+    in.ewald = ew;
+    in.probe = pr;
+    in.temperature = temperature;
+    in.seed = seed;
+    in.n_per_block = n_per_block;
+    in.n_blocks = n_blocks;
+    in.access = access;
+    std::vector<PointRun> runs(1);
+    if (int rc = widom_prepare(in, runs[0])) return rc;
+    const FixedScratch fixed{g_ctx.noalloc_scratch, (size_t)PM_NOALLOC_SCRATCH_BYTES, "pm_widom_noalloc"};
+    return execute_points(runs, MODE_WIDOM, precision, out_block_sums, false, nullptr, &fixed);
 }
 
 int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* ew,

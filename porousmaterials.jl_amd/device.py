@@ -274,6 +274,21 @@ def widom(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision,
     return sums
 
 
+def widom_noalloc(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision, acc=None):
+    """Block sums (n_blocks, 3) of a Widom run through pm_widom_noalloc: the
+    same run as :func:`widom` in the library's fixed scratch array, with no
+    device allocation (the framework's image list must exist: one
+    :func:`widom` call with the same framework and force field builds it)."""
+    lib = _lib.load()
+    ff = _ff_handle(ff)
+    sums = np.zeros(3 * n_blocks)
+    check(lib.pm_widom_noalloc(fw.ptr, ff.ptr, ew.ptr if ew is not None else None, pr.ptr,
+                               float(temperature), int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_blocks),
+                               int(n_per_block), precision_code(precision), dptr(sums),
+                               acc.ptr if acc is not None else None))
+    return sums.reshape(n_blocks, 3)
+
+
 def energy_grid(fw, ff, ew, pr, n_pts, precision) -> np.ndarray:
     lib = _lib.load()
     ff = _ff_handle(ff)
