@@ -13,6 +13,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pm_gcmc.h"
@@ -55,6 +56,10 @@ struct Ctx {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     void* noalloc_scratch = nullptr;  // device address of ::scratch on this device
+    // pinned host staging of grid downloads and the events that mark its pieces
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    std::vector<hipEvent_t> piece_ev;
     // host staging for the per-call LJ table: outlives the asynchronous copy
     std::vector<unsigned char> host_tab;
     double cell_points = DEFAULT_CELL_POINTS;
@@ -86,6 +91,87 @@ int ensure_scratch(size_t bytes) {
     size_t want = std::max(bytes, (size_t)1 << 20);
     PM_HIP(hipMalloc(&g_ctx.scratch, want));
     g_ctx.scratch_bytes = want;
+    return PM_OK;
+}
+
+// Pinned host staging for grid downloads (grown on demand, held until the
+// process ends).
+int ensure_pinned(size_t bytes) {
+    if (bytes <= g_ctx.pinned_bytes) return PM_OK;
+    if (g_ctx.pinned) {
+        PM_HIP(hipStreamSynchronize(g_ctx.stream));
+        PM_HIP(hipHostFree(g_ctx.pinned));
+        g_ctx.pinned = nullptr;
+        g_ctx.pinned_bytes = 0;
+    }
+    const size_t want = std::max(bytes, (size_t)1 << 24);
+    PM_HIP(hipHostMalloc(&g_ctx.pinned, want, hipHostMallocDefault));
+    g_ctx.pinned_bytes = want;
+    return PM_OK;
+}
+
+int ensure_piece_events(size_t n) {
+    while (g_ctx.piece_ev.size() < n) {
+        hipEvent_t e = nullptr;
+        PM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        g_ctx.piece_ev.push_back(e);
+    }
+    return PM_OK;
+}
+
+// One piece of a grid download: elements [e0, e0 + n) of a grid, from the
+// device grid (FP32 or FP64) through the pinned staging into the caller's
+// FP64 buffer.
+struct GridPiece {
+    const unsigned char* dev;
+    unsigned char* pin;
+    double* dst;
+    long long n;
+};
+
+constexpr size_t PIECE_BYTES = (size_t)4 << 20;   // bytes of one download piece
+constexpr size_t MAX_PINNED = (size_t)2 << 30;    // larger downloads copy to the caller directly
+constexpr unsigned MAX_HOST_THREADS = 16;
+
+// Enqueues the downloads of every piece (each followed by its event) and
+// widens / copies them into the callers' buffers on up to MAX_HOST_THREADS
+// host threads as the pieces arrive, so the copy over the host link overlaps
+// the host-side conversion.  Returns once every piece is in place.
+int download_grids(std::vector<GridPiece>& pieces, bool f32) {
+    if (pieces.empty()) return PM_OK;
+    if (int rc = ensure_piece_events(pieces.size())) return rc;
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    for (size_t k = 0; k < pieces.size(); ++k) {
+        PM_HIP(hipMemcpyAsync(pieces[k].pin, pieces[k].dev, (size_t)pieces[k].n * es, hipMemcpyDeviceToHost,
+                              g_ctx.stream));
+        PM_HIP(hipEventRecord(g_ctx.piece_ev[k], g_ctx.stream));
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = (unsigned)std::min<size_t>(std::min(hw, MAX_HOST_THREADS), pieces.size());
+    std::vector<hipError_t> errs(nt, hipSuccess);
+    auto work = [&](unsigned t) {
+        for (size_t k = t; k < pieces.size(); k += nt) {
+            const hipError_t e = hipEventSynchronize(g_ctx.piece_ev[k]);
+            if (e != hipSuccess) {
+                errs[t] = e;
+                return;
+            }
+            const GridPiece& pc = pieces[k];
+            if (f32) {
+                const float* src = reinterpret_cast<const float*>(pc.pin);
+                for (long long i = 0; i < pc.n; ++i) pc.dst[i] = (double)src[i];
+            } else {
+                std::memcpy(pc.dst, pc.pin, (size_t)pc.n * sizeof(double));
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(nt);
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    for (hipError_t e : errs)
+        if (e != hipSuccess) return fail(PM_ERR_DEVICE, std::string("grid download: ") + hipGetErrorString(e));
     return PM_OK;
 }
 
@@ -1112,6 +1198,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
     unsigned long long cand = 0;
     PM_HIP(hipMemcpyAsync(&cand, d_cand, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream));
+    std::vector<GridPiece> pieces;
     std::vector<std::vector<float>> tmp;
     double pairs = 0.0;
     if (widom) {
@@ -1122,21 +1209,51 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
                                   hipMemcpyDeviceToHost, g_ctx.stream));
         for (const PointRun& r : runs) pairs += (double)r.ra.n_blocks * r.ra.n_per_block * r.ns * r.fw->n_rep;
     } else {
-        tmp.resize(fp64 ? 0 : n_runs);
+        // grids go to the callers' FP64 buffers in pieces through pinned
+        // staging (FP32 device grids are widened on the host)
+        const size_t es = fp64 ? sizeof(double) : sizeof(float);
+        size_t total = 0;
         for (int i = 0; i < n_runs; ++i) {
-            const PointRun& r = runs[i];
-            pairs += (double)r.n_points * r.fw->n_rep;
-            if (fp64) {
-                PM_HIP(hipMemcpyAsync(r.host_out, base + o_grid + grid_off[i], (size_t)r.n_points * sizeof(double),
-                                      hipMemcpyDeviceToHost, g_ctx.stream));
-            } else {
-                tmp[i].resize((size_t)r.n_points);
-                PM_HIP(hipMemcpyAsync(tmp[i].data(), base + o_grid + grid_off[i], (size_t)r.n_points * sizeof(float),
-                                      hipMemcpyDeviceToHost, g_ctx.stream));
+            pairs += (double)runs[i].n_points * runs[i].fw->n_rep;
+            total += (size_t)runs[i].n_points * es;
+        }
+        const bool staged = total <= MAX_PINNED;
+        if (staged) {
+            if (int rc = ensure_pinned(total)) return rc;
+            const long long per = (long long)std::max(PIECE_BYTES, total / 256 + 1) / (long long)es;
+            size_t pin_off = 0;
+            for (int i = 0; i < n_runs; ++i) {
+                const PointRun& r = runs[i];
+                for (long long e0 = 0; e0 < r.n_points; e0 += per) {
+                    const long long n = std::min(per, r.n_points - e0);
+                    pieces.push_back(GridPiece{base + o_grid + grid_off[i] + (size_t)e0 * es,
+                                               static_cast<unsigned char*>(g_ctx.pinned) + pin_off, r.host_out + e0,
+                                               n});
+                    pin_off += (size_t)n * es;
+                }
+            }
+            if (int rc = download_grids(pieces, !fp64)) return rc;
+        } else {
+            // beyond the staging limit: straight to the callers' buffers (FP64
+            // grids), or through a pageable buffer per grid (FP32 grids, widened
+            // below)
+            tmp.resize(fp64 ? 0 : n_runs);
+            for (int i = 0; i < n_runs; ++i) {
+                const PointRun& r = runs[i];
+                if (fp64) {
+                    PM_HIP(hipMemcpyAsync(r.host_out, base + o_grid + grid_off[i],
+                                          (size_t)r.n_points * sizeof(double), hipMemcpyDeviceToHost, g_ctx.stream));
+                } else {
+                    tmp[i].resize((size_t)r.n_points);
+                    PM_HIP(hipMemcpyAsync(tmp[i].data(), base + o_grid + grid_off[i],
+                                          (size_t)r.n_points * sizeof(float), hipMemcpyDeviceToHost, g_ctx.stream));
+                }
             }
         }
     }
     if (int rc = finish_timing(pairs)) return rc;
+    for (size_t i = 0; i < tmp.size(); ++i)
+        for (long long p = 0; p < runs[i].n_points; ++p) runs[i].host_out[p] = (double)tmp[i][(size_t)p];
     {
         float ms = 0.f;
         PM_HIP(hipEventElapsedTime(&ms, g_ctx.evp0, g_ctx.evp1));
@@ -1144,9 +1261,6 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     }
     g_ctx.last_pair_ms = pair_ms;
     g_ctx.last_cand = (double)cand;
-    if (!widom && !fp64)
-        for (int i = 0; i < n_runs; ++i)
-            for (long long p = 0; p < runs[i].n_points; ++p) runs[i].host_out[p] = (double)tmp[i][p];
     return PM_OK;
 }
 
