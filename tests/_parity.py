@@ -131,13 +131,18 @@ def block_sums_from_energies(vdw, es, n_per_block, temperature):
 
 
 def repulsive_block_allowance(vdw, es, n_per_block, tol, temperature):
-    """Per block, (S allowance, <U> allowance): zeros where some insertion has
-    E <= 0 K.  Where every insertion is repulsive (E > 0, possible only in
-    small samples of dense frameworks, K_H of order e^(-E/T) with E ~ 10^5 K),
-    the relative changes of S and of <U> that the per-insertion tolerance
-    implies: an error dE_i = tol (|E_vdw,i| + |E_es,i| + T) moves weight i by
-    dE_i / T, so S by sum_i w_i expm1(dE_i / T) / S and <U> by
-    sum_i w_i |E_i - <U>| dE_i / T / (S |<U>|) + tol."""
+    """Per block, (S allowance, <U> allowance) from the per-insertion
+    tolerance: an error dE_i = tol (|E_vdw,i| + |E_es,i| + T) moves weight i by
+    dE_i / T and <U> = sum w E / sum w directly by dE_i, so
+
+      S:   sum_i w_i expm1(dE_i / T) / S           (blocks whose every insertion
+                                                      is repulsive; 0 elsewhere)
+      <U>: sum_i w_i dE_i (1 + |E_i - <U>| / T) / (S |<U>|)   (every block).
+
+    The S allowance covers small samples of dense frameworks (K_H of order
+    e^(-E/T) with E ~ 10^5 K); the <U> allowance matters only where <U> is
+    small beside the pair terms it cancels from (E = 51 K from 4330 K of LJ and
+    -4279 K of Ewald, say), and is ~tol elsewhere."""
     vdw = np.asarray(vdw, dtype=float).reshape(-1, n_per_block)
     es = np.asarray(es, dtype=float).reshape(-1, n_per_block)
     tot = vdw + es
@@ -146,17 +151,17 @@ def repulsive_block_allowance(vdw, es, n_per_block, tol, temperature):
     with np.errstate(over="ignore", invalid="ignore"):
         for b in range(tot.shape[0]):
             fin = np.isfinite(tot[b])
-            if not fin.any() or tot[b][fin].min() <= 0.0:
+            if not fin.any():
                 continue
             E = tot[b][fin]
-            w = np.exp(-E / temperature)
-            if w.sum() == 0.0:
-                continue
+            w = np.exp(-(E - E.min()) / temperature)
             dE = tol * (np.abs(vdw[b][fin]) + np.abs(es[b][fin]) + temperature)
-            arg = np.minimum(dE / temperature, 700.0)
-            s_out[b] = float((w * np.expm1(arg)).sum() / w.sum())
             u = float((w * E).sum() / w.sum())
-            u_out[b] = float((w * np.abs(E - u) * dE / temperature).sum() / (w.sum() * abs(u))) + tol
+            if u != 0.0:
+                u_out[b] = float((w * dE * (1.0 + np.abs(E - u) / temperature)).sum() / (w.sum() * abs(u)))
+            if E.min() > 0.0 and np.exp(-E.min() / temperature) > 0.0:
+                arg = np.minimum(dE / temperature, 700.0)
+                s_out[b] = float((w * np.expm1(arg)).sum() / w.sum())
     return s_out, u_out
 
 

@@ -148,14 +148,18 @@ def test_co2_workload_dlnKH_full_size_vs_dual_oracle(pm, co2_full, co2_dual_full
     full 10^6 insertions (pm_widom_gradient) against the oracle's exact
     (dual-number) derivatives on the same stream: 1e-8 (FP64) and 1e-4
     (mixed) relative, floor 1e-6 of the largest component; per-block
-    weighted derivative sums to the same bars."""
+    weighted derivative sums to the same bars, floor 1e-6 (FP64) or 1e-3
+    (mixed) of the block's largest component."""
     r = co2_full
     ft, G, S = co2_dual_full
     fw, ff, ew, pr = r.handles
     sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, 298.0, r.seed, N_BLOCKS, N_PER_BLOCK, precision, r.ff)
     got = grads[:, ft, :]
     rtol = 1e-8 if precision == "fp64" else 1e-4
-    blk = max(float((np.abs(got[b] - G[b]) / np.maximum(np.abs(G[b]), 1e-6 * np.abs(G[b]).max())).max())
+    # per-block sums of a component can cancel across insertions: floor at a
+    # fraction of the block's largest component (1e-6 FP64, 1e-3 mixed)
+    floor = 1e-6 if precision == "fp64" else 1e-3
+    blk = max(float((np.abs(got[b] - G[b]) / np.maximum(np.abs(G[b]), floor * np.abs(G[b]).max())).max())
               for b in range(N_BLOCKS))
     g_dev = -got.sum(axis=0) / (298.0 * sums[:, 0].sum())
     g_ref = -G.sum(axis=0) / (298.0 * S)
@@ -220,10 +224,12 @@ def test_screening_batch_every_run_vs_oracle(pm, screening_oracle, precision):
     stand-alone run equals the batch bit for bit) vs the oracle.  These are
     dense frameworks and 20-insertion blocks: most blocks hold only repulsive
     insertions (E > 0, up to 10^7 K).  Blocks with an insertion at E <= 0 K
-    are held to 1e-4 (mixed) in K_H and <U>; an all-repulsive block, whose K_H
-    is e^(-E/T) of a pose with E ~ 10^5 K, to the changes of S and <U> its
-    energies' tolerance implies (the largest error over such blocks is
-    printed)."""
+    are held to 1e-4 (mixed) in K_H; an all-repulsive block, whose K_H is
+    e^(-E/T) of a pose with E ~ 10^5 K, to the change of S its energies'
+    tolerance implies (the largest error over such blocks is printed).  <U> is
+    held to 1e-4, or where a block's <U> cancels from much larger pair terms
+    (E = 51 K from 4330 K of LJ and -4279 K of Ewald), to the change its
+    energies' tolerance implies."""
     say(f"case: screening batch, 512 runs x 5 x 20 insertions vs oracle ({precision})")
     handles, runs = [], []
     for c, name, seed, v, e in screening_oracle:
@@ -245,7 +251,7 @@ def test_screening_batch_every_run_vs_oracle(pm, screening_oracle, precision):
         rep = (allow > 0.0) & (want[:, 0] > 0.0)
         n_rep += int(rep.sum())
         attr = ~rep
-        es, eu = check_block_sums(g[attr], want[attr], KH_RTOL[precision], what)
+        es, eu = check_block_sums(g[attr], want[attr], KH_RTOL[precision], what, None, u_allow[attr])
         es_r, eu_r = check_block_sums(g[rep], want[rep], KH_RTOL[precision], what + " (repulsive)", allow[rep],
                                       u_allow[rep])
         worst = [max(worst[0], rel_w), max(worst[1], es), max(worst[2], eu, eu_r), max(worst[3], es_r),

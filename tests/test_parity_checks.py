@@ -151,6 +151,9 @@ def test_weighted_mask_and_repulsive_allowance():
     m = weighted_mask(tot, groups)
     assert m.tolist() == [True, True, False, False, True, False, False, False]
     allow, u_allow = repulsive_block_allowance(tot, np.zeros_like(tot), 4, 1e-5, 298.0)
-    assert allow[0] == 0.0 and u_allow[0] == 0.0
+    assert allow[0] == 0.0 and 1e-5 < u_allow[0] < 1e-4
     assert allow[1] == pytest.approx(math.expm1(1e-5 * (3e4 + 298.0) / 298.0), rel=1e-6)
-    assert u_allow[1] >= 1e-5
+    assert u_allow[1] == pytest.approx(1e-5 * (3e4 + 298.0) / 3e4, rel=1e-6)
+    # a <U> that cancels from large pair terms gets the allowance its terms imply
+    _, u_c = repulsive_block_allowance(np.array([4330.0]), np.array([-4279.0]), 1, 1e-5, 298.0)
+    assert u_c[0] == pytest.approx(1e-5 * (4330.0 + 4279.0 + 298.0) / 51.0, rel=1e-9)
