@@ -520,7 +520,7 @@ def main():
             "tuning": {k: tuning[k] for k in ("cell_points", "chunk_r_max", "source")}},
         "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "point_pair_kernel<float, false>", "kernel_ms": p_ms,
+                     "kernel": "point_pair_kernel<float, 0>", "kernel_ms": p_ms,
                      "flop_per_launch": pair_flop,
                      "candidate_pairs_per_point": cand / n_points,
                      "pruning_factor": (n_points * xr.n_atoms) / cand},

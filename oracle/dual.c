@@ -17,7 +17,11 @@
  *     dE/d(eps_t, sigma_t, q_t) for listed framework species t: eps_t and
  *     sigma_t are the framework-side pure parameters fed through the mixing
  *     rule (the probe's held fixed), q_t a uniform shift of the charge of
- *     every framework atom of species t.
+ *     every framework atom of species t.  Optionally also the condition of
+ *     each derivative as a sum: the sum of the magnitudes of its summands (per
+ *     LJ pair, per real-space pair times the site charge, per site's Fourier
+ *     part times the site charge), the scale of the rounding error of any
+ *     evaluation that rounds each summand to a relative precision.
  *   orc_pose_grad: energies of explicit poses and dE/d(site xyz), Cartesian.
  *
  * Positions, cutoff tests and nearest images follow oracle.c exactly; the
@@ -165,7 +169,7 @@ static void dual_mix(const orc_param_dual* p, int32_t t, int32_t s, dnum* eps, d
 }
 
 int32_t orc_widom_param_grad(const orc_widom_cfg* c, const orc_param_dual* p, uint64_t g0, int64_t count,
-                             double* out_vdw, double* out_es, double* out_d) {
+                             double* out_vdw, double* out_es, double* out_d, double* out_cond) {
     if (p->n_ft * 3 > ORC_DN || c->n_lj > 16 || c->n_q > 16) return 1;
     const int n_sp = c->n_species;
     /* dual mixed LJ tables [framework species][probe species] */
@@ -225,6 +229,7 @@ int32_t orc_widom_param_grad(const orc_widom_cfg* c, const orc_param_dual* p, ui
             }
         }
         dnum vdw = dn_c(0.0);
+        double cond[ORC_DN] = {0.0};  /* sum of |summand derivative| */
         int overlap = 0;
         for (int s = 0; s < c->n_lj && !overlap; ++s) {
             const int ps = c->lj_species[s];
@@ -239,7 +244,9 @@ int32_t orc_widom_param_grad(const orc_widom_cfg* c, const orc_param_dual* p, ui
                 if (r2 < c->r2_cutoff) {
                     if (r2 < R2_OVERLAP) { overlap = 1; break; }
                     const int t = c->species[j];
-                    vdw = dn_add(vdw, dn_lj(dn_c(r2), sig2_t[t * n_sp + ps], eps_t[t * n_sp + ps]));
+                    const dnum e = dn_lj(dn_c(r2), sig2_t[t * n_sp + ps], eps_t[t * n_sp + ps]);
+                    vdw = dn_add(vdw, e);
+                    for (int k = 0; k < nd; ++k) cond[k] += fabs(e.d[k]);
                 }
             }
         }
@@ -257,7 +264,9 @@ int32_t orc_widom_param_grad(const orc_widom_cfg* c, const orc_param_dual* p, ui
                     const double r = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
                     if (r < c->sr_cutoff) {
                         const dnum q = dual_charge(p, c->species[j], c->q[j]);
-                        sr = dn_add(sr, dn_scale(q, erfc(r * c->alpha) / r / fpe));
+                        const dnum e = dn_scale(q, erfc(r * c->alpha) / r / fpe);
+                        sr = dn_add(sr, e);
+                        for (int k = 0; k < nd; ++k) cond[k] += fabs(c->q_site[s] * e.d[k]);
                     }
                 }
                 const double xu[3] = {qxf[3 * s] * c->rep[0], qxf[3 * s + 1] * c->rep[1], qxf[3 * s + 2] * c->rep[2]};
@@ -268,12 +277,15 @@ int32_t orc_widom_param_grad(const orc_widom_cfg* c, const orc_param_dual* p, ui
                     const double ang = 2.0 * M_PI * t;
                     lr = dn_add(lr, dn_add(dn_scale(are[k], cos(ang)), dn_scale(aim[k], sin(ang))));
                 }
+                for (int k = 0; k < nd; ++k) cond[k] += fabs(c->q_site[s] * lr.d[k]);
                 es = dn_add(es, dn_scale(dn_add(sr, lr), c->q_site[s]));
             }
         }
         out_vdw[ii] = overlap ? INFINITY : vdw.v;
         out_es[ii] = overlap ? 0.0 : es.v;
         for (int k = 0; k < nd; ++k) out_d[ii * nd + k] = overlap ? 0.0 : vdw.d[k] + es.d[k];
+        if (out_cond)
+            for (int k = 0; k < nd; ++k) out_cond[ii * nd + k] = overlap ? 0.0 : cond[k];
     }
     free(eps_t);
     free(sig2_t);

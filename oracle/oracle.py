@@ -75,7 +75,7 @@ def lib():
                                                  ctypes.c_double]),
             "orc_widom_scales": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, D]),
             "orc_widom_param_grad": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
-                                                      ctypes.c_int64, D, D, D]),
+                                                      ctypes.c_int64, D, D, D, D]),
             "orc_pose_grad": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, I32, D, ctypes.c_int64, D, D, D]),
             "orc_energy_grid_scales": (ctypes.c_int32, [ctypes.c_void_p, I32, D]),
         }
@@ -421,12 +421,14 @@ class OracleSystem:
         return out
 
     # -- dual-number derivatives (oracle/dual.c) -----------------------------
-    def widom_param_grad(self, seed, g0, count, pure_eps, pure_sig, ft):
+    def widom_param_grad(self, seed, g0, count, pure_eps, pure_sig, ft, cond=False):
         """Insertions [g0, g0 + count) of the stream: (E_vdw, E_es, dE) with
         dE (count, len(ft), 3) = dE/d(eps_t, sigma_t, q_t) for the force-field
         species ft (framework side; Lorentz-Berthelot mixing of the pure
         parameters pure_eps / pure_sig, indexed by force-field species; q_t a
-        uniform shift of every framework atom of species t)."""
+        uniform shift of every framework atom of species t).  cond=True adds a
+        fourth array of dE's shape: each derivative's sum of summand
+        magnitudes (per pair and per site's Fourier part)."""
         ft = np.asarray(ft, dtype=np.int32)
         pe, ps = _c(pure_eps), _c(pure_sig)
         p = _ParamDual(len(ft), _i(ft), _d(pe), _d(ps), self.unit_xf.shape[0], _d(self.unit_xf),
@@ -434,11 +436,12 @@ class OracleSystem:
         self.cfg.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         v, e = np.zeros(count), np.zeros(count)
         d = np.zeros((count, len(ft), 3))
+        cd = np.zeros((count, len(ft), 3)) if cond else None
         rc = lib().orc_widom_param_grad(ctypes.byref(self.cfg), ctypes.byref(p), int(g0), int(count), _d(v), _d(e),
-                                        _d(d))
+                                        _d(d), _d(cd) if cond else None)
         if rc:
             raise ValueError("orc_widom_param_grad: too many species or sites")
-        return v, e, d
+        return (v, e, d, cd) if cond else (v, e, d)
 
     def pose_grad(self, site_species, site_q, site_xf):
         """Energies (n_poses,) and dE/d(site Cartesian xyz) (n_poses, n_sites,

@@ -36,7 +36,6 @@ namespace pmk {
 constexpr int WG = 256;      // work-items per workgroup (4 waves)
 constexpr int TILE = 256;    // framework atoms per LDS tile
 constexpr int KTILE = 128;   // k-vectors per LDS tile (pose kernel)
-constexpr int FT = 2048;     // Fourier coefficients per LDS tile (gradient Fourier pass)
 constexpr double R2_OVERLAP = 0.1;  // src/vdw_energetics.jl:2
 
 enum { MODE_WIDOM = 0, MODE_GRID = 2 };

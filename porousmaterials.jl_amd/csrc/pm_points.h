@@ -128,7 +128,9 @@ struct GroupArgs {
     double* out_sr;
     double* out_lr;
     unsigned long long* cand_pairs;  // evaluated candidate pairs (reporting)
-    double* out_g;           // gradient runs: per slot GQ x GT per-type sums
+    double* out_g;           // gradient runs: GQ x GT per-type sums per slot, sum-major:
+                             // sum (q GT + t) of slot p at out_g[(q GT + t) g_stride + p]
+    long long g_stride;
 };
 
 // A pointer into read-only device memory (the constant address space: the
@@ -470,54 +472,56 @@ __device__ inline void pair_term(double r2, double q, float qf, const LJP<Real>&
     }
 }
 
-// pair_term with the per-type derivative sums of a gradient run: cE += u^2 - u
-// (d LJ / d eps_ts = 4 (u^2 - u), u = (sigma_ts^2 / r^2)^3), cS += d LJ / d sigma_ts,
-// cQ += erfc(alpha r) / r, all for in-cutoff pairs, in Real (the caller folds
-// them into FP64 per tile).  The energy terms are computed exactly as in
-// pair_term.
+// pair_term with the sums of a gradient run, for a candidate of the framework
+// type the caller is summing: sw += u^2 - u and su += u over pairs inside the
+// LJ cutoff (u = (sigma_ts^2 / r^2)^3), sq += erfc(alpha r) / r over pairs
+// inside the real-space cutoff, all in Real.  d LJ / d eps_ts = 4 (u^2 - u) and
+// d LJ / d sigma_ts = (4 eps_ts / sigma_ts)(12 u^2 - 6 u) = csig (12 sw + 6 su),
+// so the caller forms both per type from sw and su.  The energy terms are
+// pair_term's, operation for operation: a gradient run's energies equal a
+// plain run's bit for bit.
 template <int ESM, typename Real>
-__device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<Real>& p, Real csig, bool is_lj,
-                                      double r2lj, double r2sr, const ErfcTabs& et, bool& ovl, Real& plj,
-                                      Real& psr, Real& cE, Real& cS, Real& cQ) {
-    const float inv_h = et.inv_h;
-    const float4* tab = et.tab;
+__device__ inline void pair_term_grad(double r2, double q, float qf, const LJP<Real>& p, bool is_lj, double r2lj,
+                                      double r2sr, const ErfcTabs& et, bool& ovl, Real& plj, Real& psr, Real& sw,
+                                      Real& su, Real& sq) {
     const bool in = r2 < r2lj;
     ovl |= in & (r2 < R2_OVERLAP) & is_lj;
-    const bool lj_in = in & is_lj;
     if constexpr (sizeof(Real) == 4) {
-        // 1/r^2 exactly as pair_term computes it
-        const float s = p.sig2 * ((ESM != 0) ? __builtin_amdgcn_rsqf((float)r2) * __builtin_amdgcn_rsqf((float)r2)
-                                             : __builtin_amdgcn_rcpf((float)r2));
-        const float u = s * s * s;
-        const float w = fmaf(u, u, -u);
-        plj = fmaf(p.eps4, in ? w : 0.0f, plj);
-        cE += lj_in ? w : 0.0f;
-        cS += lj_in ? csig * u * fmaf(12.0f, u, -6.0f) : 0.0f;
-    } else {
-        const double y = rsq64(r2);  // 1/r^2 exactly as pair_term computes it
-        const double s = p.sig2 * (y * y);
-        const double u = s * s * s;
-        plj += in ? p.eps4 * u * (u - 1.0) : 0.0;
-        cE += lj_in ? u * (u - 1.0) : 0.0;
-        cS += lj_in ? csig * u * (12.0 * u - 6.0) : 0.0;
-    }
-    if constexpr (ESM != 0) {
-        const bool ins = (ESM == 1) ? in : (r2 < r2sr);
-        if constexpr (sizeof(Real) == 4) {
-            const float rf2 = (float)r2;
-            const float rinv = __builtin_amdgcn_rsqf(rf2);
-            const float x = fminf(rf2 * rinv * inv_h, ERFC_XMAX);
+        const float rf2 = (float)r2;
+        float rinv = 0.0f, s;
+        if constexpr (ESM != 0) {
+            rinv = __builtin_amdgcn_rsqf(rf2);
+            s = p.sig2 * (rinv * rinv);
+        } else {
+            s = p.sig2 * __builtin_amdgcn_rcpf(rf2);
+        }
+        const float s3 = s * s * s;
+        const float wm = in ? fmaf(s3, s3, -s3) : 0.0f;
+        plj = fmaf(p.eps4, wm, plj);
+        sw += wm;
+        su += in ? s3 : 0.0f;
+        if constexpr (ESM != 0) {
+            const bool inr = (ESM == 1) ? in : (r2 < r2sr);
+            const float x = fminf(rf2 * rinv * et.inv_h, ERFC_XMAX);
             const int i = (int)x;
             const float t = __builtin_amdgcn_fractf(x);
-            const float4 c = tab[i];
-            const float e = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-            psr = fmaf((ins & (qf != 0.0f)) ? qf * rinv : 0.0f, e, psr);
-            cQ += ins ? e * rinv : 0.0f;
-        } else {
-            const double y = rsq64(r2);
+            const float4 c = et.tab[i];
+            const float ec = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+            psr = fmaf((inr & (qf != 0.0f)) ? qf * rinv : 0.0f, ec, psr);
+            sq += inr ? ec * rinv : 0.0f;
+        }
+    } else {
+        const double y = rsq64(r2);
+        const double s = p.sig2 * (y * y);
+        const double s3 = s * s * s;
+        plj += in ? p.eps4 * s3 * (s3 - 1.0) : 0.0;
+        sw += in ? s3 * (s3 - 1.0) : 0.0;
+        su += in ? s3 : 0.0;
+        if constexpr (ESM != 0) {
+            const bool inr = (ESM == 1) ? in : (r2 < r2sr);
             const double ec = erfc_tab64(r2 * y, et);
-            psr += (ins & (q != 0.0)) ? q * y * ec : 0.0;
-            cQ += ins ? ec * y : 0.0;
+            psr += (inr & (q != 0.0)) ? q * y * ec : 0.0;
+            sq += inr ? ec * y : 0.0;
         }
     }
 }
@@ -536,68 +540,76 @@ struct PairLds {
     int* t;         // candidate type * nsite
     float* qf;      // candidate charge in FP32
     int* sel;       // selected tiles of the current batch
+    int* tcnt;      // gradient runs: per wave's tile, the kept candidates of each type
     const float4* erfc;    // FP32 kernels
     const double2* erfc64;  // FP64 kernels
     int* wcnt;      // per-wave candidate counts
 };
 
-// Per-type derivative sums of a gradient run's point.  The pair terms of the
-// current framework type accumulate in Real (fE, fS, fQ), folded into FP64
-// (cE, cS, cQ) at the end of every tile and when the type changes, as the
-// energies are; the FP64 sums of a type are added to g[] when the type
-// changes.  The type is the same for every lane of the wave (all lanes read
-// the same candidate), so the flush is a uniform branch; the tiles of an image
-// list are sorted by species inside each spatial bin, so it runs about once
-// per species per bin.
-template <typename Real>
+// Per-type derivative sums of a gradient run's point, FP64, for NT framework
+// types: g[k] = sum (u^2 - u) (d LJ / d eps_ts = 4 of it), g[NT + k] =
+// d LJ / d sigma_ts, g[2 NT + k] = sum erfc(alpha r) / r.  A run of candidates
+// of one type accumulates in Real (pair_term_grad's sw, su, sq) and is folded
+// in here once.  The image-list path folds with the type known at compile time
+// (add), the all-atoms path with the type in a register (add_dyn, every entry
+// updated: a conditional update of one entry would be folded into a
+// dynamically indexed array held in scratch memory).
+template <typename Real, int NT>
 struct TypeSums {
-    int cur = -1;  // current type * nsite
-    double cE = 0.0, cS = 0.0, cQ = 0.0;
-    Real fE = (Real)0, fS = (Real)0, fQ = (Real)0;
-    double g[3 * GT];
+    double g[3 * NT];
+    // the all-atoms path: the type (times nsite) being summed and its sums
+    int cur = -1;
+    Real sw = (Real)0, su = (Real)0, sq = (Real)0, csig = (Real)0;
     __device__ TypeSums() {
 #pragma unroll
-        for (int k = 0; k < 3 * GT; ++k) g[k] = 0.0;
+        for (int k = 0; k < 3 * NT; ++k) g[k] = 0.0;
     }
-    __device__ __attribute__((always_inline)) inline void fold() {
-        cE += (double)fE;
-        cS += (double)fS;
-        cQ += (double)fQ;
-        fE = fS = fQ = (Real)0;
+    __device__ __attribute__((always_inline)) inline void add(int k, Real w, Real u, Real q, Real cs, bool is_lj) {
+        g[k] += is_lj ? (double)w : 0.0;
+        g[NT + k] += is_lj ? (double)cs * (12.0 * (double)w + 6.0 * (double)u) : 0.0;
+        g[2 * NT + k] += (double)q;
     }
-    __device__ __attribute__((always_inline)) inline void flush(int nsite) {
-        fold();
-        if (cur >= 0) {
-            const int t = cur / nsite;
-            // every entry is updated (a conditional update of one entry would be
-            // folded into a dynamically indexed array held in scratch memory)
+    __device__ __attribute__((always_inline)) inline void add_dyn(int t, bool is_lj) {
+        const double e = is_lj ? (double)sw : 0.0;
+        const double d = is_lj ? (double)csig * (12.0 * (double)sw + 6.0 * (double)su) : 0.0;
+        const double c = (double)sq;
 #pragma unroll
-            for (int k = 0; k < GT; ++k) {
-                g[k] += (k == t) ? cE : 0.0;
-                g[GT + k] += (k == t) ? cS : 0.0;
-                g[2 * GT + k] += (k == t) ? cQ : 0.0;
-            }
+        for (int k = 0; k < NT; ++k) {
+            g[k] += (k == t) ? e : 0.0;
+            g[NT + k] += (k == t) ? d : 0.0;
+            g[2 * NT + k] += (k == t) ? c : 0.0;
         }
-        cE = cS = cQ = 0.0;
+        sw = su = sq = (Real)0;
     }
-    __device__ __attribute__((always_inline)) inline void type(int tt, int nsite) {
+    // all-atoms path: a candidate of type tt (times nsite) with d/d sigma factor cs
+    __device__ __attribute__((always_inline)) inline void type(int tt, int nsite, Real cs, bool is_lj) {
         if (tt != cur) {
-            flush(nsite);
+            if (cur >= 0) add_dyn(cur / nsite, is_lj);
             cur = tt;
+            csig = cs;
         }
+    }
+    __device__ __attribute__((always_inline)) inline void flush(int nsite, bool is_lj) {
+        if (cur >= 0) add_dyn(cur / nsite, is_lj);
     }
 };
 
 // The pair loops of one workgroup's chunk.  Fast path: scan the image list,
 // keep the images within r_cmax + chunk_r of the chunk centre by a
 // wave-ballot compaction into LDS, evaluate every lane's point against them.
-// Slow path: every replicated atom, nearest image per pair.  GRAD adds the
-// per-type derivative sums of a gradient run.
-template <int ESM, typename Real, bool GRAD>
+// Slow path: every replicated atom, nearest image per pair.  NT > 0 (a
+// gradient run of at most NT framework types) adds the per-type derivative
+// sums: on the fast path the images of a tile come in type order (the image
+// list is sorted by species inside each spatial bin, and a tile never spans
+// two bins), so each tile's kept candidates are runs of one type each, counted
+// per type at compaction; a run is summed with its type's parameters held in
+// registers and folded into the type's FP64 sums once.
+template <int ESM, typename Real, int NT>
 __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs& a, bool valid, double px, double py, double pz, double cf0,
                                    double cf1, double cf2, int site, const LJP<Real>* s_lj, const Real* s_gc,
                                    PairLds L, double& acc_lj, double& acc_sr, bool& ovl, unsigned long long& n_cand,
-                                   TypeSums<Real>& ts) {
+                                   TypeSums<Real, (NT > 0 ? NT : 1)>& ts) {
+    constexpr bool GRAD = NT > 0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double f11 = a.f11, f12 = a.f12, f13 = a.f13, f22 = a.f22, f23 = a.f23, f33 = a.f33;
     const double r2lj = a.r2_lj, r2sr = a.r2_sr;
@@ -660,8 +672,19 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                 }
                 const unsigned long long m = __ballot(keep);
                 const int before = __popcll(m & ((1ull << lane) - 1ull));
+                int tc[NT > 0 ? NT : 1];
+                if constexpr (GRAD) {
+#pragma unroll
+                    for (int k = 0; k < NT; ++k) tc[k] = __popcll(__ballot(keep && ty == k * nsite));
+                }
                 __syncthreads();  // the previous round's candidates (and the tile counts) are consumed
-                if (lane == 0) L.wcnt[wave] = __popcll(m);
+                if (lane == 0) {
+                    L.wcnt[wave] = __popcll(m);
+                    if constexpr (GRAD) {
+#pragma unroll
+                        for (int k = 0; k < NT; ++k) L.tcnt[wave * NT + k] = tc[k];
+                    }
+                }
                 __syncthreads();
                 int woff[WG / 64 + 1];
                 woff[0] = 0;
@@ -678,33 +701,55 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                 __syncthreads();
                 n_cand += (unsigned long long)woff[WG / 64];
                 if (wave_live) {
-                    // the candidates of each tile accumulate in Real, flushed to FP64
+                    if constexpr (GRAD) {
+                        // tile by tile (the energies' FP32 sums fold per tile, as below),
+                        // type run by type run inside each tile
+                        int j = 0;
+                        for (int gi = 0; gi < WG / 64; ++gi) {
+                            Real plj = (Real)0, psr = (Real)0;
 #pragma unroll
-                    for (int gi = 0; gi < WG / 64; ++gi) {
-                        Real plj = (Real)0, psr = (Real)0;
-                        // two candidates per iteration share the LDS address registers (FP32 energies)
-                        constexpr int PAIR_UNROLL = (sizeof(Real) == 4 && !GRAD) ? 2 : 1;
+                            for (int k = 0; k < NT; ++k) {
+                                const int n = __builtin_amdgcn_readfirstlane(L.tcnt[gi * NT + k]);
+                                if (n == 0) continue;
+                                const LJP<Real> p = s_lj[k * nsite + site];
+                                const Real cs = s_gc[k * nsite + site];
+                                Real sw = (Real)0, su = (Real)0, sq = (Real)0;
+                                const int je = j + n;
+#pragma unroll 2
+                                for (; j < je; ++j) {
+                                    const double4 c = L.c[j];
+                                    const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
+                                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                                    const float qj = sizeof(Real) == 4 ? unpack_q(c.w) : 0.0f;
+                                    pair_term_grad<ESM, Real>(r2, c.w, qj, p, is_lj, r2lj, r2sr, et, ovl, plj, psr, sw,
+                                                              su, sq);
+                                }
+                                ts.add(k, sw, su, sq, cs, is_lj);
+                            }
+                            acc_lj += (double)plj;
+                            acc_sr += (double)psr;
+                        }
+                    } else {
+                        // the candidates of each tile accumulate in Real, flushed to FP64
+#pragma unroll
+                        for (int gi = 0; gi < WG / 64; ++gi) {
+                            Real plj = (Real)0, psr = (Real)0;
+                            // two candidates per iteration share the LDS address registers (FP32 energies)
+                            constexpr int PAIR_UNROLL = sizeof(Real) == 4 ? 2 : 1;
 #pragma unroll PAIR_UNROLL
-                        for (int j = woff[gi]; j < woff[gi + 1]; ++j) {
-                            const double4 c = L.c[j];
-                            const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
-                            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                            const bool f32 = sizeof(Real) == 4;
-                            const int tj = f32 ? unpack_t(c.w) : L.t[j];
-                            const float qj = f32 ? unpack_q(c.w) : 0.0f;  // the FP64 terms take c.w
-                            if constexpr (GRAD) {
-                                const int tt = __builtin_amdgcn_readfirstlane(tj);
-                                ts.type(tt, nsite);
-                                pair_term_grad<ESM, Real>(r2, c.w, qj, s_lj[tt + site], s_gc[tt + site], is_lj,
-                                                          r2lj, r2sr, et, ovl, plj, psr, ts.fE, ts.fS, ts.fQ);
-                            } else {
+                            for (int j = woff[gi]; j < woff[gi + 1]; ++j) {
+                                const double4 c = L.c[j];
+                                const double ddx = c.x - prx, ddy = c.y - pry, ddz = c.z - prz;
+                                const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                                const bool f32 = sizeof(Real) == 4;
+                                const int tj = f32 ? unpack_t(c.w) : L.t[j];
+                                const float qj = f32 ? unpack_q(c.w) : 0.0f;  // the FP64 terms take c.w
                                 pair_term<ESM, Real>(r2, c.w, qj, s_lj[tj + site], is_lj, r2lj, r2sr, et, ovl, plj,
                                                      psr);
                             }
+                            acc_lj += (double)plj;
+                            acc_sr += (double)psr;
                         }
-                        acc_lj += (double)plj;
-                        acc_sr += (double)psr;
-                        if constexpr (GRAD) ts.fold();
                     }
                 }
             }
@@ -737,9 +782,9 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                         const double r2 = cx * cx + cy * cy + cz * cz;
                         if constexpr (GRAD) {
                             const int tt = __builtin_amdgcn_readfirstlane(L.t[j]);
-                            ts.type(tt, nsite);
-                            pair_term_grad<ESM, Real>(r2, at.w, L.qf[j], s_lj[tt + site], s_gc[tt + site], is_lj,
-                                                      r2lj, r2sr, et, ovl, plj, psr, ts.fE, ts.fS, ts.fQ);
+                            ts.type(tt, nsite, s_gc[tt + site], is_lj);
+                            pair_term_grad<ESM, Real>(r2, at.w, L.qf[j], s_lj[tt + site], is_lj, r2lj, r2sr, et,
+                                                      ovl, plj, psr, ts.sw, ts.su, ts.sq);
                         } else {
                             pair_term<ESM, Real>(r2, at.w, L.qf[j], s_lj[L.t[j] + site], is_lj, r2lj, r2sr, et, ovl,
                                                  plj, psr);
@@ -747,12 +792,12 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                     }
                     acc_lj += (double)plj;
                     acc_sr += (double)psr;
-                    if constexpr (GRAD) ts.fold();
+                    if constexpr (GRAD) ts.flush(nsite, is_lj);  // FP64 per 64 candidates, as the energies
                 }
             }
         }
     }
-    if constexpr (GRAD) ts.flush(nsite);
+    if constexpr (GRAD) ts.flush(nsite, is_lj);
 }
 
 // LDS: TILE candidate records, types and FP32 charges, the selected-tile list,
@@ -764,10 +809,13 @@ constexpr int pair_lds_fixed() {
            (sizeof(Real) == 4 ? ERFC_SEG * (int)sizeof(float4) : 3 * ERFC64_SEG * (int)sizeof(double2));
 }
 
-template <typename Real, bool GRAD>
+// NT = 0: energies.  NT > 0: a gradient run of at most NT framework types.
+template <typename Real, int NT>
 __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
+    constexpr bool GRAD = NT > 0;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_wcnt[WG / 64];
+    __shared__ int s_tcnt[(WG / 64) * (NT > 0 ? NT : 1)];
     PairLds L;
     L.c = reinterpret_cast<double4*>(smem);
     L.t = reinterpret_cast<int*>(smem + TILE * sizeof(double4));
@@ -778,6 +826,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     double2* s_erfc64 = reinterpret_cast<double2*>(s_erfc);
     L.erfc64 = s_erfc64;
     L.wcnt = s_wcnt;
+    L.tcnt = s_tcnt;
     LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + pair_lds_fixed<Real>());
 
     const int tid = threadIdx.x;
@@ -847,23 +896,29 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     double acc_lj = 0.0, acc_sr = 0.0;
     bool ovl = false;
     unsigned long long n_cand = 0;
-    TypeSums<Real> ts;
+    TypeSums<Real, (NT > 0 ? NT : 1)> ts;
     if (a.esm == 0)
-        chunk_pairs<0, Real, GRAD>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
-                                   n_cand, ts);
+        chunk_pairs<0, Real, NT>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+                                 n_cand, ts);
     else if (a.esm == 1)
-        chunk_pairs<1, Real, GRAD>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
-                                   n_cand, ts);
+        chunk_pairs<1, Real, NT>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+                                 n_cand, ts);
     else
-        chunk_pairs<2, Real, GRAD>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
-                                   n_cand, ts);
+        chunk_pairs<2, Real, NT>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+                                 n_cand, ts);
     if (valid) {
         g.out_lj[pid] = ovl ? INFINITY : acc_lj;
         g.out_sr[pid] = acc_sr;
         if constexpr (GRAD) {
-            double* o = g.out_g + pid * (GQ * GT);
+            // (types NT .. GT - 1 of the fixed stride are never read)
+            double* o = g.out_g + pid;
+            const long long gs = g.g_stride;
 #pragma unroll
-            for (int k = 0; k < 3 * GT; ++k) o[k] = ts.g[k];
+            for (int k = 0; k < NT; ++k) {
+                o[k * gs] = ts.g[k];
+                o[(GT + k) * gs] = ts.g[NT + k];
+                o[(2 * GT + k) * gs] = ts.g[2 * NT + k];
+            }
         }
     }
     if (g.cand_pairs != nullptr) {
@@ -1251,76 +1306,90 @@ __global__ __launch_bounds__(64) void type_structure_factor_kernel(const double4
     }
 }
 
-// Per-type Fourier sums of every charged point of a gradient run (FP64,
-// the recurrences of point_fourier_kernel over the per-type boxes).
-__global__ __launch_bounds__(WG) void point_fourier_grad_kernel(GroupArgs g) {
-    __shared__ double2 s_a[FT];
+// Per-type Fourier sums of every charged point of a gradient run, FP64:
+// F_t(u) = sum over the kept m of Re(A_t(m) e^{-i 2 pi m.u}) for NT framework
+// types.  Each (ma, mb) row runs over its kept mc range [lo, hi] directly: the
+// phase z_a^ma z_b^mb z_c^mc steps along the row by z_c, and z_c^lo is carried
+// from row to row by single steps of z_c or its conjugate (the kept ranges of
+// neighbouring rows differ by a few entries).  Per kept entry and point that
+// is 2 NT FMAs for the sums and one complex product for the phase; the
+// coefficients are read by scalar loads shared by the FOURIER_PTS points of a
+// work-item, as in point_fourier_kernel.
+template <int NT>
+__global__ __launch_bounds__(FOURIER_WG) void point_fourier_grad_kernel(GroupArgs g) {
     const int tid = threadIdx.x;
-    const long long p0 = (long long)blockIdx.x * WG;
+    const long long p0 = (long long)blockIdx.x * WG;  // WG slots per workgroup, FOURIER_PTS per work-item
     const SegArgs sg = g.segs[find_segment<SEG_BY_SLOT>(g.segs, g.n_seg, p0)];
     const RunArgs& a = g.runs[sg.run];
     if (a.esm == 0) return;  // uniform: the whole workgroup leaves
-    const long long pid = p0 + tid, lp = pid - sg.slot_base;
-    bool valid = lp < sg.n_slots;
-    const double4 p = valid ? g.pos[pid] : make_double4(0.0, 0.0, 0.0, 0.0);
-    valid = valid && p.w != 0.0;
-    const int site = (int)(lp % a.nsite);
-    const bool charged = valid && ((a.site_q_mask >> site) & 1u);
-    double t0 = p.x * a.repd[0], t1 = p.y * a.repd[1], t2 = p.z * a.repd[2];
-    t0 -= rint(t0);
-    t1 -= rint(t1);
-    t2 -= rint(t2);
-    double s, c;
-    sincospi(2.0 * t0, &s, &c);
-    const double2 Ea = make_double2(c, -s);
-    sincospi(2.0 * t1, &s, &c);
-    const double2 Eb = make_double2(c, -s);
-    sincospi(2.0 * t2, &s, &c);
-    const double2 Ec = make_double2(c, -s);
+    FourierPoint P[FOURIER_PTS];
+#pragma unroll
+    for (int k = 0; k < FOURIER_PTS; ++k) P[k] = fourier_point(g, sg, a, p0 + tid + k * FOURIER_WG);
     // wave-uniform: scalar loop control (the loads are not provably uniform)
     const int Ma = __builtin_amdgcn_readfirstlane(a.Ma), Mb = __builtin_amdgcn_readfirstlane(a.Mb),
               Mc = __builtin_amdgcn_readfirstlane(a.Mc);
-    double2 Eb_neg = make_double2(1.0, 0.0), Ec_neg = make_double2(1.0, 0.0);
-    const double2 Eb_c = make_double2(Eb.x, -Eb.y), Ec_c = make_double2(Ec.x, -Ec.y);
-    for (int q = 0; q < Mb; ++q) Eb_neg = cmul(Eb_neg, Eb_c);
-    for (int q = 0; q < Mc; ++q) Ec_neg = cmul(Ec_neg, Ec_c);
-
+    double2 Eb_neg[FOURIER_PTS], Pa[FOURIER_PTS], Pb[FOURIER_PTS], Zc[FOURIER_PTS], Z[FOURIER_PTS];
+    double acc[FOURIER_PTS][NT];
+#pragma unroll
+    for (int k = 0; k < FOURIER_PTS; ++k) {
+        Eb_neg[k] = make_double2(1.0, 0.0);
+        const double2 Eb_c = make_double2(P[k].Eb.x, -P[k].Eb.y);
+        for (int q = 0; q < Mb; ++q) Eb_neg[k] = cmul(Eb_neg[k], Eb_c);
+        Pa[k] = make_double2(1.0, 0.0);
+        Pb[k] = Eb_neg[k];
+        Zc[k] = make_double2(P[k].zr, P[k].zi);
+        Z[k] = make_double2(1.0, 0.0);  // z_c^zc_at
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[k][t] = 0.0;
+    }
+    int zc_at = 0;
     const int rowlen = 2 * Mc + 1, nb = 2 * Mb + 1;
     const int n_rows = (Ma + 1) * nb;
-    const int rows_per_tile = max(1, (FT / GT) / rowlen);
-    double2 Pa = make_double2(1.0, 0.0);
-    double2 Pb = Eb_neg;
-    double acc[GT];
+    const gconst_ptr<double> At = uniform_ptr(reinterpret_cast<const double*>(a.A_t));
+    const gconst_ptr<int> krows = uniform_ptr(reinterpret_cast<const int*>(a.krows));
+    for (int r = 0; r < n_rows; ++r) {
+        if (r > 0 && (r % nb) == 0) {  // next m_a
 #pragma unroll
-    for (int t = 0; t < GT; ++t) acc[t] = 0.0;
-    for (int r0 = 0; r0 < n_rows; r0 += rows_per_tile) {
-        const int r1 = min(n_rows, r0 + rows_per_tile);
-        const int n_ent = (r1 - r0) * rowlen * GT;
-        __syncthreads();
-        for (int e = tid; e < n_ent; e += WG) s_a[e] = a.A_t[(long long)r0 * rowlen * GT + e];
-        __syncthreads();
-        for (int r = r0; r < r1; ++r) {
-            if (r > 0 && (r % nb) == 0) {  // next m_a
-                Pa = cmul(Pa, Ea);
-                Pb = cmul(Pa, Eb_neg);
+            for (int k = 0; k < FOURIER_PTS; ++k) {
+                Pa[k] = cmul(Pa[k], P[k].Ea);
+                Pb[k] = cmul(Pa[k], Eb_neg[k]);
             }
-            double2 P = cmul(Pb, Ec_neg);
-            const double2* row = s_a + (r - r0) * rowlen * GT;
-            for (int mc = 0; mc < rowlen; ++mc) {
-#pragma unroll
-                for (int t = 0; t < GT; ++t) {
-                    const double2 A = row[mc * GT + t];
-                    acc[t] += A.x * P.x - A.y * P.y;
-                }
-                P = cmul(P, Ec);
-            }
-            Pb = cmul(Pb, Eb);
         }
-    }
-    if (valid) {
-        double* o = g.out_g + pid * (GQ * GT) + 3 * GT;
+        const int lo = krows[2 * r], hi = krows[2 * r + 1];
+        if (lo <= hi) {
+            for (; zc_at < lo; ++zc_at)
 #pragma unroll
-        for (int t = 0; t < GT; ++t) o[t] = charged ? acc[t] : 0.0;
+                for (int k = 0; k < FOURIER_PTS; ++k) Z[k] = cmul(Z[k], Zc[k]);
+            for (; zc_at > lo; --zc_at)
+#pragma unroll
+                for (int k = 0; k < FOURIER_PTS; ++k) Z[k] = cmul(Z[k], make_double2(Zc[k].x, -Zc[k].y));
+            double2 Q[FOURIER_PTS];
+#pragma unroll
+            for (int k = 0; k < FOURIER_PTS; ++k) Q[k] = cmul(Pb[k], Z[k]);
+            // entry (r, mc): 2 GT doubles from At + 2 GT ((r rowlen) + mc + Mc)
+            const gconst_ptr<double> row = At + 2LL * GT * ((long long)r * rowlen + Mc);
+            for (int mc = lo; mc <= hi; ++mc) {
+                const gconst_ptr<double> e = row + 2LL * GT * mc;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const double Ar = e[2 * t], Ai = e[2 * t + 1];
+#pragma unroll
+                    for (int k = 0; k < FOURIER_PTS; ++k) acc[k][t] = fma(Ar, Q[k].x, fma(-Ai, Q[k].y, acc[k][t]));
+                }
+#pragma unroll
+                for (int k = 0; k < FOURIER_PTS; ++k) Q[k] = cmul(Q[k], Zc[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < FOURIER_PTS; ++k) Pb[k] = cmul(Pb[k], P[k].Eb);
+    }
+#pragma unroll
+    for (int k = 0; k < FOURIER_PTS; ++k) {
+        if (P[k].valid) {
+            double* o = g.out_g + 3 * GT * g.g_stride + P[k].pid;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) o[t * g.g_stride] = P[k].charged ? acc[k][t] : 0.0;
+        }
     }
 }
 
@@ -1352,15 +1421,17 @@ __global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
         for (int s = 0; s < ns; ++s) {
             vdw += g.out_lj[p0 + s];
             if (a.esm != 0) es += a.site_q[s] * (g.out_sr[p0 + s] * a.inv4pe + g.out_lr[p0 + s]);
-            const double* gs = g.out_g + (p0 + s) * (GQ * GT);
+            const double* gs = g.out_g + (p0 + s);
+            const long long st = g.g_stride;
 #pragma unroll
             for (int t = 0; t < GT; ++t) {
                 if (t < a.n_gt) {
                     const double4 J = a.jac[t * ns + s];
-                    const double re = 4.0 * gs[t], rs = gs[GT + t];
+                    const double re = 4.0 * gs[t * st], rs = gs[(GT + t) * st];
                     de[t] += re * J.x + rs * J.z;
                     dsg[t] += re * J.y + rs * J.w;
-                    if (a.esm != 0) dq[t] += a.site_q[s] * (gs[2 * GT + t] * a.inv4pe + gs[3 * GT + t]);
+                    if (a.esm != 0)
+                        dq[t] += a.site_q[s] * (gs[(2 * GT + t) * st] * a.inv4pe + gs[(3 * GT + t) * st]);
                 }
             }
         }

@@ -839,9 +839,43 @@ int grid_prepare(const pm_grid_run& in, PointRun& r) {
     return PM_OK;
 }
 
-template <typename Real, bool GRAD>
+template <typename Real, int NT>
 void pair_launch_t(const GroupArgs& g, int n_wg, size_t shm) {
-    hipLaunchKernelGGL((point_pair_kernel<Real, GRAD>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, g);
+    hipLaunchKernelGGL((point_pair_kernel<Real, NT>), dim3(n_wg), dim3(WG), shm, g_ctx.stream, g);
+}
+
+// Compiled type counts of the gradient kernels: a gradient run of n_gt
+// framework types runs the smallest one that holds it (its per-type FP64 sums
+// take 6 NT registers per point).
+int grad_nt(int n_gt) { return n_gt <= 2 ? 2 : n_gt <= 4 ? 4 : n_gt <= 6 ? 6 : GT; }
+
+template <typename Real>
+void pair_launch_grad(const GroupArgs& g, int n_wg, size_t shm, int nt) {
+    switch (nt) {
+        case 2: pair_launch_t<Real, 2>(g, n_wg, shm); break;
+        case 4: pair_launch_t<Real, 4>(g, n_wg, shm); break;
+        case 6: pair_launch_t<Real, 6>(g, n_wg, shm); break;
+        default: pair_launch_t<Real, GT>(g, n_wg, shm); break;
+    }
+}
+
+template <int NT>
+void fourier_grad_launch_t(const GroupArgs& g, unsigned n_wg) {
+    hipLaunchKernelGGL(point_fourier_grad_kernel<NT>, dim3(n_wg), dim3(FOURIER_WG), 0, g_ctx.stream, g);
+}
+
+// The Fourier sums' work is proportional to the type count: one kernel per count.
+void fourier_grad_launch(const GroupArgs& g, unsigned n_wg, int n_gt) {
+    switch (n_gt) {
+        case 1: fourier_grad_launch_t<1>(g, n_wg); break;
+        case 2: fourier_grad_launch_t<2>(g, n_wg); break;
+        case 3: fourier_grad_launch_t<3>(g, n_wg); break;
+        case 4: fourier_grad_launch_t<4>(g, n_wg); break;
+        case 5: fourier_grad_launch_t<5>(g, n_wg); break;
+        case 6: fourier_grad_launch_t<6>(g, n_wg); break;
+        case 7: fourier_grad_launch_t<7>(g, n_wg); break;
+        default: fourier_grad_launch_t<GT>(g, n_wg); break;
+    }
 }
 
 unsigned grid1d(long long n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -864,6 +898,11 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
                    double* out_gsums = nullptr, const FixedScratch* fixed = nullptr) {
     const bool fp64 = precision == PM_FP64;
     const int n_runs = (int)runs.size();
+    int n_gt = 0, nt_grad = 0;  // gradient runs: framework types, and the pair kernel's compiled count
+    if (grad) {
+        for (const auto& r : runs) n_gt = std::max(n_gt, r.ra.n_gt);
+        nt_grad = grad_nt(n_gt);
+    }
     // image lists and paths
     for (auto& r : runs) {
         RunArgs& a = r.ra;
@@ -1108,6 +1147,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     g.out_lr = reinterpret_cast<double*>(base + o_lr);
     g.cand_pairs = d_cand;
     g.out_g = grad ? reinterpret_cast<double*>(base + o_g) : nullptr;
+    g.g_stride = max_slots;
     const size_t shm = (fp64 ? pair_lds_fixed<double>() : pair_lds_fixed<float>()) + std::max<size_t>(lj_lds, 16);
 
     PM_HIP(hipMemsetAsync(d_cand, 0, sizeof(unsigned long long), g_ctx.stream));
@@ -1138,11 +1178,11 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         }
         PM_HIP(hipEventRecord(g_ctx.evp0, g_ctx.stream));
         if (grad) {
-            if (fp64) pair_launch_t<double, true>(g, (int)pair_wg, shm);
-            else pair_launch_t<float, true>(g, (int)pair_wg, shm);
+            if (fp64) pair_launch_grad<double>(g, (int)pair_wg, shm, nt_grad);
+            else pair_launch_grad<float>(g, (int)pair_wg, shm, nt_grad);
         } else {
-            if (fp64) pair_launch_t<double, false>(g, (int)pair_wg, shm);
-            else pair_launch_t<float, false>(g, (int)pair_wg, shm);
+            if (fp64) pair_launch_t<double, 0>(g, (int)pair_wg, shm);
+            else pair_launch_t<float, 0>(g, (int)pair_wg, shm);
         }
         PM_HIP(hipEventRecord(g_ctx.evp1, g_ctx.stream));
         const long long slot_wg = (gr.n_slots + WG - 1) / WG;
@@ -1174,8 +1214,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
                 hipLaunchKernelGGL(grid_fourier_a_kernel, dim3(grid1d(n_p, 256)), dim3(256), 0, g_ctx.stream, f);
             }
         }
-        if (grad && gr.es)
-            hipLaunchKernelGGL(point_fourier_grad_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
+        if (grad && gr.es) fourier_grad_launch(g, (unsigned)slot_wg, n_gt);
         if (widom && grad)
             hipLaunchKernelGGL(widom_combine_grad_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
         else if (widom)
@@ -1271,7 +1310,6 @@ int ensure_type_boxes(pm_ewald* ew, const pm_framework* fw) {
     if (ew->d_box_t) return PM_OK;
     if ((int)fw->species_global.size() > GT)
         return fail(PM_ERR_ARG, "gradient runs support at most PM_MAX_GRAD_TYPES framework species");
-    if (2 * ew->Mc + 1 > FT / GT) return fail(PM_ERR_ARG, "k-vector box too wide for the gradient pass");
     const size_t box_n = (size_t)(ew->Ma + 1) * (2 * ew->Mb + 1) * (2 * ew->Mc + 1);
     std::vector<double4> ua((size_t)std::max<int64_t>(fw->n_unit, 1));
     for (int64_t u = 0; u < fw->n_unit; ++u)

@@ -193,3 +193,39 @@ def check_block_sums(got, want, rtol, what, s_allow=None, u_allow=None):
             scale = abs(u_w) + abs(p_w)
             assert abs(p_g - p_w) <= ulim * scale, f"{what}: block {b}: <U_{name}> {p_g!r} vs {p_w!r}"
     return worst_s, worst_u
+
+
+def derivative_sum_bounds(vdw, es, d, cond, n_per_block, tol, temperature):
+    """Bounds on the error of the per-block weighted derivative sums
+    G_b = sum_i w_i dE_i/d theta (w_i = e^(-E_i/T)) and of
+    g = d ln K_H / d theta = -sum_b G_b / (T sum_b S_b), from the oracle's
+    per-insertion derivatives d (n, species, 3) and their summand conditions
+    cond (sum of the magnitudes of the pair and Fourier summands behind each
+    derivative, oracle/dual.c).  With each summand carrying relative rounding
+    tol, a derivative errs by up to tol * cond_i, and an energy by up to
+    dE_i = tol (|E_vdw| + |E_es| + T), which moves weight i by dE_i / T:
+
+      |dG_b| <= tol * sum_i w_i (cond_i + |d_i| (|E_vdw| + |E_es| + T) / T)
+      |dg|   <= (sum_b |dG_b| + |sum_b G_b| dS / S) / (T S),
+                 dS / S = tol * sum_i w_i (|E_vdw| + |E_es| + T) / (T S).
+
+    Derivatives of a neutral probe's energy by a framework charge cancel
+    across its sites (sum_s q_s phi_t(x_s)), so cond_i can exceed |d_i| by
+    10^3-10^4 there; elsewhere it is within a few |d_i|.  Returns
+    (per-block bounds (blocks, species, 3), bound on g (species, 3))."""
+    vdw = np.asarray(vdw, dtype=float)
+    es = np.asarray(es, dtype=float)
+    tot = vdw + es
+    fin = np.isfinite(tot)
+    with np.errstate(over="ignore", invalid="ignore"):
+        w = np.where(fin, np.exp(-np.where(fin, tot, 0.0) / temperature), 0.0)
+        mag = np.where(fin, np.abs(np.where(fin, vdw, 0.0)) + np.abs(es) + temperature, 0.0)
+    ww = w[:, None, None]
+    per = tol * ww * (np.abs(cond) + np.abs(d) * (mag / temperature)[:, None, None])
+    nb = tot.shape[0] // n_per_block
+    bb = per.reshape(nb, n_per_block, *d.shape[1:]).sum(axis=1)
+    S = w.sum()
+    G = (ww * d).sum(axis=0)
+    ds = tol * float((w * mag).sum()) / (temperature * S)
+    bg = (bb.sum(axis=0) + np.abs(G) * ds) / (temperature * S)
+    return bb, bg

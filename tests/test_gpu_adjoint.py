@@ -7,14 +7,17 @@ the oracle's FP64 ln K_H on the same insertion stream, to 1e-4 relative
 atom of species t (h = 1e-4 of their mean |q|).  The per-block weighted
 derivative sums sum_i w_i dE_i/d theta are checked against the oracle's
 dual-number derivatives (oracle/dual.c, the forward-mode check) on the same
-stream: 1e-8 (FP64) and 1e-4 (mixed) relative, floor 1e-8 / 1e-3 of the
-block's largest component (1e-6 of the largest for d ln K_H / d theta).  The pass's block sums must equal pm_widom's bit
-for bit."""
+stream, on the image-list and the all-atoms pair paths: FP64 to 1e-8
+relative (floor 1e-8 of the block's largest component); mixed within the
+derivative-sum bounds of tests/_parity.py (TOL 1e-5 per summand and per
+energy).  The pass's block sums must equal pm_widom's bit for bit."""
 import math
 import sys
 
 import numpy as np
 import pytest
+
+from _parity import TINY_S, TOL, derivative_sum_bounds
 
 pytestmark = pytest.mark.gpu
 
@@ -110,12 +113,9 @@ def test_dlnKH_matches_oracle_finite_differences(pm, orc, adj, precision):
     assert err.max() <= 1e-4
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_block_derivative_sums_match_dual_oracle(pm, orc, adj, precision):
+def _dual_check(pm, orc, adj, precision, nb, seed):
     case, fw, ew, pr = adj
     ff, x, co2 = case.ljff, case.crystal, case.probes["CO2"]
-    nb, seed = 2000, 31
-    say(f"case: sum_i w_i dE_i/d theta per block, CO2 5 x {nb} insertions, vs the dual-number oracle ({precision})")
     sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, T, seed, N_BLOCKS, nb, precision, ff)
     lj_off, q_off = pm.site_offsets(co2)
     o = orc.OracleSystem((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
@@ -125,7 +125,7 @@ def test_block_derivative_sums_match_dual_oracle(pm, orc, adj, precision):
     pe = np.array([ff.pure_epsilon[s] for s in ff.species])
     ps = np.array([ff.pure_sigma[s] for s in ff.species])
     ft = sorted({ff.index(s) for s in x.species})
-    v, e, d = o.widom_param_grad(seed, 0, N_BLOCKS * nb, pe, ps, ft)
+    v, e, d, cd = o.widom_param_grad(seed, 0, N_BLOCKS * nb, pe, ps, ft, cond=True)
     E = v + e
     fin = np.isfinite(E)
     w = np.where(fin, np.exp(-np.where(fin, E, 0.0) / T), 0.0)
@@ -133,22 +133,47 @@ def test_block_derivative_sums_match_dual_oracle(pm, orc, adj, precision):
                      for b in range(N_BLOCKS)])                      # (blocks, species, 3)
     got = grads[:, ft, :]
     assert np.all(grads[:, [k for k in range(grads.shape[1]) if k not in ft], :] == 0.0)
-    rtol, floor = (1e-8, 1e-8) if precision == "fp64" else (1e-4, 1e-6)
-    # per-block sums of a component can cancel across insertions (floor 1e-3
-    # of the block's largest component on the mixed path)
-    bfloor = floor if precision == "fp64" else 1e-3
-    worst = 0.0
-    for b in range(N_BLOCKS):
-        scale = np.maximum(np.abs(want[b]), bfloor * np.abs(want[b]).max())
-        err = np.abs(got[b] - want[b]) / scale
-        worst = max(worst, float(err.max()))
     S = sums[:, 0].sum()
     g_dev = -got.sum(axis=0) / (T * S)
     g_dual = -want.sum(axis=0) / (T * w.sum())
-    dl = np.abs(g_dev - g_dual) / np.maximum(np.abs(g_dual), floor * np.abs(g_dual).max())
-    say(f"   per-block weighted derivative sums: max relative deviation {worst:.2e}; d ln K_H/d theta: "
-        f"{dl.max():.2e}")
-    assert worst <= rtol and dl.max() <= rtol
+    dl_floor = 1e-8 if precision == "fp64" else 1e-6
+    dl = np.abs(g_dev - g_dual) / np.maximum(np.abs(g_dual), dl_floor * np.abs(g_dual).max())
+    if precision == "fp64":
+        worst = 0.0
+        for b in range(N_BLOCKS):
+            scale = np.maximum(np.abs(want[b]), 1e-8 * np.abs(want[b]).max())
+            worst = max(worst, float((np.abs(got[b] - want[b]) / scale).max()))
+        say(f"   per-block weighted derivative sums: max relative deviation {worst:.2e}; d ln K_H/d theta: "
+            f"{dl.max():.2e}")
+        assert worst <= 1e-8 and dl.max() <= 1e-8
+    else:
+        bb, bg = derivative_sum_bounds(v, e, d, cd, nb, TOL["mixed"], T)
+        # (a block whose weights all underflow has zero sums and a zero bound)
+        rb = float((np.abs(got - want) / np.maximum(bb, TINY_S)).max())
+        rg = float((np.abs(g_dev - g_dual) / bg).max())
+        say(f"   d ln K_H/d theta max relative deviation {dl.max():.2e}; error / bound: per-block sums {rb:.2e}, "
+            f"d ln K_H/d theta {rg:.2e}")
+        assert rb <= 1.0 and rg <= 1.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_block_derivative_sums_match_dual_oracle(pm, orc, adj, precision):
+    say(f"case: sum_i w_i dE_i/d theta per block, CO2 5 x 2000 insertions, vs the dual-number oracle ({precision})")
+    _dual_check(pm, orc, adj, precision, 2000, 31)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_all_atoms_path_derivative_sums_match_dual_oracle(pm, orc, adj, precision):
+    """The pair kernel's all-atoms path (one cell per run: the chunk radius
+    exceeds the image-list path's limit), where a candidate's type changes
+    from atom to atom."""
+    say(f"case: all-atoms pair path, CO2 5 x 2000 insertions, vs the dual-number oracle ({precision})")
+    before = pm.device.get_tuning()
+    try:
+        pm.device.set_tuning(65536, 0.5)
+        _dual_check(pm, orc, adj, precision, 2000, 37)
+    finally:
+        pm.device.set_tuning(before["cell_points"], before["chunk_r_max"])
 
 
 def test_henry_coefficient_gradient_api(pm):

@@ -24,8 +24,8 @@ import math
 import numpy as np
 import pytest
 
-from _parity import (TOL, block_sums_from_energies, check_block_sums, check_energies, describe,
-                     repulsive_block_allowance, say)
+from _parity import (TINY_S, TOL, block_sums_from_energies, check_block_sums, check_energies, derivative_sum_bounds,
+                     describe, repulsive_block_allowance, say)
 from conftest import golden
 
 pytestmark = pytest.mark.gpu
@@ -131,7 +131,7 @@ def co2_dual_full(pm, orc, co2_full):
     ps = np.array([ff.pure_sigma[s] for s in ff.species])
     ft = sorted({ff.index(s) for s in x.species})
     say(f"case: CO2 workload: dual-number oracle derivatives of {N_BLOCKS} x {N_PER_BLOCK} insertions")
-    v, e, d = r.o.widom_param_grad(r.seed, 0, N_BLOCKS * N_PER_BLOCK, pe, ps, ft)
+    v, e, d, cd = r.o.widom_param_grad(r.seed, 0, N_BLOCKS * N_PER_BLOCK, pe, ps, ft, cond=True)
     fin = np.isfinite(r.v)
     assert np.array_equal(np.isfinite(v), fin)
     assert np.max(np.abs(v[fin] + e[fin] - r.v[fin] - r.e[fin]) / (np.abs(r.v[fin]) + np.abs(r.e[fin]) + 1.0)) < 1e-10
@@ -139,36 +139,43 @@ def co2_dual_full(pm, orc, co2_full):
     w = np.where(fin, np.exp(-np.where(fin, E, 0.0) / 298.0), 0.0)
     G = np.stack([(w[b * N_PER_BLOCK:(b + 1) * N_PER_BLOCK, None, None]
                    * d[b * N_PER_BLOCK:(b + 1) * N_PER_BLOCK]).sum(axis=0) for b in range(N_BLOCKS)])
-    return ft, G, w.sum()
+    bounds = derivative_sum_bounds(v, e, d, cd, N_PER_BLOCK, TOL["mixed"], 298.0)
+    return ft, G, w.sum(), bounds
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_co2_workload_dlnKH_full_size_vs_dual_oracle(pm, co2_full, co2_dual_full, precision):
     """d ln K_H / d(eps_t, sigma_t, q_t) of the 5 framework species at the
     full 10^6 insertions (pm_widom_gradient) against the oracle's exact
-    (dual-number) derivatives on the same stream: 1e-8 (FP64) and 1e-4
-    (mixed) relative, floor 1e-6 of the largest component; per-block
-    weighted derivative sums to the same bars, floor 1e-6 (FP64) or 1e-3
-    (mixed) of the block's largest component."""
+    (dual-number) derivatives on the same stream.  FP64: 1e-8 relative, floor
+    1e-6 of the largest component (per block: of the block's largest).
+    Mixed: within the derivative-sum bounds of tests/_parity.py at TOL 1e-5
+    (each summand of a derivative rounded to 1e-5 of itself, each weight moved
+    by the energy tolerance): the charge derivatives of a neutral probe cancel
+    across its sites by up to 10^4, which no FP32 pair arithmetic resolves to
+    1e-4 of the result."""
     r = co2_full
-    ft, G, S = co2_dual_full
+    ft, G, S, (bb, bg) = co2_dual_full
     fw, ff, ew, pr = r.handles
     sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, 298.0, r.seed, N_BLOCKS, N_PER_BLOCK, precision, r.ff)
     got = grads[:, ft, :]
-    rtol = 1e-8 if precision == "fp64" else 1e-4
-    # per-block sums of a component can cancel across insertions: floor at a
-    # fraction of the block's largest component (1e-6 FP64, 1e-3 mixed)
-    floor = 1e-6 if precision == "fp64" else 1e-3
-    blk = max(float((np.abs(got[b] - G[b]) / np.maximum(np.abs(G[b]), floor * np.abs(G[b]).max())).max())
-              for b in range(N_BLOCKS))
     g_dev = -got.sum(axis=0) / (298.0 * sums[:, 0].sum())
     g_ref = -G.sum(axis=0) / (298.0 * S)
     err = np.abs(g_dev - g_ref) / np.maximum(np.abs(g_ref), 1e-6 * np.abs(g_ref).max())
     for i, t in enumerate(ft):
         say(f"   {r.ff.species[t]}: d ln K_H/d eps {g_dev[i, 0]:+.8e} (oracle {g_ref[i, 0]:+.8e}), d/d sigma "
             f"{g_dev[i, 1]:+.8e} ({g_ref[i, 1]:+.8e}), d/d q {g_dev[i, 2]:+.8e} ({g_ref[i, 2]:+.8e})")
-    say(f"   {precision}: d ln K_H/d theta max relative deviation {err.max():.2e}; per-block sums {blk:.2e}")
-    assert err.max() <= rtol and blk <= rtol
+    if precision == "fp64":
+        blk = max(float((np.abs(got[b] - G[b]) / np.maximum(np.abs(G[b]), 1e-6 * np.abs(G[b]).max())).max())
+                  for b in range(N_BLOCKS))
+        say(f"   fp64: d ln K_H/d theta max relative deviation {err.max():.2e}; per-block sums {blk:.2e}")
+        assert err.max() <= 1e-8 and blk <= 1e-8
+    else:
+        rb = float((np.abs(got - G) / np.maximum(bb, TINY_S)).max())
+        rg = float((np.abs(g_dev - g_ref) / bg).max())
+        say(f"   mixed: d ln K_H/d theta max relative deviation {err.max():.2e}; error / bound: per-block sums "
+            f"{rb:.2e}, d ln K_H/d theta {rg:.2e}")
+        assert rb <= 1.0 and rg <= 1.0
 
 
 @pytest.fixture(scope="module")

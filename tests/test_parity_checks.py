@@ -1,14 +1,15 @@
 """The parity checks of tests/_parity.py catch the mistakes they are there
-for.  Mistakes are planted in copies of the oracle's own energies (on the
-CPU): the checks must pass the oracle's energies unchanged and fail each
-planted error, in particular an error on the insertion that carries a block's
-K_H that the condition-scale bound alone would let through."""
+for.  Mistakes are planted in copies of the oracle's own energies and
+derivatives (on the CPU): the checks must pass the oracle's values unchanged
+(and moved by rounding-sized amounts) and fail each planted error, in
+particular an error on the insertion that carries a block's K_H that the
+condition-scale bound alone would let through."""
 import math
 
 import numpy as np
 import pytest
 
-from _parity import (TOL, block_sums_from_energies, check_block_sums, check_energies,
+from _parity import (TOL, block_sums_from_energies, check_block_sums, check_energies, derivative_sum_bounds,
                      repulsive_block_allowance, weight_ratio, weighted_mask)
 from conftest import golden
 
@@ -157,3 +158,83 @@ def test_weighted_mask_and_repulsive_allowance():
     # a <U> that cancels from large pair terms gets the allowance its terms imply
     _, u_c = repulsive_block_allowance(np.array([4330.0]), np.array([-4279.0]), 1, 1e-5, 298.0)
     assert u_c[0] == pytest.approx(1e-5 * (4330.0 + 4279.0 + 298.0) / 51.0, rel=1e-9)
+
+
+@pytest.fixture(scope="module")
+def co2_dual(pm, orc):
+    """The CO2 workload's framework (5 charged species), 5 x 2000 insertions:
+    the dual oracle's derivatives and their summand conditions."""
+    case = pm.synthetic.co2_widom_case()
+    x, ff, co2 = case.crystal, case.ljff, case.probes["CO2"]
+    lj_off, q_off = pm.site_offsets(co2)
+    o = orc.OracleSystem((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
+                         [ff.index(s) for s in x.species], x.atom_q, ff.epsilon_table, ff.sigma2_table,
+                         math.sqrt(ff.r2_cutoff), lj_off.T, [ff.index(s) for s in co2.atom_species], q_off.T,
+                         co2.charges_q, ewald_precision=case.ewald_precision)
+    pe = np.array([ff.pure_epsilon[s] for s in ff.species])
+    ps = np.array([ff.pure_sigma[s] for s in ff.species])
+    ft = sorted({ff.index(s) for s in x.species})
+    nb = 2000
+    v, e, d, cd = o.widom_param_grad(31, 0, 5 * nb, pe, ps, ft, cond=True)
+    v2, e2, d2 = o.widom_param_grad(31, 0, 5 * nb, pe, ps, ft)
+    assert np.array_equal(d, d2) and np.array_equal(v, v2) and np.array_equal(e, e2)
+    return nb, v, e, d, cd
+
+
+def _sums(nb, v, e, d):
+    tot = v + e
+    fin = np.isfinite(tot)
+    w = np.where(fin, np.exp(-np.where(fin, tot, 0.0) / 298.0), 0.0)
+    G = (w[:, None, None] * d).reshape(-1, nb, *d.shape[1:]).sum(axis=1)
+    return G, -G.sum(axis=0) / (298.0 * w.sum())
+
+
+def test_derivative_condition_bounds_the_derivative(co2_dual):
+    """A sum of summands is no larger than the sum of their magnitudes, and a
+    neutral probe's charge derivatives cancel across its sites: their
+    condition is far above the derivative itself."""
+    nb, v, e, d, cd = co2_dual
+    fin = np.isfinite(v + e)
+    assert np.all(cd[fin] >= np.abs(d[fin]) * (1.0 - 1e-12))
+    assert np.all(cd[~fin] == 0.0)
+    G, _ = _sums(nb, v, e, d)
+    tot = v + e
+    w = np.where(fin, np.exp(-np.where(fin, tot, 0.0) / 298.0), 0.0)
+    C = (w[:, None, None] * cd).sum(axis=0)
+    assert np.max(C[:, 2] / np.abs(G.sum(axis=0)[:, 2])) > 100.0   # q_t: cancellation across sites
+    assert np.max(C[:, 0] / np.abs(G.sum(axis=0)[:, 0])) < 10.0    # eps_t: none to speak of
+
+
+def test_derivative_bounds_pass_rounding_sized_errors(co2_dual):
+    """Every summand of every derivative moved by up to 0.5 TOL of itself (as
+    cond bounds) and every energy by 0.5 TOL (|E_vdw| + |E_es| + T): within
+    the bounds."""
+    nb, v, e, d, cd = co2_dual
+    bb, bg = derivative_sum_bounds(v, e, d, cd, nb, TOL["mixed"], 298.0)
+    rng = np.random.Generator(np.random.PCG64(5))
+    fin = np.isfinite(v + e)
+    dd = d + 0.5 * TOL["mixed"] * cd * rng.choice([-1.0, 1.0], size=d.shape)
+    ee = e + np.where(fin, 0.5 * TOL["mixed"] * (np.abs(np.where(fin, v, 0.0)) + np.abs(e) + 298.0), 0.0)
+    G0, g0 = _sums(nb, v, e, d)
+    G1, g1 = _sums(nb, v, ee, dd)
+    assert np.all(np.abs(G1 - G0) <= bb)
+    assert np.all(np.abs(g1 - g0) <= bg)
+
+
+@pytest.mark.parametrize("mistake", ["eps sign", "sigma 1%", "q 1%"])
+def test_derivative_bounds_catch_planted_errors(co2_dual, mistake):
+    """A sign error in one species' d/d eps, a 1% error in every d/d sigma,
+    a 1% error in every d/d q: each fails the bounds."""
+    nb, v, e, d, cd = co2_dual
+    bb, bg = derivative_sum_bounds(v, e, d, cd, nb, TOL["mixed"], 298.0)
+    bad = d.copy()
+    if mistake == "eps sign":
+        bad[:, 1, 0] = -bad[:, 1, 0]
+    elif mistake == "sigma 1%":
+        bad[:, :, 1] *= 1.01
+    else:
+        bad[:, :, 2] *= 1.01
+    G0, g0 = _sums(nb, v, e, d)
+    G1, g1 = _sums(nb, v, e, bad)
+    assert np.max(np.abs(G1 - G0) / bb) > 1.0
+    assert np.max(np.abs(g1 - g0) / bg) > 1.0
