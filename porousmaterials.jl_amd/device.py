@@ -289,11 +289,23 @@ def widom_noalloc(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, prec
     return sums.reshape(n_blocks, 3)
 
 
-def energy_grid(fw, ff, ew, pr, n_pts, precision) -> np.ndarray:
+def _grid_out(out, n):
+    """The caller's output buffer (C-contiguous float64 of n values, written
+    in place: a buffer reused across calls skips the operating system's
+    zeroing of fresh pages), or a new one."""
+    if out is None:
+        return np.zeros(n)
+    if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous
+            and out.flags.writeable and out.size == n):
+        raise ValueError(f"out must be a writeable C-contiguous float64 array of {n} values")
+    return out.reshape(-1)
+
+
+def energy_grid(fw, ff, ew, pr, n_pts, precision, out=None) -> np.ndarray:
     lib = _lib.load()
     ff = _ff_handle(ff)
     n = np.array(n_pts, dtype=np.int32)
-    out = np.zeros(int(np.prod(n_pts)))
+    out = _grid_out(out, int(np.prod(n_pts)))
     check(lib.pm_energy_grid(fw.ptr, ff.ptr if ff is not None else None,
                              ew.ptr if ew is not None else None, pr.ptr, iptr(n),
                              precision_code(precision), dptr(out)))
@@ -336,19 +348,22 @@ def widom_batch(runs, precision):
     return out
 
 
-def energy_grid_batch(runs, precision):
+def energy_grid_batch(runs, precision, outs=None):
     """Many energy grids in one call (pm_energy_grid_batch).
 
     ``runs``: sequence of (fw, ff, ew, pr, n_pts).  Returns a list of flat
-    i-fastest arrays in the order of ``runs``."""
+    i-fastest arrays in the order of ``runs`` (``outs``: the caller's buffers,
+    one per run, written in place)."""
     lib = _lib.load()
     arr = (_lib.GridRun * len(runs))()
-    outs, keep = [], []
+    if outs is not None and len(outs) != len(runs):
+        raise ValueError("outs must hold one buffer per run")
+    res, keep = [], []
     for i, (fw, ff, ew, pr, n_pts) in enumerate(runs):
         ff = _ff_handle(ff)
         keep.append(ff)
-        out = np.zeros(int(np.prod(n_pts)))
-        outs.append(out)
+        out = _grid_out(None if outs is None else outs[i], int(np.prod(n_pts)))
+        res.append(out)
         r = arr[i]
         r.framework = fw.ptr
         r.forcefield = ff.ptr if ff is not None else None
@@ -358,7 +373,7 @@ def energy_grid_batch(runs, precision):
             r.n_pts[d] = int(n_pts[d])
         r.out = dptr(out)
     check(lib.pm_energy_grid_batch(len(runs), arr, precision_code(precision)))
-    return outs
+    return res
 
 
 def widom_gradient(fw, ff, ew, pr, temperature, seed, n_blocks, n_per_block, precision, ljff, acc=None):

@@ -100,3 +100,23 @@ def test_small_grids_80(pm, orc):
             worst = max(worst, rw)
     say(f"   32 x 80^3 point-charge grids: {total_pts / (total_ms * 1e-3):.3e} grid points/s; "
         f"max weighted error {worst:.2e} of |E|+T")
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_grids_into_caller_buffers(pm, precision):
+    """pm_energy_grid and pm_energy_grid_batch writing into buffers the caller
+    reuses give the same grids, bit for bit, as into new ones."""
+    from porousmaterials_jl_amd.grid import _grid_setup
+    cases = pm.synthetic.small_grid_cases()[:4]
+    say(f"case: 4 80^3 ion grids into reused host buffers ({precision})")
+    runs = [_grid_setup(c.crystal, c.probes["ion"], c.ljff, c.meta["n_pts"], "K", False, False,
+                        c.ewald_precision, electrostatics=True)[0] for c in cases]
+    want = [pm.device.energy_grid(*r, precision) for r in runs]
+    bufs = [np.full_like(w, np.nan) for w in want]
+    for _ in range(2):
+        got = pm.device.energy_grid_batch(runs, precision, outs=bufs)
+        assert all(g is b for g, b in zip(got, bufs))
+        assert all(np.array_equal(b, w) for b, w in zip(bufs, want))
+    buf = np.full_like(want[0], np.nan)
+    assert pm.device.energy_grid(*runs[0], precision, out=buf) is buf
+    assert np.array_equal(buf, want[0])

@@ -354,3 +354,19 @@ def test_gcmc_results_round_trip_and_dataframe(pm, tmp_path):
     assert [float(v[0]) for v in df["⟨N⟩ (mmol/g)"]] == [1.0, 2.0, 3.0]
     assert all(df["xtal"] == "SBMOF-1.cif") and all(isinstance(g, pm.Grid) for g in df["density grid"])
     assert all(r == (3, 6, 2) for r in df["repfactors"])
+
+
+def test_grid_output_buffer_validation(pm):
+    """A caller's grid buffer must be a writeable C-contiguous float64 array of
+    the grid's size (checked before any device call)."""
+    from porousmaterials_jl_amd.device import _grid_out
+    assert _grid_out(None, 6).shape == (6,)
+    b = np.zeros((2, 3))
+    assert _grid_out(b, 6).base is b or np.shares_memory(_grid_out(b, 6), b)
+    for bad in (np.zeros(6, dtype=np.float32), np.zeros(7), np.zeros((3, 4))[:, :2], np.zeros(12)[::2]):
+        with pytest.raises(ValueError, match="out must be"):
+            _grid_out(bad, 6)
+    ro = np.zeros(6)
+    ro.flags.writeable = False
+    with pytest.raises(ValueError):
+        _grid_out(ro, 6)
