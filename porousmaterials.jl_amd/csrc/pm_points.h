@@ -202,9 +202,13 @@ __device__ inline bool access_open(const RunArgs& a, const double* u) {
 }
 
 // ------------------------------------------------------------ Widom gen ----
-// One work-item per insertion slot: pose from the stream, sites wrapped into
-// [0,1)^3 (integer shifts change neither the nearest-image pairs nor the
-// Fourier phases m.(R o x)), binned into the run's cells.  An insertion in a
+// One work-item per insertion slot: pose from the stream, sites folded into
+// the first unit cell [0, 1 / rep)^3 of the replicated box (shifts by unit-cell
+// vectors change neither a site's set of pair distances -- the framework has
+// the unit cell's period -- nor the Fourier phases m.(R o x)), binned into the
+// run's cells over that unit cell: a cell of a given point count is rep_a rep_b
+// rep_c times smaller than over the whole box, and so is the slack of a
+// chunk's candidate sphere beyond the cutoff.  An insertion in a
 // blocked pocket of the run's accessibility grid emits no point (its slots
 // look like padding); the combine kernels give it E_vdw = E_es = +Inf
 // (src/henry.jl:225-232).
@@ -241,9 +245,12 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     // in flight together (a vector load after an atomic would wait for it).
     const gconst_ptr<double> offp = uniform_ptr(&a.off[0][0]);
     const gconst_ptr<double> c2fp = uniform_ptr(&a.c2f[0]);
-    double c2f[9];
+    const gconst_ptr<double> repp = uniform_ptr(&a.repd[0]);
+    double c2f[9], repd[3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) c2f[k] = c2fp[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) repd[k] = repp[k];
     unsigned cid[PM_MAX_SITES], rank[PM_MAX_SITES];
 #pragma unroll
     for (int s = 0; s < PM_MAX_SITES; ++s) {
@@ -255,9 +262,10 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
             const double t2 = R[2] * y0 + R[5] * y1 + R[8] * y2;
             y0 = t0; y1 = t1; y2 = t2;
         }
-        double x = u[0] + (c2f[0] * y0 + c2f[3] * y1 + c2f[6] * y2);
-        double y = u[1] + (c2f[1] * y0 + c2f[4] * y1 + c2f[7] * y2);
-        double z = u[2] + (c2f[2] * y0 + c2f[5] * y1 + c2f[8] * y2);
+        // unit-cell fractional coordinates in [0, 1), then back to the box's
+        double x = (u[0] + (c2f[0] * y0 + c2f[3] * y1 + c2f[6] * y2)) * repd[0];
+        double y = (u[1] + (c2f[1] * y0 + c2f[4] * y1 + c2f[7] * y2)) * repd[1];
+        double z = (u[2] + (c2f[2] * y0 + c2f[5] * y1 + c2f[8] * y2)) * repd[2];
         x -= floor(x);
         y -= floor(y);
         z -= floor(z);
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
         const int ib = min((int)(y * n1), n1 - 1);
         const int ic = min((int)(z * n2), n2 - 1);
         cid[s] = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
-        g.pos[p0 + s] = make_double4(x, y, z, 1.0);
+        g.pos[p0 + s] = make_double4(x / repd[0], y / repd[1], z / repd[2], 1.0);
         // the count before this point is its rank in the cell (the order within
         // a cell changes no point's energy)
         rank[s] = atomicAdd(&g.count[cid[s]], 1u);
@@ -852,9 +860,10 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
         const int c = (int)(it.x - sg.cell_base);
         const int n0 = a.ncell[0], n1 = a.ncell[1];
         const int ia = c % n0, ib = (c / n0) % n1, ic = c / (n0 * n1);
-        cf0 = (ia + 0.5) / n0;
-        cf1 = (ib + 0.5) / n1;
-        cf2 = (ic + 0.5) / a.ncell[2];
+        // cells tile the first unit cell of the replicated box
+        cf0 = (ia + 0.5) / (n0 * a.repd[0]);
+        cf1 = (ib + 0.5) / (n1 * a.repd[1]);
+        cf2 = (ic + 0.5) / (a.ncell[2] * a.repd[2]);
         const unsigned k = g.point_off[it.x] + it.y * WG + tid;
         valid = k < g.point_off[it.x + 1];
         pid = valid ? (long long)g.order[k] : sg.slot_base;

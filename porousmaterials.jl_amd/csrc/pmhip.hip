@@ -485,9 +485,11 @@ int run_poses(const pm_framework* fw, const pm_forcefield* ff, const pm_ewald* e
 
 // ---------------------------------------------------- point engine ----
 // Image list for radius r: every periodic image (x_unit + t) / rep whose
-// replicated-box fractional coordinates lie within [-m_a, 1 + m_a] on each
-// axis, m_a = r / width_a, so that every image within r of any point of the
-// box [0,1)^3 is present; stored in Cartesian coordinates.  Lists are cached
+// replicated-box fractional coordinates lie within [-m_a, 1 / rep_a + m_a] on
+// each axis, m_a = r / width_a, so that every image within r of any point of
+// the first unit cell [0, 1 / rep]^3 of the replicated box is present (the
+// point engine's points all lie there: Widom sites are folded into it, grids
+// span it); stored in Cartesian coordinates.  Lists are cached
 // per framework and radius, so a run's list -- and with it the order of its
 // FP32 partial sums -- depends on the framework and the radius alone.
 const pm_framework::Images* find_images(const pm_framework* fw, double r) {
@@ -505,7 +507,7 @@ int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
     double m[3];
     for (int i = 0; i < 3; ++i) m[i] = r / fw->width[i];
     int64_t est = fw->n_unit;
-    for (int i = 0; i < 3; ++i) est *= (int64_t)std::ceil(fw->rep[i] * (1.0 + 2.0 * m[i])) + 1;
+    for (int i = 0; i < 3; ++i) est *= (int64_t)std::ceil(1.0 + 2.0 * m[i] * fw->rep[i]) + 1;
     if (est > MAX_IMAGES) return fail(PM_ERR_ARG, "image list too large");
     std::vector<double4> img;
     std::vector<int32_t> ty;
@@ -524,7 +526,7 @@ int get_images(pm_framework* fw, double r, const pm_framework::Images** out) {
         int lo[3], hi[3];
         for (int a = 0; a < 3; ++a) {
             lo[a] = (int)std::ceil(-m[a] * fw->rep[a] - x[a]);
-            hi[a] = (int)std::floor((1.0 + m[a]) * fw->rep[a] - x[a]);
+            hi[a] = (int)std::floor(1.0 + m[a] * fw->rep[a] - x[a]);
         }
         for (int ta = lo[0]; ta <= hi[0]; ++ta)
             for (int tb = lo[1]; tb <= hi[1]; ++tb)
@@ -677,18 +679,20 @@ void fill_run_pair(RunArgs& a, const pm_framework* fw, const pm_forcefield* ff, 
     }
 }
 
-// Cells of a Widom run's counting sort: about cell_points points each, half
-// diagonal <= chunk_r_max where the point density allows, at most
+// Cells of a Widom run's counting sort over the first unit cell of the
+// replicated box (where the sites are folded): about cell_points points each,
+// half diagonal <= chunk_r_max where the point density allows, at most
 // MAX_RUN_CELLS cells.
 void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double* cell_r) {
-    double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
+    const double n_cells_box = (double)fw->rep[0] * fw->rep[1] * fw->rep[2];
+    double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]) / n_cells_box;
     double h = std::cbrt(vol * g_ctx.cell_points / std::max(n_points, 1.0));
     double scale = 1.0;
     int prev[3] = {1, 1, 1};
     double prev_r = 0.0;
     for (int iter = 0; iter < 64; ++iter) {
         for (int i = 0; i < 3; ++i)
-            ncell[i] = (int)std::min(1024.0, std::max(1.0, std::round(fw->width[i] / h * scale)));
+            ncell[i] = (int)std::min(1024.0, std::max(1.0, std::round(fw->width[i] / fw->rep[i] / h * scale)));
         if (iter > 0 && (double)ncell[0] * ncell[1] * ncell[2] > (double)MAX_RUN_CELLS) {
             for (int i = 0; i < 3; ++i) ncell[i] = prev[i];
             *cell_r = prev_r;
@@ -696,9 +700,9 @@ void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double*
         }
         double ea[3], eb[3], ec[3];
         for (int i = 0; i < 3; ++i) {
-            ea[i] = fw->F[i] / ncell[0];
-            eb[i] = fw->F[3 + i] / ncell[1];
-            ec[i] = fw->F[6 + i] / ncell[2];
+            ea[i] = fw->F[i] / (ncell[0] * fw->rep[0]);
+            eb[i] = fw->F[3 + i] / (ncell[1] * fw->rep[1]);
+            ec[i] = fw->F[6 + i] / (ncell[2] * fw->rep[2]);
         }
         *cell_r = half_diagonal(ea, eb, ec);
         if (*cell_r <= g_ctx.chunk_r_max) return;

@@ -298,7 +298,7 @@ def _grid_out(out, n):
     if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous
             and out.flags.writeable and out.size == n):
         raise ValueError(f"out must be a writeable C-contiguous float64 array of {n} values")
-    return out.reshape(-1)
+    return out if out.ndim == 1 else out.reshape(-1)
 
 
 def energy_grid(fw, ff, ew, pr, n_pts, precision, out=None) -> np.ndarray:

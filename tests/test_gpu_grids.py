@@ -115,8 +115,8 @@ def test_grids_into_caller_buffers(pm, precision):
     bufs = [np.full_like(w, np.nan) for w in want]
     for _ in range(2):
         got = pm.device.energy_grid_batch(runs, precision, outs=bufs)
-        assert all(g is b for g, b in zip(got, bufs))
+        assert all(np.shares_memory(g, b) for g, b in zip(got, bufs))
         assert all(np.array_equal(b, w) for b, w in zip(bufs, want))
     buf = np.full_like(want[0], np.nan)
-    assert pm.device.energy_grid(*runs[0], precision, out=buf) is buf
+    assert np.shares_memory(pm.device.energy_grid(*runs[0], precision, out=buf), buf)
     assert np.array_equal(buf, want[0])
