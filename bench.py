@@ -33,6 +33,8 @@ METRIC = "Widom insertions/s on 1 MI355X, mixed vs FP64 GPU vs CPU, at <=1e-4 re
 C_GEOM = 21   # per (site, replicated atom) candidate pair
 C_LJ = 10     # per in-cutoff LJ pair
 C_SR = 20     # per in-cutoff Ewald real-space pair
+C_LJ_GRAD = 2  # gradient runs, per in-cutoff LJ pair: the sums of u^2 - u and u
+C_SR_GRAD = 2  # gradient runs, per in-cutoff real-space pair: the sum of erfc(alpha r) / r
 C_K = 14      # per (charge site, unit-cell k-vector)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6
@@ -236,7 +238,9 @@ def screening_leg(pm, steps, n_per_block=20000):
         "runs": len(runs), "insertions": n_ins, "value": n_ins / dt, "unit": "insertions/s",
         "ms_per_call": dt * 1e3, "device_pipeline_ms": float(np.mean(pipe)), "pair_kernel_ms": p_ms,
         "pair_kernel_tflops": flop / (p_ms * 1e-3) / 1e12,
-        "fp64_value": n_ins / dt64, "host_setup_s": setup_s,
+        "roofline": roofline("point_pair_kernel<float, 0>", flop, p_ms, PEAK_FP32_TFLOPS, "screening",
+                             note="flop, time and traffic per call: the sum over the call's launch groups"),
+        "fp64_value": n_ins / dt64, "fp64_calls": 1, "host_setup_s": setup_s,
     }
 
 
@@ -263,7 +267,7 @@ def grids_leg(pm, steps):
             f += npts * C_SR * in_cutoff(n_rep, omega, r_c)
         return f
 
-    def measure(case, probe_name, precision):
+    def measure(key, case, probe_name, precision):
         probe = case.probes[probe_name]
         run, _ = _grid_setup(case.crystal, probe, case.ljff, case.meta["n_pts"], "K", False, False,
                              case.ewald_precision, electrostatics=True)
@@ -283,18 +287,22 @@ def grids_leg(pm, steps):
         wall_new = (time.perf_counter() - t) / steps
         npts = float(np.prod(case.meta["n_pts"]))
         p_ms = float(np.mean(pair))
-        tf = pair_flop(case, probe, float(np.mean(cand)), npts) / (p_ms * 1e-3) / 1e12
         peak = PEAK_FP32_TFLOPS if precision == "mixed" else PEAK_FP64_TFLOPS
+        kernel = "point_pair_kernel<float, 0>" if precision == "mixed" else "point_pair_kernel<double, 0>"
+        rl = roofline(kernel, pair_flop(case, probe, float(np.mean(cand)), npts), p_ms, peak,
+                      f"grid_{key}_{precision}")
+        tf = rl["achieved"]
         return {"points": int(npts), "value": npts / (float(np.mean(pipe)) * 1e-3), "unit": "grid points/s",
                 "wall_value": npts / wall, "wall_value_new_buffer": npts / wall_new,
                 "device_pipeline_ms": float(np.mean(pipe)), "pair_kernel_ms": p_ms,
-                "pair_kernel_tflops": tf, "frac": tf / peak, "candidates_per_point": float(np.mean(cand)) / npts}
+                "pair_kernel_tflops": tf, "frac": tf / peak, "candidates_per_point": float(np.mean(cand)) / npts,
+                "roofline": rl}
 
     out = {}
     small = pm.synthetic.lj_grid_case()
     large = pm.synthetic.large_grid_case()
     for key, case, probe_name in (("ch4_50", small, "CH4"), ("ion_200", large, "ion"), ("ch4_200", large, "CH4")):
-        out[key] = {p: measure(case, probe_name, p) for p in ("mixed", "fp64")}
+        out[key] = {p: measure(key, case, probe_name, p) for p in ("mixed", "fp64")}
         print(f"grid {key}: mixed {out[key]['mixed']['value']:.3e} points/s, fp64 "
               f"{out[key]['fp64']['value']:.3e} points/s", file=sys.stderr, flush=True)
     cases = pm.synthetic.small_grid_cases()
@@ -324,17 +332,31 @@ def grids_leg(pm, steps):
     return out
 
 
-def adjoint_leg(pm, case, fw, ff, ew, pr, xr, n_per_block, seed):
-    """Gradients on the CO2 workload: (b) one adjoint Widom pass over the
-    full 10^6 insertions (pm_widom_gradient, mixed), (a) forces of 4096 CO2
-    poses (pm_pose_gradient, mixed) and their batched minimisation (FP64)."""
+def adjoint_leg(pm, case, fw, ff, ew, pr, xr, ep, n_per_block, seed, steps, minimise=True):
+    """Gradients on the CO2 workload: (b) the adjoint Widom pass over the
+    full 10^6 insertions (pm_widom_gradient, mixed; ``steps`` timed calls),
+    (a) forces of 4096 CO2 poses (pm_pose_gradient, mixed) and, unless
+    ``minimise`` is off, their batched minimisation (FP64)."""
     co2 = case.probes["CO2"]
     pm.device.widom_gradient(fw, ff, ew, pr, case.temperature, seed, 5, n_per_block, "mixed", case.ljff)
+    pipe, pair, cands = [], [], []
     t = time.perf_counter()
-    sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, case.temperature, seed, 5, n_per_block, "mixed",
-                                           case.ljff)
-    dt_b = time.perf_counter() - t
-    k_ms = pm.device.last_kernel_stats()[0]
+    for _ in range(steps):
+        sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, case.temperature, seed, 5, n_per_block, "mixed",
+                                               case.ljff)
+        pipe.append(pm.device.last_kernel_stats()[0])
+        pk, pc = pm.device.last_pair_kernel_stats()
+        pair.append(pk)
+        cands.append(pc)
+    dt_b = (time.perf_counter() - t) / steps
+    k_ms, p_ms, cand = float(np.mean(pipe)), float(np.mean(pair)), float(np.mean(cands))
+    n_ins = 5 * n_per_block
+    r_lj, r_sr = math.sqrt(case.ljff.r2_cutoff), ep.sr_cutoff_r
+    flop = C_GEOM * cand + n_ins * (co2.n_atoms * in_cutoff(xr.n_atoms, xr.box.Omega, r_lj) * (C_LJ + C_LJ_GRAD)
+                                    + co2.n_charges * in_cutoff(xr.n_atoms, xr.box.Omega, r_sr)
+                                    * (C_SR + C_SR_GRAD))
+    nt = len({case.ljff.index(sp) for sp in case.crystal.species})
+    nt_kernel = 2 if nt <= 2 else 4 if nt <= 4 else 6 if nt <= 6 else 8
     S = float(sums[:, 0].sum())
     g = -grads.sum(axis=0) / (case.temperature * S)
     lj_off, _ = pm.site_offsets(co2)
@@ -351,23 +373,29 @@ def adjoint_leg(pm, case, fw, ff, ew, pr, xr, n_per_block, seed):
     pm.device.pose_gradients(fw, ff, ew, species, list(co2.charges_q), poses, "mixed")
     dt_a = time.perf_counter() - t
     a_ms = pm.device.last_kernel_stats()[0]
-    rep = np.array(pm.replication_factors(case.crystal.box, case.ljff))
-    starts = np.mod((com @ xr.box.c_to_f.T) * rep[None, :], 1.0)
-    t = time.perf_counter()
-    _, e_min, info = pm.find_energy_minima(case.crystal, co2, case.ljff, starts, max_iter=200)
-    dt_m = time.perf_counter() - t
-    print(f"adjoint: pass {dt_b * 1e3:.1f} ms, 4096 pose forces {dt_a * 1e3:.1f} ms, 4096 minimisations "
-          f"{dt_m:.1f} s", file=sys.stderr, flush=True)
-    return {
-        "widom_gradient": {"insertions": 5 * n_per_block, "value": 5 * n_per_block / dt_b,
-                           "unit": "insertions/s", "device_pipeline_ms": k_ms,
-                           "parameters": "d ln K_H / d(eps_t, sigma_t, q_t), 5 framework species",
-                           "dlnKH_deps_T1": float(g[case.ljff.index("T1"), 0])},
+    out = {
+        "widom_gradient": {"insertions": n_ins, "value": n_ins / dt_b, "steps": steps,
+                           "unit": "insertions/s", "device_pipeline_ms": k_ms, "pair_kernel_ms": p_ms,
+                           "parameters": f"d ln K_H / d(eps_t, sigma_t, q_t), {nt} framework species",
+                           "dlnKH_deps_T1": float(g[case.ljff.index("T1"), 0]),
+                           "roofline": roofline(f"point_pair_kernel<float, {nt_kernel}>", flop, p_ms,
+                                                PEAK_FP32_TFLOPS, "adjoint")},
         "pose_forces": {"poses": 4096, "value": 4096 / dt_a, "unit": "poses/s", "device_ms": a_ms},
-        "minimise_4096": {"seconds": dt_m, "iterations": int(info["iterations"]),
-                          "device_calls": int(info["device_calls"]),
-                          "converged": int(np.sum(info["converged"]))},
     }
+    msg = f"adjoint: pass {dt_b * 1e3:.2f} ms (device {k_ms:.2f} ms, pair kernel {p_ms:.2f} ms), " \
+          f"4096 pose forces {dt_a * 1e3:.1f} ms"
+    if minimise:
+        rep = np.array(pm.replication_factors(case.crystal.box, case.ljff))
+        starts = np.mod((com @ xr.box.c_to_f.T) * rep[None, :], 1.0)
+        t = time.perf_counter()
+        _, e_min, info = pm.find_energy_minima(case.crystal, co2, case.ljff, starts, max_iter=200)
+        dt_m = time.perf_counter() - t
+        out["minimise_4096"] = {"seconds": dt_m, "iterations": int(info["iterations"]),
+                                "device_calls": int(info["device_calls"]),
+                                "converged": int(np.sum(info["converged"]))}
+        msg += f", 4096 minimisations {dt_m:.1f} s"
+    print(msg, file=sys.stderr, flush=True)
+    return out
 
 
 def fp64_leg(pm, run, n_ins, nb, case, sums32, value32, steps, flop_of):
@@ -391,9 +419,12 @@ def fp64_leg(pm, run, n_ins, nb, case, sums32, value32, steps, flop_of):
     r64 = pm.block_statistics(s64, nb, case.temperature, 1.0)
     r32 = pm.block_statistics(sums32, nb, case.temperature, 1.0)
     kh = "henry coefficient [mol/(m³-bar)]"
-    tf = flop_of(float(np.mean(c64s))) / (p64 * 1e-3) / 1e12
+    rl = roofline("point_pair_kernel<double, 0>", flop_of(float(np.mean(c64s))), p64, PEAK_FP64_TFLOPS,
+                  "widom_fp64")
+    tf = rl["achieved"]
     out = {"value": n_ins / dt64, "steps": steps, "device_pipeline_ms": k64, "pair_kernel_ms": p64,
            "pair_kernel_tflops": tf, "peak_tflops": PEAK_FP64_TFLOPS, "frac": tf / PEAK_FP64_TFLOPS,
+           "roofline": rl,
            "kh_rel_diff_mixed_vs_fp64": abs(r32[kh] - r64[kh]) / abs(r64[kh]),
            "u_rel_diff_mixed_vs_fp64": abs(r32["⟨U⟩ (K)"] - r64["⟨U⟩ (K)"]) / abs(r64["⟨U⟩ (K)"])}
     if value32 is not None:
@@ -403,15 +434,31 @@ def fp64_leg(pm, run, n_ins, nb, case, sums32, value32, steps, flop_of):
     return out
 
 
-def _traffic_record():
-    """HBM bytes per launch of the pair kernel measured with rocprofv3
-    FETCH_SIZE / WRITE_SIZE counters (scaled by a plain copy of known size),
-    committed as profiles/pair_kernel_traffic.json; None if absent."""
-    path = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
+def _traffic_record(key):
+    """HBM bytes per call of a leg's pair kernel, measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters (scaled by a plain copy of known size)
+    and committed in profiles/kernel_traffic.json under ``key``; None if
+    absent."""
+    path = os.path.join(ROOT, "profiles", "kernel_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)
+        return json.load(f).get(key)
+
+
+def roofline(kernel, flop, kernel_ms, peak, traffic_key, **extra):
+    """The roofline object of a leg's dominant kernel: algorithmic TFLOP/s
+    over the kernel's HIP-event time against the published VALU peak, and
+    its measured HBM bytes per call when committed."""
+    achieved = flop / (kernel_ms * 1e-3) / 1e12
+    out = {"bound": "valu-fp32" if peak == PEAK_FP32_TFLOPS else "valu-fp64", "achieved": achieved, "peak": peak,
+           "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": kernel, "kernel_ms": kernel_ms,
+           "flop_per_launch": flop, **extra}
+    t = _traffic_record(traffic_key)
+    if t is not None:
+        out["traffic"] = t["bytes_per_launch"]
+        out["traffic_source"] = t["source"]
+    return out
 
 
 def main():
@@ -427,6 +474,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (gradient) leg")
     ap.add_argument("--no-grids", action="store_true", help="skip the energy-grid leg")
     ap.add_argument("--no-gcmc", action="store_true", help="skip the grand-canonical Monte Carlo leg")
+    ap.add_argument("--no-minimise", action="store_true",
+                    help="adjoint leg: skip the batched minimisation of 4096 poses")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (pm_copy_probe) three times: the "
                          "calibration launch for FETCH_SIZE / WRITE_SIZE counter runs")
@@ -473,7 +522,8 @@ def main():
                                                            * C_LJ + co2.n_charges
                                                            * in_cutoff(xr.n_atoms, xr.box.Omega, r_sr) * C_SR))
         elif args.only == "adjoint":
-            res = adjoint_leg(pm, case, fw, ff, ew, pr, xr, nb, seed)
+            res = adjoint_leg(pm, case, fw, ff, ew, pr, xr, ep, nb, seed, max(1, min(args.steps, 5)),
+                              minimise=not args.no_minimise)
         elif args.only == "screening":
             res = screening_leg(pm, max(1, min(args.steps, 3)))
         elif args.only == "grids":
@@ -516,7 +566,6 @@ def main():
     # C_SR per in-cutoff pair
     pair_flop = (C_GEOM * cand + n_ins * (co2.n_atoms * in_cutoff(xr.n_atoms, xr.box.Omega, r_lj) * C_LJ
                                          + co2.n_charges * in_cutoff(xr.n_atoms, xr.box.Omega, r_sr) * C_SR))
-    achieved = pair_flop / (p_ms * 1e-3) / 1e12
     flop_ap = flops_all_pairs(xr.n_atoms, xr.box.Omega, r_lj, co2.n_atoms, co2.n_charges, n_k, r_sr)
     out = {
         "metric": METRIC, "value": value, "unit": "insertions/s", "n_gpus": 1,
@@ -530,12 +579,9 @@ def main():
                          f"{n_ins} insertions = 5 blocks x {nb}"),
             "insertions_per_step": n_ins, "precision": "mixed",
             "tuning": {k: tuning[k] for k in ("cell_points", "chunk_r_max", "source")}},
-        "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "point_pair_kernel<float, 0>", "kernel_ms": p_ms,
-                     "flop_per_launch": pair_flop,
-                     "candidate_pairs_per_point": cand / n_points,
-                     "pruning_factor": (n_points * xr.n_atoms) / cand},
+        "roofline": roofline("point_pair_kernel<float, 0>", pair_flop, p_ms, PEAK_FP32_TFLOPS, "widom_mixed",
+                             candidate_pairs_per_point=cand / n_points,
+                             pruning_factor=(n_points * xr.n_atoms) / cand),
         "device_pipeline_ms": k_ms,
         "all_pairs_equivalent_tflops": flop_ap * n_ins / (k_ms * 1e-3) / 1e12,
     }
@@ -546,7 +592,8 @@ def main():
         out["kh_rel_diff_mixed_vs_fp64"] = out["fp64"].pop("kh_rel_diff_mixed_vs_fp64")
         out["u_rel_diff_mixed_vs_fp64"] = out["fp64"].pop("u_rel_diff_mixed_vs_fp64")
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_leg(pm, case, fw, ff, ew, pr, xr, nb, seed)
+        out["adjoint"] = adjoint_leg(pm, case, fw, ff, ew, pr, xr, ep, nb, seed, max(1, min(args.steps, 5)),
+                                     minimise=not args.no_minimise)
     if not args.no_screening:
         out["screening"] = screening_leg(pm, max(1, min(args.steps, 3)))
     if not args.no_grids:
@@ -566,10 +613,6 @@ def main():
         _lib.check(_lib.load().pm_copy_probe(256 << 20, 3, ctypes.byref(ms)))
         out["copy_probe"] = {"bytes": 256 << 20, "ms": ms.value,
                              "GB_per_s": 2 * (256 << 20) / (ms.value * 1e-3) / 1e9}
-    traffic = _traffic_record()
-    if traffic is not None:
-        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
-        out["roofline"]["traffic_source"] = traffic["source"]
     print(json.dumps(out), flush=True)
 
 

@@ -310,7 +310,7 @@ int pm_energy_grid_batch(int32_t n_runs, const pm_grid_run* runs, int32_t precis
  * Point-engine tuning (no counterpart in the reference, whose CPU loops have
  * no tiling).  The chunk shapes only schedule work: every energy is a
  * function of the point and its run alone, so no setting changes a result.
- *   PM_TUNE_CELL_POINTS  target points per Widom cell (default 192)
+ *   PM_TUNE_CELL_POINTS  target points per Widom cell (default 448)
  *   PM_TUNE_CHUNK_R_MAX  largest chunk radius (A) served by the image-list
  *                        path; wider chunks take the all-atoms path (default 6)
  * Settings are process-wide.
