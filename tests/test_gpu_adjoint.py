@@ -33,15 +33,7 @@ def say(msg):
 
 @pytest.fixture(scope="module")
 def adj(pm):
-    case = pm.synthetic.co2_widom_case()
-    x, ff = case.crystal, case.ljff
-    rep = pm.replication_factors(x.box, ff)
-    xr = pm.replicate(x, rep)
-    ep = pm.setup_Ewald_sum(xr.box, math.sqrt(ff.r2_cutoff), eps=case.ewald_precision, verbose=False)
-    fw = pm.device.framework(xr, ff)
-    ew = pm.device.ewald(xr, fw, ep)
-    pr = pm.device.probe(case.probes["CO2"], ff)
-    return case, fw, ew, pr
+    return _handles(pm, pm.synthetic.co2_widom_case())
 
 
 def _oracle_lnS(orc, pm, case, ljff, q_atoms):
@@ -113,6 +105,17 @@ def test_dlnKH_matches_oracle_finite_differences(pm, orc, adj, precision):
     assert err.max() <= 1e-4
 
 
+def _handles(pm, case):
+    x, ff = case.crystal, case.ljff
+    rep = pm.replication_factors(x.box, ff)
+    xr = pm.replicate(x, rep)
+    ep = pm.setup_Ewald_sum(xr.box, math.sqrt(ff.r2_cutoff), eps=case.ewald_precision, verbose=False)
+    fw = pm.device.framework(xr, ff)
+    ew = pm.device.ewald(xr, fw, ep)
+    pr = pm.device.probe(case.probes["CO2"], ff)
+    return case, fw, ew, pr
+
+
 def _dual_check(pm, orc, adj, precision, nb, seed):
     case, fw, ew, pr = adj
     ff, x, co2 = case.ljff, case.crystal, case.probes["CO2"]
@@ -174,6 +177,23 @@ def test_all_atoms_path_derivative_sums_match_dual_oracle(pm, orc, adj, precisio
         _dual_check(pm, orc, adj, precision, 2000, 37)
     finally:
         pm.device.set_tuning(before["cell_points"], before["chunk_r_max"])
+
+
+@pytest.mark.parametrize("n_species", [1, 2, 3, 7, 8])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gradient_kernels_for_every_species_count(pm, orc, precision, n_species):
+    """Frameworks of 1, 2, 3, 7 and 8 species run the gradient pair kernels
+    compiled for 2, 4 and 8 species and the Fourier kernels for 1, 2, 3, 7
+    and 8: block sums equal pm_widom's bit for bit, derivative sums meet the
+    dual-number oracle as above."""
+    case = pm.synthetic.co2_widom_case(seed=40 + n_species, n_species=n_species)
+    say(f"case: {n_species}-species framework, CO2 5 x 1000 insertions ({precision})")
+    h = _handles(pm, case)
+    _, fw, ew, pr = h
+    s_g, _ = pm.device.widom_gradient(fw, case.ljff, ew, pr, T, 43, 5, 1000, precision, case.ljff)
+    s_w = pm.device.widom(fw, case.ljff, ew, pr, T, 43, 5, 1000, precision)
+    assert np.array_equal(s_g, s_w)
+    _dual_check(pm, orc, h, precision, 1000, 43)
 
 
 def test_henry_coefficient_gradient_api(pm):
