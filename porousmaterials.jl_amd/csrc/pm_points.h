@@ -285,9 +285,10 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     }
 }
 
-__global__ void fill_u32_kernel(unsigned* p, long long n, unsigned v) {
+__global__ void fill_u32_kernel(unsigned* p, long long n, unsigned v, unsigned long long* zero) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+    if (zero != nullptr && i == 0) *zero = 0ull;
 }
 
 // ---------------------------------------------------------------- scan ----

@@ -62,6 +62,10 @@ struct Ctx {
     std::vector<hipEvent_t> piece_ev;
     // host staging for the per-call LJ table: outlives the asynchronous copy
     std::vector<unsigned char> host_tab;
+    // pinned host staging of the point engine's per-call upload (run records,
+    // segments, tables) and download (candidate count, block sums)
+    void* io_pin = nullptr;
+    size_t io_pin_bytes = 0;
     double cell_points = DEFAULT_CELL_POINTS;
     double chunk_r_max = DEFAULT_CHUNK_R_MAX;
 };
@@ -107,6 +111,26 @@ int ensure_pinned(size_t bytes) {
     const size_t want = std::max(bytes, (size_t)1 << 24);
     PM_HIP(hipHostMalloc(&g_ctx.pinned, want, hipHostMallocDefault));
     g_ctx.pinned_bytes = want;
+    return PM_OK;
+}
+
+// The pinned upload/download staging of the point engine, grown on demand when
+// `may_alloc` (held until the process ends); nullptr when it is smaller than
+// `bytes` and may not grow, and the caller then copies through pageable memory.
+int io_staging(size_t bytes, bool may_alloc, unsigned char** out) {
+    *out = nullptr;
+    if (bytes > g_ctx.io_pin_bytes && may_alloc) {
+        if (g_ctx.io_pin) {
+            PM_HIP(hipStreamSynchronize(g_ctx.stream));
+            PM_HIP(hipHostFree(g_ctx.io_pin));
+            g_ctx.io_pin = nullptr;
+            g_ctx.io_pin_bytes = 0;
+        }
+        const size_t want = std::max(bytes, (size_t)1 << 20);
+        PM_HIP(hipHostMalloc(&g_ctx.io_pin, want, hipHostMallocDefault));
+        g_ctx.io_pin_bytes = want;
+    }
+    if (bytes <= g_ctx.io_pin_bytes) *out = static_cast<unsigned char*>(g_ctx.io_pin);
     return PM_OK;
 }
 
@@ -1051,7 +1075,12 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const size_t stage_bytes = ar.used;
     const bool widom = mode == MODE_WIDOM;
     const size_t o_part = ar.take(widom ? (size_t)part_total * sizeof(double) : 0);
-    const size_t o_sums = ar.take(widom ? (size_t)blocks_total * 3 * sizeof(double) : 0);
+    // what the call downloads, in one piece: the candidate-pair counter, then
+    // the block sums, then the block derivative sums
+    const size_t sums_bytes = widom ? (size_t)blocks_total * 3 * sizeof(double) : 0;
+    const size_t gsums_bytes = grad ? (size_t)blocks_total * 3 * GT * sizeof(double) : 0;
+    const size_t down_bytes = sizeof(unsigned long long) + sums_bytes + gsums_bytes;
+    const size_t o_down = ar.take(down_bytes);
     const size_t o_grid = ar.take(widom ? 0 : (size_t)grid_total);
     const size_t o_pos = ar.take(widom ? (size_t)max_slots * sizeof(double4) : 0);
     const size_t o_cell = ar.take(widom ? (size_t)max_slots * sizeof(uint2) : 0);
@@ -1066,7 +1095,6 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const size_t o_span = ar.take(widom ? (size_t)max_spans * 2 * sizeof(unsigned) : 0);
     const size_t o_items = ar.take(widom ? (size_t)max_items * sizeof(uint4) : 0);
     const size_t o_nitems = ar.take(sizeof(unsigned) * 4);
-    const size_t o_cand = ar.take(sizeof(unsigned long long));
     const size_t o_g = ar.take(grad ? (size_t)max_slots * GQ * GT * sizeof(double) : 0);
     // separable Fourier part of charged grids: phases, B and C of the largest run
     size_t gf_elems = 0;
@@ -1079,7 +1107,8 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
                                                   (size_t)(a.Ma + 1) * (2 * a.Mb + 1) * n3 + (size_t)(a.Ma + 1) * n2 * n3);
             }
     const size_t o_gf = ar.take(gf_elems * sizeof(double2));
-    const size_t o_gsums = ar.take(grad ? (size_t)blocks_total * 3 * GT * sizeof(double) : 0);
+    const size_t o_sums = o_down + sizeof(unsigned long long);
+    const size_t o_gsums = o_sums + sums_bytes;
     unsigned char* base = nullptr;
     if (fixed) {
         if (ar.used > fixed->bytes)
@@ -1108,22 +1137,29 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             a.grid_f = fp64 ? nullptr : reinterpret_cast<float*>(base + o_grid + grid_off[i]);
         }
     }
-    std::vector<unsigned char>& stage = g_ctx.host_tab;
-    stage.assign(stage_bytes, 0);
+    // pinned staging when it is there or may be allocated: upload at its
+    // start, download after it (the upload is done before the kernels run)
+    unsigned char* pin = nullptr;
+    const size_t up_bytes = align256(stage_bytes);
+    if (int rc = io_staging(up_bytes + down_bytes, !fixed, &pin)) return rc;
+    std::vector<unsigned char>& host_stage = g_ctx.host_tab;
+    if (pin == nullptr) host_stage.resize(stage_bytes);
+    unsigned char* const stage = pin != nullptr ? pin : host_stage.data();
+    unsigned char* const down = pin != nullptr ? pin + up_bytes : nullptr;
+    std::memset(stage, 0, stage_bytes);
     for (int i = 0; i < n_runs; ++i) {
-        std::memcpy(stage.data() + o_runs + i * sizeof(RunArgs), &runs[i].ra, sizeof(RunArgs));
+        std::memcpy(stage + o_runs + i * sizeof(RunArgs), &runs[i].ra, sizeof(RunArgs));
         if (!runs[i].tab.empty())
-            std::memcpy(stage.data() + o_tabs + runs[i].tab_off, runs[i].tab.data(), runs[i].tab.size());
+            std::memcpy(stage + o_tabs + runs[i].tab_off, runs[i].tab.data(), runs[i].tab.size());
         if (grad) {
             if (!runs[i].gtab.empty())
-                std::memcpy(stage.data() + o_tabs + runs[i].gtab_off, runs[i].gtab.data(), runs[i].gtab.size());
+                std::memcpy(stage + o_tabs + runs[i].gtab_off, runs[i].gtab.data(), runs[i].gtab.size());
             if (!runs[i].jac.empty())
-                std::memcpy(stage.data() + o_tabs + runs[i].jac_off, runs[i].jac.data(),
-                            runs[i].jac.size() * sizeof(double4));
+                std::memcpy(stage + o_tabs + runs[i].jac_off, runs[i].jac.data(), runs[i].jac.size() * sizeof(double4));
         }
     }
-    if (!segs.empty()) std::memcpy(stage.data() + o_segs, segs.data(), segs.size() * sizeof(SegArgs));
-    PM_HIP(hipMemcpyAsync(base, stage.data(), stage_bytes, hipMemcpyHostToDevice, g_ctx.stream));
+    if (!segs.empty()) std::memcpy(stage + o_segs, segs.data(), segs.size() * sizeof(SegArgs));
+    PM_HIP(hipMemcpyAsync(base, stage, stage_bytes, hipMemcpyHostToDevice, g_ctx.stream));
 
     const RunArgs* d_runs = reinterpret_cast<const RunArgs*>(base + o_runs);
     const SegArgs* d_segs = reinterpret_cast<const SegArgs*>(base + o_segs);
@@ -1133,7 +1169,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     unsigned* d_span_p = reinterpret_cast<unsigned*>(base + o_span);
     unsigned* d_span_s = d_span_p + max_spans;
     unsigned* d_nitems = reinterpret_cast<unsigned*>(base + o_nitems);
-    unsigned long long* d_cand = reinterpret_cast<unsigned long long*>(base + o_cand);
+    unsigned long long* d_cand = reinterpret_cast<unsigned long long*>(base + o_down);
     double* d_sums = reinterpret_cast<double*>(base + o_sums);
 
     GroupArgs g{};
@@ -1154,7 +1190,11 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     g.g_stride = max_slots;
     const size_t shm = (fp64 ? pair_lds_fixed<double>() : pair_lds_fixed<float>()) + std::max<size_t>(lj_lds, 16);
 
-    PM_HIP(hipMemsetAsync(d_cand, 0, sizeof(unsigned long long), g_ctx.stream));
+    // the counter is cleared by a kernel (Widom: the first cell-count fill),
+    // which a caller's graph capture records safely
+    if (!widom)
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, g_ctx.stream, static_cast<unsigned*>(nullptr), 0LL,
+                           0u, d_cand);
     PM_HIP(hipEventRecord(g_ctx.ev0, g_ctx.stream));
     float pair_ms = 0.f;
     for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -1166,7 +1206,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             const long long n_cells = gr.n_cells;
             const long long n_spans = (n_cells + SCAN_SPAN - 1) / SCAN_SPAN;
             hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_cells + 1, 256)), dim3(256), 0, g_ctx.stream, d_count,
-                               n_cells + 1, 0u);
+                               n_cells + 1, 0u, gi == 0 ? d_cand : nullptr);
             hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
             hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)n_spans), dim3(SCAN_WG), 0, g_ctx.stream, d_count,
                                (int)n_cells, d_span_p, d_span_s);
@@ -1240,16 +1280,19 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     }
     PM_HIP(hipEventRecord(g_ctx.ev1, g_ctx.stream));
     unsigned long long cand = 0;
-    PM_HIP(hipMemcpyAsync(&cand, d_cand, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream));
+    if (down != nullptr) {
+        PM_HIP(hipMemcpyAsync(down, d_cand, down_bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+    } else {
+        PM_HIP(hipMemcpyAsync(&cand, d_cand, sizeof(cand), hipMemcpyDeviceToHost, g_ctx.stream));
+        if (widom)
+            PM_HIP(hipMemcpyAsync(out_sums, d_sums, sums_bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+        if (grad)
+            PM_HIP(hipMemcpyAsync(out_gsums, base + o_gsums, gsums_bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+    }
     std::vector<GridPiece> pieces;
     std::vector<std::vector<float>> tmp;
     double pairs = 0.0;
     if (widom) {
-        PM_HIP(hipMemcpyAsync(out_sums, d_sums, (size_t)blocks_total * 3 * sizeof(double), hipMemcpyDeviceToHost,
-                              g_ctx.stream));
-        if (grad)
-            PM_HIP(hipMemcpyAsync(out_gsums, base + o_gsums, (size_t)blocks_total * 3 * GT * sizeof(double),
-                                  hipMemcpyDeviceToHost, g_ctx.stream));
         for (const PointRun& r : runs) pairs += (double)r.ra.n_blocks * r.ra.n_per_block * r.ns * r.fw->n_rep;
     } else {
         // grids go to the callers' FP64 buffers in pieces through pinned
@@ -1295,6 +1338,11 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         }
     }
     if (int rc = finish_timing(pairs)) return rc;
+    if (down != nullptr) {
+        std::memcpy(&cand, down, sizeof(cand));
+        if (widom) std::memcpy(out_sums, down + sizeof(cand), sums_bytes);
+        if (grad) std::memcpy(out_gsums, down + sizeof(cand) + sums_bytes, gsums_bytes);
+    }
     for (size_t i = 0; i < tmp.size(); ++i)
         for (long long p = 0; p < runs[i].n_points; ++p) runs[i].host_out[p] = (double)tmp[i][(size_t)p];
     {
