@@ -553,6 +553,7 @@ struct PairLds {
     const float4* erfc;    // FP32 kernels
     const double2* erfc64;  // FP64 kernels
     int* wcnt;      // per-wave candidate counts
+    double* sph;    // per-wave bounding-box and radius partials of the workgroup's sphere
 };
 
 // Per-type derivative sums of a gradient run's point, FP64, for NT framework
@@ -603,9 +604,11 @@ struct TypeSums {
     }
 };
 
-// The pair loops of one workgroup's chunk.  Fast path: scan the image list,
-// keep the images within r_cmax + chunk_r of the chunk centre by a
-// wave-ballot compaction into LDS, evaluate every lane's point against them.
+// The pair loops of one workgroup's chunk.  Fast path: bound the workgroup's
+// points by a sphere, scan the image list, keep the images within r_cmax of
+// the sphere by a wave-ballot compaction into LDS, evaluate every lane's point
+// against them (an image outside a point's cutoffs adds exact zeros, so the
+// sums do not depend on the sphere).
 // Slow path: every replicated atom, nearest image per pair.  NT > 0 (a
 // gradient run of at most NT framework types) adds the per-type derivative
 // sums: on the fast path the images of a tile come in type order (the image
@@ -614,8 +617,8 @@ struct TypeSums {
 // per type at compaction; a run is summed with its type's parameters held in
 // registers and folded into the type's FP64 sums once.
 template <int ESM, typename Real, int NT>
-__device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs& a, bool valid, double px, double py, double pz, double cf0,
-                                   double cf1, double cf2, int site, const LJP<Real>* s_lj, const Real* s_gc,
+__device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs& a, bool valid, double px, double py, double pz,
+                                   int site, const LJP<Real>* s_lj, const Real* s_gc,
                                    PairLds L, double& acc_lj, double& acc_sr, bool& ovl, unsigned long long& n_cand,
                                    TypeSums<Real, (NT > 0 ? NT : 1)>& ts) {
     constexpr bool GRAD = NT > 0;
@@ -635,10 +638,36 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
         const double prx = f11 * px + f12 * py + f13 * pz;
         const double pry = f22 * py + f23 * pz;
         const double prz = f33 * pz;
-        const double ccx = f11 * cf0 + f12 * cf1 + f13 * cf2;
-        const double ccy = f22 * cf1 + f23 * cf2;
-        const double ccz = f33 * cf2;
-        const double Rc = a.r_cmax + a.chunk_r + 1e-3;
+        // The workgroup's sphere: the centre of its valid points' bounding box
+        // and their largest distance from it (inside the chunk's sphere, and
+        // smaller where the points leave the chunk's corners empty).
+        double bx[6] = {valid ? prx : INFINITY,  valid ? pry : INFINITY,  valid ? prz : INFINITY,
+                        valid ? -prx : INFINITY, valid ? -pry : INFINITY, valid ? -prz : INFINITY};
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < 6; ++i) bx[i] = fmin(bx[i], __shfl_xor(bx[i], off, 64));
+        if (lane == 0)
+#pragma unroll
+            for (int i = 0; i < 6; ++i) L.sph[wave * 8 + i] = bx[i];
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < WG / 64; ++w)
+#pragma unroll
+            for (int i = 0; i < 6; ++i) bx[i] = fmin(bx[i], L.sph[w * 8 + i]);
+        const double ccx = 0.5 * (bx[0] - bx[3]), ccy = 0.5 * (bx[1] - bx[4]), ccz = 0.5 * (bx[2] - bx[5]);
+        double d2 = 0.0;
+        if (valid) {
+            const double ex = prx - ccx, ey = pry - ccy, ez = prz - ccz;
+            d2 = ex * ex + ey * ey + ez * ez;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
+        if (lane == 0) L.sph[wave * 8 + 6] = d2;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < WG / 64; ++w) d2 = fmax(d2, L.sph[w * 8 + 6]);
+        const double Rc = a.r_cmax + sqrt(d2) + 1e-3;
         const double R2 = Rc * Rc;
         const int n_tiles = a.n_tiles;
         for (int tb = 0; tb < n_tiles; tb += WG) {
@@ -825,6 +854,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_wcnt[WG / 64];
     __shared__ int s_tcnt[(WG / 64) * (NT > 0 ? NT : 1)];
+    __shared__ double s_sph[(WG / 64) * 8];
     PairLds L;
     L.c = reinterpret_cast<double4*>(smem);
     L.t = reinterpret_cast<int*>(smem + TILE * sizeof(double4));
@@ -836,6 +866,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     L.erfc64 = s_erfc64;
     L.wcnt = s_wcnt;
     L.tcnt = s_tcnt;
+    L.sph = s_sph;
     LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + pair_lds_fixed<Real>());
 
     const int tid = threadIdx.x;
@@ -852,23 +883,15 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     const RunArgs& a = g.runs[sg.run];
     Real* s_gc = reinterpret_cast<Real*>(s_lj + a.n_types * a.nsite);
 
-    double cf0, cf1, cf2;  // chunk centre (fractional)
     bool valid;
     long long pid;
     double px, py, pz;
     int site;
     if (g.mode == MODE_WIDOM) {
-        const int c = (int)(it.x - sg.cell_base);
-        const int n0 = a.ncell[0], n1 = a.ncell[1];
-        const int ia = c % n0, ib = (c / n0) % n1, ic = c / (n0 * n1);
-        // cells tile the first unit cell of the replicated box
-        cf0 = (ia + 0.5) / (n0 * a.repd[0]);
-        cf1 = (ib + 0.5) / (n1 * a.repd[1]);
-        cf2 = (ic + 0.5) / (a.ncell[2] * a.repd[2]);
         const unsigned k = g.point_off[it.x] + it.y * WG + tid;
         valid = k < g.point_off[it.x + 1];
         pid = valid ? (long long)g.order[k] : sg.slot_base;
-        const double4 p = valid ? g.pos[pid] : make_double4(cf0, cf1, cf2, 0.0);
+        const double4 p = valid ? g.pos[pid] : make_double4(0.0, 0.0, 0.0, 0.0);
         px = p.x; py = p.y; pz = p.z;
         site = (int)((pid - sg.slot_base) % a.nsite);
     } else {
@@ -884,9 +907,6 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
         px = grid_coord(i, n1, a.repd[0]);
         py = grid_coord(j, n2, a.repd[1]);
         pz = grid_coord(k, n3, a.repd[2]);
-        cf0 = grid_coord(i0 + 0.5 * (a.brick[0] - 1), n1, a.repd[0]);
-        cf1 = grid_coord(j0 + 0.5 * (a.brick[1] - 1), n2, a.repd[1]);
-        cf2 = grid_coord(k0 + 0.5 * (a.brick[2] - 1), n3, a.repd[2]);
         site = 0;
     }
     const int n_tab = a.n_types * a.nsite;
@@ -908,13 +928,13 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     unsigned long long n_cand = 0;
     TypeSums<Real, (NT > 0 ? NT : 1)> ts;
     if (a.esm == 0)
-        chunk_pairs<0, Real, NT>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+        chunk_pairs<0, Real, NT>(a, valid, px, py, pz, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
                                  n_cand, ts);
     else if (a.esm == 1)
-        chunk_pairs<1, Real, NT>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+        chunk_pairs<1, Real, NT>(a, valid, px, py, pz, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
                                  n_cand, ts);
     else
-        chunk_pairs<2, Real, NT>(a, valid, px, py, pz, cf0, cf1, cf2, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
+        chunk_pairs<2, Real, NT>(a, valid, px, py, pz, site, s_lj, s_gc, L, acc_lj, acc_sr, ovl,
                                  n_cand, ts);
     if (valid) {
         g.out_lj[pid] = ovl ? INFINITY : acc_lj;
