@@ -245,7 +245,7 @@ int pm_widom_gradient(const pm_framework* fw, const pm_forcefield* ff, const pm_
  * partials, block sums, staged tables) live in a fixed device array of
  * PM_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the
  * process ends; a run whose buffers exceed it fails with PM_ERR_ARG before any
- * device work (about 210 bytes per insertion of a 3-site probe: some 3e5 CO2
+ * device work (about 235 bytes per insertion of a 3-site probe: some 2.8e5 CO2
  * insertions per call).  The framework's image list for the force field's
  * cutoff must already exist (any pm_widom call with the same framework and
  * force field builds it); otherwise PM_ERR_STATE.  It calls no hipMalloc,

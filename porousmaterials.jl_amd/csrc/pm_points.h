@@ -6,9 +6,11 @@
 // of point slots.  Each stage is one launch over every segment of a group:
 //
 //   widom_gen_kernel        poses from the counter stream, sites wrapped into
-//                           [0,1)^3 and binned into the run's cells
-//   scan_*_kernel,
-//   scatter_points_kernel,
+//                           [0,1)^3, binned into the run's cells and given
+//                           slots within super-bins of cells
+//   scan_excl_kernel,
+//   stage_points_kernel,
+//   cell_sort_kernel,
 //   slice_items_kernel      counting sort by cell; work items (cell, slice)
 //   point_pair_kernel       LJ + real-space Ewald of a chunk of points (one
 //                           slice of a cell, or one brick of a grid) against
@@ -42,8 +44,6 @@ constexpr int ERFC_SEG = 512;      // cubic segments of erfc(alpha r) on [0, r_s
 constexpr float ERFC_XMAX = (float)ERFC_SEG - (float)ERFC_SEG / 16777216.0f;
 constexpr int ERFC64_SEG = 512;    // quintic segments of erfc(alpha r) on [0, r_sr] (FP64 path)
 constexpr int SCAN_WG = 1024;      // work-items of the scan kernels
-constexpr int SCAN_PER = 4;        // cells per work-item in the scan kernels
-constexpr int SCAN_SPAN = SCAN_WG * SCAN_PER;
 constexpr int GT = PM_MAX_GRAD_TYPES;  // framework types of a gradient run (fixed stride)
 constexpr int GQ = 4;                  // per-type sums per point: d/d eps, d/d sigma, erfc/r, Fourier
 constexpr int GK = 3 + 3 * GT;         // per-workgroup partials of a gradient run
@@ -118,8 +118,9 @@ struct GroupArgs {
     const SegArgs* segs;
     int n_seg, mode;
     double4* pos;            // Widom: per slot, fractional xyz in [0,1)^3, w = 1 (valid) or 0
-    uint2* cell;             // Widom: per slot, (cell in the group or 0xffffffff, rank within the cell)
-    unsigned* count;         // per cell
+    uint2* cell;             // Widom: per slot, (cell in the group or 0xffffffff, slot within its super-bin)
+    unsigned* sb_count;      // Widom: points per super-bin of 2^sb_shift cells
+    int sb_shift;
     const unsigned* point_off;
     const uint4* items;      // (cell, slice, segment, 0)
     const unsigned* n_items;
@@ -212,7 +213,16 @@ __device__ inline bool access_open(const RunArgs& a, const double* u) {
 // blocked pocket of the run's accessibility grid emits no point (its slots
 // look like padding); the combine kernels give it E_vdw = E_es = +Inf
 // (src/henry.jl:225-232).
+//
+// Binning is the first of two counting passes.  Cells come in super-bins of
+// 2^sb_shift consecutive cells; the workgroup counts its points per super-bin
+// in LDS and takes each super-bin's slots with one global atomic, so a point
+// leaves with (cell, slot within its super-bin).  One device-wide atomic per
+// point would cost more than the rest of the kernel.
+constexpr int SB_SPAN_MAX = 4096;   // super-bins one segment's cells may span
+constexpr int SB_CELLS_MAX = 4096;  // cells per super-bin (2^sb_shift)
 __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
+    __shared__ unsigned s_sb[SB_SPAN_MAX];
     const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
     const RunArgs& a = g.runs[sg.run];
     const int lw = (int)blockIdx.x - sg.wg_base;  // workgroup within the segment
@@ -222,6 +232,12 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     const bool valid = i < a.n_per_block;
     const int ns = a.nsite;
     const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
+    const int n0 = a.ncell[0], n1 = a.ncell[1], n2 = a.ncell[2];
+    // the super-bins of this segment's cells
+    const int S = g.sb_shift;
+    const unsigned sb_lo = sg.cell_base >> S;
+    const int span = (int)(((sg.cell_base + (unsigned)(n0 * n1 * n2) - 1u) >> S) - sb_lo) + 1;
+    for (int k = threadIdx.x; k < span; k += WG) s_sb[k] = 0u;
     double u[6];
     bool emit = valid;
     if (valid) {
@@ -229,59 +245,65 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
         insertion_uniforms(a.seed, gi, u);
         if (a.acc != nullptr) emit = access_open(a, u);
     }
+    __syncthreads();  // the super-bin counts are clear
+    unsigned cid[PM_MAX_SITES], slot[PM_MAX_SITES];
     if (!emit) {
         for (int s = 0; s < ns; ++s) {
             g.pos[p0 + s] = make_double4(0.0, 0.0, 0.0, 0.0);
             g.cell[p0 + s] = make_uint2(0xffffffffu, 0u);
         }
-        return;
-    }
-    double R[9];
-    const bool rotate = a.rotate != 0;
-    if (rotate) arvo_rotation(&u[3], R);
-    const int n0 = a.ncell[0], n1 = a.ncell[1], n2 = a.ncell[2];
-    // The site offsets and the basis change come in by scalar loads before the
-    // first atomic, so that the returning atomics of an insertion's sites are
-    // in flight together (a vector load after an atomic would wait for it).
-    const gconst_ptr<double> offp = uniform_ptr(&a.off[0][0]);
-    const gconst_ptr<double> c2fp = uniform_ptr(&a.c2f[0]);
-    const gconst_ptr<double> repp = uniform_ptr(&a.repd[0]);
-    double c2f[9], repd[3];
+    } else {
+        double R[9];
+        const bool rotate = a.rotate != 0;
+        if (rotate) arvo_rotation(&u[3], R);
+        const gconst_ptr<double> offp = uniform_ptr(&a.off[0][0]);
+        const gconst_ptr<double> c2fp = uniform_ptr(&a.c2f[0]);
+        const gconst_ptr<double> repp = uniform_ptr(&a.repd[0]);
+        double c2f[9], repd[3];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) c2f[k] = c2fp[k];
+        for (int k = 0; k < 9; ++k) c2f[k] = c2fp[k];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) repd[k] = repp[k];
-    unsigned cid[PM_MAX_SITES], rank[PM_MAX_SITES];
+        for (int k = 0; k < 3; ++k) repd[k] = repp[k];
 #pragma unroll
-    for (int s = 0; s < PM_MAX_SITES; ++s) {
-        if (s >= ns) break;
-        double y0 = offp[3 * s], y1 = offp[3 * s + 1], y2 = offp[3 * s + 2];
-        if (rotate) {
-            const double t0 = R[0] * y0 + R[3] * y1 + R[6] * y2;
-            const double t1 = R[1] * y0 + R[4] * y1 + R[7] * y2;
-            const double t2 = R[2] * y0 + R[5] * y1 + R[8] * y2;
-            y0 = t0; y1 = t1; y2 = t2;
+        for (int s = 0; s < PM_MAX_SITES; ++s) {
+            if (s >= ns) break;
+            double y0 = offp[3 * s], y1 = offp[3 * s + 1], y2 = offp[3 * s + 2];
+            if (rotate) {
+                const double t0 = R[0] * y0 + R[3] * y1 + R[6] * y2;
+                const double t1 = R[1] * y0 + R[4] * y1 + R[7] * y2;
+                const double t2 = R[2] * y0 + R[5] * y1 + R[8] * y2;
+                y0 = t0; y1 = t1; y2 = t2;
+            }
+            // unit-cell fractional coordinates in [0, 1), then back to the box's
+            double x = (u[0] + (c2f[0] * y0 + c2f[3] * y1 + c2f[6] * y2)) * repd[0];
+            double y = (u[1] + (c2f[1] * y0 + c2f[4] * y1 + c2f[7] * y2)) * repd[1];
+            double z = (u[2] + (c2f[2] * y0 + c2f[5] * y1 + c2f[8] * y2)) * repd[2];
+            x -= floor(x);
+            y -= floor(y);
+            z -= floor(z);
+            const int ia = min((int)(x * n0), n0 - 1);
+            const int ib = min((int)(y * n1), n1 - 1);
+            const int ic = min((int)(z * n2), n2 - 1);
+            cid[s] = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
+            g.pos[p0 + s] = make_double4(x / repd[0], y / repd[1], z / repd[2], 1.0);
+            // the count before this point is its rank among the workgroup's
+            // points of the super-bin (the order within a cell changes no
+            // point's energy)
+            slot[s] = atomicAdd(&s_sb[(cid[s] >> S) - sb_lo], 1u);
         }
-        // unit-cell fractional coordinates in [0, 1), then back to the box's
-        double x = (u[0] + (c2f[0] * y0 + c2f[3] * y1 + c2f[6] * y2)) * repd[0];
-        double y = (u[1] + (c2f[1] * y0 + c2f[4] * y1 + c2f[7] * y2)) * repd[1];
-        double z = (u[2] + (c2f[2] * y0 + c2f[5] * y1 + c2f[8] * y2)) * repd[2];
-        x -= floor(x);
-        y -= floor(y);
-        z -= floor(z);
-        const int ia = min((int)(x * n0), n0 - 1);
-        const int ib = min((int)(y * n1), n1 - 1);
-        const int ic = min((int)(z * n2), n2 - 1);
-        cid[s] = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
-        g.pos[p0 + s] = make_double4(x / repd[0], y / repd[1], z / repd[2], 1.0);
-        // the count before this point is its rank in the cell (the order within
-        // a cell changes no point's energy)
-        rank[s] = atomicAdd(&g.count[cid[s]], 1u);
     }
+    __syncthreads();  // the workgroup's counts are complete
+    for (int k = threadIdx.x; k < span; k += WG) {
+        const unsigned n = s_sb[k];
+        if (n != 0u) s_sb[k] = atomicAdd(&g.sb_count[sb_lo + k], n);
+    }
+    __syncthreads();  // each super-bin's first slot for this workgroup
+    if (emit) {
 #pragma unroll
-    for (int s = 0; s < PM_MAX_SITES; ++s) {
-        if (s >= ns) break;
-        g.cell[p0 + s] = make_uint2(cid[s], rank[s]);
+        for (int s = 0; s < PM_MAX_SITES; ++s) {
+            if (s >= ns) break;
+            g.cell[p0 + s] = make_uint2(cid[s], s_sb[(cid[s] >> S) - sb_lo] + slot[s]);
+        }
     }
 }
 
@@ -291,10 +313,12 @@ __global__ void fill_u32_kernel(unsigned* p, long long n, unsigned v, unsigned l
     if (zero != nullptr && i == 0) *zero = 0ull;
 }
 
-// ---------------------------------------------------------------- scan ----
-// Exclusive scans over cells of the point counts and of the slice counts
-// (ceil(count / WG)) in three launches: per-span totals, a scan of the span
-// totals, and the per-span scans.
+// ---------------------------------------------------------------- sort ----
+// The points of a Widom group sorted by cell in two counting passes: the
+// generator's per-super-bin slots, a scan over super-bins, the points staged
+// by super-bin, then one workgroup per super-bin counts and places them by
+// cell in LDS.  Cells also get their slices (ceil(count / WG) work items each):
+// offsets within the super-bin, then a scan over super-bins.
 __device__ inline void scan_pair_1024(unsigned& p, unsigned& s, unsigned& tot_p, unsigned& tot_s) {
     // exclusive scan of (p, s) over the 1024 work-items; also the totals
     __shared__ unsigned w_p[SCAN_WG / 64], w_s[SCAN_WG / 64];
@@ -320,85 +344,94 @@ __device__ inline void scan_pair_1024(unsigned& p, unsigned& s, unsigned& tot_p,
     s = bs + is - s;
 }
 
-__global__ __launch_bounds__(SCAN_WG) void scan_reduce_kernel(const unsigned* count, int n_cells,
-                                                              unsigned* span_p, unsigned* span_s) {
-    const long long c0 = (long long)blockIdx.x * SCAN_SPAN + (long long)threadIdx.x * SCAN_PER;
-    unsigned p = 0, s = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) {
-        const unsigned v = c0 + k < n_cells ? count[c0 + k] : 0u;
-        p += v;
-        s += (v + WG - 1) / WG;
-    }
-    unsigned tp, ts;
-    scan_pair_1024(p, s, tp, ts);
-    if (threadIdx.x == 0) { span_p[blockIdx.x] = tp; span_s[blockIdx.x] = ts; }
-}
-
-// One workgroup: exclusive scan of the span totals in place; the grand totals
-// go to point_off[n_cells], slice_off[n_cells] and *n_slices.
-__global__ __launch_bounds__(SCAN_WG) void scan_spans_kernel(unsigned* span_p, unsigned* span_s, int n_spans,
-                                                             unsigned* point_off, unsigned* slice_off,
-                                                             int n_cells, unsigned* n_slices) {
-    unsigned run_p = 0, run_s = 0;
-    for (int base = 0; base < n_spans; base += SCAN_WG) {
-        const int i = base + threadIdx.x;
-        unsigned p = i < n_spans ? span_p[i] : 0u, s = i < n_spans ? span_s[i] : 0u;
-        unsigned tp, ts;
+// One workgroup: exclusive scan of in[0, n) into out[0, n], out[n] = the
+// total, also stored at *total.
+__global__ __launch_bounds__(SCAN_WG) void scan_excl_kernel(const unsigned* in, int n, unsigned* out,
+                                                            unsigned* total) {
+    unsigned run = 0;
+    for (int base = 0; base < n; base += SCAN_WG) {
+        const int k = base + threadIdx.x;
+        unsigned p = k < n ? in[k] : 0u, s = 0u, tp, ts;
         scan_pair_1024(p, s, tp, ts);
-        if (i < n_spans) { span_p[i] = run_p + p; span_s[i] = run_s + s; }
-        run_p += tp;
-        run_s += ts;
+        if (k < n) out[k] = run + p;
+        run += tp;
     }
     if (threadIdx.x == 0) {
-        point_off[n_cells] = run_p;
-        slice_off[n_cells] = run_s;
-        *n_slices = run_s;
+        out[n] = run;
+        *total = run;
     }
 }
 
-__global__ __launch_bounds__(SCAN_WG) void scan_down_kernel(const unsigned* count, int n_cells,
-                                                            const unsigned* span_p, const unsigned* span_s,
-                                                            unsigned* point_off, unsigned* slice_off) {
-    const long long c0 = (long long)blockIdx.x * SCAN_SPAN + (long long)threadIdx.x * SCAN_PER;
-    unsigned v[SCAN_PER];
-    unsigned p = 0, s = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) {
-        v[k] = c0 + k < n_cells ? count[c0 + k] : 0u;
-        p += v[k];
-        s += (v[k] + WG - 1) / WG;
-    }
-    unsigned tp, ts;
-    scan_pair_1024(p, s, tp, ts);
-    p += span_p[blockIdx.x];
-    s += span_s[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) {
-        if (c0 + k < n_cells) {
-            point_off[c0 + k] = p;
-            slice_off[c0 + k] = s;
-        }
-        p += v[k];
-        s += (v[k] + WG - 1) / WG;
-    }
-}
-
-__global__ void scatter_points_kernel(const uint2* cell, long long n_pts, const unsigned* point_off,
-                                      unsigned* order) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Every point to its super-bin's stretch of the staging array: (slot, cell).
+__global__ void stage_points_kernel(const uint2* cell, long long n_pts, const unsigned* sb_off, int sb_shift,
+                                    uint2* stage) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pts) return;
     const uint2 c = cell[p];
     if (c.x == 0xffffffffu) return;
-    order[point_off[c.x] + c.y] = (unsigned)p;
+    stage[sb_off[c.x >> sb_shift] + c.y] = make_uint2((unsigned)p, c.x);
 }
 
-__global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_off, const SegArgs* segs,
-                                   int n_seg, uint4* items, int n_cells) {
+// The second counting pass, one workgroup per super-bin: the super-bin's
+// points counted per cell in LDS, the cells' point offsets (the super-bin's
+// points are a contiguous stretch of the sorted order) and, within the
+// super-bin, their slice offsets (ceil(count / WG) slices per cell), then every
+// point to its cell's stretch of `order`.
+__global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, const unsigned* sb_off, int sb_shift,
+                                                            int n_cells, unsigned* count, unsigned* point_off,
+                                                            unsigned* slice_loc, unsigned* sb_slices,
+                                                            unsigned* order) {
+    __shared__ unsigned s_n[SB_CELLS_MAX];
+    const int tid = threadIdx.x;
+    const int sb = blockIdx.x;
+    const int c_lo = sb << sb_shift;
+    const int nc = min(1 << sb_shift, n_cells - c_lo);
+    const unsigned lo = sb_off[sb], hi = sb_off[sb + 1];
+    for (int k = tid; k < nc; k += SCAN_WG) s_n[k] = 0u;
+    __syncthreads();
+    for (unsigned i = lo + tid; i < hi; i += SCAN_WG) atomicAdd(&s_n[stage[i].y - c_lo], 1u);
+    __syncthreads();
+    // each work-item a run of consecutive cells
+    const int per = (nc + SCAN_WG - 1) / SCAN_WG;
+    const int k0 = tid * per;
+    unsigned p = 0u, s = 0u;
+    for (int j = 0; j < per; ++j) {
+        const int k = k0 + j;
+        if (k < nc) {
+            const unsigned v = s_n[k];
+            p += v;
+            s += (v + WG - 1) / WG;
+        }
+    }
+    unsigned tp, ts;
+    scan_pair_1024(p, s, tp, ts);  // (its barriers also order the reads above before the writes below)
+    p += lo;
+    for (int j = 0; j < per; ++j) {
+        const int k = k0 + j;
+        if (k < nc) {
+            const unsigned v = s_n[k];
+            count[c_lo + k] = v;
+            point_off[c_lo + k] = p;
+            slice_loc[c_lo + k] = s;
+            s_n[k] = p;  // from here on: the next free position of the cell
+            p += v;
+            s += (v + WG - 1) / WG;
+        }
+    }
+    if (tid == 0) sb_slices[sb] = ts;
+    __syncthreads();
+    for (unsigned i = lo + tid; i < hi; i += SCAN_WG) {
+        const uint2 e = stage[i];
+        order[atomicAdd(&s_n[e.y - c_lo], 1u)] = e.x;
+    }
+}
+
+__global__ void slice_items_kernel(const unsigned* count, const unsigned* slice_loc, const unsigned* sb_slice_off,
+                                   int sb_shift, const SegArgs* segs, int n_seg, uint4* items, int n_cells) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n_cells) return;
     const int si = find_segment<SEG_BY_CELL>(segs, n_seg, c);
-    unsigned ns = (count[c] + WG - 1) / WG, base = slice_off[c];
+    unsigned ns = (count[c] + WG - 1) / WG, base = sb_slice_off[c >> sb_shift] + slice_loc[c];
     for (unsigned k = 0; k < ns; ++k) items[base + k] = make_uint4((unsigned)c, k, (unsigned)si, 0u);
 }
 

@@ -43,6 +43,9 @@ constexpr long long MAX_IMAGES = 1LL << 26;       // images per list
 constexpr size_t MAX_IMAGE_LISTS = 8;             // image lists (radii) per framework
 constexpr double MAX_RUN_CELLS = 1 << 22;         // cells of one Widom run
 constexpr long long MAX_GROUP_CELLS = 1LL << 24;  // cells of one launch group
+constexpr double SB_POINTS = 32768.0;             // target points per super-bin of the Widom sort
+constexpr int SB_SHIFT_MAX = 12;                  // 2^12 = SB_CELLS_MAX cells per super-bin
+static_assert((1 << SB_SHIFT_MAX) == pmk::SB_CELLS_MAX, "super-bin cells fit the sort kernel's LDS");
 
 thread_local std::string g_err;
 std::mutex g_mu;
@@ -978,6 +981,8 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         int n_seg = 0;
         long long n_wg = 0, n_slots = 0, n_cells = 0;
         bool es = false;
+        int sb_shift = 0;    // Widom: super-bins of 2^sb_shift cells
+        long long n_sb = 0;
     };
     std::vector<SegArgs> segs;
     std::vector<Group> groups;
@@ -1031,11 +1036,26 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         }
     }
     close_group();
-    long long max_slots = 0, max_cells = 0, max_items = 0;
-    for (const Group& gr : groups) {
+    long long max_slots = 0, max_cells = 0, max_items = 0, max_sb = 0;
+    for (Group& gr : groups) {
         max_slots = std::max(max_slots, gr.n_slots);
         max_cells = std::max(max_cells, gr.n_cells);
         max_items = std::max(max_items, gr.n_cells + (gr.n_slots + WG - 1) / WG + 1);
+        if (mode != MODE_WIDOM) continue;
+        // super-bins of about SB_POINTS points, and no segment's cells spanning
+        // more than SB_SPAN_MAX of them
+        const double per_cell = (double)gr.n_slots / (double)std::max(gr.n_cells, 1LL);
+        int S = (int)std::lround(std::log2(std::max(1.0, SB_POINTS / std::max(per_cell, 1.0))));
+        S = std::min(std::max(S, 0), SB_SHIFT_MAX);
+        for (int si = 0; si < gr.n_seg; ++si) {
+            const SegArgs& sg = segs[gr.seg0 + si];
+            const long long c0 = sg.cell_base, c1 = c0 + runs[sg.run].n_cells - 1;
+            while (S < SB_SHIFT_MAX && (c1 >> S) - (c0 >> S) + 1 > SB_SPAN_MAX) ++S;
+            if ((c1 >> S) - (c0 >> S) + 1 > SB_SPAN_MAX) return fail(PM_ERR_ARG, "too many Widom cells in one run");
+        }
+        gr.sb_shift = S;
+        gr.n_sb = (gr.n_cells + (1LL << S) - 1) >> S;
+        max_sb = std::max(max_sb, gr.n_sb);
     }
 
     // LJ tables (host), per-run output offsets
@@ -1088,11 +1108,14 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const size_t o_lj = ar.take((size_t)max_slots * sizeof(double));
     const size_t o_sr = ar.take((size_t)max_slots * sizeof(double));
     const size_t o_lr = ar.take((size_t)max_slots * sizeof(double));
-    const long long max_spans = max_cells / SCAN_SPAN + 1;
     const size_t o_count = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
     const size_t o_poff = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
-    const size_t o_soff = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
-    const size_t o_span = ar.take(widom ? (size_t)max_spans * 2 * sizeof(unsigned) : 0);
+    const size_t o_sloc = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
+    const size_t o_sbc = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
+    const size_t o_sbo = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
+    const size_t o_sbs = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
+    const size_t o_sbso = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
+    const size_t o_stage = ar.take(widom ? (size_t)max_slots * sizeof(uint2) : 0);
     const size_t o_items = ar.take(widom ? (size_t)max_items * sizeof(uint4) : 0);
     const size_t o_nitems = ar.take(sizeof(unsigned) * 4);
     const size_t o_g = ar.take(grad ? (size_t)max_slots * GQ * GT * sizeof(double) : 0);
@@ -1165,9 +1188,12 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const SegArgs* d_segs = reinterpret_cast<const SegArgs*>(base + o_segs);
     unsigned* d_count = reinterpret_cast<unsigned*>(base + o_count);
     unsigned* d_poff = reinterpret_cast<unsigned*>(base + o_poff);
-    unsigned* d_soff = reinterpret_cast<unsigned*>(base + o_soff);
-    unsigned* d_span_p = reinterpret_cast<unsigned*>(base + o_span);
-    unsigned* d_span_s = d_span_p + max_spans;
+    unsigned* d_sloc = reinterpret_cast<unsigned*>(base + o_sloc);
+    unsigned* d_sbc = reinterpret_cast<unsigned*>(base + o_sbc);
+    unsigned* d_sbo = reinterpret_cast<unsigned*>(base + o_sbo);
+    unsigned* d_sbs = reinterpret_cast<unsigned*>(base + o_sbs);
+    unsigned* d_sbso = reinterpret_cast<unsigned*>(base + o_sbso);
+    uint2* d_stage = reinterpret_cast<uint2*>(base + o_stage);
     unsigned* d_nitems = reinterpret_cast<unsigned*>(base + o_nitems);
     unsigned long long* d_cand = reinterpret_cast<unsigned long long*>(base + o_down);
     double* d_sums = reinterpret_cast<double*>(base + o_sums);
@@ -1177,7 +1203,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     g.mode = mode;
     g.pos = reinterpret_cast<double4*>(base + o_pos);
     g.cell = reinterpret_cast<uint2*>(base + o_cell);
-    g.count = d_count;
+    g.sb_count = d_sbc;
     g.point_off = d_poff;
     g.items = reinterpret_cast<const uint4*>(base + o_items);
     g.n_items = d_nitems;
@@ -1203,21 +1229,23 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         g.n_seg = gr.n_seg;
         long long pair_wg = gr.n_wg;
         if (widom) {
-            const long long n_cells = gr.n_cells;
-            const long long n_spans = (n_cells + SCAN_SPAN - 1) / SCAN_SPAN;
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_cells + 1, 256)), dim3(256), 0, g_ctx.stream, d_count,
-                               n_cells + 1, 0u, gi == 0 ? d_cand : nullptr);
+            const long long n_cells = gr.n_cells, n_sb = gr.n_sb;
+            const int S = gr.sb_shift;
+            g.sb_shift = S;
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_sb + 1, 256)), dim3(256), 0, g_ctx.stream, d_sbc,
+                               n_sb + 1, 0u, gi == 0 ? d_cand : nullptr);
             hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
-            hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)n_spans), dim3(SCAN_WG), 0, g_ctx.stream, d_count,
-                               (int)n_cells, d_span_p, d_span_s);
-            hipLaunchKernelGGL(scan_spans_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_span_p, d_span_s,
-                               (int)n_spans, d_poff, d_soff, (int)n_cells, d_nitems);
-            hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)n_spans), dim3(SCAN_WG), 0, g_ctx.stream, d_count,
-                               (int)n_cells, d_span_p, d_span_s, d_poff, d_soff);
-            hipLaunchKernelGGL(scatter_points_kernel, dim3(grid1d(gr.n_slots, 256)), dim3(256), 0, g_ctx.stream,
-                               g.cell, gr.n_slots, d_poff, const_cast<unsigned*>(g.order));
+            // super-bin offsets; their total (the group's points) also closes point_off
+            hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_sbc, (int)n_sb, d_sbo,
+                               d_poff + n_cells);
+            hipLaunchKernelGGL(stage_points_kernel, dim3(grid1d(gr.n_slots, 256)), dim3(256), 0, g_ctx.stream,
+                               g.cell, gr.n_slots, d_sbo, S, d_stage);
+            hipLaunchKernelGGL(cell_sort_kernel, dim3((unsigned)n_sb), dim3(SCAN_WG), 0, g_ctx.stream, d_stage, d_sbo,
+                               S, (int)n_cells, d_count, d_poff, d_sloc, d_sbs, const_cast<unsigned*>(g.order));
+            hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_sbs, (int)n_sb, d_sbso,
+                               d_nitems);
             hipLaunchKernelGGL(slice_items_kernel, dim3(grid1d(n_cells, 256)), dim3(256), 0, g_ctx.stream, d_count,
-                               d_soff, g.segs, gr.n_seg, const_cast<uint4*>(g.items), (int)n_cells);
+                               d_sloc, d_sbso, S, g.segs, gr.n_seg, const_cast<uint4*>(g.items), (int)n_cells);
             pair_wg = n_cells + (gr.n_slots + WG - 1) / WG;  // >= the slices the scan counts
         }
         PM_HIP(hipEventRecord(g_ctx.evp0, g_ctx.stream));
