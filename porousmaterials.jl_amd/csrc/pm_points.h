@@ -118,8 +118,9 @@ struct GroupArgs {
     const SegArgs* segs;
     int n_seg, mode;
     double4* pos;            // Widom: per slot, fractional xyz in [0,1)^3, w = 1 (valid) or 0
-    uint2* cell;             // Widom: per slot, (cell in the group or 0xffffffff, slot within its super-bin)
-    unsigned* sb_count;      // Widom: points per super-bin of 2^sb_shift cells
+    uint2* cell;             // Widom: per slot, (slab << 24 | cell in the group, or 0xffffffff;
+                             // copy << 28 | slot within the super-bin copy)
+    unsigned* sb_count;      // Widom: points per counter copy of each super-bin of 2^sb_shift cells
     int sb_shift;
     const unsigned* point_off;
     const uint4* items;      // (cell, slice, segment, 0)
@@ -216,11 +217,21 @@ __device__ inline bool access_open(const RunArgs& a, const double* u) {
 //
 // Binning is the first of two counting passes.  Cells come in super-bins of
 // 2^sb_shift consecutive cells; the workgroup counts its points per super-bin
-// in LDS and takes each super-bin's slots with one global atomic, so a point
-// leaves with (cell, slot within its super-bin).  One device-wide atomic per
-// point would cost more than the rest of the kernel.
-constexpr int SB_SPAN_MAX = 4096;   // super-bins one segment's cells may span
-constexpr int SB_CELLS_MAX = 4096;  // cells per super-bin (2^sb_shift)
+// in LDS and takes each super-bin's slots with one global atomic on one of
+// SB_COPIES counters of the super-bin (workgroup index mod SB_COPIES, so that
+// no counter serialises every workgroup's atomics), and a point leaves with
+// (cell, copy << 28 | slot within its copy of the super-bin).  One device-wide
+// atomic per point would cost more than the rest of the kernel.
+constexpr int SB_SPAN_MAX = 4160;   // super-bins one segment's cells may span
+constexpr int SB_CELLS_MAX = 1024;  // cells per super-bin (2^sb_shift)
+constexpr int SB_COPIES = 16;       // counters per super-bin
+constexpr unsigned SB_SLOT_MASK = 0x0fffffffu;
+// Within a cell the points are ordered by CELL_ZBINS slabs along the cell's
+// third axis, so that a cell's first slice (WG points) takes its low slabs and
+// the workgroup's candidate sphere shrinks; the slab rides in the top bits of
+// the cell index (cells of a group < 2^24).
+constexpr int CELL_ZBINS = 4;
+constexpr unsigned CELL_MASK = 0x00ffffffu;
 __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     __shared__ unsigned s_sb[SB_SPAN_MAX];
     const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
@@ -246,7 +257,7 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
         if (a.acc != nullptr) emit = access_open(a, u);
     }
     __syncthreads();  // the super-bin counts are clear
-    unsigned cid[PM_MAX_SITES], slot[PM_MAX_SITES];
+    unsigned cid[PM_MAX_SITES], slot[PM_MAX_SITES], zb[PM_MAX_SITES];
     if (!emit) {
         for (int s = 0; s < ns; ++s) {
             g.pos[p0 + s] = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -284,7 +295,9 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
             const int ia = min((int)(x * n0), n0 - 1);
             const int ib = min((int)(y * n1), n1 - 1);
             const int ic = min((int)(z * n2), n2 - 1);
+            const int iz = min((int)((z * n2 - ic) * CELL_ZBINS), CELL_ZBINS - 1);
             cid[s] = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
+            zb[s] = (unsigned)iz << 24;
             g.pos[p0 + s] = make_double4(x / repd[0], y / repd[1], z / repd[2], 1.0);
             // the count before this point is its rank among the workgroup's
             // points of the super-bin (the order within a cell changes no
@@ -293,16 +306,17 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
         }
     }
     __syncthreads();  // the workgroup's counts are complete
+    const unsigned copy = blockIdx.x % SB_COPIES;
     for (int k = threadIdx.x; k < span; k += WG) {
         const unsigned n = s_sb[k];
-        if (n != 0u) s_sb[k] = atomicAdd(&g.sb_count[sb_lo + k], n);
+        if (n != 0u) s_sb[k] = atomicAdd(&g.sb_count[(sb_lo + k) * SB_COPIES + copy], n);
     }
     __syncthreads();  // each super-bin's first slot for this workgroup
     if (emit) {
 #pragma unroll
         for (int s = 0; s < PM_MAX_SITES; ++s) {
             if (s >= ns) break;
-            g.cell[p0 + s] = make_uint2(cid[s], s_sb[(cid[s] >> S) - sb_lo] + slot[s]);
+            g.cell[p0 + s] = make_uint2(zb[s] | cid[s], (copy << 28) | (s_sb[(cid[s] >> S) - sb_lo] + slot[s]));
         }
     }
 }
@@ -362,34 +376,49 @@ __global__ __launch_bounds__(SCAN_WG) void scan_excl_kernel(const unsigned* in, 
     }
 }
 
-// Every point to its super-bin's stretch of the staging array: (slot, cell).
+// Every point to its super-bin's stretch of the staging array, (slot, cell);
+// sb_off holds the offsets of the super-bins' counter copies, copy-minor.
 __global__ void stage_points_kernel(const uint2* cell, long long n_pts, const unsigned* sb_off, int sb_shift,
                                     uint2* stage) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pts) return;
     const uint2 c = cell[p];
     if (c.x == 0xffffffffu) return;
-    stage[sb_off[c.x >> sb_shift] + c.y] = make_uint2((unsigned)p, c.x);
+    stage[sb_off[((c.x & CELL_MASK) >> sb_shift) * SB_COPIES + (c.y >> 28)] + (c.y & SB_SLOT_MASK)] =
+        make_uint2((unsigned)p, c.x);
 }
 
 // The second counting pass, one workgroup per super-bin: the super-bin's
-// points counted per cell in LDS, the cells' point offsets (the super-bin's
-// points are a contiguous stretch of the sorted order) and, within the
-// super-bin, their slice offsets (ceil(count / WG) slices per cell), then every
-// point to its cell's stretch of `order`.
+// points counted per (cell, slab) in LDS, the cells' point offsets (the
+// super-bin's points are a contiguous stretch of the sorted order) and, within
+// the super-bin, their slice offsets (ceil(count / WG) slices per cell), then
+// every point to its (cell, slab)'s stretch of `order`.
 __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, const unsigned* sb_off, int sb_shift,
                                                             int n_cells, unsigned* count, unsigned* point_off,
                                                             unsigned* slice_loc, unsigned* sb_slices,
                                                             unsigned* order) {
-    __shared__ unsigned s_n[SB_CELLS_MAX];
+    __shared__ unsigned s_n[SB_CELLS_MAX * CELL_ZBINS];
     const int tid = threadIdx.x;
     const int sb = blockIdx.x;
     const int c_lo = sb << sb_shift;
     const int nc = min(1 << sb_shift, n_cells - c_lo);
-    const unsigned lo = sb_off[sb], hi = sb_off[sb + 1];
-    for (int k = tid; k < nc; k += SCAN_WG) s_n[k] = 0u;
+    const int nk = nc * CELL_ZBINS;
+    const unsigned lo = sb_off[sb * SB_COPIES], hi = sb_off[(sb + 1) * SB_COPIES];
+    for (int k = tid; k < nk; k += SCAN_WG) s_n[k] = 0u;
     __syncthreads();
-    for (unsigned i = lo + tid; i < hi; i += SCAN_WG) atomicAdd(&s_n[stage[i].y - c_lo], 1u);
+    // four staged points per work-item in flight at once; key (cell, slab)
+    for (unsigned i0 = lo + tid; i0 < hi; i0 += 4 * SCAN_WG) {
+        unsigned cl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned i = i0 + j * SCAN_WG;
+            cl[j] = i < hi ? stage[i].y : 0xffffffffu;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (cl[j] != 0xffffffffu)
+                atomicAdd(&s_n[((cl[j] & CELL_MASK) - c_lo) * CELL_ZBINS + (cl[j] >> 24)], 1u);
+    }
     __syncthreads();
     // each work-item a run of consecutive cells
     const int per = (nc + SCAN_WG - 1) / SCAN_WG;
@@ -398,7 +427,9 @@ __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, 
     for (int j = 0; j < per; ++j) {
         const int k = k0 + j;
         if (k < nc) {
-            const unsigned v = s_n[k];
+            unsigned v = 0u;
+#pragma unroll
+            for (int z = 0; z < CELL_ZBINS; ++z) v += s_n[k * CELL_ZBINS + z];
             p += v;
             s += (v + WG - 1) / WG;
         }
@@ -409,20 +440,32 @@ __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, 
     for (int j = 0; j < per; ++j) {
         const int k = k0 + j;
         if (k < nc) {
-            const unsigned v = s_n[k];
-            count[c_lo + k] = v;
-            point_off[c_lo + k] = p;
+            const unsigned p0 = p;
+#pragma unroll
+            for (int z = 0; z < CELL_ZBINS; ++z) {
+                const unsigned v = s_n[k * CELL_ZBINS + z];
+                s_n[k * CELL_ZBINS + z] = p;  // from here on: the next free position of the slab
+                p += v;
+            }
+            count[c_lo + k] = p - p0;
+            point_off[c_lo + k] = p0;
             slice_loc[c_lo + k] = s;
-            s_n[k] = p;  // from here on: the next free position of the cell
-            p += v;
-            s += (v + WG - 1) / WG;
+            s += (p - p0 + WG - 1) / WG;
         }
     }
     if (tid == 0) sb_slices[sb] = ts;
     __syncthreads();
-    for (unsigned i = lo + tid; i < hi; i += SCAN_WG) {
-        const uint2 e = stage[i];
-        order[atomicAdd(&s_n[e.y - c_lo], 1u)] = e.x;
+    for (unsigned i0 = lo + tid; i0 < hi; i0 += 4 * SCAN_WG) {
+        uint2 e[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned i = i0 + j * SCAN_WG;
+            e[j] = i < hi ? stage[i] : make_uint2(0u, 0xffffffffu);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (e[j].y != 0xffffffffu)
+                order[atomicAdd(&s_n[((e[j].y & CELL_MASK) - c_lo) * CELL_ZBINS + (e[j].y >> 24)], 1u)] = e[j].x;
     }
 }
 

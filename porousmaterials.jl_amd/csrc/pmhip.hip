@@ -44,8 +44,12 @@ constexpr size_t MAX_IMAGE_LISTS = 8;             // image lists (radii) per fra
 constexpr double MAX_RUN_CELLS = 1 << 22;         // cells of one Widom run
 constexpr long long MAX_GROUP_CELLS = 1LL << 24;  // cells of one launch group
 constexpr double SB_POINTS = 32768.0;             // target points per super-bin of the Widom sort
-constexpr int SB_SHIFT_MAX = 12;                  // 2^12 = SB_CELLS_MAX cells per super-bin
+constexpr int SB_SHIFT_MAX = 10;                  // 2^10 = SB_CELLS_MAX cells per super-bin
 static_assert((1 << SB_SHIFT_MAX) == pmk::SB_CELLS_MAX, "super-bin cells fit the sort kernel's LDS");
+static_assert(MAX_POINT_SLOTS <= (long long)pmk::SB_SLOT_MASK + 1, "a super-bin slot fits below the copy bits");
+static_assert(MAX_GROUP_CELLS <= (long long)pmk::CELL_MASK + 1 && MAX_RUN_CELLS <= (double)pmk::CELL_MASK + 1,
+              "a group's cell index fits below the slab bits");
+static_assert(MAX_RUN_CELLS / (1 << SB_SHIFT_MAX) + 1 <= pmk::SB_SPAN_MAX, "a run's cells span few enough super-bins");
 
 thread_local std::string g_err;
 std::mutex g_mu;
@@ -1111,8 +1115,8 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
     const size_t o_count = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
     const size_t o_poff = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
     const size_t o_sloc = ar.take(widom ? (size_t)(max_cells + 1) * sizeof(unsigned) : 0);
-    const size_t o_sbc = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
-    const size_t o_sbo = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
+    const size_t o_sbc = ar.take(widom ? (size_t)(max_sb * SB_COPIES + 1) * sizeof(unsigned) : 0);
+    const size_t o_sbo = ar.take(widom ? (size_t)(max_sb * SB_COPIES + 1) * sizeof(unsigned) : 0);
     const size_t o_sbs = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
     const size_t o_sbso = ar.take(widom ? (size_t)(max_sb + 1) * sizeof(unsigned) : 0);
     const size_t o_stage = ar.take(widom ? (size_t)max_slots * sizeof(uint2) : 0);
@@ -1232,11 +1236,13 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             const long long n_cells = gr.n_cells, n_sb = gr.n_sb;
             const int S = gr.sb_shift;
             g.sb_shift = S;
-            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_sb + 1, 256)), dim3(256), 0, g_ctx.stream, d_sbc,
-                               n_sb + 1, 0u, gi == 0 ? d_cand : nullptr);
+            const long long n_sbc = n_sb * SB_COPIES;
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_sbc + 1, 256)), dim3(256), 0, g_ctx.stream, d_sbc,
+                               n_sbc + 1, 0u, gi == 0 ? d_cand : nullptr);
             hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
-            // super-bin offsets; their total (the group's points) also closes point_off
-            hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_sbc, (int)n_sb, d_sbo,
+            // offsets of the super-bins' counter copies; their total (the
+            // group's points) also closes point_off
+            hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_sbc, (int)n_sbc, d_sbo,
                                d_poff + n_cells);
             hipLaunchKernelGGL(stage_points_kernel, dim3(grid1d(gr.n_slots, 256)), dim3(256), 0, g_ctx.stream,
                                g.cell, gr.n_slots, d_sbo, S, d_stage);
