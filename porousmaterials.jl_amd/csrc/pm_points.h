@@ -109,6 +109,8 @@ struct SegArgs {
     int w0;               // Widom: first insertion workgroup of the run in this segment
     int wg_base;          // first gen/combine workgroup (Widom) or brick (grid) of the segment
     unsigned cell_base;   // Widom: first cell of the segment
+    int gen_base;         // Widom: first generator workgroup of the segment
+    int pad;
     long long slot_base;  // first point slot of the segment (a multiple of WG)
     long long n_slots;    // point slots of the segment (grid: the grid's points)
 };
@@ -150,7 +152,7 @@ __device__ inline gconst_ptr<T> uniform_ptr(const T* p) {
     return reinterpret_cast<gconst_ptr<T>>(((unsigned long long)hi << 32) | lo);
 }
 
-enum { SEG_BY_WG = 0, SEG_BY_CELL = 1, SEG_BY_SLOT = 2 };
+enum { SEG_BY_WG = 0, SEG_BY_CELL = 1, SEG_BY_SLOT = 2, SEG_BY_GEN = 3 };
 
 // Last segment whose first workgroup / cell / slot is <= v.
 template <int KEY>
@@ -158,8 +160,9 @@ __device__ inline int find_segment(const SegArgs* s, int n, long long v) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const long long k = KEY == SEG_BY_WG ? (long long)s[mid].wg_base
+        const long long k = KEY == SEG_BY_WG     ? (long long)s[mid].wg_base
                             : KEY == SEG_BY_CELL ? (long long)s[mid].cell_base
+                            : KEY == SEG_BY_GEN  ? (long long)s[mid].gen_base
                                                  : s[mid].slot_base;
         if (k <= v) lo = mid;
         else hi = mid - 1;
@@ -215,13 +218,16 @@ __device__ inline bool access_open(const RunArgs& a, const double* u) {
 // look like padding); the combine kernels give it E_vdw = E_es = +Inf
 // (src/henry.jl:225-232).
 //
-// Binning is the first of two counting passes.  Cells come in super-bins of
+// A generator workgroup covers GEN_CHUNKS consecutive insertion workgroups of
+// one segment (WG insertions each).  Binning is the first of two counting
+// passes.  Cells come in super-bins of
 // 2^sb_shift consecutive cells; the workgroup counts its points per super-bin
 // in LDS and takes each super-bin's slots with one global atomic on one of
 // SB_COPIES counters of the super-bin (workgroup index mod SB_COPIES, so that
 // no counter serialises every workgroup's atomics), and a point leaves with
 // (cell, copy << 28 | slot within its copy of the super-bin).  One device-wide
 // atomic per point would cost more than the rest of the kernel.
+constexpr int GEN_CHUNKS = 4;       // insertion workgroups per generator workgroup
 constexpr int SB_SPAN_MAX = 4160;   // super-bins one segment's cells may span
 constexpr int SB_CELLS_MAX = 1024;  // cells per super-bin (2^sb_shift)
 constexpr int SB_COPIES = 16;       // counters per super-bin
@@ -232,23 +238,28 @@ constexpr unsigned SB_SLOT_MASK = 0x0fffffffu;
 // the cell index (cells of a group < 2^24).
 constexpr int CELL_ZBINS = 4;
 constexpr unsigned CELL_MASK = 0x00ffffffu;
-__global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
+__global__ __launch_bounds__(WG * GEN_CHUNKS) void widom_gen_kernel(GroupArgs g) {
+    constexpr int GEN_WG = WG * GEN_CHUNKS;
     __shared__ unsigned s_sb[SB_SPAN_MAX];
-    const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
+    const SegArgs sg = g.segs[find_segment<SEG_BY_GEN>(g.segs, g.n_seg, blockIdx.x)];
     const RunArgs& a = g.runs[sg.run];
-    const int lw = (int)blockIdx.x - sg.wg_base;  // workgroup within the segment
-    const int w = sg.w0 + lw;                      // insertion workgroup within the run
-    const int b = w / a.chunks_per_block, c = w - b * a.chunks_per_block;
-    const long long i = (long long)c * WG + threadIdx.x;
-    const bool valid = i < a.n_per_block;
     const int ns = a.nsite;
-    const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
+    const int t = threadIdx.x % WG;
+    // insertion workgroup within the segment; past the segment's last one the
+    // work-item writes nothing (those slots are another segment's)
+    const int lw = ((int)blockIdx.x - sg.gen_base) * GEN_CHUNKS + (int)threadIdx.x / WG;
+    const bool in_seg = (long long)lw * WG * ns < sg.n_slots;
+    const int w = sg.w0 + lw;  // insertion workgroup within the run
+    const int b = w / a.chunks_per_block, c = w - b * a.chunks_per_block;
+    const long long i = (long long)c * WG + t;
+    const bool valid = in_seg && i < a.n_per_block;
+    const long long p0 = sg.slot_base + ((long long)lw * WG + t) * ns;
     const int n0 = a.ncell[0], n1 = a.ncell[1], n2 = a.ncell[2];
     // the super-bins of this segment's cells
     const int S = g.sb_shift;
     const unsigned sb_lo = sg.cell_base >> S;
     const int span = (int)(((sg.cell_base + (unsigned)(n0 * n1 * n2) - 1u) >> S) - sb_lo) + 1;
-    for (int k = threadIdx.x; k < span; k += WG) s_sb[k] = 0u;
+    for (int k = threadIdx.x; k < span; k += GEN_WG) s_sb[k] = 0u;
     double u[6];
     bool emit = valid;
     if (valid) {
@@ -258,7 +269,8 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     }
     __syncthreads();  // the super-bin counts are clear
     unsigned cid[PM_MAX_SITES], slot[PM_MAX_SITES], zb[PM_MAX_SITES];
-    if (!emit) {
+    if (!in_seg) {
+    } else if (!emit) {
         for (int s = 0; s < ns; ++s) {
             g.pos[p0 + s] = make_double4(0.0, 0.0, 0.0, 0.0);
             g.cell[p0 + s] = make_uint2(0xffffffffu, 0u);
@@ -307,7 +319,7 @@ __global__ __launch_bounds__(WG) void widom_gen_kernel(GroupArgs g) {
     }
     __syncthreads();  // the workgroup's counts are complete
     const unsigned copy = blockIdx.x % SB_COPIES;
-    for (int k = threadIdx.x; k < span; k += WG) {
+    for (int k = threadIdx.x; k < span; k += GEN_WG) {
         const unsigned n = s_sb[k];
         if (n != 0u) s_sb[k] = atomicAdd(&g.sb_count[(sb_lo + k) * SB_COPIES + copy], n);
     }

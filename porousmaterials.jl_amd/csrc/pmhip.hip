@@ -984,6 +984,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         size_t seg0 = 0;
         int n_seg = 0;
         long long n_wg = 0, n_slots = 0, n_cells = 0;
+        long long n_gen = 0;  // Widom: generator workgroups (GEN_CHUNKS insertion workgroups each)
         bool es = false;
         int sb_shift = 0;    // Widom: super-bins of 2^sb_shift cells
         long long n_sb = 0;
@@ -1013,11 +1014,13 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
                 s.w0 = (int)w;
                 s.wg_base = (int)cur.n_wg;
                 s.cell_base = (unsigned)cur.n_cells;
+                s.gen_base = (int)cur.n_gen;
                 s.slot_base = cur.n_slots;
                 s.n_slots = take * per_wg;
                 segs.push_back(s);
                 cur.n_seg++;
                 cur.n_wg += take;
+                cur.n_gen += (take + GEN_CHUNKS - 1) / GEN_CHUNKS;
                 cur.n_slots += take * per_wg;
                 cur.n_cells += r.n_cells;
                 cur.es = cur.es || r.es;
@@ -1239,7 +1242,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             const long long n_sbc = n_sb * SB_COPIES;
             hipLaunchKernelGGL(fill_u32_kernel, dim3(grid1d(n_sbc + 1, 256)), dim3(256), 0, g_ctx.stream, d_sbc,
                                n_sbc + 1, 0u, gi == 0 ? d_cand : nullptr);
-            hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
+            hipLaunchKernelGGL(widom_gen_kernel, dim3((unsigned)gr.n_gen), dim3(WG * GEN_CHUNKS), 0, g_ctx.stream, g);
             // offsets of the super-bins' counter copies; their total (the
             // group's points) also closes point_off
             hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_sbc, (int)n_sbc, d_sbo,
