@@ -1130,7 +1130,9 @@ __global__ __launch_bounds__(FOURIER_WG) void point_fourier_kernel(GroupArgs g) 
     // wave-uniform: scalar loop control (the loads are not provably uniform)
     const int Ma = __builtin_amdgcn_readfirstlane(a.Ma), Mb = __builtin_amdgcn_readfirstlane(a.Mb),
               Mc = __builtin_amdgcn_readfirstlane(a.Mc);
-    double2 Eb_neg[FOURIER_PTS], Pa[FOURIER_PTS], Pb[FOURIER_PTS];  // Pb: z_a^ma z_b^mb of the current row
+    // per m_a: T = sum_mb z_b^(mb + Mb) R(ma, mb) by Horner in z_b over the rows
+    // (mb from Mb down to -Mb), then acc += Re(z_a^ma z_b^-Mb T)
+    double2 Eb_neg[FOURIER_PTS], Pa[FOURIER_PTS];  // z_b^-Mb, z_a^ma
     double acc[FOURIER_PTS];
 #pragma unroll
     for (int k = 0; k < FOURIER_PTS; ++k) {
@@ -1138,68 +1140,78 @@ __global__ __launch_bounds__(FOURIER_WG) void point_fourier_kernel(GroupArgs g) 
         const double2 Eb_c = make_double2(P[k].Eb.x, -P[k].Eb.y);
         for (int q = 0; q < Mb; ++q) Eb_neg[k] = cmul(Eb_neg[k], Eb_c);
         Pa[k] = make_double2(1.0, 0.0);
-        Pb[k] = Eb_neg[k];
         acc[k] = 0.0;
     }
 
     const int rowlen = 2 * Mc + 1, nb = 2 * Mb + 1;
-    const int n_rows = (Ma + 1) * nb;
     // (plain scalars through these pointers: the vector types have no copy
     // from that address space)
     const gconst_ptr<double> Abox = uniform_ptr(reinterpret_cast<const double*>(a.A));
     const gconst_ptr<int> krows = uniform_ptr(reinterpret_cast<const int*>(a.krows));
-    for (int r = 0; r < n_rows; ++r) {
-        if (r > 0 && (r % nb) == 0) {  // next m_a
+    for (int ma = 0; ma <= Ma; ++ma) {
+        double Tr[FOURIER_PTS], Ti[FOURIER_PTS];
 #pragma unroll
-            for (int k = 0; k < FOURIER_PTS; ++k) {
-                Pa[k] = cmul(Pa[k], P[k].Ea);
-                Pb[k] = cmul(Pa[k], Eb_neg[k]);
-            }
-        }
-        const int2 lh = make_int2(krows[2 * r], krows[2 * r + 1]);
-        if (lh.x <= lh.y) {
-            const gconst_ptr<double> row = Abox + 2 * ((long long)r * rowlen + Mc);  // mc in [-Mc, Mc]
-            // Every inner FMA carries a negated operand (-q * -z where the sign
-            // is not wanted), which keeps it in the three-operand form that
-            // takes the scalar coefficient directly; the products are exact
-            // negations of each other, so the sums are unchanged.
-            double qr[FOURIER_PTS], qi[FOURIER_PTS];  // sum_{mc >= 0} A_mc z^mc
+        for (int k = 0; k < FOURIER_PTS; ++k) Tr[k] = Ti[k] = 0.0;
+        for (int jb = nb - 1; jb >= 0; --jb) {
+            const int r = ma * nb + jb;
+            const int2 lh = make_int2(krows[2 * r], krows[2 * r + 1]);
+            double Rr[FOURIER_PTS], Ri[FOURIER_PTS];  // the row's sum_mc A z_c^mc
 #pragma unroll
-            for (int k = 0; k < FOURIER_PTS; ++k) qr[k] = qi[k] = 0.0;
+            for (int k = 0; k < FOURIER_PTS; ++k) Rr[k] = Ri[k] = 0.0;
+            if (lh.x <= lh.y) {
+                const gconst_ptr<double> row = Abox + 2 * ((long long)r * rowlen + Mc);  // mc in [-Mc, Mc]
+                // Every inner FMA carries a negated operand (-q * -z where the sign
+                // is not wanted), which keeps it in the three-operand form that
+                // takes the scalar coefficient directly; the products are exact
+                // negations of each other, so the sums are unchanged.
+                double qr[FOURIER_PTS], qi[FOURIER_PTS];  // sum_{mc >= 0} A_mc z^mc
+#pragma unroll
+                for (int k = 0; k < FOURIER_PTS; ++k) qr[k] = qi[k] = 0.0;
 #pragma unroll 4
-            for (int mc = lh.y; mc >= 0; --mc) {
-                const double Ar = row[2 * mc], Ai = row[2 * mc + 1];
+                for (int mc = lh.y; mc >= 0; --mc) {
+                    const double Ar = row[2 * mc], Ai = row[2 * mc + 1];
+#pragma unroll
+                    for (int k = 0; k < FOURIER_PTS; ++k) {
+                        const double nr = fma(qr[k], P[k].zr, fma(-qi[k], P[k].zi, Ar));
+                        qi[k] = fma(qr[k], P[k].zi, fma(-qi[k], nzr[k], Ai));
+                        qr[k] = nr;
+                    }
+                }
+                double nr_[FOURIER_PTS], ni_[FOURIER_PTS];  // sum_{j >= 1} A_{-j} (z*)^{j-1}
+#pragma unroll
+                for (int k = 0; k < FOURIER_PTS; ++k) nr_[k] = ni_[k] = 0.0;
+#pragma unroll 4
+                for (int j = -lh.x; j >= 1; --j) {
+                    const double Ar = row[-2 * j], Ai = row[-2 * j + 1];
+#pragma unroll
+                    for (int k = 0; k < FOURIER_PTS; ++k) {
+                        const double tr = fma(nr_[k], P[k].zr, fma(-ni_[k], nzi[k], Ar));
+                        ni_[k] = fma(ni_[k], P[k].zr, fma(-nr_[k], P[k].zi, Ai));
+                        nr_[k] = tr;
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < FOURIER_PTS; ++k) {
-                    const double nr = fma(qr[k], P[k].zr, fma(-qi[k], P[k].zi, Ar));
-                    qi[k] = fma(qr[k], P[k].zi, fma(-qi[k], nzr[k], Ai));
-                    qr[k] = nr;
-                }
-            }
-            double nr_[FOURIER_PTS], ni_[FOURIER_PTS];  // sum_{j >= 1} A_{-j} (z*)^{j-1}
-#pragma unroll
-            for (int k = 0; k < FOURIER_PTS; ++k) nr_[k] = ni_[k] = 0.0;
-#pragma unroll 4
-            for (int j = -lh.x; j >= 1; --j) {
-                const double Ar = row[-2 * j], Ai = row[-2 * j + 1];
-#pragma unroll
-                for (int k = 0; k < FOURIER_PTS; ++k) {
-                    const double tr = fma(nr_[k], P[k].zr, fma(-ni_[k], nzi[k], Ar));
-                    ni_[k] = fma(ni_[k], P[k].zr, fma(-nr_[k], P[k].zi, Ai));
-                    nr_[k] = tr;
+                    // R = q + z* n
+                    const double zr = P[k].zr, zi = P[k].zi;
+                    Rr[k] = qr[k] + (zr * nr_[k] + zi * ni_[k]);
+                    Ri[k] = qi[k] + (zr * ni_[k] - zi * nr_[k]);
                 }
             }
 #pragma unroll
             for (int k = 0; k < FOURIER_PTS; ++k) {
-                // R = q + z* n;  acc += Re(Pb R)
-                const double zr = P[k].zr, zi = P[k].zi;
-                const double Rr = qr[k] + (zr * nr_[k] + zi * ni_[k]);
-                const double Ri = qi[k] + (zr * ni_[k] - zi * nr_[k]);
-                acc[k] += Pb[k].x * Rr - Pb[k].y * Ri;
+                // T = T z_b + R
+                const double tr = fma(Tr[k], P[k].Eb.x, fma(-Ti[k], P[k].Eb.y, Rr[k]));
+                Ti[k] = fma(Tr[k], P[k].Eb.y, fma(Ti[k], P[k].Eb.x, Ri[k]));
+                Tr[k] = tr;
             }
         }
 #pragma unroll
-        for (int k = 0; k < FOURIER_PTS; ++k) Pb[k] = cmul(Pb[k], P[k].Eb);
+        for (int k = 0; k < FOURIER_PTS; ++k) {
+            const double2 Q = cmul(Pa[k], Eb_neg[k]);
+            acc[k] += Q.x * Tr[k] - Q.y * Ti[k];
+            Pa[k] = cmul(Pa[k], P[k].Ea);
+        }
     }
 #pragma unroll
     for (int k = 0; k < FOURIER_PTS; ++k)
