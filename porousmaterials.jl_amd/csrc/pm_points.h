@@ -81,6 +81,7 @@ struct RunArgs {
     long long n_per_block;
     int rotate, n_blocks, chunks_per_block, block_base;
     int ncell[3];
+    int zbins;               // slabs per cell along the third axis (points are ordered by slab)
     double off[PM_MAX_SITES][3];  // site offsets from the COM (Cartesian)
     double site_q[PM_MAX_SITES];
     double temperature;
@@ -229,14 +230,15 @@ __device__ inline bool access_open(const RunArgs& a, const double* u) {
 // atomic per point would cost more than the rest of the kernel.
 constexpr int GEN_CHUNKS = 4;       // insertion workgroups per generator workgroup
 constexpr int SB_SPAN_MAX = 4160;   // super-bins one segment's cells may span
-constexpr int SB_CELLS_MAX = 1024;  // cells per super-bin (2^sb_shift)
+constexpr int SB_KEYS_MAX = 4096;   // (cell, slab) keys per super-bin: 2^sb_shift x the group's slab stride
 constexpr int SB_COPIES = 16;       // counters per super-bin
 constexpr unsigned SB_SLOT_MASK = 0x0fffffffu;
-// Within a cell the points are ordered by CELL_ZBINS slabs along the cell's
-// third axis, so that a cell's first slice (WG points) takes its low slabs and
-// the workgroup's candidate sphere shrinks; the slab rides in the top bits of
-// the cell index (cells of a group < 2^24).
-constexpr int CELL_ZBINS = 4;
+// Cells are columns along the third axis, and within a cell the points are
+// ordered by `zbins` slabs along that axis, so that a slice of WG consecutive
+// points is a compact piece of the column and the workgroup's candidate sphere
+// is small; the slab rides in the top bits of the cell index (cells of a group
+// < 2^24, slabs <= ZBINS_MAX).
+constexpr int ZBINS_MAX = 255;
 constexpr unsigned CELL_MASK = 0x00ffffffu;
 __global__ __launch_bounds__(WG * GEN_CHUNKS) void widom_gen_kernel(GroupArgs g) {
     constexpr int GEN_WG = WG * GEN_CHUNKS;
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(WG * GEN_CHUNKS) void widom_gen_kernel(GroupArgs g)
             const int ia = min((int)(x * n0), n0 - 1);
             const int ib = min((int)(y * n1), n1 - 1);
             const int ic = min((int)(z * n2), n2 - 1);
-            const int iz = min((int)((z * n2 - ic) * CELL_ZBINS), CELL_ZBINS - 1);
+            const int iz = min((int)((z * n2 - ic) * a.zbins), a.zbins - 1);
             cid[s] = sg.cell_base + (unsigned)(ia + n0 * (ib + n1 * ic));
             zb[s] = (unsigned)iz << 24;
             g.pos[p0 + s] = make_double4(x / repd[0], y / repd[1], z / repd[2], 1.0);
@@ -406,15 +408,15 @@ __global__ void stage_points_kernel(const uint2* cell, long long n_pts, const un
 // the super-bin, their slice offsets (ceil(count / WG) slices per cell), then
 // every point to its (cell, slab)'s stretch of `order`.
 __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, const unsigned* sb_off, int sb_shift,
-                                                            int n_cells, unsigned* count, unsigned* point_off,
-                                                            unsigned* slice_loc, unsigned* sb_slices,
-                                                            unsigned* order) {
-    __shared__ unsigned s_n[SB_CELLS_MAX * CELL_ZBINS];
+                                                            int zstride, int n_cells, unsigned* count,
+                                                            unsigned* point_off, unsigned* slice_loc,
+                                                            unsigned* sb_slices, unsigned* order) {
+    __shared__ unsigned s_n[SB_KEYS_MAX];
     const int tid = threadIdx.x;
     const int sb = blockIdx.x;
     const int c_lo = sb << sb_shift;
     const int nc = min(1 << sb_shift, n_cells - c_lo);
-    const int nk = nc * CELL_ZBINS;
+    const int nk = nc * zstride;
     const unsigned lo = sb_off[sb * SB_COPIES], hi = sb_off[(sb + 1) * SB_COPIES];
     for (int k = tid; k < nk; k += SCAN_WG) s_n[k] = 0u;
     __syncthreads();
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, 
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             if (cl[j] != 0xffffffffu)
-                atomicAdd(&s_n[((cl[j] & CELL_MASK) - c_lo) * CELL_ZBINS + (cl[j] >> 24)], 1u);
+                atomicAdd(&s_n[((cl[j] & CELL_MASK) - c_lo) * zstride + (cl[j] >> 24)], 1u);
     }
     __syncthreads();
     // each work-item a run of consecutive cells
@@ -440,8 +442,7 @@ __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, 
         const int k = k0 + j;
         if (k < nc) {
             unsigned v = 0u;
-#pragma unroll
-            for (int z = 0; z < CELL_ZBINS; ++z) v += s_n[k * CELL_ZBINS + z];
+            for (int z = 0; z < zstride; ++z) v += s_n[k * zstride + z];
             p += v;
             s += (v + WG - 1) / WG;
         }
@@ -453,10 +454,9 @@ __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, 
         const int k = k0 + j;
         if (k < nc) {
             const unsigned p0 = p;
-#pragma unroll
-            for (int z = 0; z < CELL_ZBINS; ++z) {
-                const unsigned v = s_n[k * CELL_ZBINS + z];
-                s_n[k * CELL_ZBINS + z] = p;  // from here on: the next free position of the slab
+            for (int z = 0; z < zstride; ++z) {
+                const unsigned v = s_n[k * zstride + z];
+                s_n[k * zstride + z] = p;  // from here on: the next free position of the slab
                 p += v;
             }
             count[c_lo + k] = p - p0;
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(SCAN_WG) void cell_sort_kernel(const uint2* stage, 
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             if (e[j].y != 0xffffffffu)
-                order[atomicAdd(&s_n[((e[j].y & CELL_MASK) - c_lo) * CELL_ZBINS + (e[j].y >> 24)], 1u)] = e[j].x;
+                order[atomicAdd(&s_n[((e[j].y & CELL_MASK) - c_lo) * zstride + (e[j].y >> 24)], 1u)] = e[j].x;
     }
 }
 

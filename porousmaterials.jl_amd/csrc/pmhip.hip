@@ -41,15 +41,15 @@ constexpr double DEFAULT_CELL_POINTS = 448.0; // target points per Widom cell
 constexpr long long MAX_POINT_SLOTS = 1LL << 24;  // point slots per launch group
 constexpr long long MAX_IMAGES = 1LL << 26;       // images per list
 constexpr size_t MAX_IMAGE_LISTS = 8;             // image lists (radii) per framework
-constexpr double MAX_RUN_CELLS = 1 << 22;         // cells of one Widom run
 constexpr long long MAX_GROUP_CELLS = 1LL << 24;  // cells of one launch group
 constexpr double SB_POINTS = 32768.0;             // target points per super-bin of the Widom sort
-constexpr int SB_SHIFT_MAX = 10;                  // 2^10 = SB_CELLS_MAX cells per super-bin
-static_assert((1 << SB_SHIFT_MAX) == pmk::SB_CELLS_MAX, "super-bin cells fit the sort kernel's LDS");
+constexpr double ZSLAB_POINTS = 64.0;             // target points per slab of a Widom cell
+constexpr int MAX_RUN_CELLS = 65536;              // cells of one Widom run
 static_assert(MAX_POINT_SLOTS <= (long long)pmk::SB_SLOT_MASK + 1, "a super-bin slot fits below the copy bits");
-static_assert(MAX_GROUP_CELLS <= (long long)pmk::CELL_MASK + 1 && MAX_RUN_CELLS <= (double)pmk::CELL_MASK + 1,
-              "a group's cell index fits below the slab bits");
-static_assert(MAX_RUN_CELLS / (1 << SB_SHIFT_MAX) + 1 <= pmk::SB_SPAN_MAX, "a run's cells span few enough super-bins");
+static_assert(MAX_GROUP_CELLS <= (long long)pmk::CELL_MASK + 1, "a group's cell index fits below the slab bits");
+static_assert(pmk::ZBINS_MAX < 256 && pmk::SB_KEYS_MAX / 16 >= pmk::ZBINS_MAX,
+              "slabs fit the top byte of the cell index, and 16 cells of them a super-bin");
+static_assert(MAX_RUN_CELLS / 16 + 1 <= pmk::SB_SPAN_MAX, "a run's cells span few enough super-bins of 16 cells");
 
 thread_local std::string g_err;
 std::mutex g_mu;
@@ -710,36 +710,66 @@ void fill_run_pair(RunArgs& a, const pm_framework* fw, const pm_forcefield* ff, 
     }
 }
 
-// Cells of a Widom run's counting sort over the first unit cell of the
-// replicated box (where the sites are folded): about cell_points points each,
-// half diagonal <= chunk_r_max where the point density allows, at most
-// MAX_RUN_CELLS cells.
-void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double* cell_r) {
+// The cells of a Widom run over the first unit cell, from h, the edge of a
+// cube holding cell_points points (the tuning knob).  Where such a cube holds
+// at least a slice (WG points), the cells are columns along the third axis
+// with an h x h cross-section and the full height of the unit cell (stacked
+// only when a column would need more than ZBINS_MAX slabs), and within a cell
+// the points are ordered by zbins slabs of about ZSLAB_POINTS points, so that
+// WG consecutive points of a cell fill a compact piece of the column.  Sparser
+// runs keep cubes of edge h and one slab.  chunk_r is the half diagonal of the
+// piece of a cell that a slice fills (the scale of a workgroup's candidate
+// sphere), which the image-list path needs at most chunk_r_max: h shrinks
+// until it is, while the run has at most MAX_RUN_CELLS cells.
+void choose_cells(const pm_framework* fw, double n_points, int ncell[3], double* chunk_r, int* zbins) {
     const double n_cells_box = (double)fw->rep[0] * fw->rep[1] * fw->rep[2];
-    double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]) / n_cells_box;
-    double h = std::cbrt(vol * g_ctx.cell_points / std::max(n_points, 1.0));
-    double scale = 1.0;
-    int prev[3] = {1, 1, 1};
-    double prev_r = 0.0;
+    const double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]) / n_cells_box;
+    const double n_pts = std::max(n_points, 1.0);
+    double h = std::cbrt(vol * g_ctx.cell_points / n_pts);
+    auto axis_cells = [&](int i, double edge) {
+        return (int)std::min(1024.0, std::max(1.0, std::round(fw->width[i] / fw->rep[i] / edge)));
+    };
+    bool have = false;
     for (int iter = 0; iter < 64; ++iter) {
-        for (int i = 0; i < 3; ++i)
-            ncell[i] = (int)std::min(1024.0, std::max(1.0, std::round(fw->width[i] / fw->rep[i] / h * scale)));
-        if (iter > 0 && (double)ncell[0] * ncell[1] * ncell[2] > (double)MAX_RUN_CELLS) {
-            for (int i = 0; i < 3; ++i) ncell[i] = prev[i];
-            *cell_r = prev_r;
-            return;
+        int n[3] = {axis_cells(0, h), axis_cells(1, h), axis_cells(2, h)};
+        int zb = 1;
+        if (n_pts / ((double)n[0] * n[1] * n[2]) >= WG) {
+            const double pts_col = n_pts / ((double)n[0] * n[1]);
+            const long long zb_want = std::max(1LL, std::llround(pts_col / ZSLAB_POINTS));
+            n[2] = (int)((zb_want + ZBINS_MAX - 1) / ZBINS_MAX);
+            zb = (int)std::max(1LL, std::llround((double)zb_want / n[2]));
         }
+        const double cells = (double)n[0] * n[1] * n[2];
+        if (cells > (double)MAX_RUN_CELLS) {
+            if (have) return;  // the finer cells of the previous step stand
+            h *= 1.1;          // too fine from the start: coarser
+            continue;
+        }
+        // the piece of a cell that WG consecutive points fill
+        const double frac_c = std::min(1.0, (double)WG * cells / n_pts);
         double ea[3], eb[3], ec[3];
         for (int i = 0; i < 3; ++i) {
-            ea[i] = fw->F[i] / (ncell[0] * fw->rep[0]);
-            eb[i] = fw->F[3 + i] / (ncell[1] * fw->rep[1]);
-            ec[i] = fw->F[6 + i] / (ncell[2] * fw->rep[2]);
+            ea[i] = fw->F[i] / (n[0] * fw->rep[0]);
+            eb[i] = fw->F[3 + i] / (n[1] * fw->rep[1]);
+            ec[i] = fw->F[6 + i] / (n[2] * fw->rep[2]) * frac_c;
         }
-        *cell_r = half_diagonal(ea, eb, ec);
-        if (*cell_r <= g_ctx.chunk_r_max) return;
-        for (int i = 0; i < 3; ++i) prev[i] = ncell[i];
-        prev_r = *cell_r;
-        scale *= 1.25;
+        for (int i = 0; i < 3; ++i) ncell[i] = n[i];
+        *zbins = zb;
+        *chunk_r = half_diagonal(ea, eb, ec);
+        have = true;
+        if (*chunk_r <= g_ctx.chunk_r_max) return;
+        h /= 1.25;
+    }
+    if (!have) {  // (not reached for cell_points >= 1; one cell, all-atoms path)
+        double ea[3], eb[3], ec[3];
+        for (int i = 0; i < 3; ++i) {
+            ea[i] = fw->F[i] / fw->rep[0];
+            eb[i] = fw->F[3 + i] / fw->rep[1];
+            ec[i] = fw->F[6 + i] / fw->rep[2];
+        }
+        ncell[0] = ncell[1] = ncell[2] = 1;
+        *zbins = 1;
+        *chunk_r = half_diagonal(ea, eb, ec);
     }
 }
 
@@ -811,7 +841,7 @@ int widom_prepare(const pm_widom_run& in, PointRun& r) {
         a.site_q[s] = pr->site[s].q;
     }
     const double pts = std::min((double)in.n_blocks * (double)in.n_per_block * ns, (double)MAX_POINT_SLOTS);
-    choose_cells(fw, pts, a.ncell, &a.chunk_r);
+    choose_cells(fw, pts, a.ncell, &a.chunk_r, &a.zbins);
     r.n_cells = (int64_t)a.ncell[0] * a.ncell[1] * a.ncell[2];
     const double vol = std::fabs(fw->F[0] * fw->F[4] * fw->F[8]);
     r.cost = (double)fw->n_rep / vol * std::pow(a.r_cmax + a.chunk_r, 3);
@@ -987,6 +1017,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         long long n_gen = 0;  // Widom: generator workgroups (GEN_CHUNKS insertion workgroups each)
         bool es = false;
         int sb_shift = 0;    // Widom: super-bins of 2^sb_shift cells
+        int zstride = 1;     // Widom: the sort's slabs per cell (the largest of the group's runs)
         long long n_sb = 0;
     };
     std::vector<SegArgs> segs;
@@ -1049,17 +1080,23 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         max_cells = std::max(max_cells, gr.n_cells);
         max_items = std::max(max_items, gr.n_cells + (gr.n_slots + WG - 1) / WG + 1);
         if (mode != MODE_WIDOM) continue;
-        // super-bins of about SB_POINTS points, and no segment's cells spanning
-        // more than SB_SPAN_MAX of them
+        // super-bins of about SB_POINTS points; the sort's (cell, slab) keys
+        // of a super-bin (2^S cells x the group's largest slab count) within
+        // SB_KEYS_MAX, and no segment's cells spanning more than SB_SPAN_MAX
+        int zstride = 1;
+        for (int si = 0; si < gr.n_seg; ++si) zstride = std::max(zstride, runs[segs[gr.seg0 + si].run].ra.zbins);
+        int s_max = 0;
+        while (((2LL << s_max) * zstride) <= SB_KEYS_MAX) ++s_max;
         const double per_cell = (double)gr.n_slots / (double)std::max(gr.n_cells, 1LL);
         int S = (int)std::lround(std::log2(std::max(1.0, SB_POINTS / std::max(per_cell, 1.0))));
-        S = std::min(std::max(S, 0), SB_SHIFT_MAX);
+        S = std::min(std::max(S, 0), s_max);
         for (int si = 0; si < gr.n_seg; ++si) {
             const SegArgs& sg = segs[gr.seg0 + si];
             const long long c0 = sg.cell_base, c1 = c0 + runs[sg.run].n_cells - 1;
-            while (S < SB_SHIFT_MAX && (c1 >> S) - (c0 >> S) + 1 > SB_SPAN_MAX) ++S;
+            while (S < s_max && (c1 >> S) - (c0 >> S) + 1 > SB_SPAN_MAX) ++S;
             if ((c1 >> S) - (c0 >> S) + 1 > SB_SPAN_MAX) return fail(PM_ERR_ARG, "too many Widom cells in one run");
         }
+        gr.zstride = zstride;
         gr.sb_shift = S;
         gr.n_sb = (gr.n_cells + (1LL << S) - 1) >> S;
         max_sb = std::max(max_sb, gr.n_sb);
@@ -1250,7 +1287,8 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
             hipLaunchKernelGGL(stage_points_kernel, dim3(grid1d(gr.n_slots, 256)), dim3(256), 0, g_ctx.stream,
                                g.cell, gr.n_slots, d_sbo, S, d_stage);
             hipLaunchKernelGGL(cell_sort_kernel, dim3((unsigned)n_sb), dim3(SCAN_WG), 0, g_ctx.stream, d_stage, d_sbo,
-                               S, (int)n_cells, d_count, d_poff, d_sloc, d_sbs, const_cast<unsigned*>(g.order));
+                               S, gr.zstride, (int)n_cells, d_count, d_poff, d_sloc, d_sbs,
+                               const_cast<unsigned*>(g.order));
             hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(SCAN_WG), 0, g_ctx.stream, d_sbs, (int)n_sb, d_sbso,
                                d_nitems);
             hipLaunchKernelGGL(slice_items_kernel, dim3(grid1d(n_cells, 256)), dim3(256), 0, g_ctx.stream, d_count,
@@ -2247,7 +2285,7 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
     // candidate giving the same layout runs the same launches, so it could only
     // win by timing noise and is not timed again (the wider radius limit, timed
     // first, stands).
-    std::vector<std::array<int, 4>> seen;
+    std::vector<std::array<int, 5>> seen;
     for (double cr : crs)
         for (double cp : cps) {
             g_ctx.cell_points = cp;
@@ -2260,7 +2298,8 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
                     return rc;
                 }
                 const RunArgs& ra = runs[0].ra;
-                const std::array<int, 4> layout{ra.ncell[0], ra.ncell[1], ra.ncell[2], ra.chunk_r <= cr ? 1 : 0};
+                const std::array<int, 5> layout{ra.ncell[0], ra.ncell[1], ra.ncell[2], ra.zbins,
+                                                ra.chunk_r <= cr ? 1 : 0};
                 if (std::find(seen.begin(), seen.end(), layout) != seen.end()) continue;
                 seen.push_back(layout);
             }
