@@ -310,7 +310,9 @@ int pm_energy_grid_batch(int32_t n_runs, const pm_grid_run* runs, int32_t precis
  * Point-engine tuning (no counterpart in the reference, whose CPU loops have
  * no tiling).  The chunk shapes only schedule work: every energy is a
  * function of the point and its run alone, so no setting changes a result.
- *   PM_TUNE_CELL_POINTS  target points per Widom cell (default 448)
+ *   PM_TUNE_CELL_POINTS  points in a cube whose edge sets the cross-section
+ *                        of a Widom cell (a column along the third axis, or
+ *                        that cube for sparse runs; default 320)
  *   PM_TUNE_CHUNK_R_MAX  largest chunk radius (A) served by the image-list
  *                        path; wider chunks take the all-atoms path (default 6)
  * Settings are process-wide.

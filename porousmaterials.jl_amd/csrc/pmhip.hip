@@ -37,7 +37,7 @@ namespace {
 // Point-engine tunables.
 // Point-engine tunables (defaults; pm_set_tuning / pm_autotune_widom change them).
 constexpr double DEFAULT_CHUNK_R_MAX = 6.0;   // A: largest chunk radius of the image-list path
-constexpr double DEFAULT_CELL_POINTS = 448.0; // target points per Widom cell
+constexpr double DEFAULT_CELL_POINTS = 320.0; // points in the cube that sets a Widom cell's cross-section
 constexpr long long MAX_POINT_SLOTS = 1LL << 24;  // point slots per launch group
 constexpr long long MAX_IMAGES = 1LL << 26;       // images per list
 constexpr size_t MAX_IMAGE_LISTS = 8;             // image lists (radii) per framework
@@ -2270,9 +2270,8 @@ int pm_autotune_widom(const pm_framework* fw, const pm_forcefield* ff, const pm_
         std::vector<PointRun> probe_run(1);
         if (int rc = widom_prepare(in, probe_run[0])) return rc;  // validates the arguments
     }
-    // cells of about 448 points fill 1.75 workgroups with a Poisson spread that
-    // rarely spills into a third, one-wave workgroup (512 often does); the
-    // timing is not monotonic in the cell size, so the list is dense there
+    // the cube that sets a cell's cross-section (or, for sparse runs, the cell);
+    // the timing is not monotonic in it, so the list is dense at 256-640
     const double cps[] = {64.0, 128.0, 192.0, 256.0, 320.0, 384.0, 448.0, 512.0, 576.0, 640.0, 768.0, 896.0, 1280.0};
     // the widest radius limit first: a narrower one must be more than 1% faster
     // to be kept, since a wider limit keeps sparser runs (other frameworks,
