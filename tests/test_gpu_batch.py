@@ -137,14 +137,15 @@ def test_chunk_tuning_changes_no_result(pm):
     """Per-point energies do not depend on the cell layout: pm_widom gives the
     same block sums bit for bit under any (cell points, chunk radius) the
     image-list path serves, and after pm_autotune_widom."""
-    say("case: CO2 5 x 20000 under five chunk tunings and after the autotuner (mixed)")
+    say("case: CO2 5 x 20000 under six chunk tunings and after the autotuner (mixed)")
     case = pm.synthetic.co2_widom_case()
     fw, ff, ew, pr, _ = _handles(pm, case, "CO2")
     before = pm.device.get_tuning()
     try:
         pm.device.set_tuning(640, 3.0)
         want = pm.device.widom(fw, ff, ew, pr, 298.0, 11, 5, 20000, "mixed")
-        for cp, cr in ((192, 3.0), (256, 3.0), (1280, 3.0), (1280, 6.0), (384, 4.5)):
+        # (4096: 4 x 4 columns of about 19,000 points, two stacked cells of 146 slabs each)
+        for cp, cr in ((192, 3.0), (256, 3.0), (1280, 3.0), (1280, 6.0), (384, 4.5), (4096, 6.0)):
             pm.device.set_tuning(cp, cr)
             got = pm.device.widom(fw, ff, ew, pr, 298.0, 11, 5, 20000, "mixed")
             assert np.array_equal(got, want), (cp, cr)
