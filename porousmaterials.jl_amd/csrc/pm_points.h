@@ -1167,7 +1167,7 @@ __global__ __launch_bounds__(FOURIER_WG) void point_fourier_kernel(GroupArgs g) 
                 double qr[FOURIER_PTS], qi[FOURIER_PTS];  // sum_{mc >= 0} A_mc z^mc
 #pragma unroll
                 for (int k = 0; k < FOURIER_PTS; ++k) qr[k] = qi[k] = 0.0;
-#pragma unroll 4
+#pragma unroll 2
                 for (int mc = lh.y; mc >= 0; --mc) {
                     const double Ar = row[2 * mc], Ai = row[2 * mc + 1];
 #pragma unroll
@@ -1180,7 +1180,7 @@ __global__ __launch_bounds__(FOURIER_WG) void point_fourier_kernel(GroupArgs g) 
                 double nr_[FOURIER_PTS], ni_[FOURIER_PTS];  // sum_{j >= 1} A_{-j} (z*)^{j-1}
 #pragma unroll
                 for (int k = 0; k < FOURIER_PTS; ++k) nr_[k] = ni_[k] = 0.0;
-#pragma unroll 4
+#pragma unroll 2
                 for (int j = -lh.x; j >= 1; --j) {
                     const double Ar = row[-2 * j], Ai = row[-2 * j + 1];
 #pragma unroll
