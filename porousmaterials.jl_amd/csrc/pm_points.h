@@ -1498,6 +1498,7 @@ __global__ __launch_bounds__(FOURIER_WG) void point_fourier_grad_kernel(GroupArg
             for (int k = 0; k < FOURIER_PTS; ++k) Q[k] = cmul(Pb[k], Z[k]);
             // entry (r, mc): 2 GT doubles from At + 2 GT ((r rowlen) + mc + Mc)
             const gconst_ptr<double> row = At + 2LL * GT * ((long long)r * rowlen + Mc);
+#pragma unroll 2
             for (int mc = lo; mc <= hi; ++mc) {
                 const gconst_ptr<double> e = row + 2LL * GT * mc;
 #pragma unroll
