@@ -1531,6 +1531,9 @@ __global__ __launch_bounds__(FOURIER_WG) void point_fourier_grad_kernel(GroupArg
 //   dE/d sigma_t = sum_s 4 RE_st d eps_ts/d sigma_t + RS_st d sigma_ts/d sigma_t
 //   dE/d q_t     = sum_s q_s (RQ_st / (4 pi eps0) + LR_st)
 // Workgroup partials: (S, W_vdw, W_es, then w dE/d theta for every type).
+// NT >= every run's type count (the group's largest): a site's 4 NT sums are
+// loaded together, ahead of the arithmetic, and types a run lacks add nothing.
+template <int NT>
 __global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
     const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
     const RunArgs& a = g.runs[sg.run];
@@ -1549,20 +1552,31 @@ __global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
         vdw = INFINITY;
         es = INFINITY;
     } else if (valid) {
-        for (int s = 0; s < ns; ++s) {
+        const long long st = g.g_stride;
+#pragma unroll
+        for (int s = 0; s < PM_MAX_SITES; ++s) {
+            if (s >= ns) break;
             vdw += g.out_lj[p0 + s];
             if (a.esm != 0) es += a.site_q[s] * (g.out_sr[p0 + s] * a.inv4pe + g.out_lr[p0 + s]);
+            // (every sum array exists for GT types; those a run lacks hold zeros
+            // or, for a run without charges, are not used)
             const double* gs = g.out_g + (p0 + s);
-            const long long st = g.g_stride;
+            double w0[NT], w1[NT], w2[NT], w3[NT];
 #pragma unroll
-            for (int t = 0; t < GT; ++t) {
+            for (int t = 0; t < NT; ++t) {
+                w0[t] = gs[t * st];
+                w1[t] = gs[(GT + t) * st];
+                w2[t] = gs[(2 * GT + t) * st];
+                w3[t] = gs[(3 * GT + t) * st];
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
                 if (t < a.n_gt) {
                     const double4 J = a.jac[t * ns + s];
-                    const double re = 4.0 * gs[t * st], rs = gs[(GT + t) * st];
+                    const double re = 4.0 * w0[t], rs = w1[t];
                     de[t] += re * J.x + rs * J.z;
                     dsg[t] += re * J.y + rs * J.w;
-                    if (a.esm != 0)
-                        dq[t] += a.site_q[s] * (gs[(2 * GT + t) * st] * a.inv4pe + gs[(3 * GT + t) * st]);
+                    if (a.esm != 0) dq[t] += a.site_q[s] * (w2[t] * a.inv4pe + w3[t]);
                 }
             }
         }

@@ -929,6 +929,25 @@ void fourier_grad_launch_t(const GroupArgs& g, unsigned n_wg) {
     hipLaunchKernelGGL(point_fourier_grad_kernel<NT>, dim3(n_wg), dim3(FOURIER_WG), 0, g_ctx.stream, g);
 }
 
+template <int NT>
+void combine_grad_launch_t(const GroupArgs& g, unsigned n_wg) {
+    hipLaunchKernelGGL(widom_combine_grad_kernel<NT>, dim3(n_wg), dim3(WG), 0, g_ctx.stream, g);
+}
+
+// The gradient combine for the group's largest type count.
+void combine_grad_launch(const GroupArgs& g, unsigned n_wg, int n_gt) {
+    switch (n_gt) {
+        case 1: combine_grad_launch_t<1>(g, n_wg); break;
+        case 2: combine_grad_launch_t<2>(g, n_wg); break;
+        case 3: combine_grad_launch_t<3>(g, n_wg); break;
+        case 4: combine_grad_launch_t<4>(g, n_wg); break;
+        case 5: combine_grad_launch_t<5>(g, n_wg); break;
+        case 6: combine_grad_launch_t<6>(g, n_wg); break;
+        case 7: combine_grad_launch_t<7>(g, n_wg); break;
+        default: combine_grad_launch_t<GT>(g, n_wg); break;
+    }
+}
+
 // The Fourier sums' work is proportional to the type count: one kernel per count.
 void fourier_grad_launch(const GroupArgs& g, unsigned n_wg, int n_gt) {
     switch (n_gt) {
@@ -1335,7 +1354,7 @@ int execute_points(std::vector<PointRun>& runs, int mode, int precision, double*
         }
         if (grad && gr.es) fourier_grad_launch(g, (unsigned)slot_wg, n_gt);
         if (widom && grad)
-            hipLaunchKernelGGL(widom_combine_grad_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
+            combine_grad_launch(g, (unsigned)gr.n_wg, n_gt);
         else if (widom)
             hipLaunchKernelGGL(widom_combine_kernel, dim3((unsigned)gr.n_wg), dim3(WG), 0, g_ctx.stream, g);
         else hipLaunchKernelGGL(grid_combine_kernel, dim3((unsigned)slot_wg), dim3(WG), 0, g_ctx.stream, g);
