@@ -221,23 +221,23 @@ __device__ inline bool access_open(const RunArgs& a, const double* u) {
 //
 // A generator workgroup covers GEN_CHUNKS consecutive insertion workgroups of
 // one segment (WG insertions each).  Binning is the first of two counting
-// passes.  Cells come in super-bins of
-// 2^sb_shift consecutive cells; the workgroup counts its points per super-bin
-// in LDS and takes each super-bin's slots with one global atomic on one of
-// SB_COPIES counters of the super-bin (workgroup index mod SB_COPIES, so that
-// no counter serialises every workgroup's atomics), and a point leaves with
-// (cell, copy << 28 | slot within its copy of the super-bin).  One device-wide
-// atomic per point would cost more than the rest of the kernel.
+// passes.  Cells come in super-bins of 2^sb_shift consecutive cells; the
+// workgroup counts its points per super-bin in LDS and takes each super-bin's
+// slots with one global atomic on one of SB_COPIES counters of the super-bin
+// (workgroup index mod SB_COPIES, so that no counter serialises every
+// workgroup's atomics), and a point leaves with (slab << 24 | cell,
+// copy << 28 | slot within its copy of the super-bin).  One device-wide atomic
+// per point would cost more than the rest of the kernel.
 constexpr int GEN_CHUNKS = 4;       // insertion workgroups per generator workgroup
 constexpr int SB_SPAN_MAX = 4160;   // super-bins one segment's cells may span
 constexpr int SB_KEYS_MAX = 4096;   // (cell, slab) keys per super-bin: 2^sb_shift x the group's slab stride
 constexpr int SB_COPIES = 16;       // counters per super-bin
 constexpr unsigned SB_SLOT_MASK = 0x0fffffffu;
-// Cells are columns along the third axis, and within a cell the points are
-// ordered by `zbins` slabs along that axis, so that a slice of WG consecutive
-// points is a compact piece of the column and the workgroup's candidate sphere
-// is small; the slab rides in the top bits of the cell index (cells of a group
-// < 2^24, slabs <= ZBINS_MAX).
+// Cells are columns along the third axis (cubes, with one slab, for sparse
+// runs), and within a cell the points are ordered by `zbins` slabs along that
+// axis, so that a slice of WG consecutive points is a compact piece of the
+// column and the workgroup's candidate sphere is small; the slab rides in the
+// top bits of the cell index (cells of a group < 2^24, slabs <= ZBINS_MAX).
 constexpr int ZBINS_MAX = 255;
 constexpr unsigned CELL_MASK = 0x00ffffffu;
 __global__ __launch_bounds__(WG * GEN_CHUNKS) void widom_gen_kernel(GroupArgs g) {
