@@ -121,6 +121,29 @@ def test_co2_workload_full_size_vs_oracle(pm, co2_full, precision):
         assert math.isclose(rs[k], ro[k], rel_tol=KH_RTOL[precision]), (k, rs[k], ro[k])
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_co2_workload_full_size_cell_layouts(pm, co2_full, precision):
+    """Every insertion's energies and every block sum of the 10^6-insertion
+    CO2 workload are the same bit for bit under cube cells (64 points: one
+    wave each), column cells (320: 20 x 23 columns of 102 slabs) and stacked
+    column cells (4096: 9 x 10 x 3 cells of 174 slabs), all on the image-list
+    path: the cell layout only schedules work."""
+    fw, ff, ew, pr = co2_full.handles
+    before = pm.device.get_tuning()
+    try:
+        got = []
+        for cp in (64, 320, 4096):
+            say(f"   {precision}: cell points {cp}")
+            pm.device.set_tuning(cp, 6.0)
+            got.append(pm.device.widom(fw, ff, ew, pr, 298.0, co2_full.seed, N_BLOCKS, N_PER_BLOCK, precision,
+                                       return_energies=True))
+        for cp, (sums, en) in zip((320, 4096), got[1:]):
+            assert np.array_equal(sums, got[0][0]), cp
+            assert np.array_equal(en, got[0][1]), cp
+    finally:
+        pm.device.set_tuning(before["cell_points"], before["chunk_r_max"])
+
+
 @pytest.fixture(scope="module")
 def co2_dual_full(pm, orc, co2_full):
     """The oracle's dual-number derivatives dE_i/d(eps_t, sigma_t, q_t) of
