@@ -1531,54 +1531,41 @@ __global__ __launch_bounds__(FOURIER_WG) void point_fourier_grad_kernel(GroupArg
 //   dE/d sigma_t = sum_s 4 RE_st d eps_ts/d sigma_t + RS_st d sigma_ts/d sigma_t
 //   dE/d q_t     = sum_s q_s (RQ_st / (4 pi eps0) + LR_st)
 // Workgroup partials: (S, W_vdw, W_es, then w dE/d theta for every type).
-// NT >= every run's type count (the group's largest): a site's 4 NT sums are
-// loaded together, ahead of the arithmetic, and types a run lacks add nothing.
+// Two passes over the workgroup's WG nsite contiguous slots.  First a
+// work-item per insertion: energies, weight and the (S, W) terms, exactly as
+// widom_combine_kernel, with the weight (or -1 for an insertion that adds
+// nothing) left in LDS.  Then a work-item per slot, lane-contiguous so the
+// 4 NT per-type sum arrays are read coalesced: slot j = insertion j / nsite,
+// site j % nsite adds w (Jacobian . sums) for every type.  NT >= every run's
+// type count (the group's largest); types a run lacks add nothing.
 template <int NT>
 __global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
+    __shared__ double s_w[WG];
+    __shared__ double4 s_jac[NT * PM_MAX_SITES];
+    __shared__ double s_q[PM_MAX_SITES];
+    const int tid = threadIdx.x;
     const SegArgs sg = g.segs[find_segment<SEG_BY_WG>(g.segs, g.n_seg, blockIdx.x)];
     const RunArgs& a = g.runs[sg.run];
     const int lw = (int)blockIdx.x - sg.wg_base, w = sg.w0 + lw;
     const int b = w / a.chunks_per_block, c = w - b * a.chunks_per_block;
-    const long long i = (long long)c * WG + threadIdx.x;
+    const long long i = (long long)c * WG + tid;
     const bool valid = i < a.n_per_block;
-    const int ns = a.nsite;
-    const long long p0 = sg.slot_base + ((long long)lw * WG + threadIdx.x) * ns;
+    const int ns = a.nsite, n_gt = a.n_gt;
+    const long long q0 = sg.slot_base + (long long)lw * WG * ns;  // the workgroup's first slot
+    const long long p0 = q0 + (long long)tid * ns;
+    if (tid < n_gt * ns) s_jac[tid] = a.jac[tid];
+    if (tid < ns) s_q[tid] = a.site_q[tid];
     double vdw = 0.0, es = 0.0;
-    double de[GT], dsg[GT], dq[GT];
-#pragma unroll
-    for (int t = 0; t < GT; ++t) { de[t] = 0.0; dsg[t] = 0.0; dq[t] = 0.0; }
     const bool blocked = valid && a.acc != nullptr && g.pos[p0].w == 0.0;
     if (blocked) {
         vdw = INFINITY;
         es = INFINITY;
     } else if (valid) {
-        const long long st = g.g_stride;
 #pragma unroll
         for (int s = 0; s < PM_MAX_SITES; ++s) {
             if (s >= ns) break;
             vdw += g.out_lj[p0 + s];
             if (a.esm != 0) es += a.site_q[s] * (g.out_sr[p0 + s] * a.inv4pe + g.out_lr[p0 + s]);
-            // (every sum array exists for GT types; those a run lacks hold zeros
-            // or, for a run without charges, are not used)
-            const double* gs = g.out_g + (p0 + s);
-            double w0[NT], w1[NT], w2[NT], w3[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                w0[t] = gs[t * st];
-                w1[t] = gs[(GT + t) * st];
-                w2[t] = gs[(2 * GT + t) * st];
-                w3[t] = gs[(3 * GT + t) * st];
-            }
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                if (t < a.n_gt) {
-                    const double4 J = a.jac[t * ns + s];
-                    const double re = 4.0 * w0[t], rs = w1[t];
-                    de[t] += re * J.x + rs * J.z;
-                    dsg[t] += re * J.y + rs * J.w;
-                    if (a.esm != 0) dq[t] += a.site_q[s] * (w2[t] * a.inv4pe + w3[t]);
-                }
-            }
         }
     }
     const double wgt = exp(-(vdw + es) / a.temperature);
@@ -1589,15 +1576,49 @@ __global__ __launch_bounds__(WG) void widom_combine_grad_kernel(GroupArgs g) {
         a.out_e[2 * gi] = vdw;
         a.out_e[2 * gi + 1] = es;
     }
+    s_w[tid] = use ? wgt : -1.0;
     double v[GK];
     v[0] = valid ? wgt : 0.0;
     v[1] = use ? wgt * vdw : 0.0;
     v[2] = use ? wgt * es : 0.0;
+    double de[GT], dsg[GT], dq[GT];
+#pragma unroll
+    for (int t = 0; t < GT; ++t) { de[t] = 0.0; dsg[t] = 0.0; dq[t] = 0.0; }
+    __syncthreads();
+    // (every sum array exists for GT types; those a run lacks hold zeros or,
+    // for a run without charges, are not used)
+    const long long st = g.g_stride;
+    const bool charged = a.esm != 0;
+    for (int j = tid; j < WG * ns; j += WG) {
+        const int ins = j / ns, s = j - ins * ns;
+        const double wj = s_w[ins];
+        if (wj < 0.0) continue;
+        const double* gs = g.out_g + (q0 + j);
+        double w0[NT], w1[NT], w2[NT], w3[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            w0[t] = gs[t * st];
+            w1[t] = gs[(GT + t) * st];
+            w2[t] = charged ? gs[(2 * GT + t) * st] : 0.0;
+            w3[t] = charged ? gs[(3 * GT + t) * st] : 0.0;
+        }
+        const double qs = s_q[s];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            if (t < n_gt) {
+                const double4 J = s_jac[t * ns + s];
+                const double re = 4.0 * w0[t], rs = w1[t];
+                de[t] += wj * (re * J.x + rs * J.z);
+                dsg[t] += wj * (re * J.y + rs * J.w);
+                if (charged) dq[t] += wj * (qs * (w2[t] * a.inv4pe + w3[t]));
+            }
+        }
+    }
 #pragma unroll
     for (int t = 0; t < GT; ++t) {
-        v[3 + 3 * t] = use ? wgt * de[t] : 0.0;
-        v[4 + 3 * t] = use ? wgt * dsg[t] : 0.0;
-        v[5 + 3 * t] = use ? wgt * dq[t] : 0.0;
+        v[3 + 3 * t] = de[t];
+        v[4 + 3 * t] = dsg[t];
+        v[5 + 3 * t] = dq[t];
     }
     double* part = a.partials + (long long)GK * w;
     block_sum_k<GK>(v, part, part + 3);
