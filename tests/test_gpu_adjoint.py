@@ -1,4 +1,5 @@
-"""Adjoint Widom pass (pm_widom_gradient) on the CO2 workload's framework.
+"""Adjoint Widom pass (pm_widom_gradient) on the CO2 workload's framework
+(CO2, and probes of one and of five sites).
 
 d ln K_H / d theta for theta = (eps_t, sigma_t, q_t) of the 5 framework
 species is checked against central finite differences (h = 1e-4 relative) of
@@ -105,20 +106,20 @@ def test_dlnKH_matches_oracle_finite_differences(pm, orc, adj, precision):
     assert err.max() <= 1e-4
 
 
-def _handles(pm, case):
+def _handles(pm, case, probe="CO2"):
     x, ff = case.crystal, case.ljff
     rep = pm.replication_factors(x.box, ff)
     xr = pm.replicate(x, rep)
     ep = pm.setup_Ewald_sum(xr.box, math.sqrt(ff.r2_cutoff), eps=case.ewald_precision, verbose=False)
     fw = pm.device.framework(xr, ff)
     ew = pm.device.ewald(xr, fw, ep)
-    pr = pm.device.probe(case.probes["CO2"], ff)
+    pr = pm.device.probe(case.probes[probe], ff)
     return case, fw, ew, pr
 
 
-def _dual_check(pm, orc, adj, precision, nb, seed):
+def _dual_check(pm, orc, adj, precision, nb, seed, probe="CO2"):
     case, fw, ew, pr = adj
-    ff, x, co2 = case.ljff, case.crystal, case.probes["CO2"]
+    ff, x, co2 = case.ljff, case.crystal, case.probes[probe]
     sums, grads = pm.device.widom_gradient(fw, ff, ew, pr, T, seed, N_BLOCKS, nb, precision, ff)
     lj_off, q_off = pm.site_offsets(co2)
     o = orc.OracleSystem((x.box.a, x.box.b, x.box.c, x.box.alpha, x.box.beta, x.box.gamma), x.xf.T,
@@ -153,7 +154,9 @@ def _dual_check(pm, orc, adj, precision, nb, seed):
         bb, bg = derivative_sum_bounds(v, e, d, cd, nb, TOL["mixed"], T)
         # (a block whose weights all underflow has zero sums and a zero bound)
         rb = float((np.abs(got - want) / np.maximum(bb, TINY_S)).max())
-        rg = float((np.abs(g_dev - g_dual) / bg).max())
+        # (a component with a zero bound, e.g. the charge derivatives of an
+        # uncharged probe, must match exactly)
+        rg = float((np.abs(g_dev - g_dual) / np.maximum(bg, np.finfo(float).tiny)).max())
         say(f"   d ln K_H/d theta max relative deviation {dl.max():.2e}; error / bound: per-block sums {rb:.2e}, "
             f"d ln K_H/d theta {rg:.2e}")
         assert rb <= 1.0 and rg <= 1.0
@@ -194,6 +197,35 @@ def test_gradient_kernels_for_every_species_count(pm, orc, precision, n_species)
     s_w = pm.device.widom(fw, case.ljff, ew, pr, T, 43, 5, 1000, precision)
     assert np.array_equal(s_g, s_w)
     _dual_check(pm, orc, h, precision, 1000, 43)
+
+
+def _five_site_probe(pm):
+    """CO2's three LJ atoms on the x axis and two off-atom charges +-0.4 e on
+    the y axis: five sites, two of them charge-only."""
+    ax = np.array([[0.0, -1.16, 1.16], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    qx = np.array([[0.0, 0.0], [0.55, -0.55], [0.0, 0.0]])
+    return pm.Molecule("CO2_q5", ["C_CO2", "O_CO2", "O_CO2"], ax, [0.4, -0.4], qx, np.zeros(3))
+
+
+@pytest.mark.parametrize("probe", ["CH4", "CO2_q5"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gradient_other_site_counts(pm, orc, precision, probe):
+    """Probes of one site (CH4: uncharged, so no charge derivatives) and of
+    five sites (three LJ-only, two charge-only) on the CO2 workload's
+    framework: block sums equal pm_widom's bit for bit, derivative sums meet
+    the dual-number oracle as above."""
+    case = pm.synthetic.co2_widom_case()
+    case.probes["CH4"] = pm.lj_sphere("CH4")
+    case.probes["CO2_q5"] = _five_site_probe(pm)
+    say(f"case: {probe} probe, 5 x 1000 insertions ({precision})")
+    h = _handles(pm, case, probe)
+    _, fw, ew, pr = h
+    s_g, grads = pm.device.widom_gradient(fw, case.ljff, ew, pr, T, 47, 5, 1000, precision, case.ljff)
+    s_w = pm.device.widom(fw, case.ljff, ew, pr, T, 47, 5, 1000, precision)
+    assert np.array_equal(s_g, s_w)
+    if probe == "CH4":
+        assert np.all(grads[:, :, 2] == 0.0)
+    _dual_check(pm, orc, h, precision, 1000, 47, probe)
 
 
 def test_henry_coefficient_gradient_api(pm):
