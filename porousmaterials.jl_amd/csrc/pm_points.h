@@ -47,6 +47,7 @@ constexpr int SCAN_WG = 1024;      // work-items of the scan kernels
 constexpr int GT = PM_MAX_GRAD_TYPES;  // framework types of a gradient run (fixed stride)
 constexpr int GQ = 4;                  // per-type sums per point: d/d eps, d/d sigma, erfc/r, Fourier
 constexpr int GK = 3 + 3 * GT;         // per-workgroup partials of a gradient run
+static_assert(7 * GT <= 64, "a tile's cumulative type counts (at most 64, 7 bits each) fill one 64-bit word");
 
 // One Widom run or grid of a call (device memory).
 struct RunArgs {
@@ -637,7 +638,8 @@ struct PairLds {
     int* t;         // candidate type * nsite
     float* qf;      // candidate charge in FP32
     int* sel;       // selected tiles of the current batch
-    int* tcnt;      // gradient runs: per wave's tile, the kept candidates of each type
+    unsigned* ends; // gradient runs: per wave's tile, the cumulative kept candidates of types 0..k,
+                    // 7 bits per type (two words)
     const float4* erfc;    // FP32 kernels
     const double2* erfc64;  // FP64 kernels
     int* wcnt;      // per-wave candidate counts
@@ -807,8 +809,16 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                 if (lane == 0) {
                     L.wcnt[wave] = __popcll(m);
                     if constexpr (GRAD) {
+                        // cumulative counts, 7 bits per type (a tile holds at most 64)
+                        unsigned long long ends = 0ull;
+                        int cum = 0;
 #pragma unroll
-                        for (int k = 0; k < NT; ++k) L.tcnt[wave * NT + k] = tc[k];
+                        for (int k = 0; k < NT; ++k) {
+                            cum += tc[k];
+                            ends |= (unsigned long long)cum << (7 * k);
+                        }
+                        L.ends[2 * wave] = (unsigned)ends;
+                        L.ends[2 * wave + 1] = (unsigned)(ends >> 32);
                     }
                 }
                 __syncthreads();
@@ -829,18 +839,23 @@ __device__ __attribute__((always_inline)) inline void chunk_pairs(const RunArgs&
                 if (wave_live) {
                     if constexpr (GRAD) {
                         // tile by tile (the energies' FP32 sums fold per tile, as below),
-                        // type run by type run inside each tile
+                        // type run by type run inside each tile; the runs' ends come
+                        // from one scalar word per tile
                         int j = 0;
                         for (int gi = 0; gi < WG / 64; ++gi) {
                             Real plj = (Real)0, psr = (Real)0;
+                            const int j0 = j;
+                            const unsigned long long ends =
+                                (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(L.ends[2 * gi]) |
+                                (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(L.ends[2 * gi + 1])
+                                    << 32;
 #pragma unroll
                             for (int k = 0; k < NT; ++k) {
-                                const int n = __builtin_amdgcn_readfirstlane(L.tcnt[gi * NT + k]);
-                                if (n == 0) continue;
+                                const int je = j0 + (int)((ends >> (7 * k)) & 127ull);
+                                if (je == j) continue;
                                 const LJP<Real> p = s_lj[k * nsite + site];
                                 const Real cs = s_gc[k * nsite + site];
                                 Real sw = (Real)0, su = (Real)0, sq = (Real)0;
-                                const int je = j + n;
 #pragma unroll 2
                                 for (; j < je; ++j) {
                                     const double4 c = L.c[j];
@@ -941,7 +956,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     constexpr bool GRAD = NT > 0;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_wcnt[WG / 64];
-    __shared__ int s_tcnt[(WG / 64) * (NT > 0 ? NT : 1)];
+    __shared__ unsigned s_ends[(WG / 64) * 2];
     __shared__ double s_sph[(WG / 64) * 8];
     PairLds L;
     L.c = reinterpret_cast<double4*>(smem);
@@ -953,7 +968,7 @@ __global__ __launch_bounds__(WG) void point_pair_kernel(GroupArgs g) {
     double2* s_erfc64 = reinterpret_cast<double2*>(s_erfc);
     L.erfc64 = s_erfc64;
     L.wcnt = s_wcnt;
-    L.tcnt = s_tcnt;
+    L.ends = s_ends;
     L.sph = s_sph;
     LJP<Real>* s_lj = reinterpret_cast<LJP<Real>*>(smem + pair_lds_fixed<Real>());
 
